@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Benchmark of the H(curl) curl-curl + sigma-mass operator path on one MI355X.
+
+Workload (the single-GPU configuration the headline metric is quoted on):
+48^3 hex unit cube, Nedelec order 3 (9,082,800 DOFs), nu = 1, two-region
+sigma/dt (1e4 on elements with centroid in [0.25,0.75]^3, 1e-6 elsewhere), PEC
+on all faces.  A "step" is one FP64 partial-assembly operator apply y = A x over
+the whole mesh, with x, y and the quadrature data resident in HBM.
+
+Printed: one JSON line with value = FP64 apply GDOF/s, a roofline object for the
+apply (algorithmic bytes B = 8*2N + 8*12*E*Q^3 per launch over the measured
+launch time), the CPU baseline (the oracle's OpenMP PA apply on the host cores,
+bounded sample) and the solve figures (FP64 Jacobi-PCG vs FP32 apply + FP64
+refinement, both to a true relative residual of 1e-8).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+FP64_PEAK_TFLOPS = 78.6
+
+METRIC = "ND curl-curl+mass apply GDOF/s (%HBM peak); solves/s + p99 at 1e-8 residual"
+
+
+def two_region(n, inner=1e4, outer=1e-6):
+    c = (np.arange(n) + 0.5) / n
+    ins = (c >= 0.25) & (c <= 0.75)
+    Z, Y, X = np.meshgrid(ins, ins, ins, indexing="ij")
+    return np.where((X & Y & Z).ravel(), inner, outer).astype(np.float64)
+
+
+def apply_bytes(N, E, p, s_vec=8, s_q=8, nrhs=1):
+    Q = p + 1
+    return s_vec * 2 * N * nrhs + s_q * 12 * E * Q ** 3
+
+
+def time_applies(torch, fn, steps, warmup):
+    st = torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(steps):
+        fn()
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps  # ms per apply
+
+
+def cpu_baseline(n, p, sig, budget_s):
+    """The oracle's OpenMP FP64 PA apply on this host (bounded sample)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O
+    O.build()
+    m = O.Mesh(n)
+    qd = O.pa_setup(m, p, 1.0, sig)
+    ess = O.ess_mask(m, p)
+    x = O.rand_uniform(2, O.ndofs(m, p))
+    O.pa_apply(m, p, qd, ess, x)  # warm-up
+    times = []
+    t_start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        O.pa_apply(m, p, qd, ess, x)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start > budget_s or len(times) >= 50:
+            break
+    t = float(np.median(times))
+    N = O.ndofs(m, p)
+    return {"value": N / t / 1e9, "unit": "GDOF/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"{len(times)} FP64 PA applies of the same 48^3 p=3 operator (median "
+                      f"{t * 1e3:.1f} ms each, oracle/hc_oracle.c, OpenMP)",
+            "ms_per_apply": t * 1e3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=48)
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--solve-maxit", type=int, default=20000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "metric": METRIC, "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    import hephaestus_amd as H
+
+    n, p = args.n, args.order
+    sig = two_region(n)
+    E = n ** 3
+    t0 = time.perf_counter()
+    op = H.Operator(n, p, 1.0, sig).setup(1.0)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    N = op.ndofs
+    print(f"[bench] operator {n}^3 p={p} N={N} setup {setup_s:.2f}s device bytes "
+          f"{op.device_bytes / 1e9:.2f} GB", file=sys.stderr, flush=True)
+    x = torch.empty(N, dtype=torch.float64, device="cuda")
+    op.fill_uniform(2, x)
+    y = torch.empty_like(x)
+    ms = time_applies(torch, lambda: op.apply(x, y), args.steps, args.warmup)
+    B = apply_bytes(N, E, p)
+    gdofs = N / (ms * 1e-3) / 1e9
+    achieved = B / (ms * 1e-3) / 1e9
+    print(f"[bench] FP64 apply {ms * 1e3:.1f} us  {gdofs:.1f} GDOF/s  {achieved:.0f} GB/s",
+          file=sys.stderr, flush=True)
+    # FP32 apply
+    x32 = x.float()
+    y32 = torch.empty_like(x32)
+    ms32 = time_applies(torch, lambda: op.apply_f32(x32, y32), args.steps, args.warmup)
+    B32 = apply_bytes(N, E, p, 4, 4)
+    print(f"[bench] FP32 apply {ms32 * 1e3:.1f} us  {N / ms32 / 1e6:.1f} GDOF/s  "
+          f"{B32 / ms32 / 1e6:.0f} GB/s", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC,
+        "value": round(gdofs, 3),
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (splitmix64 x, uniform [-1,1) on free DOFs)",
+        "config": {"workload": f"FP64 PA apply, {n}^3 hex unit cube, ND order {p}, N={N}, nu=1, "
+                               f"sigma/dt two-region 1e4/1e-6, PEC all faces",
+                   "n": n, "order": p, "ndofs": N, "elements": E},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_apply_tile + k_assemble_planes (one apply)",
+                     "bytes_per_launch": B},
+        "fp32_apply": {"ms": round(ms32, 5), "GDOF_s": round(N / ms32 / 1e6, 3),
+                       "GB_s": round(B32 / ms32 / 1e6, 1)},
+    }
+    if not args.no_solve:
+        b = torch.empty_like(x)
+        op.fill_uniform(2, b)
+        sol = {}
+        for name, mode in (("fp64", H.SOLVE_FP64), ("fp32_ir", H.SOLVE_FP32_IR)):
+            xs = torch.zeros_like(b)
+            t0 = time.perf_counter()
+            st = op.solve(b, xs, rtol=1e-8, maxit=args.solve_maxit, mode=mode, raise_on_fail=False)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            sol[name] = {"iterations": st["iterations"], "outer": st["outer_iterations"],
+                         "converged": bool(st["converged"]), "true_rel_res": st["rel_res"],
+                         "solve_ms": round(st["solve_ms"], 3), "wall_s": round(wall, 3),
+                         "solves_per_s": round(1000.0 / st["solve_ms"], 4) if st["solve_ms"] > 0 else None}
+            print(f"[bench] solve {name}: {sol[name]}", file=sys.stderr, flush=True)
+        out["solves"] = sol
+    if not args.no_cpu:
+        del op
+        torch.cuda.empty_cache()
+        cb = cpu_baseline(n, p, sig, args.cpu_budget)
+        out["cpu_baseline"] = cb
+        print(f"[bench] CPU baseline {cb}", file=sys.stderr, flush=True)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,208 @@
+"""ctypes binding of libhcurl.so (the C ABI in include/hc_hcurl.h).
+
+PyTorch is only plumbing here: device buffers are torch tensors whose data
+pointers are handed to the C ABI, and the HIP stream is torch's current stream.
+The library is loaded after torch so that both share one HIP runtime.  There is
+no fallback: if libhcurl.so is missing or no HIP device is present the calls
+raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhcurl.so")
+
+HC_OK = 0
+HC_EINVAL = -1
+HC_ENOMEM = -2
+HC_EHIP = -3
+HC_ENOTCONV = -4
+HC_ENAN = -5
+HC_EBUSY = -6
+
+FACE_ALL = 0x3F
+PREC_FP64 = 0x1
+PREC_FP32 = 0x2
+SOLVE_FP64 = 0
+SOLVE_FP32_IR = 1
+
+# Entry points declared in include/hc_hcurl.h (checked by the CPU test-suite).
+EXPORTS = [
+    "hc_op_create", "hc_op_set_coefficients", "hc_op_setup", "hc_op_ndofs", "hc_op_ess_count",
+    "hc_op_device_bytes", "hc_op_apply", "hc_op_apply_f32", "hc_op_diagonal", "hc_pcg_solve",
+    "hc_op_destroy", "hc_last_error", "hc_fill_uniform",
+]
+
+
+class HcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("vert", C.POINTER(C.c_double)),
+                ("x0", C.c_double * 3), ("len", C.c_double * 3),
+                ("pec_faces", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("outer_iterations", C.c_int), ("converged", C.c_int),
+                ("status", C.c_int), ("rel_res", C.c_double), ("solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def load():
+    """Load libhcurl.so (after torch, so that one HIP runtime is shared)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    import torch  # noqa: F401  (HIP runtime first)
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc) first")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    lib.hc_op_create.argtypes = [C.POINTER(MeshDesc), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.c_int, C.POINTER(vp)]
+    lib.hc_op_set_coefficients.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.hc_op_setup.argtypes = [vp, C.c_double, vp]
+    lib.hc_op_ndofs.argtypes = [vp]
+    lib.hc_op_ndofs.restype = C.c_int64
+    lib.hc_op_ess_count.argtypes = [vp, C.POINTER(C.c_int64)]
+    lib.hc_op_device_bytes.argtypes = [vp]
+    lib.hc_op_device_bytes.restype = C.c_int64
+    lib.hc_op_apply.argtypes = [vp, vp, vp, C.c_int, vp]
+    lib.hc_op_apply_f32.argtypes = [vp, vp, vp, C.c_int, vp]
+    lib.hc_op_diagonal.argtypes = [vp, vp, vp]
+    lib.hc_pcg_solve.argtypes = [vp, vp, vp, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(SolveStats), vp]
+    lib.hc_op_destroy.argtypes = [vp]
+    lib.hc_last_error.restype = C.c_char_p
+    lib.hc_fill_uniform.argtypes = [vp, C.c_uint64, vp, C.c_int64, vp]
+    _lib = lib
+    return lib
+
+
+def _check(rc, what=""):
+    if rc != HC_OK:
+        msg = _lib.hc_last_error().decode()
+        raise HcError(rc, f"{what}: {msg}" if what else msg)
+
+
+def _stream(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+class Operator:
+    """dt*nu curl-curl + sigma mass on a structured hex mesh (ND order p), PEC
+    faces eliminated.  Vectors are torch tensors on the HIP device."""
+
+    def __init__(self, n, order, nu, sigma, precision=PREC_FP64 | PREC_FP32, pec_faces=FACE_ALL,
+                 vert=None, x0=(0.0, 0.0, 0.0), length=(1.0, 1.0, 1.0)):
+        lib = load()
+        if np.isscalar(n):
+            n = (int(n),) * 3
+        self.n = tuple(int(v) for v in n)
+        self.order = int(order)
+        E = self.n[0] * self.n[1] * self.n[2]
+        self.nelem = E
+        nu = np.broadcast_to(np.asarray(nu, dtype=np.float64), (E,)).copy()
+        sigma = np.broadcast_to(np.asarray(sigma, dtype=np.float64), (E,)).copy()
+        self._vert = None if vert is None else np.ascontiguousarray(vert, dtype=np.float64)
+        desc = MeshDesc(self.n[0], self.n[1], self.n[2],
+                        self._vert.ctypes.data_as(C.POINTER(C.c_double)) if self._vert is not None else None,
+                        (C.c_double * 3)(*x0), (C.c_double * 3)(*length), int(pec_faces))
+        h = C.c_void_p()
+        _check(lib.hc_op_create(C.byref(desc), self.order,
+                                nu.ctypes.data_as(C.POINTER(C.c_double)),
+                                sigma.ctypes.data_as(C.POINTER(C.c_double)), int(precision), C.byref(h)),
+               "hc_op_create")
+        self._h = h
+        self.ndofs = int(lib.hc_op_ndofs(h))
+
+    def set_coefficients(self, nu, sigma):
+        E = self.nelem
+        nu = np.broadcast_to(np.asarray(nu, dtype=np.float64), (E,)).copy()
+        sigma = np.broadcast_to(np.asarray(sigma, dtype=np.float64), (E,)).copy()
+        _check(_lib.hc_op_set_coefficients(self._h, nu.ctypes.data_as(C.POINTER(C.c_double)),
+                                           sigma.ctypes.data_as(C.POINTER(C.c_double))), "set_coefficients")
+
+    def setup(self, dt=1.0, stream=None):
+        _check(_lib.hc_op_setup(self._h, float(dt), _stream(stream)), "hc_op_setup")
+        return self
+
+    @property
+    def ess_count(self):
+        v = C.c_int64()
+        _check(_lib.hc_op_ess_count(self._h, C.byref(v)))
+        return int(v.value)
+
+    @property
+    def device_bytes(self):
+        return int(_lib.hc_op_device_bytes(self._h))
+
+    def apply(self, x, y=None, stream=None):
+        import torch
+        assert x.dtype == torch.float64 and x.is_cuda and x.is_contiguous()
+        if y is None:
+            y = torch.empty_like(x)
+        nrhs = 1 if x.dim() == 1 else x.shape[1]
+        _check(_lib.hc_op_apply(self._h, _ptr(x), _ptr(y), nrhs, _stream(stream)), "hc_op_apply")
+        return y
+
+    def apply_f32(self, x, y=None, stream=None):
+        import torch
+        assert x.dtype == torch.float32 and x.is_cuda and x.is_contiguous()
+        if y is None:
+            y = torch.empty_like(x)
+        nrhs = 1 if x.dim() == 1 else x.shape[1]
+        _check(_lib.hc_op_apply_f32(self._h, _ptr(x), _ptr(y), nrhs, _stream(stream)), "hc_op_apply_f32")
+        return y
+
+    def diagonal(self, stream=None):
+        import torch
+        d = torch.empty(self.ndofs, dtype=torch.float64, device="cuda")
+        _check(_lib.hc_op_diagonal(self._h, _ptr(d), _stream(stream)), "hc_op_diagonal")
+        return d
+
+    def solve(self, b, x, rtol=1e-8, maxit=10000, mode=SOLVE_FP64, stream=None, raise_on_fail=True):
+        st = SolveStats()
+        nrhs = 1 if b.dim() == 1 else b.shape[1]
+        rc = _lib.hc_pcg_solve(self._h, _ptr(b), _ptr(x), nrhs, float(rtol), int(maxit), int(mode),
+                               C.byref(st), _stream(stream))
+        if rc != HC_OK and (raise_on_fail or rc not in (HC_ENOTCONV,)):
+            _check(rc, "hc_pcg_solve")
+        d = st.as_dict()
+        d["rc"] = rc
+        return d
+
+    def fill_uniform(self, seed, v, stream=None):
+        _check(_lib.hc_fill_uniform(self._h, C.c_uint64(seed), _ptr(v), v.numel(), _stream(stream)),
+               "hc_fill_uniform")
+        return v
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.hc_op_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

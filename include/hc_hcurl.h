@@ -1,0 +1,155 @@
+/*
+ * hc_hcurl.h -- C ABI of the MI355X-native H(curl) curl-curl + sigma-mass
+ * operator library (libhcurl.so).
+ *
+ * Reference interfaces these entry points replace (paths relative to the
+ * hephaestus source root; MFEM/hypre calls marked [ext]):
+ *
+ *   hc_op_create    mfem::ParBilinearForm on the ND space + the
+ *                   CurlCurlKernel / VectorFEMassKernel coefficient binding:
+ *                   src/kernels/curl_curl_kernel.cpp:11-18 (Init resolves the
+ *                   coefficient), src/kernels/vector_fe_mass_kernel.cpp:11-18,
+ *                   src/equation_systems/equation_system.cpp:324-347
+ *                   (BuildBilinearForms creates the form per test variable).
+ *   hc_op_setup     ParBilinearForm::Assemble [ext] as driven by
+ *                   EquationSystem::BuildBilinearForms (equation_system.cpp:345)
+ *                   and TimeDependentEquationSystem::SetTimeStep
+ *                   (equation_system.cpp:413-426: re-assemble when dt changes);
+ *                   here a partial-assembly setup of quadrature-point factors.
+ *   hc_op_apply     EquationSystem::Mult -> HypreParMatrix::Mult [ext]
+ *                   (equation_system.cpp:200-204) on the system formed by
+ *                   FormLinearSystem (equation_system.cpp:132-190): y = A x with
+ *                   essential (PEC) rows/cols eliminated, A_ess,ess = I.
+ *   hc_op_apply_f32 the same operator applied in FP32 (no reference
+ *                   counterpart; the inner operator of FP32+FP64 refinement).
+ *   hc_op_diagonal  HypreDiagScale's diagonal [ext]
+ *                   (src/solvers/hephaestus_solvers.hpp:52).
+ *   hc_pcg_solve    DefaultJacobiPCGSolver::Mult = HyprePCG + HypreDiagScale
+ *                   [ext] (hephaestus_solvers.hpp:34-57), called from
+ *                   TimeDomainEquationSystemProblemOperator::ImplicitSolve
+ *                   (src/problem_operators/time_domain_equation_system_problem_operator.cpp:46-48).
+ *   hc_op_destroy   ~ParBilinearForm [ext].
+ *   hc_last_error   MFEM_ABORT's message (the reference aborts; this library
+ *                   returns a negative status and keeps the message).
+ *
+ * Conventions
+ *   - Every entry point returns HC_OK (0) or a negative HC_E* status and never
+ *     aborts; hc_last_error() returns the calling thread's last message.
+ *   - Vector arguments are DEVICE pointers (hipMalloc'd or torch-allocated);
+ *     the mesh vertex and coefficient arrays of hc_op_create are HOST pointers.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *   - A handle is not thread-safe: one host thread per handle at a time.
+ *   - DOF numbering: component c = x,y,z has n_d*p points along its own axis
+ *     (open, Gauss-Legendre) and n_d*p+1 along the others (closed,
+ *     Gauss-Lobatto); x-component first, each component x-fastest.  All edges
+ *     and faces are oriented along +axes (no sign flips).
+ *   - Multi-RHS vectors use the layout [dof][rhs] (rhs fastest).
+ */
+#ifndef HC_HCURL_H
+#define HC_HCURL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HC_OK 0
+#define HC_EINVAL (-1)   /* bad argument */
+#define HC_ENOMEM (-2)   /* device allocation failed */
+#define HC_EHIP (-3)     /* HIP runtime error */
+#define HC_ENOTCONV (-4) /* solver did not reach the tolerance in maxit */
+#define HC_ENAN (-5)     /* NaN/Inf met in a reduction */
+#define HC_EBUSY (-6)    /* resource budget exceeded (admission refused) */
+
+/* pec_faces bits: faces with essential tangential (PEC) conditions. */
+#define HC_FACE_XMIN 0x01
+#define HC_FACE_XMAX 0x02
+#define HC_FACE_YMIN 0x04
+#define HC_FACE_YMAX 0x08
+#define HC_FACE_ZMIN 0x10
+#define HC_FACE_ZMAX 0x20
+#define HC_FACE_ALL 0x3F
+
+/* precision flags of hc_op_create */
+#define HC_PREC_FP64 0x1
+#define HC_PREC_FP32 0x2
+
+/* solve modes of hc_pcg_solve */
+#define HC_SOLVE_FP64 0    /* plain FP64 Jacobi-PCG */
+#define HC_SOLVE_FP32_IR 1 /* FP32 Jacobi-PCG inner solves + FP64 refinement */
+
+typedef struct hc_op_s * hc_op_t;
+
+typedef struct
+{
+  int nx, ny, nz;     /* elements per direction */
+  const double * vert; /* host; (nx+1)(ny+1)(nz+1)*3 coords, x fastest; NULL = box */
+  double x0[3];       /* box origin (vert == NULL) */
+  double len[3];      /* box edge lengths (vert == NULL) */
+  int pec_faces;      /* HC_FACE_* bits */
+} hc_mesh_desc;
+
+typedef struct
+{
+  int iterations;       /* total inner PCG iterations (all RHS, max over RHS when batched) */
+  int outer_iterations; /* refinement steps (HC_SOLVE_FP32_IR), else 0 */
+  int converged;        /* 1 if every RHS met rtol */
+  int status;           /* HC_OK or HC_ENOTCONV / HC_ENAN */
+  double rel_res;       /* max over RHS of the final FP64 true relative residual */
+  double solve_ms;      /* device time of the solve (hipEvents) */
+} hc_solve_stats;
+
+/* Create an operator on a structured hex mesh.  nu_e / sigma_e: per-element
+ * curl-curl and mass coefficients (host arrays, nx*ny*nz each).  precision:
+ * HC_PREC_* bits selecting which quadrature-data copies are kept. */
+int hc_op_create(const hc_mesh_desc * mesh,
+                 int order,
+                 const double * nu_e,
+                 const double * sigma_e,
+                 int precision,
+                 hc_op_t * out);
+/* Replace the per-element coefficients (host arrays); takes effect at the next
+ * hc_op_setup. */
+int hc_op_set_coefficients(hc_op_t op, const double * nu_e, const double * sigma_e);
+/* Partial-assembly setup for a(u,w) = (dt*nu curl u, curl w) + (sigma u, w):
+ * FP64 quadrature-point factors (and the FP32 copy when requested) plus the
+ * Jacobi diagonal.  Re-running with an unchanged dt (|ddt| <= 1e-12 dt) and
+ * unchanged coefficients is a no-op. */
+int hc_op_setup(hc_op_t op, double dt, void * stream);
+int64_t hc_op_ndofs(hc_op_t op);
+int hc_op_ess_count(hc_op_t op, int64_t * n_ess);
+/* Device bytes owned by the handle. */
+int64_t hc_op_device_bytes(hc_op_t op);
+
+/* y = A x (PEC eliminated).  x, y: device, layout [dof][rhs]. */
+int hc_op_apply(hc_op_t op, const double * x, double * y, int nrhs, void * stream);
+int hc_op_apply_f32(hc_op_t op, const float * x, float * y, int nrhs, void * stream);
+/* Jacobi diagonal (1 on PEC DOFs) into device array d. */
+int hc_op_diagonal(hc_op_t op, double * d, void * stream);
+
+/* Jacobi-PCG on A x = b (b, x device; x holds the initial guess).  Stopping
+ * rule ||b - A x||_2 <= rtol ||b||_2 per RHS, checked on the recursive residual
+ * and confirmed with the FP64 operator.  maxit bounds the inner iterations of
+ * each PCG; mode HC_SOLVE_FP64 or HC_SOLVE_FP32_IR. */
+int hc_pcg_solve(hc_op_t op,
+                 const double * b,
+                 double * x,
+                 int nrhs,
+                 double rtol,
+                 int maxit,
+                 int mode,
+                 hc_solve_stats * stats,
+                 void * stream);
+
+int hc_op_destroy(hc_op_t op);
+const char * hc_last_error(void);
+
+/* Fill device vector v (n entries) with b_i = 2*u01(splitmix64_i(seed)) - 1,
+ * zeroed on the operator's PEC DOFs when op != NULL. */
+int hc_fill_uniform(hc_op_t op, uint64_t seed, double * v, int64_t n, void * stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
