@@ -242,6 +242,7 @@ ensure_solver_work(hc_op_t op, bool need32)
       return rc;
     if (hipHostMalloc((void **)&op->st_host, 2 * sizeof(hc::PcgState), hipHostMallocDefault) != hipSuccess)
       return fail(HC_ENOMEM, "hipHostMalloc failed");
+    HC_TRY_HIP(hipMemset(op->st_dev, 0, 2 * sizeof(hc::PcgState)));
   }
   if (need32 && !op->r32)
   {
@@ -551,12 +552,30 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
   hc::PcgState fin{};
   if (mode == HC_SOLVE_FP64)
   {
-    rc = run_pcg<double>(op, op->qd64, op->tb64, op->part64, op->dinv64, b, x, op->r64, op->p64, op->q64, false,
-                         rtol, maxit, st, fin);
-    if (rc)
-      return rc;
-    s.iterations = fin.iters;
-    s.status = fin.status;
+    // PCG; when the recursive residual meets rtol but the true FP64 residual
+    // does not (round-off drift near attainable accuracy), restart from the
+    // current x with the true residual (at most 4 restarts).
+    int total = 0;
+    for (int restart = 0; restart < 5 && total < maxit; restart++)
+    {
+      rc = run_pcg<double>(op, op->qd64, op->tb64, op->part64, op->dinv64, b, x, op->r64, op->p64, op->q64, false,
+                           rtol, maxit - total, st, fin);
+      if (rc)
+        return rc;
+      total += fin.iters;
+      s.status = fin.status;
+      if (fin.status != 0 || !fin.done)
+        break;
+      HC_TRY_HIP(hc::apply_op<double>(op->g, op->segs, op->tb64, op->qd64, x, op->q64, op->part64, nullptr, st));
+      HC_TRY_HIP(hc::residual64(b, op->q64, op->r64, op->N, wo, true, st));
+      hc::PcgState chk{};
+      if ((rc = read_state(op, 1, st, chk)))
+        return rc;
+      if (chk.rr <= rtol * rtol * chk.bb)
+        break;
+      s.outer_iterations++;
+    }
+    s.iterations = total;
   }
   else if (mode == HC_SOLVE_FP32_IR)
   {
@@ -617,7 +636,12 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
   if (s.status == HC_ENAN)
     return fail(HC_ENAN, "hc_pcg_solve: NaN/Inf met in a reduction");
   if (!s.converged)
-    return fail(HC_ENOTCONV, "hc_pcg_solve: not converged, true relative residual " + std::to_string(s.rel_res));
+  {
+    char buf[160];
+    std::snprintf(buf, sizeof(buf), "hc_pcg_solve: not converged after %d iterations, true relative residual %.3e",
+                  s.iterations, s.rel_res);
+    return fail(HC_ENOTCONV, buf);
+  }
   return HC_OK;
 }
 

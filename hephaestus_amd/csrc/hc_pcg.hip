@@ -231,7 +231,13 @@ k_residual_fin(const double * __restrict__ part, PcgState * s, int set_bb)
   {
     s->rr = o[0];
     if (set_bb)
+    {
+      // start of a new outer sequence
       s->bb = o[1];
+      s->status = 0;
+      s->done = 0;
+      s->iters = 0;
+    }
     if (bad(o[0]))
       s->status = -5;
   }

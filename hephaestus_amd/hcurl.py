@@ -185,7 +185,7 @@ class Operator:
         nrhs = 1 if b.dim() == 1 else b.shape[1]
         rc = _lib.hc_pcg_solve(self._h, _ptr(b), _ptr(x), nrhs, float(rtol), int(maxit), int(mode),
                                C.byref(st), _stream(stream))
-        if rc != HC_OK and (raise_on_fail or rc not in (HC_ENOTCONV,)):
+        if rc != HC_OK and (raise_on_fail or rc not in (HC_ENOTCONV, HC_ENAN)):
             _check(rc, "hc_pcg_solve")
         d = st.as_dict()
         d["rc"] = rc
