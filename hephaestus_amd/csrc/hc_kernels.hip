@@ -3,20 +3,13 @@
 //                     assembly replacement of ParBilinearForm::Assemble as
 //                     driven from EquationSystem::BuildBilinearForms
 //                     (src/equation_systems/equation_system.cpp:324-347);
-//   * k_apply_tile    sum-factorised element apply of dt*nu K + sigma M over a
-//                     tile of elements, LDS-staged, deterministic in-tile
-//                     assembly; the tile-interior DOFs are final, the DOFs on
-//                     tile faces go to a partials buffer;
+//   * (the tiled element apply k_apply_tile lives in hc_apply.hip)
 //   * k_assemble_planes  sums the (2 or 4) tile partials of every tile-face DOF
 //                     in a fixed order (bitwise-reproducible results);
 //   * k_diagonal      Jacobi diagonal, DOF-parallel gather (no atomics).
 // The apply is the operator EquationSystem::Mult (equation_system.cpp:200-204)
 // on the PEC-eliminated system of FormLinearSystem (equation_system.cpp:132-190):
 // y_free = (A x_free)_free, y_ess = x_ess.
-//
-// The operator is HBM-bound (about 2-3 flop/byte at p = 3): no MFMA; the tile
-// kernel streams the 12 per-quadrature-point factors once, reads each x DOF of
-// the tile box once and writes each y DOF once (plus the face partials).
 #include "hc_kernels.hpp"
 
 namespace hc
@@ -138,646 +131,6 @@ k_cast_f32(const double * __restrict__ a, float * __restrict__ b, int64_t n)
 {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     b[i] = (float)a[i];
-}
-
-// ---------------------------------------------------------------------------
-// Tiled sum-factorised apply
-// ---------------------------------------------------------------------------
-
-template <int P, int TX, int TY, int TZ>
-struct TileCfg
-{
-  static constexpr int O = P, D = P + 1, Q = P + 1, QQ = Q * Q;
-  static constexpr int TE = TX * TY * TZ;
-  static constexpr int NP = TE * QQ;
-  static constexpr int NTHR = (NP + 63) / 64 * 64;
-  static constexpr int NC = O * D * D;
-  static constexpr int ND = 3 * NC;
-  static constexpr int B00 = TX * P, B01 = TY * P + 1, B02 = TZ * P + 1;
-  static constexpr int B10 = TX * P + 1, B11 = TY * P, B12 = TZ * P + 1;
-  static constexpr int B20 = TX * P + 1, B21 = TY * P + 1, B22 = TZ * P;
-  static constexpr int BOX0 = B00 * B01 * B02, BOX1 = B10 * B11 * B12, BOX2 = B20 * B21 * B22;
-  static constexpr int BOX = BOX0 + BOX1 + BOX2;
-  static constexpr int cmax(int a, int b) { return a > b ? a : b; }
-  static constexpr int SA = cmax(D * D * Q, 2 * D * O * Q);
-  static constexpr int SB = cmax(2 * D * QQ, 3 * O * QQ);
-  static constexpr int UNI = cmax(BOX, TE * ND);
-};
-
-// Decode a box index into (component, l0, l1, l2).
-template <class C>
-__device__ inline void
-box_decode(int i, int & c, int & l0, int & l1, int & l2)
-{
-  int r, b0, b1;
-  if (i < C::BOX0)
-  {
-    c = 0;
-    r = i;
-    b0 = C::B00;
-    b1 = C::B01;
-  }
-  else if (i < C::BOX0 + C::BOX1)
-  {
-    c = 1;
-    r = i - C::BOX0;
-    b0 = C::B10;
-    b1 = C::B11;
-  }
-  else
-  {
-    c = 2;
-    r = i - C::BOX0 - C::BOX1;
-    b0 = C::B20;
-    b1 = C::B21;
-  }
-  l0 = r % b0;
-  r /= b0;
-  l1 = r % b1;
-  l2 = r / b1;
-}
-
-template <int P, typename T, int TX, int TY, int TZ>
-__global__ void __launch_bounds__((TileCfg<P, TX, TY, TZ>::NTHR))
-k_apply_tile(Geom g,
-             Tables<T> tb,
-             const T * __restrict__ qd,
-             const T * __restrict__ x,
-             T * __restrict__ y,
-             T * __restrict__ part,
-             const int * __restrict__ done)
-{
-  using C = TileCfg<P, TX, TY, TZ>;
-  constexpr int O = C::O, D = C::D, Q = C::Q, QQ = C::QQ, SA = C::SA, SB = C::SB;
-  constexpr int NT = C::NTHR;
-  if (done && *done)
-    return;
-  __shared__ T sU[C::UNI];
-  __shared__ T sA[C::TE * SA];
-  __shared__ T sB[C::TE * SB];
-  const int tid = threadIdx.x;
-  int bid = blockIdx.x;
-  const int tx = bid % g.nt[0];
-  bid /= g.nt[0];
-  const int ty = bid % g.nt[1];
-  const int tz = bid / g.nt[1];
-  const int e0[3] = {tx * TX, ty * TY, tz * TZ};
-  const int64_t N = g.off[3];
-
-  // ---- 1. gather the tile's DOF box (PEC DOFs read as zero) ----
-  for (int i = tid; i < C::BOX; i += NT)
-  {
-    int c, l0, l1, l2;
-    box_decode<C>(i, c, l0, l1, l2);
-    const int64_t i0 = (int64_t)e0[0] * P + l0, i1 = (int64_t)e0[1] * P + l1, i2 = (int64_t)e0[2] * P + l2;
-    T v = T(0);
-    if (i0 < g.cd[c][0] && i1 < g.cd[c][1] && i2 < g.cd[c][2] && !is_ess(g, c, i0, i1, i2))
-      v = x[g.off[c] + (i2 * g.cd[c][1] + i1) * g.cd[c][0] + i0];
-    sU[i] = v;
-  }
-  __syncthreads();
-
-  // pencil owned by this thread in the quadrature-point stages
-  const int pel = tid / QQ;
-  const int pqx = (tid % QQ) % Q, pqy = (tid % QQ) / Q;
-  const bool pact = tid < C::NP;
-  T U[3][Q], Cu[3][Q];
-#pragma unroll
-  for (int q = 0; q < Q; q++)
-    Cu[0][q] = Cu[1][q] = Cu[2][q] = T(0);
-
-  // ---------------- forward: component x, X[dz][dy][dx] (D,D,O) ----------------
-  for (int it = tid; it < C::TE * D * D; it += NT)
-  {
-    const int el = it / (D * D), r = it % (D * D), dy = r % D, dz = r / D;
-    const int l0 = el % TX, l1 = (el / TX) % TY, l2 = el / (TX * TY);
-    const T * X = sU + ((l2 * P + dz) * C::B01 + (l1 * P + dy)) * C::B00 + l0 * P;
-    T xv[O];
-#pragma unroll
-    for (int dx = 0; dx < O; dx++)
-      xv[dx] = X[dx];
-#pragma unroll
-    for (int qx = 0; qx < Q; qx++)
-    {
-      T s = 0;
-#pragma unroll
-      for (int dx = 0; dx < O; dx++)
-        s += tb.Bo[qx * O + dx] * xv[dx];
-      sA[el * SA + (dz * D + dy) * Q + qx] = s;
-    }
-  }
-  __syncthreads();
-  for (int it = tid; it < C::TE * D * Q; it += NT)
-  {
-    const int el = it / (D * Q), r = it % (D * Q), qx = r % Q, dz = r / Q;
-    T a[D];
-#pragma unroll
-    for (int dy = 0; dy < D; dy++)
-      a[dy] = sA[el * SA + (dz * D + dy) * Q + qx];
-#pragma unroll
-    for (int qy = 0; qy < Q; qy++)
-    {
-      T s = 0, gg = 0;
-#pragma unroll
-      for (int dy = 0; dy < D; dy++)
-      {
-        s += tb.Bc[qy * D + dy] * a[dy];
-        gg += tb.Gc[qy * D + dy] * a[dy];
-      }
-      sB[el * SB + (dz * Q + qy) * Q + qx] = s;
-      sB[el * SB + D * QQ + (dz * Q + qy) * Q + qx] = gg;
-    }
-  }
-  __syncthreads();
-  if (pact)
-  {
-    T b1[D], b2[D];
-#pragma unroll
-    for (int dz = 0; dz < D; dz++)
-    {
-      b1[dz] = sB[pel * SB + (dz * Q + pqy) * Q + pqx];
-      b2[dz] = sB[pel * SB + D * QQ + (dz * Q + pqy) * Q + pqx];
-    }
-#pragma unroll
-    for (int qz = 0; qz < Q; qz++)
-    {
-      T u = 0, uz = 0, uy = 0;
-#pragma unroll
-      for (int dz = 0; dz < D; dz++)
-      {
-        u += tb.Bc[qz * D + dz] * b1[dz];
-        uz += tb.Gc[qz * D + dz] * b1[dz];
-        uy += tb.Bc[qz * D + dz] * b2[dz];
-      }
-      U[0][qz] = u;
-      Cu[1][qz] += uz;
-      Cu[2][qz] -= uy;
-    }
-  }
-  __syncthreads();
-
-  // ---------------- forward: component y, Y[dz][dy][dx] (D,O,D) ----------------
-  for (int it = tid; it < C::TE * D * O; it += NT)
-  {
-    const int el = it / (D * O), r = it % (D * O), dy = r % O, dz = r / O;
-    const int l0 = el % TX, l1 = (el / TX) % TY, l2 = el / (TX * TY);
-    const T * Y = sU + C::BOX0 + ((l2 * P + dz) * C::B11 + (l1 * P + dy)) * C::B10 + l0 * P;
-    T yv[D];
-#pragma unroll
-    for (int dx = 0; dx < D; dx++)
-      yv[dx] = Y[dx];
-#pragma unroll
-    for (int qx = 0; qx < Q; qx++)
-    {
-      T s = 0, gg = 0;
-#pragma unroll
-      for (int dx = 0; dx < D; dx++)
-      {
-        s += tb.Bc[qx * D + dx] * yv[dx];
-        gg += tb.Gc[qx * D + dx] * yv[dx];
-      }
-      sA[el * SA + (dz * O + dy) * Q + qx] = s;
-      sA[el * SA + D * O * Q + (dz * O + dy) * Q + qx] = gg;
-    }
-  }
-  __syncthreads();
-  for (int it = tid; it < C::TE * D * Q; it += NT)
-  {
-    const int el = it / (D * Q), r = it % (D * Q), qx = r % Q, dz = r / Q;
-    T a1[O], a2[O];
-#pragma unroll
-    for (int dy = 0; dy < O; dy++)
-    {
-      a1[dy] = sA[el * SA + (dz * O + dy) * Q + qx];
-      a2[dy] = sA[el * SA + D * O * Q + (dz * O + dy) * Q + qx];
-    }
-#pragma unroll
-    for (int qy = 0; qy < Q; qy++)
-    {
-      T s = 0, gg = 0;
-#pragma unroll
-      for (int dy = 0; dy < O; dy++)
-      {
-        s += tb.Bo[qy * O + dy] * a1[dy];
-        gg += tb.Bo[qy * O + dy] * a2[dy];
-      }
-      sB[el * SB + (dz * Q + qy) * Q + qx] = s;
-      sB[el * SB + D * QQ + (dz * Q + qy) * Q + qx] = gg;
-    }
-  }
-  __syncthreads();
-  if (pact)
-  {
-    T b1[D], b2[D];
-#pragma unroll
-    for (int dz = 0; dz < D; dz++)
-    {
-      b1[dz] = sB[pel * SB + (dz * Q + pqy) * Q + pqx];
-      b2[dz] = sB[pel * SB + D * QQ + (dz * Q + pqy) * Q + pqx];
-    }
-#pragma unroll
-    for (int qz = 0; qz < Q; qz++)
-    {
-      T u = 0, uz = 0, ux = 0;
-#pragma unroll
-      for (int dz = 0; dz < D; dz++)
-      {
-        u += tb.Bc[qz * D + dz] * b1[dz];
-        uz += tb.Gc[qz * D + dz] * b1[dz];
-        ux += tb.Bc[qz * D + dz] * b2[dz];
-      }
-      U[1][qz] = u;
-      Cu[0][qz] -= uz;
-      Cu[2][qz] += ux;
-    }
-  }
-  __syncthreads();
-
-  // ---------------- forward: component z, Z[dz][dy][dx] (O,D,D) ----------------
-  for (int it = tid; it < C::TE * O * D; it += NT)
-  {
-    const int el = it / (O * D), r = it % (O * D), dy = r % D, dz = r / D;
-    const int l0 = el % TX, l1 = (el / TX) % TY, l2 = el / (TX * TY);
-    const T * Z = sU + C::BOX0 + C::BOX1 + ((l2 * P + dz) * C::B21 + (l1 * P + dy)) * C::B20 + l0 * P;
-    T zv[D];
-#pragma unroll
-    for (int dx = 0; dx < D; dx++)
-      zv[dx] = Z[dx];
-#pragma unroll
-    for (int qx = 0; qx < Q; qx++)
-    {
-      T s = 0, gg = 0;
-#pragma unroll
-      for (int dx = 0; dx < D; dx++)
-      {
-        s += tb.Bc[qx * D + dx] * zv[dx];
-        gg += tb.Gc[qx * D + dx] * zv[dx];
-      }
-      sA[el * SA + (dz * D + dy) * Q + qx] = s;
-      sA[el * SA + O * D * Q + (dz * D + dy) * Q + qx] = gg;
-    }
-  }
-  __syncthreads();
-  for (int it = tid; it < C::TE * O * Q; it += NT)
-  {
-    const int el = it / (O * Q), r = it % (O * Q), qx = r % Q, dz = r / Q;
-    T a1[D], a2[D];
-#pragma unroll
-    for (int dy = 0; dy < D; dy++)
-    {
-      a1[dy] = sA[el * SA + (dz * D + dy) * Q + qx];
-      a2[dy] = sA[el * SA + O * D * Q + (dz * D + dy) * Q + qx];
-    }
-#pragma unroll
-    for (int qy = 0; qy < Q; qy++)
-    {
-      T s = 0, gy = 0, gx = 0;
-#pragma unroll
-      for (int dy = 0; dy < D; dy++)
-      {
-        s += tb.Bc[qy * D + dy] * a1[dy];
-        gy += tb.Gc[qy * D + dy] * a1[dy];
-        gx += tb.Bc[qy * D + dy] * a2[dy];
-      }
-      sB[el * SB + (dz * Q + qy) * Q + qx] = s;
-      sB[el * SB + O * QQ + (dz * Q + qy) * Q + qx] = gy;
-      sB[el * SB + 2 * O * QQ + (dz * Q + qy) * Q + qx] = gx;
-    }
-  }
-  __syncthreads();
-  if (pact)
-  {
-    T b1[O], b2[O], b3[O];
-#pragma unroll
-    for (int dz = 0; dz < O; dz++)
-    {
-      b1[dz] = sB[pel * SB + (dz * Q + pqy) * Q + pqx];
-      b2[dz] = sB[pel * SB + O * QQ + (dz * Q + pqy) * Q + pqx];
-      b3[dz] = sB[pel * SB + 2 * O * QQ + (dz * Q + pqy) * Q + pqx];
-    }
-#pragma unroll
-    for (int qz = 0; qz < Q; qz++)
-    {
-      T u = 0, uy = 0, ux = 0;
-#pragma unroll
-      for (int dz = 0; dz < O; dz++)
-      {
-        u += tb.Bo[qz * O + dz] * b1[dz];
-        uy += tb.Bo[qz * O + dz] * b2[dz];
-        ux += tb.Bo[qz * O + dz] * b3[dz];
-      }
-      U[2][qz] = u;
-      Cu[0][qz] += uy;
-      Cu[1][qz] -= ux;
-    }
-
-    // ---------------- quadrature-point factors ----------------
-    const int l0 = pel % TX, l1 = (pel / TX) % TY, l2 = pel / (TX * TY);
-    const int64_t ge0 = e0[0] + l0, ge1 = e0[1] + l1, ge2 = e0[2] + l2;
-    if (ge0 < g.n[0] && ge1 < g.n[1] && ge2 < g.n[2])
-    {
-      const int64_t e = (ge2 * g.n[1] + ge1) * g.n[0] + ge0;
-      const T * qe = qd + e * (12 * Q * QQ) + pqy * Q + pqx;
-#pragma unroll
-      for (int qz = 0; qz < Q; qz++)
-      {
-        const T * qq = qe + qz * 12 * QQ;
-        const T m0 = qq[0 * QQ], m1 = qq[1 * QQ], m2 = qq[2 * QQ], m3 = qq[3 * QQ], m4 = qq[4 * QQ],
-                m5 = qq[5 * QQ];
-        const T k0 = qq[6 * QQ], k1 = qq[7 * QQ], k2 = qq[8 * QQ], k3 = qq[9 * QQ], k4 = qq[10 * QQ],
-                k5 = qq[11 * QQ];
-        const T u0 = U[0][qz], u1 = U[1][qz], u2 = U[2][qz];
-        const T c0 = Cu[0][qz], c1 = Cu[1][qz], c2 = Cu[2][qz];
-        U[0][qz] = m0 * u0 + m1 * u1 + m2 * u2;
-        U[1][qz] = m1 * u0 + m3 * u1 + m4 * u2;
-        U[2][qz] = m2 * u0 + m4 * u1 + m5 * u2;
-        Cu[0][qz] = k0 * c0 + k1 * c1 + k2 * c2;
-        Cu[1][qz] = k1 * c0 + k3 * c1 + k4 * c2;
-        Cu[2][qz] = k2 * c0 + k4 * c1 + k5 * c2;
-      }
-    }
-    else
-    {
-#pragma unroll
-      for (int qz = 0; qz < Q; qz++)
-        U[0][qz] = U[1][qz] = U[2][qz] = Cu[0][qz] = Cu[1][qz] = Cu[2][qz] = T(0);
-    }
-
-    // ---------------- backward z^T, component x ----------------
-#pragma unroll
-    for (int dz = 0; dz < D; dz++)
-    {
-      T s1 = 0, s2 = 0;
-#pragma unroll
-      for (int qz = 0; qz < Q; qz++)
-      {
-        s1 += tb.Bc[qz * D + dz] * U[0][qz] + tb.Gc[qz * D + dz] * Cu[1][qz];
-        s2 -= tb.Bc[qz * D + dz] * Cu[2][qz];
-      }
-      sB[pel * SB + (dz * Q + pqy) * Q + pqx] = s1;
-      sB[pel * SB + D * QQ + (dz * Q + pqy) * Q + pqx] = s2;
-    }
-  }
-  __syncthreads();
-  // y^T, component x
-  for (int it = tid; it < C::TE * D * Q; it += NT)
-  {
-    const int el = it / (D * Q), r = it % (D * Q), qx = r % Q, dz = r / Q;
-    T s1[Q], s2[Q];
-#pragma unroll
-    for (int qy = 0; qy < Q; qy++)
-    {
-      s1[qy] = sB[el * SB + (dz * Q + qy) * Q + qx];
-      s2[qy] = sB[el * SB + D * QQ + (dz * Q + qy) * Q + qx];
-    }
-#pragma unroll
-    for (int dy = 0; dy < D; dy++)
-    {
-      T t = 0;
-#pragma unroll
-      for (int qy = 0; qy < Q; qy++)
-        t += tb.Bc[qy * D + dy] * s1[qy] + tb.Gc[qy * D + dy] * s2[qy];
-      sA[el * SA + (dz * D + dy) * Q + qx] = t;
-    }
-  }
-  __syncthreads();
-  // x^T, component x -> element vectors (sU reused)
-  for (int it = tid; it < C::TE * D * D; it += NT)
-  {
-    const int el = it / (D * D), r = it % (D * D), dy = r % D, dz = r / D;
-    T t[Q];
-#pragma unroll
-    for (int qx = 0; qx < Q; qx++)
-      t[qx] = sA[el * SA + (dz * D + dy) * Q + qx];
-#pragma unroll
-    for (int dx = 0; dx < O; dx++)
-    {
-      T s = 0;
-#pragma unroll
-      for (int qx = 0; qx < Q; qx++)
-        s += tb.Bo[qx * O + dx] * t[qx];
-      sU[el * C::ND + (dz * D + dy) * O + dx] = s;
-    }
-  }
-  // z^T, component y (writes sB only; the x^T stage above reads sA only)
-  if (pact)
-  {
-#pragma unroll
-    for (int dz = 0; dz < D; dz++)
-    {
-      T s1 = 0, s2 = 0;
-#pragma unroll
-      for (int qz = 0; qz < Q; qz++)
-      {
-        s1 += tb.Bc[qz * D + dz] * U[1][qz] - tb.Gc[qz * D + dz] * Cu[0][qz];
-        s2 += tb.Bc[qz * D + dz] * Cu[2][qz];
-      }
-      sB[pel * SB + (dz * Q + pqy) * Q + pqx] = s1;
-      sB[pel * SB + D * QQ + (dz * Q + pqy) * Q + pqx] = s2;
-    }
-  }
-  __syncthreads();
-  // y^T, component y
-  for (int it = tid; it < C::TE * D * Q; it += NT)
-  {
-    const int el = it / (D * Q), r = it % (D * Q), qx = r % Q, dz = r / Q;
-    T s1[Q], s2[Q];
-#pragma unroll
-    for (int qy = 0; qy < Q; qy++)
-    {
-      s1[qy] = sB[el * SB + (dz * Q + qy) * Q + qx];
-      s2[qy] = sB[el * SB + D * QQ + (dz * Q + qy) * Q + qx];
-    }
-#pragma unroll
-    for (int dy = 0; dy < O; dy++)
-    {
-      T ta = 0, tb2 = 0;
-#pragma unroll
-      for (int qy = 0; qy < Q; qy++)
-      {
-        ta += tb.Bo[qy * O + dy] * s1[qy];
-        tb2 += tb.Bo[qy * O + dy] * s2[qy];
-      }
-      sA[el * SA + (dz * O + dy) * Q + qx] = ta;
-      sA[el * SA + D * O * Q + (dz * O + dy) * Q + qx] = tb2;
-    }
-  }
-  __syncthreads();
-  // x^T, component y
-  for (int it = tid; it < C::TE * D * O; it += NT)
-  {
-    const int el = it / (D * O), r = it % (D * O), dy = r % O, dz = r / O;
-    T ta[Q], tb2[Q];
-#pragma unroll
-    for (int qx = 0; qx < Q; qx++)
-    {
-      ta[qx] = sA[el * SA + (dz * O + dy) * Q + qx];
-      tb2[qx] = sA[el * SA + D * O * Q + (dz * O + dy) * Q + qx];
-    }
-#pragma unroll
-    for (int dx = 0; dx < D; dx++)
-    {
-      T s = 0;
-#pragma unroll
-      for (int qx = 0; qx < Q; qx++)
-        s += tb.Bc[qx * D + dx] * ta[qx] + tb.Gc[qx * D + dx] * tb2[qx];
-      sU[el * C::ND + C::NC + (dz * O + dy) * D + dx] = s;
-    }
-  }
-  // z^T, component z
-  if (pact)
-  {
-#pragma unroll
-    for (int dz = 0; dz < O; dz++)
-    {
-      T sa = 0, sb = 0, sc = 0;
-#pragma unroll
-      for (int qz = 0; qz < Q; qz++)
-      {
-        const T bo = tb.Bo[qz * O + dz];
-        sa += bo * U[2][qz];
-        sb += bo * Cu[0][qz];
-        sc -= bo * Cu[1][qz];
-      }
-      sB[pel * SB + (dz * Q + pqy) * Q + pqx] = sa;
-      sB[pel * SB + O * QQ + (dz * Q + pqy) * Q + pqx] = sb;
-      sB[pel * SB + 2 * O * QQ + (dz * Q + pqy) * Q + pqx] = sc;
-    }
-  }
-  __syncthreads();
-  // y^T, component z
-  for (int it = tid; it < C::TE * O * Q; it += NT)
-  {
-    const int el = it / (O * Q), r = it % (O * Q), qx = r % Q, dz = r / Q;
-    T sa[Q], sb[Q], sc[Q];
-#pragma unroll
-    for (int qy = 0; qy < Q; qy++)
-    {
-      sa[qy] = sB[el * SB + (dz * Q + qy) * Q + qx];
-      sb[qy] = sB[el * SB + O * QQ + (dz * Q + qy) * Q + qx];
-      sc[qy] = sB[el * SB + 2 * O * QQ + (dz * Q + qy) * Q + qx];
-    }
-#pragma unroll
-    for (int dy = 0; dy < D; dy++)
-    {
-      T ta = 0, tb2 = 0;
-#pragma unroll
-      for (int qy = 0; qy < Q; qy++)
-      {
-        ta += tb.Bc[qy * D + dy] * sa[qy] + tb.Gc[qy * D + dy] * sb[qy];
-        tb2 += tb.Bc[qy * D + dy] * sc[qy];
-      }
-      sA[el * SA + (dz * D + dy) * Q + qx] = ta;
-      sA[el * SA + O * D * Q + (dz * D + dy) * Q + qx] = tb2;
-    }
-  }
-  __syncthreads();
-  // x^T, component z
-  for (int it = tid; it < C::TE * O * D; it += NT)
-  {
-    const int el = it / (O * D), r = it % (O * D), dy = r % D, dz = r / D;
-    T ta[Q], tb2[Q];
-#pragma unroll
-    for (int qx = 0; qx < Q; qx++)
-    {
-      ta[qx] = sA[el * SA + (dz * D + dy) * Q + qx];
-      tb2[qx] = sA[el * SA + O * D * Q + (dz * D + dy) * Q + qx];
-    }
-#pragma unroll
-    for (int dx = 0; dx < D; dx++)
-    {
-      T s = 0;
-#pragma unroll
-      for (int qx = 0; qx < Q; qx++)
-        s += tb.Bc[qx * D + dx] * ta[qx] + tb.Gc[qx * D + dx] * tb2[qx];
-      sU[el * C::ND + 2 * C::NC + (dz * D + dy) * D + dx] = s;
-    }
-  }
-  __syncthreads();
-
-  // ---- 5. in-tile assembly (fixed order) and write-out ----
-  for (int i = tid; i < C::BOX; i += NT)
-  {
-    int c, l[3];
-    box_decode<C>(i, c, l[0], l[1], l[2]);
-    const int64_t gi[3] = {(int64_t)e0[0] * P + l[0], (int64_t)e0[1] * P + l[1], (int64_t)e0[2] * P + l[2]};
-    if (gi[0] >= g.cd[c][0] || gi[1] >= g.cd[c][1] || gi[2] >= g.cd[c][2])
-      continue;
-    const int64_t gid = g.off[c] + (gi[2] * g.cd[c][1] + gi[1]) * g.cd[c][0] + gi[0];
-    // tile-face membership in the two closed directions
-    bool onp[3] = {false, false, false};
-    bool owner = true;
-    int slot = 0, bit = 1;
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-    {
-      if (a == c)
-        continue;
-      onp[a] = on_tile_plane(g, a, gi[a]);
-      if (onp[a] && l[a] != 0)
-      {
-        owner = false;
-        slot |= bit;
-      }
-      bit <<= 1;
-    }
-    if (is_ess(g, c, gi[0], gi[1], gi[2]))
-    {
-      if (owner)
-        y[gid] = x[gid];
-      continue;
-    }
-    // elements of the tile containing this DOF, per axis (at most 2)
-    int ne[3], el_[3][2], lc[3][2];
-    const int Tt[3] = {TX, TY, TZ};
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-    {
-      ne[a] = 0;
-      if (a == c)
-      {
-        el_[a][0] = l[a] / P;
-        lc[a][0] = l[a] % P;
-        ne[a] = 1;
-      }
-      else
-      {
-        const int eh = l[a] / P, lh = l[a] % P;
-        if (lh == 0 && eh > 0)
-        {
-          el_[a][ne[a]] = eh - 1;
-          lc[a][ne[a]] = P;
-          ne[a]++;
-        }
-        if (eh < Tt[a] && e0[a] + eh < g.n[a])
-        {
-          el_[a][ne[a]] = eh;
-          lc[a][ne[a]] = lh;
-          ne[a]++;
-        }
-      }
-    }
-    T s = 0;
-    for (int kz = 0; kz < ne[2]; kz++)
-      for (int ky = 0; ky < ne[1]; ky++)
-        for (int kx = 0; kx < ne[0]; kx++)
-        {
-          const int el = (el_[2][kz] * TY + el_[1][ky]) * TX + el_[0][kx];
-          const int d0 = lc[0][kx], d1 = lc[1][ky], d2 = lc[2][kz];
-          int li;
-          if (c == 0)
-            li = (d2 * D + d1) * O + d0;
-          else if (c == 1)
-            li = C::NC + (d2 * O + d1) * D + d0;
-          else
-            li = 2 * C::NC + (d2 * D + d1) * D + d0;
-          s += sU[el * C::ND + li];
-        }
-    if (!(onp[0] || onp[1] || onp[2]))
-      y[gid] = s;
-    else
-      part[slot * N + gid] = s;
-  }
 }
 
 // Sums the tile partials of DOFs on interior tile faces.  Thread space:
@@ -940,59 +293,6 @@ k_diagonal(Geom g, Tables<double> tb, const double * __restrict__ qd, double * _
 // Host launchers
 // ---------------------------------------------------------------------------
 
-template <int P, int TX, int TY, int TZ>
-static TileShape
-shape_of()
-{
-  return TileShape{TX, TY, TZ, TileCfg<P, TX, TY, TZ>::NTHR};
-}
-
-TileShape
-default_tile(int p)
-{
-  switch (p)
-  {
-    case 1:
-      return shape_of<1, 4, 4, 4>();
-    case 2:
-      return shape_of<2, 4, 4, 2>();
-    default:
-      return shape_of<3, 4, 2, 2>();
-  }
-}
-
-template <typename T>
-static hipError_t
-launch_tile(const Geom & g, const Tables<T> & tb, const T * qd, const T * x, T * y, T * part,
-            const int * done, hipStream_t st)
-{
-  const int nblk = g.nt[0] * g.nt[1] * g.nt[2];
-  switch (g.p)
-  {
-    case 1:
-    {
-      using C = TileCfg<1, 4, 4, 4>;
-      hipLaunchKernelGGL((k_apply_tile<1, T, 4, 4, 4>), dim3(nblk), dim3(C::NTHR), 0, st, g, tb, qd, x, y, part, done);
-      break;
-    }
-    case 2:
-    {
-      using C = TileCfg<2, 4, 4, 2>;
-      hipLaunchKernelGGL((k_apply_tile<2, T, 4, 4, 2>), dim3(nblk), dim3(C::NTHR), 0, st, g, tb, qd, x, y, part, done);
-      break;
-    }
-    case 3:
-    {
-      using C = TileCfg<3, 4, 2, 2>;
-      hipLaunchKernelGGL((k_apply_tile<3, T, 4, 2, 2>), dim3(nblk), dim3(C::NTHR), 0, st, g, tb, qd, x, y, part, done);
-      break;
-    }
-    default:
-      return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
 PlaneSegs
 make_plane_segs(const Geom & g)
 {
@@ -1017,25 +317,18 @@ make_plane_segs(const Geom & g)
 
 template <typename T>
 hipError_t
-apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T * qd, const T * x, T * y,
-         T * part, const int * done, hipStream_t st)
+assemble_planes(const Geom & g, const PlaneSegs & segs, const T * part, T * y, const int * done, hipStream_t st)
 {
-  hipError_t e = launch_tile<T>(g, tb, qd, x, y, part, done, st);
-  if (e != hipSuccess)
-    return e;
   const int64_t n = segs.start[segs.nseg];
   if (n > 0)
-  {
     hipLaunchKernelGGL(k_assemble_planes<T>, dim3(div_up(n, 256)), dim3(256), 0, st, g, segs, part, y, done);
-    e = hipGetLastError();
-  }
-  return e;
+  return hipGetLastError();
 }
 
-template hipError_t apply_op<double>(const Geom &, const PlaneSegs &, const Tables<double> &, const double *,
-                                     const double *, double *, double *, const int *, hipStream_t);
-template hipError_t apply_op<float>(const Geom &, const PlaneSegs &, const Tables<float> &, const float *,
-                                    const float *, float *, float *, const int *, hipStream_t);
+template hipError_t assemble_planes<double>(const Geom &, const PlaneSegs &, const double *, double *, const int *,
+                                            hipStream_t);
+template hipError_t assemble_planes<float>(const Geom &, const PlaneSegs &, const float *, float *, const int *,
+                                           hipStream_t);
 
 hipError_t
 pa_setup(const Geom & g, const Tables<double> & tb, const double * vert, const double h[3],

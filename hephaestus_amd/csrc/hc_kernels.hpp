@@ -49,6 +49,11 @@ hipError_t apply_op(const Geom & g,
                     const int * done,
                     hipStream_t st);
 
+// Sum of the tile-face partials (fixed order) into y.
+template <typename T>
+hipError_t assemble_planes(const Geom & g, const PlaneSegs & segs, const T * part, T * y, const int * done,
+                           hipStream_t st);
+
 // ---------------------------------------------------------------------------
 // Jacobi-PCG (hc_pcg.hip)
 // ---------------------------------------------------------------------------
