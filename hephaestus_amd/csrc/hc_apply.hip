@@ -25,13 +25,15 @@
 namespace hc
 {
 
-template <int P, int TX, int TY, int TZ>
+template <int P, int TX, int TY, int TZ, int SP = 1>
 struct Tile
 {
   static constexpr int O = P, D = P + 1, Q = P + 1, QQ = Q * Q;
   static constexpr int TE = TX * TY * TZ;
-  static constexpr int NP = TE * QQ;
-  static constexpr int NT = (NP + 63) / 64 * 64;
+  static constexpr int NP = TE * QQ;                // pencils
+  static constexpr int NPP = (NP + 63) / 64 * 64;   // pencils padded to whole waves
+  static constexpr int NT = NPP * SP;               // SP wave-uniform groups share each pencil's qz range
+  static constexpr int QH = (Q + SP - 1) / SP;      // quadrature points per thread along z
   // A-type arrays [qx][dz][dy], qx stride padded
   static constexpr int SQX = D * D + 1;
   static constexpr int LA = Q * SQX;
@@ -39,12 +41,12 @@ struct Tile
   static constexpr int SDZ = QQ + 4;
   static constexpr int LB = D * SDZ;
   static constexpr int pad_to(int v, int m, int r) { return ((v - r + m - 1) / m) * m + r; }
-  static constexpr int SA = pad_to(2 * LA, 16, 4);
+  static constexpr int SA = pad_to(5 * LA, 16, 4);
   static constexpr int SB = pad_to(3 * LB, 16, 4);
 };
 
-template <int P, typename T, int TX, int TY, int TZ>
-__global__ void __launch_bounds__((Tile<P, TX, TY, TZ>::NT))
+template <int P, typename T, int TX, int TY, int TZ, int MODE, int SP>
+__global__ void __launch_bounds__((Tile<P, TX, TY, TZ, SP>::NT))
 k_apply_tile(Geom g,
              Tables<T> tb,
              const T * __restrict__ qd,
@@ -53,7 +55,8 @@ k_apply_tile(Geom g,
              T * __restrict__ part,
              const int * __restrict__ done)
 {
-  using C = Tile<P, TX, TY, TZ>;
+  using C = Tile<P, TX, TY, TZ, SP>;
+  constexpr int QH = C::QH, NPP = C::NPP;
   constexpr int O = C::O, D = C::D, Q = C::Q, QQ = C::QQ;
   constexpr int SQX = C::SQX, LA = C::LA, SDZ = C::SDZ, LB = C::LB, SA = C::SA, SB = C::SB;
   constexpr int NT = C::NT, TE = C::TE;
@@ -63,45 +66,134 @@ k_apply_tile(Geom g,
   __shared__ T sB[TE * SB];
 
   const int tid = threadIdx.x;
-  int bid = blockIdx.x;
-  const int tpos[3] = {bid % g.nt[0], (bid / g.nt[0]) % g.nt[1], bid / (g.nt[0] * g.nt[1])};
-  const int e0[3] = {tpos[0] * TX, tpos[1] * TY, tpos[2] * TZ};
-  // active extent of the tile (partial tiles at the upper domain faces)
-  const int ne[3] = {min(TX, g.n[0] - e0[0]), min(TY, g.n[1] - e0[1]), min(TZ, g.n[2] - e0[2])};
+  const int ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+  int tcur = blockIdx.x;
+  if (tcur >= ntiles)
+    return;
+  // tile position, element origin and active extent (partial tiles at the
+  // upper domain faces)
+  auto tile_pos = [&](int t, int * tp, int * e0_, int * ne_) {
+    tp[0] = t % g.nt[0];
+    tp[1] = (t / g.nt[0]) % g.nt[1];
+    tp[2] = t / (g.nt[0] * g.nt[1]);
+    e0_[0] = tp[0] * TX;
+    e0_[1] = tp[1] * TY;
+    e0_[2] = tp[2] * TZ;
+    ne_[0] = min(TX, g.n[0] - e0_[0]);
+    ne_[1] = min(TY, g.n[1] - e0_[1]);
+    ne_[2] = min(TZ, g.n[2] - e0_[2]);
+  };
+  int tpos[3], e0[3], ne[3];
+  tile_pos(tcur, tpos, e0, ne);
   const int64_t N = g.off[3];
   const int cd[3][3] = {{(int)g.cd[0][0], (int)g.cd[0][1], (int)g.cd[0][2]},
                         {(int)g.cd[1][0], (int)g.cd[1][1], (int)g.cd[1][2]},
                         {(int)g.cd[2][0], (int)g.cd[2][1], (int)g.cd[2][2]}};
   const int pec = g.pec;
 
-  // ---- pencil of this thread; prefetch its quadrature data ----
-  const int pel = tid / QQ;
-  const int pqx = (tid % QQ) % Q, pqy = (tid % QQ) / Q;
+  // pencil of this thread (stages z, z^T and the quadrature points)
+  // group ph (wave-uniform) owns quadrature points qz = ph*QH + j of the pencil
+  const int ph = __builtin_amdgcn_readfirstlane(tid / NPP);
+  const int ptid = tid - ph * NPP;
+  const bool pact = ptid < C::NP;
+  const int pel = ptid / QQ;
+  const int pqx = (ptid % QQ) % Q, pqy = (ptid % QQ) / Q;
+  // run f with the group index as a compile-time constant (keeps every 1D
+  // table index static, so the table argument stays in scalar registers)
+  auto with_ph = [&](auto f) {
+    if (SP == 1 || ph == 0)
+      f(std::integral_constant<int, 0>{});
+    else
+      f(std::integral_constant<int, SP == 1 ? 0 : 1>{});
+  };
   const int pl0 = pel % TX, pl1 = (pel / TX) % TY, pl2 = pel / (TX * TY);
-  const bool pin = tid < C::NP && pl0 < ne[0] && pl1 < ne[1] && pl2 < ne[2];
-  T qf[Q][12];
-  if (pin)
-  {
-    const int64_t e = ((int64_t)(e0[2] + pl2) * g.n[1] + (e0[1] + pl1)) * g.n[0] + (e0[0] + pl0);
-    const T * qe = qd + e * (12 * Q * QQ) + pqy * Q + pqx;
+  // stage-x item of this thread: (lx fastest, dy, dz, ly, lz); TE*D*D <= NT
+  const int ilx = tid % TX, idy = (tid / TX) % D, idz = (tid / (TX * D)) % D;
+  const int ily = (tid / (TX * D * D)) % TY, ilz = tid / (TX * D * D * TY);
+
+  // ---- loads of one tile: the pencil's 12 Q quadrature factors, the item's
+  // x rows of the three components (PEC DOFs read as zero) ----
+  T qf[QH][12];
+  T xr0[O], xr1[D], xr2[D];
+  auto load_qf = [&](const int * e0_, const int * ne_) {
+    const bool pin = pact && pl0 < ne_[0] && pl1 < ne_[1] && pl2 < ne_[2];
+    // inactive lanes read element 0 (in bounds) and discard the values
+    const int64_t e = pin ? ((int64_t)(e0_[2] + pl2) * g.n[1] + (e0_[1] + pl1)) * g.n[0] + (e0_[0] + pl0) : 0;
+    const T * qe = qd + e * (12 * Q * QQ) + (pin ? pqy * Q + pqx : 0);
 #pragma unroll
-    for (int qz = 0; qz < Q; qz++)
+    for (int j = 0; j < QH; j++)
+    {
+      const int qz = ph * QH + j;
+      const bool ld = pin && qz < Q;
+      const T * qq = qe + (ld ? qz : 0) * 12 * QQ;
 #pragma unroll
       for (int k = 0; k < 12; k++)
-        qf[qz][k] = __builtin_nontemporal_load(qe + (qz * 12 + k) * QQ);
-  }
-  else
+      {
+        const T v = __builtin_nontemporal_load(qq + k * QQ);
+        qf[j][k] = ld ? v : T(0);
+      }
+    }
+  };
+  auto load_x = [&](const int * e0_, const int * ne_) {
+    const bool iin = tid < TE * D * D && ilx < ne_[0] && ily < ne_[1] && ilz < ne_[2];
+    const int iiy = (e0_[1] + ily) * P + idy, iiz = (e0_[2] + ilz) * P + idz, iix = (e0_[0] + ilx) * P;
+#pragma unroll
+    for (int d = 0; d < O; d++)
+      xr0[d] = T(0);
+#pragma unroll
+    for (int d = 0; d < D; d++)
+      xr1[d] = xr2[d] = T(0);
+    // component x: closed y, z (row PEC), open x
+    const bool ress0 = (iiy == 0 && (pec & 4)) || (iiy == cd[0][1] - 1 && (pec & 8)) ||
+                       (iiz == 0 && (pec & 16)) || (iiz == cd[0][2] - 1 && (pec & 32));
+    if (iin && !ress0)
+    {
+      const T * row = x + g.off[0] + ((int64_t)iiz * cd[0][1] + iiy) * cd[0][0] + iix;
+#pragma unroll
+      for (int d = 0; d < O; d++)
+        xr0[d] = row[d];
+    }
+    // component y: open y (dy < O), closed z (row PEC), closed x (end PEC)
+    const bool ress1 = (iiz == 0 && (pec & 16)) || (iiz == cd[1][2] - 1 && (pec & 32));
+    if (iin && idy < O && !ress1)
+    {
+      const T * row = x + g.off[1] + ((int64_t)iiz * cd[1][1] + iiy) * cd[1][0] + iix;
+#pragma unroll
+      for (int d = 0; d < D; d++)
+        xr1[d] = row[d];
+      if (iix == 0 && (pec & 1))
+        xr1[0] = T(0);
+      if (iix + P == cd[1][0] - 1 && (pec & 2))
+        xr1[P] = T(0);
+    }
+    // component z: open z (dz < O), closed y (row PEC), closed x (end PEC)
+    const bool ress2 = (iiy == 0 && (pec & 4)) || (iiy == cd[2][1] - 1 && (pec & 8));
+    if (iin && idz < O && !ress2)
+    {
+      const T * row = x + g.off[2] + ((int64_t)iiz * cd[2][1] + iiy) * cd[2][0] + iix;
+#pragma unroll
+      for (int d = 0; d < D; d++)
+        xr2[d] = row[d];
+      if (iix == 0 && (pec & 1))
+        xr2[0] = T(0);
+      if (iix + P == cd[2][0] - 1 && (pec & 2))
+        xr2[P] = T(0);
+    }
+  };
+  // MODE 0: quadrature data loaded at the quadrature-point stage (fewest
+  // registers); MODE 1: at entry; MODE 2: persistent workgroups, the loads of
+  // tile i+1 are issued right after the quadrature-point stage of tile i and
+  // fly during its backward contractions and row walkers.
+  load_x(e0, ne);
+  if (MODE >= 1)
+    load_qf(e0, ne);
+
+  for (;;)
   {
+  T U[3][QH], Cu[3][QH];
 #pragma unroll
-    for (int qz = 0; qz < Q; qz++)
-#pragma unroll
-      for (int k = 0; k < 12; k++)
-        qf[qz][k] = T(0);
-  }
-  T U[3][Q], Cu[3][Q];
-#pragma unroll
-  for (int q = 0; q < Q; q++)
-    Cu[0][q] = Cu[1][q] = Cu[2][q] = T(0);
+  for (int j = 0; j < QH; j++)
+    Cu[0][j] = Cu[1][j] = Cu[2][j] = T(0);
 
   T * const pA = sA + pel * SA; // pencil's element blocks (stage z / z^T)
   T * const pB = sB + pel * SB;
@@ -110,83 +202,41 @@ k_apply_tile(Geom g,
   // (component index as a compile-time constant: one instantiation per component)
   auto forward = [&](auto cc) {
     constexpr int c = decltype(cc)::value;
-    // ---- stage x: global rows -> A (contract dx -> qx) ----
-    // items: ex_l fastest (coalesced row loads), then dy, dz, ey_l, ez_l
-    constexpr int NI = TE * D * D;
-    for (int it = tid; it < NI; it += NT)
+    // ---- stage x: prefetched rows -> A (contract dx -> qx) ----
     {
-      const int lx = it % TX;
-      int r = it / TX;
-      const int dy = r % D;
-      r /= D;
-      const int dz = r % D;
-      r /= D;
-      const int ly = r % TY, lz = r / TY;
       const int ny_c = (c == 1) ? O : D, nz_c = (c == 2) ? O : D;
-      if (dy >= ny_c || dz >= nz_c)
-        continue;
-      const int el = (lz * TY + ly) * TX + lx;
-      const bool act = lx < ne[0] && ly < ne[1] && lz < ne[2];
-      const int iy = (e0[1] + ly) * P + dy, iz = (e0[2] + lz) * P + dz;
-      const int ix0 = (e0[0] + lx) * P;
-      bool ress = false; // whole row essential
-      if (c != 1)
-        ress = ress || (iy == 0 && (pec & 4)) || (iy == cd[c][1] - 1 && (pec & 8));
-      if (c != 2)
-        ress = ress || (iz == 0 && (pec & 16)) || (iz == cd[c][2] - 1 && (pec & 32));
-      T * A1 = sA + el * SA + dz * D + dy;
-      if (c == 0)
+      if (tid < TE * D * D && idy < ny_c && idz < nz_c)
       {
-        T xv[O];
-#pragma unroll
-        for (int d = 0; d < O; d++)
-          xv[d] = T(0);
-        if (act && !ress)
+        const int el = (ilz * TY + ily) * TX + ilx;
+        T * A1 = sA + el * SA + idz * D + idy;
+        if (c == 0)
         {
-          const T * row = x + g.off[0] + ((int64_t)iz * cd[0][1] + iy) * cd[0][0] + ix0;
 #pragma unroll
-          for (int d = 0; d < O; d++)
-            xv[d] = row[d];
-        }
-#pragma unroll
-        for (int qx = 0; qx < Q; qx++)
-        {
-          T s = 0;
-#pragma unroll
-          for (int d = 0; d < O; d++)
-            s += tb.Bo[qx * O + d] * xv[d];
-          A1[qx * SQX] = s;
-        }
-      }
-      else
-      {
-        T xv[D];
-#pragma unroll
-        for (int d = 0; d < D; d++)
-          xv[d] = T(0);
-        if (act && !ress)
-        {
-          const T * row = x + g.off[c] + ((int64_t)iz * cd[c][1] + iy) * cd[c][0] + ix0;
-#pragma unroll
-          for (int d = 0; d < D; d++)
-            xv[d] = row[d];
-          if (ix0 == 0 && (pec & 1))
-            xv[0] = T(0);
-          if (ix0 + P == cd[c][0] - 1 && (pec & 2))
-            xv[P] = T(0);
-        }
-#pragma unroll
-        for (int qx = 0; qx < Q; qx++)
-        {
-          T s = 0, gg = 0;
-#pragma unroll
-          for (int d = 0; d < D; d++)
+          for (int qx = 0; qx < Q; qx++)
           {
-            s += tb.Bc[qx * D + d] * xv[d];
-            gg += tb.Gc[qx * D + d] * xv[d];
+            T s = 0;
+#pragma unroll
+            for (int d = 0; d < O; d++)
+              s += tb.Bo[qx * O + d] * xr0[d];
+            A1[qx * SQX] = s;
           }
-          A1[qx * SQX] = s;
-          A1[LA + qx * SQX] = gg;
+        }
+        else
+        {
+#pragma unroll
+          for (int qx = 0; qx < Q; qx++)
+          {
+            T s = 0, gg = 0;
+#pragma unroll
+            for (int d = 0; d < D; d++)
+            {
+              const T v = (c == 1) ? xr1[d] : xr2[d];
+              s += tb.Bc[qx * D + d] * v;
+              gg += tb.Gc[qx * D + d] * v;
+            }
+            A1[qx * SQX] = s;
+            A1[LA + qx * SQX] = gg;
+          }
         }
       }
     }
@@ -275,86 +325,60 @@ k_apply_tile(Geom g,
       }
     }
     __syncthreads();
-    // ---- stage z: B -> registers (contract dz -> qz) ----
-    if (tid < C::NP)
+    // ---- stage z: B -> registers (contract dz -> qz), own qz range ----
+    if (pact)
     {
       const T * B1 = pB + pqy * Q + pqx;
-      if (c == 0)
+      constexpr int ND = (c == 2) ? O : D;
+      constexpr int NA = (c == 2) ? 3 : 2;
+      T b[NA][ND];
+#pragma unroll
+      for (int d = 0; d < ND; d++)
+#pragma unroll
+        for (int k = 0; k < NA; k++)
+          b[k][d] = B1[k * LB + d * SDZ];
+      with_ph([&](auto phc) {
+      constexpr int PH = decltype(phc)::value;
+#pragma unroll
+      for (int j = 0; j < QH; j++)
       {
-        T b1[D], b2[D];
+        const int qz = PH * QH + j; // past Q (odd Q split in two): tables are zero there
+        T u = 0, v1 = 0, v2 = 0;
 #pragma unroll
-        for (int d = 0; d < D; d++)
+        for (int d = 0; d < ND; d++)
         {
-          b1[d] = B1[d * SDZ];
-          b2[d] = B1[LB + d * SDZ];
-        }
-#pragma unroll
-        for (int qz = 0; qz < Q; qz++)
-        {
-          T u = 0, uz = 0, uy = 0;
-#pragma unroll
-          for (int d = 0; d < D; d++)
+          if (c == 2)
           {
-            u += tb.Bc[qz * D + d] * b1[d];
-            uz += tb.Gc[qz * D + d] * b1[d];
-            uy += tb.Bc[qz * D + d] * b2[d];
+            const T bo = tb.Bo[qz * O + d];
+            u += bo * b[0][d];
+            v1 += bo * b[1][d];
+            v2 += bo * b[2][d];
           }
-          U[0][qz] = u;
-          Cu[1][qz] += uz;
-          Cu[2][qz] -= uy;
+          else
+          {
+            u += tb.Bc[qz * D + d] * b[0][d];
+            v1 += tb.Gc[qz * D + d] * b[0][d];
+            v2 += tb.Bc[qz * D + d] * b[1][d];
+          }
+        }
+        U[c][j] = u;
+        if (c == 0)
+        {
+          Cu[1][j] += v1; // d/dz u_x
+          Cu[2][j] -= v2; // d/dy u_x
+        }
+        else if (c == 1)
+        {
+          Cu[0][j] -= v1; // d/dz u_y
+          Cu[2][j] += v2; // d/dx u_y
+        }
+        else
+        {
+          Cu[0][j] += v1; // d/dy u_z
+          Cu[1][j] -= v2; // d/dx u_z
         }
       }
-      else if (c == 1)
-      {
-        T b1[D], b2[D];
-#pragma unroll
-        for (int d = 0; d < D; d++)
-        {
-          b1[d] = B1[d * SDZ];
-          b2[d] = B1[LB + d * SDZ];
-        }
-#pragma unroll
-        for (int qz = 0; qz < Q; qz++)
-        {
-          T u = 0, uz = 0, ux = 0;
-#pragma unroll
-          for (int d = 0; d < D; d++)
-          {
-            u += tb.Bc[qz * D + d] * b1[d];
-            uz += tb.Gc[qz * D + d] * b1[d];
-            ux += tb.Bc[qz * D + d] * b2[d];
-          }
-          U[1][qz] = u;
-          Cu[0][qz] -= uz;
-          Cu[2][qz] += ux;
-        }
-      }
-      else
-      {
-        T b1[O], b2[O], b3[O];
-#pragma unroll
-        for (int d = 0; d < O; d++)
-        {
-          b1[d] = B1[d * SDZ];
-          b2[d] = B1[LB + d * SDZ];
-          b3[d] = B1[2 * LB + d * SDZ];
-        }
-#pragma unroll
-        for (int qz = 0; qz < Q; qz++)
-        {
-          T u = 0, uy = 0, ux = 0;
-#pragma unroll
-          for (int d = 0; d < O; d++)
-          {
-            u += tb.Bo[qz * O + d] * b1[d];
-            uy += tb.Bo[qz * O + d] * b2[d];
-            ux += tb.Bo[qz * O + d] * b3[d];
-          }
-          U[2][qz] = u;
-          Cu[0][qz] += uy;
-          Cu[1][qz] -= ux;
-        }
-      }
+      });
     }
     // (no barrier: the next stage x writes A only; stage z above read B)
   };
@@ -363,77 +387,93 @@ k_apply_tile(Geom g,
   forward(std::integral_constant<int, 2>{});
 
   // ======================= quadrature points =======================
+  if (MODE == 0)
+    load_qf(e0, ne);
 #pragma unroll
-  for (int qz = 0; qz < Q; qz++)
+  for (int j = 0; j < QH; j++)
   {
-    const T u0 = U[0][qz], u1 = U[1][qz], u2 = U[2][qz];
-    const T c0 = Cu[0][qz], c1 = Cu[1][qz], c2 = Cu[2][qz];
-    U[0][qz] = qf[qz][0] * u0 + qf[qz][1] * u1 + qf[qz][2] * u2;
-    U[1][qz] = qf[qz][1] * u0 + qf[qz][3] * u1 + qf[qz][4] * u2;
-    U[2][qz] = qf[qz][2] * u0 + qf[qz][4] * u1 + qf[qz][5] * u2;
-    Cu[0][qz] = qf[qz][6] * c0 + qf[qz][7] * c1 + qf[qz][8] * c2;
-    Cu[1][qz] = qf[qz][7] * c0 + qf[qz][9] * c1 + qf[qz][10] * c2;
-    Cu[2][qz] = qf[qz][8] * c0 + qf[qz][10] * c1 + qf[qz][11] * c2;
+    const T u0 = U[0][j], u1 = U[1][j], u2 = U[2][j];
+    const T c0 = Cu[0][j], c1 = Cu[1][j], c2 = Cu[2][j];
+    U[0][j] = qf[j][0] * u0 + qf[j][1] * u1 + qf[j][2] * u2;
+    U[1][j] = qf[j][1] * u0 + qf[j][3] * u1 + qf[j][4] * u2;
+    U[2][j] = qf[j][2] * u0 + qf[j][4] * u1 + qf[j][5] * u2;
+    Cu[0][j] = qf[j][6] * c0 + qf[j][7] * c1 + qf[j][8] * c2;
+    Cu[1][j] = qf[j][7] * c0 + qf[j][9] * c1 + qf[j][10] * c2;
+    Cu[2][j] = qf[j][8] * c0 + qf[j][10] * c1 + qf[j][11] * c2;
+  }
+
+  // next tile of a persistent workgroup: issue its loads now (registers of
+  // qf and the x rows are free after the quadrature-point stage)
+  const int tnext = tcur + (int)gridDim.x;
+  const bool more = (MODE == 2) && tnext < ntiles;
+  int ntp[3] = {0, 0, 0}, ne0[3] = {0, 0, 0}, nne[3] = {0, 0, 0};
+  if (more)
+  {
+    tile_pos(tnext, ntp, ne0, nne);
+    load_x(ne0, nne);
+    load_qf(ne0, nne);
   }
 
   // ======================= backward =======================
   auto backward = [&](auto cc) {
     constexpr int c = decltype(cc)::value;
     // ---- z^T: registers -> S (B region), own pencil entries only ----
-    if (tid < C::NP)
-    {
+    // Each group sums over its own qz range; with SP > 1 group 0 writes S and
+    // the other groups add theirs after a barrier (fixed order).
+    auto ztr = [&](bool add) {
+      with_ph([&](auto phc) {
+      constexpr int PH = decltype(phc)::value;
       T * S1 = pB + pqy * Q + pqx;
-      if (c == 0)
+      constexpr int ND = (c == 2) ? O : D;
+      constexpr int NA = (c == 2) ? 3 : 2;
+#pragma unroll
+      for (int d = 0; d < ND; d++)
       {
+        T sv[NA];
 #pragma unroll
-        for (int d = 0; d < D; d++)
+        for (int k = 0; k < NA; k++)
+          sv[k] = T(0);
+#pragma unroll
+        for (int j = 0; j < QH; j++)
         {
-          T s1 = 0, s2 = 0;
-#pragma unroll
-          for (int qz = 0; qz < Q; qz++)
+          const int qz = PH * QH + j; // past Q: zero table entries and zero V, W
+          if (c == 0)
           {
-            s1 += tb.Bc[qz * D + d] * U[0][qz] + tb.Gc[qz * D + d] * Cu[1][qz];
-            s2 -= tb.Bc[qz * D + d] * Cu[2][qz];
+            sv[0] += tb.Bc[qz * D + d] * U[0][j] + tb.Gc[qz * D + d] * Cu[1][j];
+            sv[1] -= tb.Bc[qz * D + d] * Cu[2][j];
           }
-          S1[d * SDZ] = s1;
-          S1[LB + d * SDZ] = s2;
-        }
-      }
-      else if (c == 1)
-      {
-#pragma unroll
-        for (int d = 0; d < D; d++)
-        {
-          T s1 = 0, s2 = 0;
-#pragma unroll
-          for (int qz = 0; qz < Q; qz++)
+          else if (c == 1)
           {
-            s1 += tb.Bc[qz * D + d] * U[1][qz] - tb.Gc[qz * D + d] * Cu[0][qz];
-            s2 += tb.Bc[qz * D + d] * Cu[2][qz];
+            sv[0] += tb.Bc[qz * D + d] * U[1][j] - tb.Gc[qz * D + d] * Cu[0][j];
+            sv[1] += tb.Bc[qz * D + d] * Cu[2][j];
           }
-          S1[d * SDZ] = s1;
-          S1[LB + d * SDZ] = s2;
-        }
-      }
-      else
-      {
-#pragma unroll
-        for (int d = 0; d < O; d++)
-        {
-          T sa = 0, sb = 0, sc = 0;
-#pragma unroll
-          for (int qz = 0; qz < Q; qz++)
+          else
           {
             const T bo = tb.Bo[qz * O + d];
-            sa += bo * U[2][qz];
-            sb += bo * Cu[0][qz];
-            sc -= bo * Cu[1][qz];
+            sv[0] += bo * U[2][j];
+            sv[1] += bo * Cu[0][j];
+            sv[2] -= bo * Cu[1][j];
           }
-          S1[d * SDZ] = sa;
-          S1[LB + d * SDZ] = sb;
-          S1[2 * LB + d * SDZ] = sc;
+        }
+#pragma unroll
+        for (int k = 0; k < NA; k++)
+        {
+          if (add)
+            S1[k * LB + d * SDZ] += sv[k];
+          else
+            S1[k * LB + d * SDZ] = sv[k];
         }
       }
+      });
+    };
+    if (pact && ph == 0)
+      ztr(false);
+#pragma unroll
+    for (int h = 1; h < SP; h++)
+    {
+      __syncthreads();
+      if (pact && ph == h)
+        ztr(true);
     }
     __syncthreads();
     // ---- y^T: S -> T (A region), contract qy -> dy ----
@@ -448,7 +488,7 @@ k_apply_tile(Geom g,
         if (dz >= nz_c)
           continue;
         const T * S1 = sB + el * SB + dz * SDZ + qx;
-        T * T1 = sA + el * SA + qx * SQX + dz * D;
+        T * T1 = sA + el * SA + (c == 0 ? 0 : (c == 1 ? LA : 3 * LA)) + qx * SQX + dz * D;
         if (c == 0)
         {
           T s1[Q], s2[Q];
@@ -518,43 +558,51 @@ k_apply_tile(Geom g,
       }
     }
     __syncthreads();
-    // ---- row stage: sum T over the elements sharing a row, contract x, write ----
+    // (the barrier above also orders this y^T before the next z^T overwrites B)
+  };
+  backward(std::integral_constant<int, 0>{});
+  backward(std::integral_constant<int, 1>{});
+  backward(std::integral_constant<int, 2>{});
+
+  // ======================= row walkers =======================
+  // One wave per component walks the tile's DOF rows of that component along
+  // x: it sums the T tensors of the (at most four) elements sharing the row in
+  // a fixed order (lower element first) and contracts along x.  For the
+  // components with a closed x axis the DOF on an element x-face inside the
+  // tile is (value of column lx-1 at dx = P) + (value of column lx at dx = 0),
+  // carried in a register.  Results go to an LDS box (B region, free now) that
+  // all threads then write out with x-contiguous stores.
+  constexpr int BX0 = TX * P, BX1 = TY * P + 1, BX2 = TZ * P + 1; // comp x box
+  constexpr int BY0 = TX * P + 1, BY1 = TY * P, BY2 = TZ * P + 1; // comp y box
+  constexpr int BZ0 = TX * P + 1, BZ1 = TY * P + 1, BZ2 = TZ * P; // comp z box
+  constexpr int OB1 = BX0 * BX1 * BX2, OB2 = OB1 + BY0 * BY1 * BY2, OBN = OB2 + BZ0 * BZ1 * BZ2;
+  static_assert(OBN <= TE * SB, "output box must fit in the B region");
+  T * const sOut = sB;
+  auto rows = [&](auto cc, int lane) {
+    constexpr int c = decltype(cc)::value;
+    constexpr int RY = (c == 1) ? TY * P : TY * P + 1;
+    constexpr int RZ = (c == 2) ? TZ * P : TZ * P + 1;
+    constexpr int RX = (c == 0) ? TX * P : TX * P + 1;
+    const T * TA = sA + (c == 0 ? 0 : (c == 1 ? LA : 3 * LA));
+    T * const OUT = sOut + (c == 0 ? 0 : (c == 1 ? OB1 : OB2));
+    for (int r = lane; r < RY * RZ; r += 64)
     {
-      // rows of component c: (lx, iy, iz); open axis has T*P points, closed T*P+1
-      constexpr int RY0 = TY * P + 1, RZ0 = TZ * P + 1;
-      const int RY = (c == 1) ? TY * P : RY0;
-      const int RZ = (c == 2) ? TZ * P : RZ0;
-      const int NR = TX * RY * RZ;
-      for (int it = tid; it < NR; it += NT)
+      const int iy = r % RY, iz = r / RY;
+      const int ehy = iy / P, lhy = iy % P, ehz = iz / P, lhz = iz % P;
+      const int ey[2] = {ehy - 1, ehy}, dyc[2] = {P, lhy};
+      const int ez[2] = {ehz - 1, ehz}, dzc[2] = {P, lhz};
+      const bool vy[2] = {c != 1 && lhy == 0 && ehy > 0, c == 1 || ehy < ne[1]};
+      const bool vz[2] = {c != 2 && lhz == 0 && ehz > 0, c == 2 || ehz < ne[2]};
+      if (!(vy[0] || vy[1]) || !(vz[0] || vz[1]))
+        continue;
+      T * const orow = OUT + (iz * RY + iy) * RX;
+      if (c == 0)
       {
-        const int lx = it % TX;
-        const int iy = (it / TX) % RY;
-        const int iz = it / (TX * RY);
-        // element candidates along y and z: slot 0 = lower element (shared face),
-        // slot 1 = the element the index falls in; summed slot 0 first
-        const int ehy = iy / P, lhy = iy % P, ehz = iz / P, lhz = iz % P;
-        const int ey[2] = {ehy - 1, ehy}, dyc[2] = {P, lhy};
-        const int ez[2] = {ehz - 1, ehz}, dzc[2] = {P, lhz};
-        const bool vy[2] = {c != 1 && lhy == 0 && ehy > 0, c == 1 || ehy < ne[1]};
-        const bool vz[2] = {c != 2 && lhz == 0 && ehz > 0, c == 2 || ehz < ne[2]};
-        const int nyc = (vy[0] || vy[1]) ? 1 : 0, nzc = (vz[0] || vz[1]) ? 1 : 0;
-        const int giy = e0[1] * P + iy, giz = e0[2] * P + iz;
-        if (lx >= ne[0] || giy >= cd[c][1] || giz >= cd[c][2] || nyc == 0 || nzc == 0)
-          continue;
-        // row-level essential test and tile-face classification (y and z)
-        bool ress = false;
-        if (c != 1)
-          ress = ress || (giy == 0 && (pec & 4)) || (giy == cd[c][1] - 1 && (pec & 8));
-        if (c != 2)
-          ress = ress || (giz == 0 && (pec & 16)) || (giz == cd[c][2] - 1 && (pec & 32));
-        const bool ylo = (c != 1) && iy == 0 && tpos[1] > 0;
-        const bool yhi = (c != 1) && iy == TY * P && tpos[1] < g.nt[1] - 1;
-        const bool zlo = (c != 2) && iz == 0 && tpos[2] > 0;
-        const bool zhi = (c != 2) && iz == TZ * P && tpos[2] < g.nt[2] - 1;
-        const int64_t rowbase = g.off[c] + ((int64_t)giz * cd[c][1] + giy) * cd[c][0];
-        if (c == 0)
+#pragma unroll
+        for (int lx = 0; lx < TX; lx++)
         {
-          // open x: each thread owns dx = 0..O-1 of element column lx
+          if (lx >= ne[0])
+            break;
           T ts[Q];
 #pragma unroll
           for (int q = 0; q < Q; q++)
@@ -566,45 +614,34 @@ k_apply_tile(Geom g,
             {
               if (!vz[kz] || !vy[ky])
                 continue;
-              const int el = (ez[kz] * TY + ey[ky]) * TX + lx;
-              const T * T1 = sA + el * SA + dzc[kz] * D + dyc[ky];
+              const T * T1 = TA + ((ez[kz] * TY + ey[ky]) * TX + lx) * SA + dzc[kz] * D + dyc[ky];
 #pragma unroll
               for (int q = 0; q < Q; q++)
                 ts[q] += T1[q * SQX];
             }
-          // slot bits: bit0 = y (upper face of a non-last tile), bit1 = z
-          const int slot = (yhi ? 1 : 0) | (zhi ? 2 : 0);
-          const bool shared = ylo || yhi || zlo || zhi;
-          const bool owner = !yhi && !zhi;
-          const int64_t base = rowbase + (e0[0] + lx) * P;
 #pragma unroll
           for (int d = 0; d < O; d++)
           {
-            if (ress)
-            {
-              if (owner)
-                y[base + d] = x[base + d];
-              continue;
-            }
-            T s = 0;
+            T v = 0;
 #pragma unroll
             for (int q = 0; q < Q; q++)
-              s += tb.Bo[q * O + d] * ts[q];
-            if (shared)
-              part[slot * N + base + d] = s;
-            else
-              y[base + d] = s;
+              v += tb.Bo[q * O + d] * ts[q];
+            orow[lx * P + d] = v;
           }
         }
-        else
+      }
+      else
+      {
+        T carry = 0;
+#pragma unroll
+        for (int lx = 0; lx < TX; lx++)
         {
-          // closed x: element column lx contributes dx = 0..P; dx = 0 also
-          // receives dx = P of column lx-1 (same tile), dx = P is written by
-          // the last active column only.
-          T ta[Q], tb2[Q], pa[Q], pb[Q];
+          if (lx >= ne[0])
+            break;
+          T ta[Q], tb2[Q];
 #pragma unroll
           for (int q = 0; q < Q; q++)
-            ta[q] = tb2[q] = pa[q] = pb[q] = T(0);
+            ta[q] = tb2[q] = T(0);
 #pragma unroll
           for (int kz = 0; kz < 2; kz++)
 #pragma unroll
@@ -612,139 +649,236 @@ k_apply_tile(Geom g,
             {
               if (!vz[kz] || !vy[ky])
                 continue;
-              const int elb = (ez[kz] * TY + ey[ky]) * TX;
-              const int off = dzc[kz] * D + dyc[ky];
-              const T * T1 = sA + (elb + lx) * SA + off;
+              const T * T1 = TA + ((ez[kz] * TY + ey[ky]) * TX + lx) * SA + dzc[kz] * D + dyc[ky];
 #pragma unroll
               for (int q = 0; q < Q; q++)
               {
                 ta[q] += T1[q * SQX];
                 tb2[q] += T1[LA + q * SQX];
               }
-              if (lx > 0)
-              {
-                const T * T0 = sA + (elb + lx - 1) * SA + off;
-#pragma unroll
-                for (int q = 0; q < Q; q++)
-                {
-                  pa[q] += T0[q * SQX];
-                  pb[q] += T0[LA + q * SQX];
-                }
-              }
             }
-          const int gx0 = (e0[0] + lx) * P;
           const bool last = (lx == ne[0] - 1);
-          // component-c slot bits: bit0 = x (closed), bit1 = the other closed axis
-          const bool olo = (c == 1) ? zlo : ylo;
-          const bool ohi = (c == 1) ? zhi : yhi;
 #pragma unroll
           for (int d = 0; d <= P; d++)
           {
-            if (d == P && !last)
-              continue;
-            const int gx = gx0 + d;
-            const bool xlo = (d == 0 && lx == 0 && tpos[0] > 0);
-            const bool xhi = (d == P && lx == TX - 1 && tpos[0] < g.nt[0] - 1);
-            const bool ess = ress || (gx == 0 && (pec & 1)) || (gx == cd[c][0] - 1 && (pec & 2));
-            const int64_t gid = rowbase + gx;
-            if (ess)
-            {
-              if (!xhi && !ohi)
-                y[gid] = x[gid];
-              continue;
-            }
-            T s = 0;
+            T v = 0;
 #pragma unroll
             for (int q = 0; q < Q; q++)
-              s += tb.Bc[q * D + d] * ta[q] + tb.Gc[q * D + d] * tb2[q];
+              v += tb.Bc[q * D + d] * ta[q] + tb.Gc[q * D + d] * tb2[q];
             if (d == 0 && lx > 0)
+              v = carry + v;
+            if (d == P)
             {
-              T sp = 0;
-#pragma unroll
-              for (int q = 0; q < Q; q++)
-                sp += tb.Bc[q * D + P] * pa[q] + tb.Gc[q * D + P] * pb[q];
-              s = sp + s;
+              carry = v;
+              if (!last)
+                continue;
             }
-            const int slot = (xhi ? 1 : 0) | (ohi ? 2 : 0);
-            if (xlo || xhi || olo || ohi)
-              part[slot * N + gid] = s;
-            else
-              y[gid] = s;
+            orow[lx * P + d] = v;
           }
         }
       }
     }
-    // (no barrier: the next z^T writes B only; this stage read A)
   };
-  backward(std::integral_constant<int, 0>{});
-  backward(std::integral_constant<int, 1>{});
-  backward(std::integral_constant<int, 2>{});
+  {
+    // component cc is walked by wave (cc mod number of waves)
+    constexpr int NW = NT / 64;
+    const int grp = tid >> 6, lane = tid & 63;
+    if (0 % NW == grp)
+      rows(std::integral_constant<int, 0>{}, lane);
+    if (1 % NW == grp)
+      rows(std::integral_constant<int, 1>{}, lane);
+    if (2 % NW == grp)
+      rows(std::integral_constant<int, 2>{}, lane);
+  }
+  __syncthreads();
+  // ---- write-out: interior DOFs to y, DOFs on a face shared with another
+  // tile to the partials (slot bit0 = lower-numbered closed axis, bit1 = the
+  // other; set when this tile holds the face as its upper face), PEC DOFs copy
+  // x (written by the tile that holds them as a lower face) ----
+  for (int i = tid; i < OBN; i += NT)
+  {
+    int c, r, b0, b1;
+    if (i < OB1)
+    {
+      c = 0;
+      r = i;
+      b0 = BX0;
+      b1 = BX1;
+    }
+    else if (i < OB2)
+    {
+      c = 1;
+      r = i - OB1;
+      b0 = BY0;
+      b1 = BY1;
+    }
+    else
+    {
+      c = 2;
+      r = i - OB2;
+      b0 = BZ0;
+      b1 = BZ1;
+    }
+    int l[3];
+    l[0] = r % b0;
+    l[1] = (r / b0) % b1;
+    l[2] = r / (b0 * b1);
+    const int Tl[3] = {TX * P, TY * P, TZ * P};
+    int gi[3], dims[3];
+    bool ok = true, ess = false, shared = false, owner = true;
+    int slot = 0, bit = 1;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+    {
+      gi[a] = e0[a] * P + l[a];
+      const int dim = g.n[a] * P + (a != c ? 1 : 0);
+      dims[a] = dim;
+      ok = ok && gi[a] < dim;
+      if (a == c)
+        continue;
+      ess = ess || (gi[a] == 0 && ((pec >> (2 * a)) & 1)) || (gi[a] == dim - 1 && ((pec >> (2 * a + 1)) & 1));
+      const bool lo = (l[a] == 0 && tpos[a] > 0);
+      const bool hi = (l[a] == Tl[a] && tpos[a] < g.nt[a] - 1);
+      shared = shared || lo || hi;
+      if (hi)
+      {
+        owner = false;
+        slot |= bit;
+      }
+      bit <<= 1;
+    }
+    if (!ok)
+      continue;
+    const int64_t offc = (c == 0) ? g.off[0] : ((c == 1) ? g.off[1] : g.off[2]);
+    const int64_t gid = offc + ((int64_t)gi[2] * dims[1] + gi[1]) * dims[0] + gi[0];
+    if (ess)
+    {
+      if (owner)
+        y[gid] = x[gid];
+    }
+    else if (shared)
+      part[slot * N + gid] = sOut[i];
+    else
+      y[gid] = sOut[i];
+  }
+  if (!more)
+    break;
+  __syncthreads(); // this tile's LDS fully consumed before the next forward pass
+  tcur = tnext;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+  {
+    tpos[a] = ntp[a];
+    e0[a] = ne0[a];
+    ne[a] = nne[a];
+  }
+  } // tile loop
 }
 
 // ---------------------------------------------------------------------------
-// launchers
+// launchers: candidate tile shapes per order (chosen by the autotuner)
 // ---------------------------------------------------------------------------
-template <int P, int TX, int TY, int TZ>
-static TileShape
-shape_of()
+template <int P, int TX, int TY, int TZ, int MODE, int SP>
+static hipError_t
+launch_one(const Geom & g, const void * tb, const void * qd, const void * x, void * y, void * part, const int * done,
+           int prec, hipStream_t st)
 {
-  return TileShape{TX, TY, TZ, Tile<P, TX, TY, TZ>::NT};
+  using C = Tile<P, TX, TY, TZ, SP>;
+  const int ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+  int nblk = ntiles;
+  if (MODE == 2)
+  {
+    // persistent: as many workgroups as can be resident at once
+    static int resident[2] = {0, 0};
+    int & r = resident[prec == 64 ? 0 : 1];
+    if (r == 0)
+    {
+      int dev = 0, ncu = 0, per = 0;
+      (void)hipGetDevice(&dev);
+      (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+      if (prec == 64)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_apply_tile<P, double, TX, TY, TZ, MODE, SP>, C::NT,
+                                                           0);
+      else
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_apply_tile<P, float, TX, TY, TZ, MODE, SP>, C::NT,
+                                                           0);
+      r = (ncu > 0 ? ncu : 256) * (per > 0 ? per : 1);
+    }
+    nblk = ntiles < r ? ntiles : r;
+  }
+  if (prec == 64)
+    hipLaunchKernelGGL((k_apply_tile<P, double, TX, TY, TZ, MODE, SP>), dim3(nblk), dim3(C::NT), 0, st, g,
+                       *(const Tables<double> *)tb, (const double *)qd, (const double *)x, (double *)y,
+                       (double *)part, done);
+  else
+    hipLaunchKernelGGL((k_apply_tile<P, float, TX, TY, TZ, MODE, SP>), dim3(nblk), dim3(C::NT), 0, st, g,
+                       *(const Tables<float> *)tb, (const float *)qd, (const float *)x, (float *)y, (float *)part,
+                       done);
+  return hipGetLastError();
+}
+
+// mode field: load mode (0, 1, 2) + 4 * (number of qz groups - 1)
+#define HC_CAND(P, TX, TY, TZ, M, SP)                                                                            \
+  {P, {TX, TY, TZ, Tile<P, TX, TY, TZ, SP>::NT}, M + 4 * (SP - 1), &launch_one<P, TX, TY, TZ, M, SP>}
+
+static const TileCandidate g_cands[] = {
+    HC_CAND(1, 4, 4, 4, 1, 1), HC_CAND(1, 4, 4, 4, 2, 1), HC_CAND(1, 8, 4, 2, 1, 1), HC_CAND(1, 4, 4, 4, 1, 2),
+    HC_CAND(1, 8, 4, 2, 0, 2), HC_CAND(1, 8, 8, 1, 0, 1),
+    HC_CAND(2, 4, 2, 2, 1, 1), HC_CAND(2, 4, 2, 2, 1, 2), HC_CAND(2, 4, 2, 2, 0, 2), HC_CAND(2, 4, 4, 2, 1, 1),
+    HC_CAND(2, 4, 4, 2, 0, 2), HC_CAND(2, 8, 2, 2, 0, 1),
+    HC_CAND(3, 4, 2, 2, 1, 1), HC_CAND(3, 4, 2, 2, 1, 2), HC_CAND(3, 4, 2, 2, 0, 2), HC_CAND(3, 4, 2, 2, 2, 2),
+    HC_CAND(3, 4, 4, 1, 1, 2), HC_CAND(3, 4, 4, 1, 0, 2), HC_CAND(3, 2, 2, 4, 1, 2), HC_CAND(3, 4, 2, 2, 0, 1),
+};
+#undef HC_CAND
+
+int
+tile_candidates(int p, const TileCandidate ** out)
+{
+  int first = -1, n = 0;
+  for (int i = 0; i < (int)(sizeof(g_cands) / sizeof(g_cands[0])); i++)
+    if (g_cands[i].p == p)
+    {
+      if (first < 0)
+        first = i;
+      n++;
+    }
+  *out = first >= 0 ? &g_cands[first] : nullptr;
+  return n;
 }
 
 TileShape
 default_tile(int p)
 {
-  switch (p)
-  {
-    case 1:
-      return shape_of<1, 4, 4, 4>();
-    case 2:
-      return shape_of<2, 4, 4, 2>();
-    default:
-      return shape_of<3, 4, 2, 2>();
-  }
-}
-
-template <typename T>
-static hipError_t
-launch_tile(const Geom & g, const Tables<T> & tb, const T * qd, const T * x, T * y, T * part,
-            const int * done, hipStream_t st)
-{
-  const int nblk = g.nt[0] * g.nt[1] * g.nt[2];
-  switch (g.p)
-  {
-    case 1:
-      hipLaunchKernelGGL((k_apply_tile<1, T, 4, 4, 4>), dim3(nblk), dim3(Tile<1, 4, 4, 4>::NT), 0, st, g, tb, qd, x,
-                         y, part, done);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_apply_tile<2, T, 4, 4, 2>), dim3(nblk), dim3(Tile<2, 4, 4, 2>::NT), 0, st, g, tb, qd, x,
-                         y, part, done);
-      break;
-    case 3:
-      hipLaunchKernelGGL((k_apply_tile<3, T, 4, 2, 2>), dim3(nblk), dim3(Tile<3, 4, 2, 2>::NT), 0, st, g, tb, qd, x,
-                         y, part, done);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  const TileCandidate * c;
+  tile_candidates(p, &c);
+  return c[0].shape;
 }
 
 template <typename T>
 hipError_t
 apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T * qd, const T * x, T * y,
-         T * part, const int * done, hipStream_t st)
+         T * part, const int * done, hipStream_t st, const TileCandidate * cand)
 {
-  hipError_t e = launch_tile<T>(g, tb, qd, x, y, part, done, st);
+  if (!cand)
+  {
+    const TileCandidate * c0;
+    const int n = tile_candidates(g.p, &c0);
+    for (int i = 0; i < n && !cand; i++)
+      if (c0[i].shape.tx == g.T[0] && c0[i].shape.ty == g.T[1] && c0[i].shape.tz == g.T[2])
+        cand = &c0[i];
+    if (!cand)
+      return hipErrorInvalidValue;
+  }
+  hipError_t e = cand->launch(g, &tb, qd, x, y, part, done, sizeof(T) == 8 ? 64 : 32, st);
   if (e != hipSuccess)
     return e;
   return assemble_planes<T>(g, segs, part, y, done, st);
 }
 
 template hipError_t apply_op<double>(const Geom &, const PlaneSegs &, const Tables<double> &, const double *,
-                                     const double *, double *, double *, const int *, hipStream_t);
+                                     const double *, double *, double *, const int *, hipStream_t,
+                                     const TileCandidate *);
 template hipError_t apply_op<float>(const Geom &, const PlaneSegs &, const Tables<float> &, const float *,
-                                    const float *, float *, float *, const int *, hipStream_t);
+                                    const float *, float *, float *, const int *, hipStream_t, const TileCandidate *);
 
 } // namespace hc

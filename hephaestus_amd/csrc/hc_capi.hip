@@ -171,10 +171,19 @@ k_zero_f(float * v, int64_t n)
 
 } // namespace
 
-struct hc_op_s
+// Apply kernel variant of one precision: geometry with the tile decomposition,
+// the tile-face thread segments and the compiled candidate.
+struct ApplyPlan
 {
   hc::Geom g{};
   hc::PlaneSegs segs{};
+  const hc::TileCandidate * cand = nullptr;
+};
+
+struct hc_op_s
+{
+  hc::Geom g{}; // base geometry (tile fields follow plan64)
+  ApplyPlan plan64, plan32;
   hc::Tables<double> tb64{};
   hc::Tables<float> tb32{};
   int p = 0;
@@ -231,6 +240,41 @@ struct hc_op_s
 namespace
 {
 
+void
+set_plan(hc_op_t op, ApplyPlan & pl, const hc::TileCandidate * c)
+{
+  pl.g = op->g;
+  pl.cand = c;
+  pl.g.T[0] = c->shape.tx;
+  pl.g.T[1] = c->shape.ty;
+  pl.g.T[2] = c->shape.tz;
+  for (int a = 0; a < 3; a++)
+    pl.g.nt[a] = (pl.g.n[a] + pl.g.T[a] - 1) / pl.g.T[a];
+  pl.segs = hc::make_plane_segs(pl.g);
+}
+
+template <typename T>
+ApplyPlan &
+plan_of(hc_op_t op)
+{
+  return sizeof(T) == 8 ? op->plan64 : op->plan32;
+}
+
+template <typename T>
+hipError_t
+apply_t(hc_op_t op, const hc::Tables<T> & tb, const T * qd, const T * x, T * y, T * part, const int * done,
+        hipStream_t st)
+{
+  ApplyPlan & pl = plan_of<T>(op);
+  return hc::apply_op<T>(pl.g, pl.segs, tb, qd, x, y, part, done, st, pl.cand);
+}
+
+hipError_t
+apply64(hc_op_t op, const double * x, double * y, const int * done, hipStream_t st)
+{
+  return apply_t<double>(op, op->tb64, op->qd64, x, y, op->part64, done, st);
+}
+
 int
 ensure_solver_work(hc_op_t op, bool need32)
 {
@@ -286,7 +330,7 @@ run_pcg(hc_op_t op,
 {
   hc::PcgWork w{op->red, op->st_dev};
   if (!zero_guess)
-    HC_TRY_HIP(hc::apply_op<T>(op->g, op->segs, tb, qd, x, q, part, nullptr, st));
+    HC_TRY_HIP(apply_t<T>(op, tb, qd, x, q, part, nullptr, st));
   HC_TRY_HIP(hc::pcg_init<T>(b, zero_guess ? nullptr : q, dinv, r, p, op->N, rtol, w, st));
   int it = 0, chunk = 8;
   int rc = read_state(op, 0, st, fin);
@@ -298,7 +342,7 @@ run_pcg(hc_op_t op,
     const int n = std::min(chunk, maxit - it);
     for (int k = 0; k < n; k++, it++)
     {
-      HC_TRY_HIP(hc::apply_op<T>(op->g, op->segs, tb, qd, p, q, part, done, st));
+      HC_TRY_HIP(apply_t<T>(op, tb, qd, p, q, part, done, st));
       HC_TRY_HIP(hc::pcg_dot_pq<T>(p, q, op->N, w, st));
       HC_TRY_HIP(hc::pcg_update_xr<T>(it, x, p, r, q, dinv, op->N, w, st));
       HC_TRY_HIP(hc::pcg_update_p<T>(it, p, r, dinv, op->N, w, st));
@@ -357,7 +401,12 @@ hc_op_create(const hc_mesh_desc * mesh, int order, const double * nu_e, const do
   }
   op->E = (int64_t)g.n[0] * g.n[1] * g.n[2];
   op->N = g.off[3];
-  op->segs = hc::make_plane_segs(g);
+  {
+    const hc::TileCandidate * cands;
+    hc::tile_candidates(order, &cands);
+    set_plan(op, op->plan64, &cands[0]);
+    set_plan(op, op->plan32, &cands[0]);
+  }
   make_tables(order, op->tb64);
   cast_tables(op->tb64, op->tb32);
   op->nu.assign(nu_e, nu_e + op->E);
@@ -492,8 +541,7 @@ hc_op_apply(hc_op_t op, const double * x, double * y, int nrhs, void * stream)
     return fail(HC_EINVAL, "hc_op_apply: hc_op_setup has not been called");
   if (nrhs != 1)
     return fail(HC_EINVAL, "hc_op_apply: nrhs > 1 is not supported by this entry point yet");
-  HC_TRY_HIP(hc::apply_op<double>(op->g, op->segs, op->tb64, op->qd64, x, y, op->part64, nullptr,
-                                  (hipStream_t)stream));
+  HC_TRY_HIP(apply64(op, x, y, nullptr, (hipStream_t)stream));
   return HC_OK;
 }
 
@@ -508,8 +556,7 @@ hc_op_apply_f32(hc_op_t op, const float * x, float * y, int nrhs, void * stream)
     return fail(HC_EINVAL, "hc_op_apply_f32: operator created without HC_PREC_FP32");
   if (nrhs != 1)
     return fail(HC_EINVAL, "hc_op_apply_f32: nrhs > 1 is not supported by this entry point yet");
-  HC_TRY_HIP(hc::apply_op<float>(op->g, op->segs, op->tb32, op->qd32, x, y, op->part32, nullptr,
-                                 (hipStream_t)stream));
+  HC_TRY_HIP(apply_t<float>(op, op->tb32, op->qd32, x, y, op->part32, nullptr, (hipStream_t)stream));
   return HC_OK;
 }
 
@@ -566,7 +613,7 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
       s.status = fin.status;
       if (fin.status != 0 || !fin.done)
         break;
-      HC_TRY_HIP(hc::apply_op<double>(op->g, op->segs, op->tb64, op->qd64, x, op->q64, op->part64, nullptr, st));
+      HC_TRY_HIP(apply64(op, x, op->q64, nullptr, st));
       HC_TRY_HIP(hc::residual64(b, op->q64, op->r64, op->N, wo, true, st));
       hc::PcgState chk{};
       if ((rc = read_state(op, 1, st, chk)))
@@ -580,7 +627,7 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
   else if (mode == HC_SOLVE_FP32_IR)
   {
     // outer residual in FP64
-    HC_TRY_HIP(hc::apply_op<double>(op->g, op->segs, op->tb64, op->qd64, x, op->q64, op->part64, nullptr, st));
+    HC_TRY_HIP(apply64(op, x, op->q64, nullptr, st));
     HC_TRY_HIP(hc::residual64(b, op->q64, op->r64, op->N, wo, true, st));
     hc::PcgState outer{};
     if ((rc = read_state(op, 1, st, outer)))
@@ -601,7 +648,7 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
         return rc;
       total += fin.iters;
       HC_TRY_HIP(hc::ir_update(x, op->x32, op->N, op->st_dev + 1, st));
-      HC_TRY_HIP(hc::apply_op<double>(op->g, op->segs, op->tb64, op->qd64, x, op->q64, op->part64, nullptr, st));
+      HC_TRY_HIP(apply64(op, x, op->q64, nullptr, st));
       HC_TRY_HIP(hc::residual64(b, op->q64, op->r64, op->N, wo, false, st));
       if ((rc = read_state(op, 1, st, outer)))
         return rc;
@@ -616,7 +663,7 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
   else
     return fail(HC_EINVAL, "hc_pcg_solve: unknown mode");
   // final true residual in FP64
-  HC_TRY_HIP(hc::apply_op<double>(op->g, op->segs, op->tb64, op->qd64, x, op->q64, op->part64, nullptr, st));
+  HC_TRY_HIP(apply64(op, x, op->q64, nullptr, st));
   HC_TRY_HIP(hc::residual64(b, op->q64, op->r64, op->N, wo, true, st));
   HC_TRY_HIP(hipEventRecord(e1, st));
   hc::PcgState outer{};
@@ -642,6 +689,121 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
                   s.iterations, s.rel_res);
     return fail(HC_ENOTCONV, buf);
   }
+  return HC_OK;
+}
+
+namespace
+{
+const hc::TileCandidate *
+find_candidate(int p, int tx, int ty, int tz, int prefetch)
+{
+  const hc::TileCandidate * c;
+  const int n = hc::tile_candidates(p, &c);
+  for (int i = 0; i < n; i++)
+    if (c[i].shape.tx == tx && c[i].shape.ty == ty && c[i].shape.tz == tz &&
+        (prefetch < 0 || c[i].mode == prefetch))
+      return &c[i];
+  return nullptr;
+}
+} // namespace
+
+int
+hc_tile_list(int order, int * out, int max)
+{
+  const hc::TileCandidate * c;
+  const int n = hc::tile_candidates(order, &c);
+  for (int i = 0; i < n && i < max && out; i++)
+  {
+    out[4 * i + 0] = c[i].shape.tx;
+    out[4 * i + 1] = c[i].shape.ty;
+    out[4 * i + 2] = c[i].shape.tz;
+    out[4 * i + 3] = c[i].mode;
+  }
+  return n;
+}
+
+int
+hc_op_set_tile(hc_op_t op, int prec_bits, int tx, int ty, int tz, int prefetch)
+{
+  if (!op || (prec_bits != 64 && prec_bits != 32))
+    return fail(HC_EINVAL, "hc_op_set_tile: bad handle or precision (64 or 32)");
+  const hc::TileCandidate * c = find_candidate(op->p, tx, ty, tz, prefetch);
+  if (!c)
+    return fail(HC_EINVAL, "hc_op_set_tile: no compiled tile variant with this shape for order " +
+                               std::to_string(op->p));
+  set_plan(op, prec_bits == 64 ? op->plan64 : op->plan32, c);
+  return HC_OK;
+}
+
+int
+hc_op_get_tile(hc_op_t op, int prec_bits, int * tx, int * ty, int * tz, int * prefetch)
+{
+  if (!op || (prec_bits != 64 && prec_bits != 32) || !tx || !ty || !tz || !prefetch)
+    return fail(HC_EINVAL, "hc_op_get_tile: bad argument");
+  const ApplyPlan & pl = prec_bits == 64 ? op->plan64 : op->plan32;
+  *tx = pl.cand->shape.tx;
+  *ty = pl.cand->shape.ty;
+  *tz = pl.cand->shape.tz;
+  *prefetch = pl.cand->mode;
+  return HC_OK;
+}
+
+int
+hc_op_autotune(hc_op_t op, int reps, double * best_us, void * stream)
+{
+  if (!op)
+    return fail(HC_EINVAL, "hc_op_autotune: null handle");
+  if (!op->setup_done)
+    return fail(HC_EINVAL, "hc_op_autotune: hc_op_setup has not been called");
+  if (reps < 1)
+    reps = 5;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = ensure_solver_work(op, op->qd32 != nullptr);
+  if (rc)
+    return rc;
+  const hc::TileCandidate * c;
+  const int n = hc::tile_candidates(op->p, &c);
+  hipEvent_t e0, e1;
+  HC_TRY_HIP(hipEventCreate(&e0));
+  HC_TRY_HIP(hipEventCreate(&e1));
+  HC_TRY_HIP(hc::fill_uniform(&op->g, 17, op->r64, op->N, st));
+  for (int prec = 64; prec >= 32; prec -= 32)
+  {
+    if (prec == 32 && !op->qd32)
+      break;
+    if (prec == 32)
+      HC_TRY_HIP(hc::cast_f32(op->r64, op->r32, op->N, st));
+    ApplyPlan & pl = prec == 64 ? op->plan64 : op->plan32;
+    float best = 1e30f;
+    const hc::TileCandidate * bestc = pl.cand;
+    for (int i = 0; i < n; i++)
+    {
+      set_plan(op, pl, &c[i]);
+      auto run = [&]() -> hipError_t {
+        return prec == 64 ? apply64(op, op->r64, op->q64, nullptr, st)
+                          : apply_t<float>(op, op->tb32, op->qd32, op->r32, op->q32, op->part32, nullptr, st);
+      };
+      for (int w = 0; w < 2; w++)
+        HC_TRY_HIP(run());
+      HC_TRY_HIP(hipEventRecord(e0, st));
+      for (int r = 0; r < reps; r++)
+        HC_TRY_HIP(run());
+      HC_TRY_HIP(hipEventRecord(e1, st));
+      HC_TRY_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      HC_TRY_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best)
+      {
+        best = ms;
+        bestc = &c[i];
+      }
+    }
+    set_plan(op, pl, bestc);
+    if (best_us)
+      best_us[prec == 64 ? 0 : 1] = 1000.0 * best / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return HC_OK;
 }
 

@@ -21,6 +21,18 @@ struct PlaneSegs
   int64_t start[7];
 };
 
+// A compiled tile variant of the apply kernel (shape + qdata prefetch).
+using TileLaunch = hipError_t (*)(const Geom & g, const void * tables, const void * qd, const void * x, void * y,
+                                  void * part, const int * done, int prec_bits, hipStream_t st);
+struct TileCandidate
+{
+  int p;
+  TileShape shape;
+  int mode; // 0: qdata loaded at the quadrature stage, 1: at entry, 2: persistent pipelined
+  TileLaunch launch;
+};
+// Candidates of order p (contiguous array); returns their count.
+int tile_candidates(int p, const TileCandidate ** out);
 TileShape default_tile(int p);
 PlaneSegs make_plane_segs(const Geom & g);
 
@@ -47,7 +59,8 @@ hipError_t apply_op(const Geom & g,
                     T * y,
                     T * part,
                     const int * done,
-                    hipStream_t st);
+                    hipStream_t st,
+                    const TileCandidate * cand = nullptr);
 
 // Sum of the tile-face partials (fixed order) into y.
 template <typename T>

@@ -34,7 +34,8 @@ SOLVE_FP32_IR = 1
 EXPORTS = [
     "hc_op_create", "hc_op_set_coefficients", "hc_op_setup", "hc_op_ndofs", "hc_op_ess_count",
     "hc_op_device_bytes", "hc_op_apply", "hc_op_apply_f32", "hc_op_diagonal", "hc_pcg_solve",
-    "hc_op_destroy", "hc_last_error", "hc_fill_uniform",
+    "hc_op_destroy", "hc_last_error", "hc_fill_uniform", "hc_op_set_tile", "hc_op_get_tile",
+    "hc_op_autotune", "hc_tile_list",
 ]
 
 
@@ -88,6 +89,11 @@ def load():
     lib.hc_op_destroy.argtypes = [vp]
     lib.hc_last_error.restype = C.c_char_p
     lib.hc_fill_uniform.argtypes = [vp, C.c_uint64, vp, C.c_int64, vp]
+    lib.hc_op_set_tile.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.hc_op_get_tile.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int)]
+    lib.hc_op_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_double), vp]
+    lib.hc_tile_list.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_int]
     _lib = lib
     return lib
 
@@ -106,6 +112,15 @@ def _stream(stream=None):
 
 def _ptr(t):
     return C.c_void_p(t.data_ptr())
+
+
+def tile_list(order):
+    """Compiled tile variants (tx, ty, tz, prefetch) of the apply kernel for an order."""
+    lib = load()
+    n = lib.hc_tile_list(int(order), None, 0)
+    buf = (C.c_int * (4 * n))()
+    lib.hc_tile_list(int(order), buf, n)
+    return [tuple(buf[4 * i:4 * i + 4]) for i in range(n)]
 
 
 class Operator:
@@ -190,6 +205,19 @@ class Operator:
         d = st.as_dict()
         d["rc"] = rc
         return d
+
+    def set_tile(self, prec_bits, tx, ty, tz, prefetch=-1):
+        _check(_lib.hc_op_set_tile(self._h, prec_bits, tx, ty, tz, prefetch), "hc_op_set_tile")
+
+    def get_tile(self, prec_bits=64):
+        v = [C.c_int() for _ in range(4)]
+        _check(_lib.hc_op_get_tile(self._h, prec_bits, *[C.byref(a) for a in v]), "hc_op_get_tile")
+        return tuple(int(a.value) for a in v)
+
+    def autotune(self, reps=5, stream=None):
+        best = (C.c_double * 2)()
+        _check(_lib.hc_op_autotune(self._h, int(reps), best, _stream(stream)), "hc_op_autotune")
+        return {"fp64_us": best[0], "fp32_us": best[1], "tile64": self.get_tile(64), "tile32": self.get_tile(32)}
 
     def fill_uniform(self, seed, v, stream=None):
         _check(_lib.hc_fill_uniform(self._h, C.c_uint64(seed), _ptr(v), v.numel(), _stream(stream)),

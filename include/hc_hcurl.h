@@ -125,6 +125,23 @@ int64_t hc_op_device_bytes(hc_op_t op);
 /* y = A x (PEC eliminated).  x, y: device, layout [dof][rhs]. */
 int hc_op_apply(hc_op_t op, const double * x, double * y, int nrhs, void * stream);
 int hc_op_apply_f32(hc_op_t op, const float * x, float * y, int nrhs, void * stream);
+/* Element-tile shape of the apply kernel.  Each order has a set of compiled
+ * tile variants: tile extent tx x ty x tz elements and a load mode (0:
+ * quadrature data loaded where it is used, 1: at kernel entry, 2: persistent
+ * workgroups that load tile i+1 while finishing tile i), plus 4 when each
+ * pencil's quadrature points are split over two wave groups.  Results differ
+ * between shapes by round-off only and are bitwise reproducible for a fixed
+ * shape.  prec_bits: 64 or 32; mode -1 in hc_op_set_tile matches any mode. */
+int hc_op_set_tile(hc_op_t op, int prec_bits, int tx, int ty, int tz, int mode);
+/* Compiled tile variants of an order: writes (tx, ty, tz, mode) for up to
+ * `max` variants into out and returns the number of variants. */
+int hc_tile_list(int order, int * out, int max);
+int hc_op_get_tile(hc_op_t op, int prec_bits, int * tx, int * ty, int * tz, int * mode);
+/* Autotuner: times every compiled tile variant on this operator (after
+ * hc_op_setup), `reps` applies each, and keeps the fastest per precision.
+ * best_us (optional, 2 entries): the winning FP64 / FP32 apply times.
+ * No reference counterpart (the reference applies a CSR matrix). */
+int hc_op_autotune(hc_op_t op, int reps, double * best_us, void * stream);
 /* Jacobi diagonal (1 on PEC DOFs) into device array d. */
 int hc_op_diagonal(hc_op_t op, double * d, void * stream);
 

@@ -129,6 +129,37 @@ def test_apply_vs_oracle(torch_cuda, hc, oracle, p, n, pec, perturb):
     assert _rel(y32, y_ref) <= 2e-6
 
 
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_every_tile_variant(torch_cuda, hc, oracle, p):
+    """Each compiled tile shape of the apply kernel gives the oracle's result
+    (several tiles per direction, partial tiles, trilinear geometry)."""
+    torch = torch_cuda
+    from hephaestus_amd import hcurl
+    n = {1: (11, 9, 6), 2: (9, 7, 5), 3: (9, 5, 7)}[p]
+    print(f"case tiles p={p} n={n}", flush=True)
+    rng = np.random.default_rng(300 + p)
+    m0 = oracle.Mesh(*n)
+    vert = m0.box_vertices() + 0.05 * rng.standard_normal(((n[0] + 1) * (n[1] + 1) * (n[2] + 1), 3)) / max(n)
+    m = oracle.Mesh(*n, vert=vert)
+    E = m.nelem
+    nu, sig = rng.uniform(0.5, 2.0, E), 10.0 ** rng.uniform(-3, 3, E)
+    qd = oracle.pa_setup(m, p, nu, sig)
+    ess = oracle.ess_mask(m, p)
+    x = oracle.rand_uniform(21, oracle.ndofs(m, p))
+    y_ref = oracle.pa_apply(m, p, qd, ess, x)
+    op = hc.Operator(n, p, nu, sig, vert=vert).setup(1.0)
+    xd = _dev(torch, x)
+    tiles = hcurl.tile_list(p)
+    assert len(tiles) >= 2
+    for (tx, ty, tz, pf) in tiles:
+        op.set_tile(64, tx, ty, tz, pf)
+        op.set_tile(32, tx, ty, tz, pf)
+        assert _rel(op.apply(xd).cpu().numpy(), y_ref) <= 1e-13, (tx, ty, tz, pf)
+        assert _rel(op.apply_f32(xd.float()).double().cpu().numpy(), y_ref) <= 2e-6, (tx, ty, tz, pf)
+    info = op.autotune(reps=2)
+    assert info["tile64"] in [t for t in tiles]
+
+
 def test_setup_dt_rule(torch_cuda, hc, oracle):
     """hc_op_setup re-assembles only when dt changes by more than 1e-12 dt
     (TimeDependentEquationSystem::SetTimeStep, equation_system.cpp:413-426)."""
