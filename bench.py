@@ -112,6 +112,9 @@ def main():
     N = op.ndofs
     print(f"[bench] operator {n}^3 p={p} N={N} setup {setup_s:.2f}s device bytes "
           f"{op.device_bytes / 1e9:.2f} GB", file=sys.stderr, flush=True)
+    tune = op.autotune(reps=5)
+    print(f"[bench] autotuned tiles: fp64 {tune['tile64']} ({tune['fp64_us']:.1f} us), "
+          f"fp32 {tune['tile32']} ({tune['fp32_us']:.1f} us)", file=sys.stderr, flush=True)
     x = torch.empty(N, dtype=torch.float64, device="cuda")
     op.fill_uniform(2, x)
     y = torch.empty_like(x)
@@ -144,7 +147,8 @@ def main():
         "data": "synthetic (splitmix64 x, uniform [-1,1) on free DOFs)",
         "config": {"workload": f"FP64 PA apply, {n}^3 hex unit cube, ND order {p}, N={N}, nu=1, "
                                f"sigma/dt two-region 1e4/1e-6, PEC all faces",
-                   "n": n, "order": p, "ndofs": N, "elements": E},
+                   "n": n, "order": p, "ndofs": N, "elements": E,
+                   "tile_fp64": list(tune["tile64"]), "tile_fp32": list(tune["tile32"])},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_apply_tile + k_assemble_planes (one apply)",

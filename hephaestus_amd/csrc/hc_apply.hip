@@ -822,12 +822,13 @@ launch_one(const Geom & g, const void * tb, const void * qd, const void * x, voi
   {P, {TX, TY, TZ, Tile<P, TX, TY, TZ, SP>::NT}, M + 4 * (SP - 1), &launch_one<P, TX, TY, TZ, M, SP>}
 
 static const TileCandidate g_cands[] = {
-    HC_CAND(1, 4, 4, 4, 1, 1), HC_CAND(1, 4, 4, 4, 2, 1), HC_CAND(1, 8, 4, 2, 1, 1), HC_CAND(1, 4, 4, 4, 1, 2),
+    // first entry of each order = default before tuning
+    HC_CAND(1, 4, 4, 4, 1, 2), HC_CAND(1, 4, 4, 4, 1, 1), HC_CAND(1, 4, 4, 4, 2, 1), HC_CAND(1, 8, 4, 2, 1, 1),
     HC_CAND(1, 8, 4, 2, 0, 2), HC_CAND(1, 8, 8, 1, 0, 1),
     HC_CAND(2, 4, 2, 2, 1, 1), HC_CAND(2, 4, 2, 2, 1, 2), HC_CAND(2, 4, 2, 2, 0, 2), HC_CAND(2, 4, 4, 2, 1, 1),
     HC_CAND(2, 4, 4, 2, 0, 2), HC_CAND(2, 8, 2, 2, 0, 1),
-    HC_CAND(3, 4, 2, 2, 1, 1), HC_CAND(3, 4, 2, 2, 1, 2), HC_CAND(3, 4, 2, 2, 0, 2), HC_CAND(3, 4, 2, 2, 2, 2),
-    HC_CAND(3, 4, 4, 1, 1, 2), HC_CAND(3, 4, 4, 1, 0, 2), HC_CAND(3, 2, 2, 4, 1, 2), HC_CAND(3, 4, 2, 2, 0, 1),
+    HC_CAND(3, 4, 2, 2, 1, 2), HC_CAND(3, 4, 2, 2, 1, 1), HC_CAND(3, 4, 2, 2, 0, 2), HC_CAND(3, 4, 2, 2, 2, 2),
+    HC_CAND(3, 4, 4, 1, 1, 2), HC_CAND(3, 2, 2, 4, 1, 2), HC_CAND(3, 4, 2, 2, 0, 1),
 };
 #undef HC_CAND
 

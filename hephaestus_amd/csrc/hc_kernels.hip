@@ -12,6 +12,8 @@
 // y_free = (A x_free)_free, y_ess = x_ess.
 #include "hc_kernels.hpp"
 
+#include <algorithm>
+
 namespace hc
 {
 
@@ -133,41 +135,52 @@ k_cast_f32(const double * __restrict__ a, float * __restrict__ b, int64_t n)
     b[i] = (float)a[i];
 }
 
-// Sums the tile partials of DOFs on interior tile faces.  Thread space:
-// segments (component c, closed axis a) x (nt[a]-1 planes) x in-plane DOFs.
+// Sums the tile partials of DOFs on interior tile faces.  Grid: x = position
+// along the plane's first in-plane axis, y = (plane, position along the second
+// in-plane axis), z = segment (component c, plane axis a).  DOFs on the
+// intersection of two plane families are summed by the lower-numbered axis.
 template <typename T>
 __global__ void
 k_assemble_planes(Geom g, PlaneSegs segs, const T * __restrict__ part, T * __restrict__ y, const int * __restrict__ done)
 {
   if (done && *done)
     return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= segs.start[segs.nseg])
-    return;
-  int s = 0;
-  while (t >= segs.start[s + 1])
-    s++;
+  const int s = blockIdx.z;
   const int c = segs.comp[s], a = segs.axis[s];
   const int b0 = (a == 0) ? 1 : 0;
   const int b1 = (a == 2) ? 1 : 2;
-  int64_t r = t - segs.start[s];
-  const int64_t n0 = g.cd[c][b0], n1 = g.cd[c][b1];
-  const int64_t j0 = r % n0;
-  r /= n0;
-  const int64_t j1 = r % n1;
-  const int64_t k = r / n1 + 1;
-  int64_t gi[3];
+  const int n0 = (int)g.cd[c][b0], n1 = (int)g.cd[c][b1];
+  const int j0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j0 >= n0)
+    return;
+  const int nrow = (g.nt[a] - 1) * n1;
+  for (int row = blockIdx.y; row < nrow; row += gridDim.y)
+  {
+  const int k = row / n1 + 1; // plane index 1 .. nt[a]-1
+  const int j1 = row % n1;
+  int gi[3];
   gi[a] = k * g.T[a] * g.p;
   gi[b0] = j0;
   gi[b1] = j1;
-  // the other closed axis
+  // the other closed axis of component c
   const int a2 = 3 - c - a;
-  const bool on2 = on_tile_plane(g, a2, gi[a2]);
+  const int step2 = g.T[a2] * g.p;
+  const bool on2 = gi[a2] > 0 && gi[a2] < g.n[a2] * g.p && (gi[a2] % step2) == 0;
   if (on2 && a2 < a)
-    return; // enumerated by the lower-numbered axis
-  if (is_ess(g, c, gi[0], gi[1], gi[2]))
-    return;
-  const int64_t gid = g.off[c] + (gi[2] * g.cd[c][1] + gi[1]) * g.cd[c][0] + gi[0];
+    continue; // summed by the lower-numbered axis
+  // essential (PEC) DOFs were written by their owning tile
+  bool ess = false;
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+  {
+    if (d == c)
+      continue;
+    const int dim = (int)g.cd[c][d];
+    ess = ess || (gi[d] == 0 && ((g.pec >> (2 * d)) & 1)) || (gi[d] == dim - 1 && ((g.pec >> (2 * d + 1)) & 1));
+  }
+  if (ess)
+    continue;
+  const int64_t gid = g.off[c] + ((int64_t)gi[2] * g.cd[c][1] + gi[1]) * g.cd[c][0] + gi[0];
   const int64_t N = g.off[3];
   // slot bits: bit 0 = lower-numbered closed axis, bit 1 = the other
   const int dlo = (c == 0) ? 1 : 0;
@@ -181,6 +194,7 @@ k_assemble_planes(Geom g, PlaneSegs segs, const T * __restrict__ part, T * __res
   if (on_lo && on_hi)
     v += part[3 * N + gid];
   y[gid] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -319,9 +333,29 @@ template <typename T>
 hipError_t
 assemble_planes(const Geom & g, const PlaneSegs & segs, const T * part, T * y, const int * done, hipStream_t st)
 {
-  const int64_t n = segs.start[segs.nseg];
-  if (n > 0)
-    hipLaunchKernelGGL(k_assemble_planes<T>, dim3(div_up(n, 256)), dim3(256), 0, st, g, segs, part, y, done);
+  if (segs.nseg == 0)
+    return hipSuccess;
+  int n0max = 1, ymax = 1;
+  for (int s = 0; s < segs.nseg; s++)
+  {
+    const int c = segs.comp[s], a = segs.axis[s];
+    const int b0 = (a == 0) ? 1 : 0;
+    const int b1 = (a == 2) ? 1 : 2;
+    n0max = std::max(n0max, (int)g.cd[c][b0]);
+    ymax = std::max(ymax, (g.nt[a] - 1) * (int)g.cd[c][b1]);
+  }
+  (void)ymax;
+  // grid.y must index (plane, j1) for every segment: use the largest
+  int gy = 0;
+  for (int s = 0; s < segs.nseg; s++)
+  {
+    const int c = segs.comp[s], a = segs.axis[s];
+    const int b1 = (a == 2) ? 1 : 2;
+    gy = std::max(gy, (g.nt[a] - 1) * (int)g.cd[c][b1]);
+  }
+  const int bx = 64;
+  hipLaunchKernelGGL(k_assemble_planes<T>, dim3((n0max + bx - 1) / bx, std::min(gy, 65535), segs.nseg), dim3(bx), 0,
+                     st, g, segs, part, y, done);
   return hipGetLastError();
 }
 
