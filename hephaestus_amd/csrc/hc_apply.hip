@@ -45,31 +45,37 @@ struct Tile
   static constexpr int SB = pad_to(3 * LB, 16, 4);
 };
 
-template <int P, typename T, int TX, int TY, int TZ, int MODE, int SP>
+// MULTI = false: one instance, described by the by-value argument d0 (its
+// fields stay in scalar registers); MULTI = true: the instances of descs[].
+template <int P, typename T, int TX, int TY, int TZ, int MODE, int SP, bool MULTI>
 __global__ void __launch_bounds__((Tile<P, TX, TY, TZ, SP>::NT))
-k_apply_tile(Geom g,
-             Tables<T> tb,
-             const T * __restrict__ qd,
-             const T * __restrict__ x,
-             T * __restrict__ y,
-             T * __restrict__ part,
-             const int * __restrict__ done)
+k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables<T> tb)
 {
   using C = Tile<P, TX, TY, TZ, SP>;
   constexpr int QH = C::QH, NPP = C::NPP;
   constexpr int O = C::O, D = C::D, Q = C::Q, QQ = C::QQ;
   constexpr int SQX = C::SQX, LA = C::LA, SDZ = C::SDZ, LB = C::LB, SA = C::SA, SB = C::SB;
   constexpr int NT = C::NT, TE = C::TE;
-  if (done && *done)
+  // ---- work item: (instance, tile, rhs), rhs fastest ----
+  const int w = blockIdx.x;
+  int di = 0;
+  if constexpr (MULTI)
+    di = __builtin_amdgcn_readfirstlane(find_instance(w, ndesc, [&](int i) { return descs[i].w0; }));
+  const ApplyDesc & ad = MULTI ? descs[di] : d0;
+  const int wl = w - ad.w0;
+  const int rhs = wl % ad.nrhs;
+  const int tcur = wl / ad.nrhs;
+  if (ad.done && ad.done[rhs * ad.done_stride])
     return;
+  const Geom & g = ad.g;
+  const T * __restrict__ qd = (const T *)ad.qd;
+  const T * __restrict__ x = (const T *)ad.x + (int64_t)rhs * g.off[3];
+  T * __restrict__ y = (T *)ad.y + (int64_t)rhs * g.off[3];
+  T * __restrict__ part = (T *)ad.part + (int64_t)rhs * 4 * g.off[3];
   __shared__ T sA[TE * SA];
   __shared__ T sB[TE * SB];
 
   const int tid = threadIdx.x;
-  const int ntiles = g.nt[0] * g.nt[1] * g.nt[2];
-  int tcur = blockIdx.x;
-  if (tcur >= ntiles)
-    return;
   // tile position, element origin and active extent (partial tiles at the
   // upper domain faces)
   auto tile_pos = [&](int t, int * tp, int * e0_, int * ne_) {
@@ -181,14 +187,11 @@ k_apply_tile(Geom g,
     }
   };
   // MODE 0: quadrature data loaded at the quadrature-point stage (fewest
-  // registers); MODE 1: at entry; MODE 2: persistent workgroups, the loads of
-  // tile i+1 are issued right after the quadrature-point stage of tile i and
-  // fly during its backward contractions and row walkers.
+  // registers); MODE 1: at entry, hidden behind the forward contractions.
   load_x(e0, ne);
   if (MODE >= 1)
     load_qf(e0, ne);
 
-  for (;;)
   {
   T U[3][QH], Cu[3][QH];
 #pragma unroll
@@ -400,18 +403,6 @@ k_apply_tile(Geom g,
     Cu[0][j] = qf[j][6] * c0 + qf[j][7] * c1 + qf[j][8] * c2;
     Cu[1][j] = qf[j][7] * c0 + qf[j][9] * c1 + qf[j][10] * c2;
     Cu[2][j] = qf[j][8] * c0 + qf[j][10] * c1 + qf[j][11] * c2;
-  }
-
-  // next tile of a persistent workgroup: issue its loads now (registers of
-  // qf and the x rows are free after the quadrature-point stage)
-  const int tnext = tcur + (int)gridDim.x;
-  const bool more = (MODE == 2) && tnext < ntiles;
-  int ntp[3] = {0, 0, 0}, ne0[3] = {0, 0, 0}, nne[3] = {0, 0, 0};
-  if (more)
-  {
-    tile_pos(tnext, ntp, ne0, nne);
-    load_x(ne0, nne);
-    load_qf(ne0, nne);
   }
 
   // ======================= backward =======================
@@ -761,18 +752,7 @@ k_apply_tile(Geom g,
     else
       y[gid] = sOut[i];
   }
-  if (!more)
-    break;
-  __syncthreads(); // this tile's LDS fully consumed before the next forward pass
-  tcur = tnext;
-#pragma unroll
-  for (int a = 0; a < 3; a++)
-  {
-    tpos[a] = ntp[a];
-    e0[a] = ne0[a];
-    ne[a] = nne[a];
   }
-  } // tile loop
 }
 
 // ---------------------------------------------------------------------------
@@ -780,55 +760,39 @@ k_apply_tile(Geom g,
 // ---------------------------------------------------------------------------
 template <int P, int TX, int TY, int TZ, int MODE, int SP>
 static hipError_t
-launch_one(const Geom & g, const void * tb, const void * qd, const void * x, void * y, void * part, const int * done,
-           int prec, hipStream_t st)
+launch_one(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, const void * tb, int prec,
+           hipStream_t st)
 {
   using C = Tile<P, TX, TY, TZ, SP>;
-  const int ntiles = g.nt[0] * g.nt[1] * g.nt[2];
-  int nblk = ntiles;
-  if (MODE == 2)
-  {
-    // persistent: as many workgroups as can be resident at once
-    static int resident[2] = {0, 0};
-    int & r = resident[prec == 64 ? 0 : 1];
-    if (r == 0)
-    {
-      int dev = 0, ncu = 0, per = 0;
-      (void)hipGetDevice(&dev);
-      (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-      if (prec == 64)
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_apply_tile<P, double, TX, TY, TZ, MODE, SP>, C::NT,
-                                                           0);
-      else
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_apply_tile<P, float, TX, TY, TZ, MODE, SP>, C::NT,
-                                                           0);
-      r = (ncu > 0 ? ncu : 256) * (per > 0 ? per : 1);
-    }
-    nblk = ntiles < r ? ntiles : r;
-  }
-  if (prec == 64)
-    hipLaunchKernelGGL((k_apply_tile<P, double, TX, TY, TZ, MODE, SP>), dim3(nblk), dim3(C::NT), 0, st, g,
-                       *(const Tables<double> *)tb, (const double *)qd, (const double *)x, (double *)y,
-                       (double *)part, done);
+  if (nwork <= 0)
+    return hipSuccess;
+  if (prec == 64 && descs)
+    hipLaunchKernelGGL((k_apply_tile<P, double, TX, TY, TZ, MODE, SP, true>), dim3(nwork), dim3(C::NT), 0, st, d0,
+                       descs, ndesc, *(const Tables<double> *)tb);
+  else if (prec == 64)
+    hipLaunchKernelGGL((k_apply_tile<P, double, TX, TY, TZ, MODE, SP, false>), dim3(nwork), dim3(C::NT), 0, st, d0,
+                       descs, ndesc, *(const Tables<double> *)tb);
+  else if (descs)
+    hipLaunchKernelGGL((k_apply_tile<P, float, TX, TY, TZ, MODE, SP, true>), dim3(nwork), dim3(C::NT), 0, st, d0,
+                       descs, ndesc, *(const Tables<float> *)tb);
   else
-    hipLaunchKernelGGL((k_apply_tile<P, float, TX, TY, TZ, MODE, SP>), dim3(nblk), dim3(C::NT), 0, st, g,
-                       *(const Tables<float> *)tb, (const float *)qd, (const float *)x, (float *)y, (float *)part,
-                       done);
+    hipLaunchKernelGGL((k_apply_tile<P, float, TX, TY, TZ, MODE, SP, false>), dim3(nwork), dim3(C::NT), 0, st, d0,
+                       descs, ndesc, *(const Tables<float> *)tb);
   return hipGetLastError();
 }
 
-// mode field: load mode (0, 1, 2) + 4 * (number of qz groups - 1)
+// mode field: load mode (0, 1) + 4 * (number of qz groups - 1)
 #define HC_CAND(P, TX, TY, TZ, M, SP)                                                                            \
   {P, {TX, TY, TZ, Tile<P, TX, TY, TZ, SP>::NT}, M + 4 * (SP - 1), &launch_one<P, TX, TY, TZ, M, SP>}
 
 static const TileCandidate g_cands[] = {
     // first entry of each order = default before tuning
-    HC_CAND(1, 4, 4, 4, 1, 2), HC_CAND(1, 4, 4, 4, 1, 1), HC_CAND(1, 4, 4, 4, 2, 1), HC_CAND(1, 8, 4, 2, 1, 1),
-    HC_CAND(1, 8, 4, 2, 0, 2), HC_CAND(1, 8, 8, 1, 0, 1),
+    HC_CAND(1, 4, 4, 4, 1, 2), HC_CAND(1, 4, 4, 4, 1, 1), HC_CAND(1, 8, 4, 2, 1, 1), HC_CAND(1, 8, 4, 2, 0, 2),
+    HC_CAND(1, 8, 8, 1, 0, 1),
     HC_CAND(2, 4, 2, 2, 1, 1), HC_CAND(2, 4, 2, 2, 1, 2), HC_CAND(2, 4, 2, 2, 0, 2), HC_CAND(2, 4, 4, 2, 1, 1),
     HC_CAND(2, 4, 4, 2, 0, 2), HC_CAND(2, 8, 2, 2, 0, 1),
-    HC_CAND(3, 4, 2, 2, 1, 2), HC_CAND(3, 4, 2, 2, 1, 1), HC_CAND(3, 4, 2, 2, 0, 2), HC_CAND(3, 4, 2, 2, 2, 2),
-    HC_CAND(3, 4, 4, 1, 1, 2), HC_CAND(3, 2, 2, 4, 1, 2), HC_CAND(3, 4, 2, 2, 0, 1),
+    HC_CAND(3, 4, 2, 2, 1, 2), HC_CAND(3, 4, 2, 2, 1, 1), HC_CAND(3, 4, 2, 2, 0, 2), HC_CAND(3, 4, 4, 1, 1, 2),
+    HC_CAND(3, 2, 2, 4, 1, 2), HC_CAND(3, 4, 2, 2, 0, 1),
 };
 #undef HC_CAND
 
@@ -858,7 +822,7 @@ default_tile(int p)
 template <typename T>
 hipError_t
 apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T * qd, const T * x, T * y,
-         T * part, const int * done, hipStream_t st, const TileCandidate * cand)
+         T * part, const int * done, hipStream_t st, const TileCandidate * cand, int nrhs, int done_stride)
 {
   if (!cand)
   {
@@ -870,16 +834,48 @@ apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T *
     if (!cand)
       return hipErrorInvalidValue;
   }
-  hipError_t e = cand->launch(g, &tb, qd, x, y, part, done, sizeof(T) == 8 ? 64 : 32, st);
+  ApplyDesc d{};
+  d.g = g;
+  d.segs = segs;
+  d.prows = (int)segs.start[segs.nseg];
+  d.qd = qd;
+  d.x = x;
+  d.y = y;
+  d.part = part;
+  d.done = done;
+  d.done_stride = done_stride;
+  d.nrhs = nrhs;
+  d.w0 = 0;
+  d.ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+  d.pw0 = 0;
+  hipError_t e = cand->launch(d, nullptr, 1, d.ntiles * nrhs, &tb, sizeof(T) == 8 ? 64 : 32, st);
   if (e != hipSuccess)
     return e;
-  return assemble_planes<T>(g, segs, part, y, done, st);
+  return assemble_planes<T>(d, nullptr, 1, d.prows * nrhs, st);
 }
 
 template hipError_t apply_op<double>(const Geom &, const PlaneSegs &, const Tables<double> &, const double *,
                                      const double *, double *, double *, const int *, hipStream_t,
-                                     const TileCandidate *);
+                                     const TileCandidate *, int, int);
 template hipError_t apply_op<float>(const Geom &, const PlaneSegs &, const Tables<float> &, const float *,
-                                    const float *, float *, float *, const int *, hipStream_t, const TileCandidate *);
+                                    const float *, float *, float *, const int *, hipStream_t, const TileCandidate *,
+                                    int, int);
+
+template <typename T>
+hipError_t
+apply_multi(const TileCandidate * cand, const Tables<T> & tb, const ApplyDesc * descs_dev, int ndesc, int nwork,
+            int npwork, hipStream_t st)
+{
+  ApplyDesc dummy{};
+  hipError_t e = cand->launch(dummy, descs_dev, ndesc, nwork, &tb, sizeof(T) == 8 ? 64 : 32, st);
+  if (e != hipSuccess)
+    return e;
+  return assemble_planes<T>(dummy, descs_dev, ndesc, npwork, st);
+}
+
+template hipError_t apply_multi<double>(const TileCandidate *, const Tables<double> &, const ApplyDesc *, int, int,
+                                        int, hipStream_t);
+template hipError_t apply_multi<float>(const TileCandidate *, const Tables<float> &, const ApplyDesc *, int, int, int,
+                                       hipStream_t);
 
 } // namespace hc

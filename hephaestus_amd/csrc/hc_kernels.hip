@@ -135,65 +135,74 @@ k_cast_f32(const double * __restrict__ a, float * __restrict__ b, int64_t n)
     b[i] = (float)a[i];
 }
 
-// Sums the tile partials of DOFs on interior tile faces.  Grid: x = position
-// along the plane's first in-plane axis, y = (plane, position along the second
-// in-plane axis), z = segment (component c, plane axis a).  DOFs on the
-// intersection of two plane families are summed by the lower-numbered axis.
-template <typename T>
-__global__ void
-k_assemble_planes(Geom g, PlaneSegs segs, const T * __restrict__ part, T * __restrict__ y, const int * __restrict__ done)
+// Sums the tile partials of DOFs on interior tile faces.  One block per
+// (instance, rhs, segment, row); a segment is (component c, plane axis a), a
+// row is (plane k, position along the second in-plane axis); the threads
+// stride the first in-plane axis.  DOFs on the intersection of two plane
+// families are summed by the lower-numbered axis; fixed summation order.
+template <typename T, bool MULTI>
+__global__ void __launch_bounds__(64)
+k_assemble_planes(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc)
 {
-  if (done && *done)
+  const int w = blockIdx.x;
+  int di = 0;
+  if constexpr (MULTI)
+    di = __builtin_amdgcn_readfirstlane(find_instance(w, ndesc, [&](int i) { return descs[i].pw0; }));
+  const ApplyDesc & d = MULTI ? descs[di] : d0;
+  const int local = w - d.pw0;
+  const int rhs = local / d.prows;
+  const int row0 = local - rhs * d.prows;
+  if (d.done && d.done[rhs * d.done_stride])
     return;
-  const int s = blockIdx.z;
-  const int c = segs.comp[s], a = segs.axis[s];
+  const Geom & g = d.g;
+  int s = 0;
+  while (s + 1 < d.segs.nseg && d.segs.start[s + 1] <= row0)
+    s++;
+  const int row = row0 - (int)d.segs.start[s];
+  const int c = d.segs.comp[s], a = d.segs.axis[s];
   const int b0 = (a == 0) ? 1 : 0;
   const int b1 = (a == 2) ? 1 : 2;
   const int n0 = (int)g.cd[c][b0], n1 = (int)g.cd[c][b1];
-  const int j0 = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j0 >= n0)
-    return;
-  const int nrow = (g.nt[a] - 1) * n1;
-  for (int row = blockIdx.y; row < nrow; row += gridDim.y)
-  {
   const int k = row / n1 + 1; // plane index 1 .. nt[a]-1
   const int j1 = row % n1;
-  int gi[3];
-  gi[a] = k * g.T[a] * g.p;
-  gi[b0] = j0;
-  gi[b1] = j1;
-  // the other closed axis of component c
-  const int a2 = 3 - c - a;
-  const int step2 = g.T[a2] * g.p;
-  const bool on2 = gi[a2] > 0 && gi[a2] < g.n[a2] * g.p && (gi[a2] % step2) == 0;
-  if (on2 && a2 < a)
-    continue; // summed by the lower-numbered axis
-  // essential (PEC) DOFs were written by their owning tile
-  bool ess = false;
-#pragma unroll
-  for (int d = 0; d < 3; d++)
-  {
-    if (d == c)
-      continue;
-    const int dim = (int)g.cd[c][d];
-    ess = ess || (gi[d] == 0 && ((g.pec >> (2 * d)) & 1)) || (gi[d] == dim - 1 && ((g.pec >> (2 * d + 1)) & 1));
-  }
-  if (ess)
-    continue;
-  const int64_t gid = g.off[c] + ((int64_t)gi[2] * g.cd[c][1] + gi[1]) * g.cd[c][0] + gi[0];
   const int64_t N = g.off[3];
-  // slot bits: bit 0 = lower-numbered closed axis, bit 1 = the other
+  const T * part = (const T *)d.part + (int64_t)rhs * 4 * N;
+  T * y = (T *)d.y + (int64_t)rhs * N;
+  const int a2 = 3 - c - a; // the other closed axis of component c
+  const int step2 = g.T[a2] * g.p;
   const int dlo = (c == 0) ? 1 : 0;
-  const bool on_lo = (a == dlo) ? true : on2;
-  const bool on_hi = (a == dlo) ? on2 : true;
-  T v = part[gid];
-  if (on_lo)
-    v += part[N + gid];
-  if (on_hi)
-    v += part[2 * N + gid];
-  if (on_lo && on_hi)
-    v += part[3 * N + gid];
-  y[gid] = v;
+  for (int j0 = threadIdx.x; j0 < n0; j0 += blockDim.x)
+  {
+    int gi[3];
+    gi[a] = k * g.T[a] * g.p;
+    gi[b0] = j0;
+    gi[b1] = j1;
+    const bool on2 = gi[a2] > 0 && gi[a2] < g.n[a2] * g.p && (gi[a2] % step2) == 0;
+    if (on2 && a2 < a)
+      continue; // summed by the lower-numbered axis
+    bool ess = false;
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+    {
+      if (q == c)
+        continue;
+      const int dim = (int)g.cd[c][q];
+      ess = ess || (gi[q] == 0 && ((g.pec >> (2 * q)) & 1)) || (gi[q] == dim - 1 && ((g.pec >> (2 * q + 1)) & 1));
+    }
+    if (ess)
+      continue; // written by the owning tile
+    const int64_t gid = g.off[c] + ((int64_t)gi[2] * g.cd[c][1] + gi[1]) * g.cd[c][0] + gi[0];
+    // slot bits: bit 0 = lower-numbered closed axis, bit 1 = the other
+    const bool on_lo = (a == dlo) ? true : on2;
+    const bool on_hi = (a == dlo) ? on2 : true;
+    T v = part[gid];
+    if (on_lo)
+      v += part[N + gid];
+    if (on_hi)
+      v += part[2 * N + gid];
+    if (on_lo && on_hi)
+      v += part[3 * N + gid];
+    y[gid] = v;
   }
 }
 
@@ -310,6 +319,7 @@ k_diagonal(Geom g, Tables<double> tb, const double * __restrict__ qd, double * _
 PlaneSegs
 make_plane_segs(const Geom & g)
 {
+  // segments (component, plane axis) with start[] = prefix of plane rows
   PlaneSegs s{};
   s.nseg = 0;
   s.start[0] = 0;
@@ -318,51 +328,38 @@ make_plane_segs(const Geom & g)
     {
       if (a == c || g.nt[a] <= 1)
         continue;
-      const int b0 = (a == 0) ? 1 : 0;
       const int b1 = (a == 2) ? 1 : 2;
-      const int64_t cnt = (int64_t)(g.nt[a] - 1) * g.cd[c][b0] * g.cd[c][b1];
+      const int64_t rows = (int64_t)(g.nt[a] - 1) * g.cd[c][b1];
       s.comp[s.nseg] = c;
       s.axis[s.nseg] = a;
-      s.start[s.nseg + 1] = s.start[s.nseg] + cnt;
+      s.start[s.nseg + 1] = s.start[s.nseg] + rows;
       s.nseg++;
     }
   return s;
 }
 
+void
+plane_rows(ApplyDesc & d)
+{
+  d.segs = make_plane_segs(d.g);
+  d.prows = (int)d.segs.start[d.segs.nseg];
+}
+
 template <typename T>
 hipError_t
-assemble_planes(const Geom & g, const PlaneSegs & segs, const T * part, T * y, const int * done, hipStream_t st)
+assemble_planes(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int npwork, hipStream_t st)
 {
-  if (segs.nseg == 0)
+  if (npwork <= 0)
     return hipSuccess;
-  int n0max = 1, ymax = 1;
-  for (int s = 0; s < segs.nseg; s++)
-  {
-    const int c = segs.comp[s], a = segs.axis[s];
-    const int b0 = (a == 0) ? 1 : 0;
-    const int b1 = (a == 2) ? 1 : 2;
-    n0max = std::max(n0max, (int)g.cd[c][b0]);
-    ymax = std::max(ymax, (g.nt[a] - 1) * (int)g.cd[c][b1]);
-  }
-  (void)ymax;
-  // grid.y must index (plane, j1) for every segment: use the largest
-  int gy = 0;
-  for (int s = 0; s < segs.nseg; s++)
-  {
-    const int c = segs.comp[s], a = segs.axis[s];
-    const int b1 = (a == 2) ? 1 : 2;
-    gy = std::max(gy, (g.nt[a] - 1) * (int)g.cd[c][b1]);
-  }
-  const int bx = 64;
-  hipLaunchKernelGGL(k_assemble_planes<T>, dim3((n0max + bx - 1) / bx, std::min(gy, 65535), segs.nseg), dim3(bx), 0,
-                     st, g, segs, part, y, done);
+  if (descs)
+    hipLaunchKernelGGL((k_assemble_planes<T, true>), dim3(npwork), dim3(64), 0, st, d0, descs, ndesc);
+  else
+    hipLaunchKernelGGL((k_assemble_planes<T, false>), dim3(npwork), dim3(64), 0, st, d0, descs, ndesc);
   return hipGetLastError();
 }
 
-template hipError_t assemble_planes<double>(const Geom &, const PlaneSegs &, const double *, double *, const int *,
-                                            hipStream_t);
-template hipError_t assemble_planes<float>(const Geom &, const PlaneSegs &, const float *, float *, const int *,
-                                           hipStream_t);
+template hipError_t assemble_planes<double>(const ApplyDesc &, const ApplyDesc *, int, int, hipStream_t);
+template hipError_t assemble_planes<float>(const ApplyDesc &, const ApplyDesc *, int, int, hipStream_t);
 
 hipError_t
 pa_setup(const Geom & g, const Tables<double> & tb, const double * vert, const double h[3],

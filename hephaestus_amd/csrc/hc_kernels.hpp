@@ -12,7 +12,8 @@ struct TileShape
   int tx, ty, tz, nthr;
 };
 
-// Thread segments of the tile-face assembly kernel.
+// Segments (component, plane axis) of the tile-face assembly; start[] is the
+// prefix of plane rows.
 struct PlaneSegs
 {
   int nseg;
@@ -21,9 +22,47 @@ struct PlaneSegs
   int64_t start[7];
 };
 
-// A compiled tile variant of the apply kernel (shape + qdata prefetch).
-using TileLaunch = hipError_t (*)(const Geom & g, const void * tables, const void * qd, const void * x, void * y,
-                                  void * part, const int * done, int prec_bits, hipStream_t st);
+// One operator instance of an apply launch.  Work items are (tile, rhs) with
+// the rhs index fastest; the items of instance i occupy [w0, w0 + ntiles*nrhs)
+// of the launch's flattened work range.  Vectors of rhs r start at r*N.
+struct ApplyDesc
+{
+  Geom g;
+  PlaneSegs segs;
+  const void * qd; // quadrature data (element type of the launch)
+  const void * x;
+  void * y;
+  void * part;     // 4 N per rhs
+  const int * done;// per-rhs flags at done[r * done_stride], or null
+  int done_stride;
+  int nrhs;
+  int w0;          // first tile work item
+  int ntiles;
+  int pw0;         // first plane-assembly work item
+  int prows;       // plane rows per rhs
+};
+
+// Locate the instance of a flattened work item: largest i with key(i) <= w.
+template <typename F>
+__device__ inline int
+find_instance(int w, int ndesc, F key)
+{
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi)
+  {
+    const int mid = (lo + hi + 1) >> 1;
+    if (key(mid) <= w)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// A compiled tile variant of the apply kernel (shape + load mode).
+// descs == nullptr: the single instance d0; otherwise ndesc instances.
+using TileLaunch = hipError_t (*)(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork,
+                                  const void * tables, int prec_bits, hipStream_t st);
 struct TileCandidate
 {
   int p;
@@ -60,12 +99,22 @@ hipError_t apply_op(const Geom & g,
                     T * part,
                     const int * done,
                     hipStream_t st,
-                    const TileCandidate * cand = nullptr);
+                    const TileCandidate * cand = nullptr,
+                    int nrhs = 1,
+                    int done_stride = 0);
+
+// Fill the plane-assembly fields of a descriptor (segments, rows per rhs).
+void plane_rows(ApplyDesc & d);
+
+// Apply over a device table of instances (all of one order, one tile variant).
+// nwork = sum of ntiles*nrhs, npwork = sum of prows*nrhs.
+template <typename T>
+hipError_t apply_multi(const TileCandidate * cand, const Tables<T> & tb, const ApplyDesc * descs_dev, int ndesc,
+                       int nwork, int npwork, hipStream_t st);
 
 // Sum of the tile-face partials (fixed order) into y.
 template <typename T>
-hipError_t assemble_planes(const Geom & g, const PlaneSegs & segs, const T * part, T * y, const int * done,
-                           hipStream_t st);
+hipError_t assemble_planes(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int npwork, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // Jacobi-PCG (hc_pcg.hip)
