@@ -45,9 +45,12 @@ struct Tile
   static constexpr int SB = pad_to(3 * LB, 16, 4);
 };
 
-// MULTI = false: one instance, described by the by-value argument d0 (its
-// fields stay in scalar registers); MULTI = true: the instances of descs[].
-template <int P, typename T, int TX, int TY, int TZ, int MODE, int SP, bool MULTI>
+// BATCH = false: one instance, described by the by-value argument d0 (its
+// fields stay in scalar registers), one rhs per work item.  BATCH = true: the
+// instances of the device table descs[]; a work item loops over a group of rb
+// right-hand sides with the tile's quadrature data held in registers (read
+// from HBM once per group).
+template <int P, typename T, int TX, int TY, int TZ, int MODE, int SP, bool BATCH>
 __global__ void __launch_bounds__((Tile<P, TX, TY, TZ, SP>::NT))
 k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables<T> tb)
 {
@@ -56,22 +59,27 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   constexpr int O = C::O, D = C::D, Q = C::Q, QQ = C::QQ;
   constexpr int SQX = C::SQX, LA = C::LA, SDZ = C::SDZ, LB = C::LB, SA = C::SA, SB = C::SB;
   constexpr int NT = C::NT, TE = C::TE;
-  // ---- work item: (instance, tile, rhs), rhs fastest ----
+  // ---- work item: (instance, tile, rhs group), group fastest; a group is
+  // rb consecutive right-hand sides (rb = 1 unless BATCH) ----
   const int w = blockIdx.x;
   int di = 0;
-  if constexpr (MULTI)
+  if constexpr (BATCH)
     di = __builtin_amdgcn_readfirstlane(find_instance(w, ndesc, [&](int i) { return descs[i].w0; }));
-  const ApplyDesc & ad = MULTI ? descs[di] : d0;
+  const ApplyDesc & ad = BATCH ? descs[di] : d0;
   const int wl = w - ad.w0;
-  const int rhs = wl % ad.nrhs;
-  const int tcur = wl / ad.nrhs;
-  if (ad.done && ad.done[rhs * ad.done_stride])
+  const int rb = BATCH ? ad.rb : 1;
+  const int ngrp = BATCH ? (ad.nrhs + rb - 1) / rb : ad.nrhs;
+  const int r_begin = (wl % ngrp) * rb;
+  const int r_end = min(ad.nrhs, r_begin + rb);
+  const int tcur = wl / ngrp;
+  if (!BATCH && ad.done && ad.done[r_begin * ad.done_stride])
     return;
   const Geom & g = ad.g;
   const T * __restrict__ qd = (const T *)ad.qd;
-  const T * __restrict__ x = (const T *)ad.x + (int64_t)rhs * g.off[3];
-  T * __restrict__ y = (T *)ad.y + (int64_t)rhs * g.off[3];
-  T * __restrict__ part = (T *)ad.part + (int64_t)rhs * 4 * g.off[3];
+  // vectors of the current right-hand side (reset per rhs below)
+  const T * x = (const T *)ad.x;
+  T * y = (T *)ad.y;
+  T * part = (T *)ad.part;
   __shared__ T sA[TE * SA];
   __shared__ T sB[TE * SB];
 
@@ -188,11 +196,17 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   };
   // MODE 0: quadrature data loaded at the quadrature-point stage (fewest
   // registers); MODE 1: at entry, hidden behind the forward contractions.
-  load_x(e0, ne);
   if (MODE >= 1)
     load_qf(e0, ne);
 
+  for (int rhs = r_begin; rhs < r_end; rhs++)
   {
+  if (BATCH && ad.done && ad.done[rhs * ad.done_stride])
+    continue;
+  x = (const T *)ad.x + (int64_t)rhs * g.off[3];
+  y = (T *)ad.y + (int64_t)rhs * g.off[3];
+  part = (T *)ad.part + (int64_t)rhs * 4 * g.off[3];
+  load_x(e0, ne);
   T U[3][QH], Cu[3][QH];
 #pragma unroll
   for (int j = 0; j < QH; j++)
@@ -752,6 +766,8 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
     else
       y[gid] = sOut[i];
   }
+  if (BATCH && rhs + 1 < r_end)
+    __syncthreads(); // the next rhs overwrites the LDS stages
   }
 }
 
@@ -822,7 +838,8 @@ default_tile(int p)
 template <typename T>
 hipError_t
 apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T * qd, const T * x, T * y,
-         T * part, const int * done, hipStream_t st, const TileCandidate * cand, int nrhs, int done_stride)
+         T * part, const int * done, hipStream_t st, const TileCandidate * cand, int nrhs, int done_stride, int rb,
+         ApplyDesc * desc_dev)
 {
   if (!cand)
   {
@@ -845,21 +862,45 @@ apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T *
   d.done = done;
   d.done_stride = done_stride;
   d.nrhs = nrhs;
+  d.rb = 1;
   d.w0 = 0;
   d.ntiles = g.nt[0] * g.nt[1] * g.nt[2];
   d.pw0 = 0;
-  hipError_t e = cand->launch(d, nullptr, 1, d.ntiles * nrhs, &tb, sizeof(T) == 8 ? 64 : 32, st);
+  hipError_t e;
+  if (rb > 1 && nrhs > 1)
+  {
+    if (!desc_dev)
+      return hipErrorInvalidValue;
+    d.rb = rb < nrhs ? rb : nrhs;
+    if ((e = hipMemcpyAsync(desc_dev, &d, sizeof(d), hipMemcpyHostToDevice, st)) != hipSuccess)
+      return e;
+    e = cand->launch(d, desc_dev, 1, d.ntiles * ((nrhs + d.rb - 1) / d.rb), &tb, sizeof(T) == 8 ? 64 : 32, st);
+  }
+  else
+    e = cand->launch(d, nullptr, 1, d.ntiles * nrhs, &tb, sizeof(T) == 8 ? 64 : 32, st);
   if (e != hipSuccess)
     return e;
   return assemble_planes<T>(d, nullptr, 1, d.prows * nrhs, st);
 }
 
+int
+default_rhs_block(int ntiles, int nrhs)
+{
+  // enough work items to fill the chip (about 8 per CU) with the fewest
+  // re-reads of the quadrature data
+  const int target = 2048;
+  int groups = ntiles >= target ? 1 : (target + ntiles - 1) / ntiles;
+  if (groups > nrhs)
+    groups = nrhs;
+  return (nrhs + groups - 1) / groups;
+}
+
 template hipError_t apply_op<double>(const Geom &, const PlaneSegs &, const Tables<double> &, const double *,
                                      const double *, double *, double *, const int *, hipStream_t,
-                                     const TileCandidate *, int, int);
+                                     const TileCandidate *, int, int, int, ApplyDesc *);
 template hipError_t apply_op<float>(const Geom &, const PlaneSegs &, const Tables<float> &, const float *,
                                     const float *, float *, float *, const int *, hipStream_t, const TileCandidate *,
-                                    int, int);
+                                    int, int, int, ApplyDesc *);
 
 template <typename T>
 hipError_t

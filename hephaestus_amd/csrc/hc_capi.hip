@@ -1,18 +1,21 @@
-// C ABI of libhcurl.so (declared in include/hc_hcurl.h).  See that header for
-// the reference interface each entry point replaces.
-#include "../../include/hc_hcurl.h"
-#include "hc_kernels.hpp"
+// C ABI of libhcurl.so (declared in include/hc_hcurl.h): operator handles.
+// See that header for the reference interface each entry point replaces.
+#include "hc_op.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
-namespace
+namespace hc
 {
 
+namespace
+{
 thread_local std::string g_last_error;
+}
 
 int
 fail(int code, const std::string & msg)
@@ -21,13 +24,18 @@ fail(int code, const std::string & msg)
   return code;
 }
 
-#define HC_TRY_HIP(expr)                                                                                        \
-  do                                                                                                             \
-  {                                                                                                              \
-    hipError_t _e = (expr);                                                                                      \
-    if (_e != hipSuccess)                                                                                        \
-      return fail(HC_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));                                   \
-  } while (0)
+const char *
+last_error()
+{
+  return g_last_error.c_str();
+}
+
+} // namespace hc
+
+using hc::fail;
+
+namespace
+{
 
 // ---------------------------------------------------------------------------
 // 1D bases: Gauss-Legendre points/weights and Gauss-Lobatto points on [0,1],
@@ -162,86 +170,13 @@ cast_tables(const hc::Tables<double> & a, hc::Tables<A> & b)
   }
 }
 
-__global__ void
-k_zero_f(float * v, int64_t n)
-{
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    v[i] = 0.0f;
-}
-
 } // namespace
-
-// Apply kernel variant of one precision: geometry with the tile decomposition,
-// the tile-face thread segments and the compiled candidate.
-struct ApplyPlan
-{
-  hc::Geom g{};
-  hc::PlaneSegs segs{};
-  const hc::TileCandidate * cand = nullptr;
-};
-
-struct hc_op_s
-{
-  hc::Geom g{}; // base geometry (tile fields follow plan64)
-  ApplyPlan plan64, plan32;
-  hc::Tables<double> tb64{};
-  hc::Tables<float> tb32{};
-  int p = 0;
-  int precision = 0;
-  int64_t E = 0, N = 0, n_ess = 0;
-  bool box = true;
-  double h[3] = {0, 0, 0};
-  std::vector<double> nu, sig;
-  bool coef_dirty = true;
-  bool setup_done = false;
-  double last_dt = 0.0;
-  int64_t bytes = 0;
-  // device
-  double * d_vert = nullptr;
-  double * d_nu = nullptr;
-  double * d_sig = nullptr;
-  double * qd64 = nullptr;
-  float * qd32 = nullptr;
-  double * diag = nullptr;
-  double * dinv64 = nullptr;
-  float * dinv32 = nullptr;
-  double * part64 = nullptr;
-  float * part32 = nullptr;
-  // solver work (allocated on first solve)
-  double *r64 = nullptr, *p64 = nullptr, *q64 = nullptr;
-  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr, *b32 = nullptr;
-  double * red = nullptr;
-  hc::PcgState * st_dev = nullptr; // [0] inner/plain, [1] outer
-  hc::PcgState * st_host = nullptr;
-  std::vector<void *> allocs;
-
-  template <typename T>
-  int alloc(T *& ptr, int64_t count)
-  {
-    void * v = nullptr;
-    const size_t sz = sizeof(T) * (size_t)(count > 0 ? count : 1);
-    if (hipMalloc(&v, sz) != hipSuccess)
-      return fail(HC_ENOMEM, "hipMalloc of " + std::to_string(sz) + " bytes failed");
-    ptr = (T *)v;
-    allocs.push_back(v);
-    bytes += (int64_t)sz;
-    return HC_OK;
-  }
-
-  ~hc_op_s()
-  {
-    for (void * v : allocs)
-      (void)hipFree(v);
-    if (st_host)
-      (void)hipHostFree(st_host);
-  }
-};
 
 namespace
 {
 
 void
-set_plan(hc_op_t op, ApplyPlan & pl, const hc::TileCandidate * c)
+set_plan(hc_op_t op, hc::ApplyPlan & pl, const hc::TileCandidate * c)
 {
   pl.g = op->g;
   pl.cand = c;
@@ -254,103 +189,29 @@ set_plan(hc_op_t op, ApplyPlan & pl, const hc::TileCandidate * c)
 }
 
 template <typename T>
-ApplyPlan &
+hc::ApplyPlan &
 plan_of(hc_op_t op)
 {
   return sizeof(T) == 8 ? op->plan64 : op->plan32;
 }
 
-template <typename T>
-hipError_t
-apply_t(hc_op_t op, const hc::Tables<T> & tb, const T * qd, const T * x, T * y, T * part, const int * done,
-        hipStream_t st)
-{
-  ApplyPlan & pl = plan_of<T>(op);
-  return hc::apply_op<T>(pl.g, pl.segs, tb, qd, x, y, part, done, st, pl.cand);
-}
-
-hipError_t
-apply64(hc_op_t op, const double * x, double * y, const int * done, hipStream_t st)
-{
-  return apply_t<double>(op, op->tb64, op->qd64, x, y, op->part64, done, st);
-}
-
-int
-ensure_solver_work(hc_op_t op, bool need32)
-{
-  if (!op->r64)
-  {
-    int rc;
-    if ((rc = op->alloc(op->r64, op->N)) || (rc = op->alloc(op->p64, op->N)) || (rc = op->alloc(op->q64, op->N)) ||
-        (rc = op->alloc(op->red, (int64_t)hc::RED_BLOCKS * 4)) || (rc = op->alloc(op->st_dev, 2)))
-      return rc;
-    if (hipHostMalloc((void **)&op->st_host, 2 * sizeof(hc::PcgState), hipHostMallocDefault) != hipSuccess)
-      return fail(HC_ENOMEM, "hipHostMalloc failed");
-    HC_TRY_HIP(hipMemset(op->st_dev, 0, 2 * sizeof(hc::PcgState)));
-  }
-  if (need32 && !op->r32)
-  {
-    int rc;
-    if ((rc = op->alloc(op->r32, op->N)) || (rc = op->alloc(op->p32, op->N)) || (rc = op->alloc(op->q32, op->N)) ||
-        (rc = op->alloc(op->x32, op->N)) || (rc = op->alloc(op->b32, op->N)))
-      return rc;
-  }
-  return HC_OK;
-}
-
-// Read the device state of solve `k` (0 inner/plain, 1 outer); synchronises.
-int
-read_state(hc_op_t op, int k, hipStream_t st, hc::PcgState & out)
-{
-  HC_TRY_HIP(hipMemcpyAsync(op->st_host + k, op->st_dev + k, sizeof(hc::PcgState), hipMemcpyDeviceToHost, st));
-  HC_TRY_HIP(hipStreamSynchronize(st));
-  out = op->st_host[k];
-  return HC_OK;
-}
-
-// Jacobi-PCG on A_T (precision T) with RHS b (T), solution x (T, initial guess
-// or zero when zero_guess).  Returns iterations; *state gets the final state.
+// y = A x for nrhs right-hand sides (rhs r at r*N) in the precision of T.
 template <typename T>
 int
-run_pcg(hc_op_t op,
-        const T * qd,
-        const hc::Tables<T> & tb,
-        T * part,
-        const T * dinv,
-        const T * b,
-        T * x,
-        T * r,
-        T * p,
-        T * q,
-        bool zero_guess,
-        double rtol,
-        int maxit,
-        hipStream_t st,
-        hc::PcgState & fin)
+apply_t(hc_op_t op, const T * x, T * y, int nrhs, const int * done, hipStream_t st)
 {
-  hc::PcgWork w{op->red, op->st_dev};
-  if (!zero_guess)
-    HC_TRY_HIP(apply_t<T>(op, tb, qd, x, q, part, nullptr, st));
-  HC_TRY_HIP(hc::pcg_init<T>(b, zero_guess ? nullptr : q, dinv, r, p, op->N, rtol, w, st));
-  int it = 0, chunk = 8;
-  int rc = read_state(op, 0, st, fin);
+  const bool f32 = sizeof(T) == 4;
+  int rc = op->ensure_ws(nrhs, f32);
   if (rc)
     return rc;
-  const int * done = &op->st_dev[0].done;
-  while (!fin.done && it < maxit)
-  {
-    const int n = std::min(chunk, maxit - it);
-    for (int k = 0; k < n; k++, it++)
-    {
-      HC_TRY_HIP(apply_t<T>(op, tb, qd, p, q, part, done, st));
-      HC_TRY_HIP(hc::pcg_dot_pq<T>(p, q, op->N, w, st));
-      HC_TRY_HIP(hc::pcg_update_xr<T>(it, x, p, r, q, dinv, op->N, w, st));
-      HC_TRY_HIP(hc::pcg_update_p<T>(it, p, r, dinv, op->N, w, st));
-    }
-    if ((rc = read_state(op, 0, st, fin)))
-      return rc;
-    chunk = std::min(chunk * 2, 64);
-  }
+  hc::ApplyPlan & pl = plan_of<T>(op);
+  const hc::Tables<T> & tb = *(const hc::Tables<T> *)(f32 ? (const void *)&op->tb32 : (const void *)&op->tb64);
+  const T * qd = (const T *)(f32 ? (const void *)op->qd32 : (const void *)op->qd64);
+  T * part = (T *)(f32 ? (void *)op->ws.part32 : (void *)op->ws.part64);
+  const int ntiles = pl.g.nt[0] * pl.g.nt[1] * pl.g.nt[2];
+  const int rb = nrhs == 1 ? 1 : (op->rhs_block > 0 ? std::min(op->rhs_block, nrhs) : hc::default_rhs_block(ntiles, nrhs));
+  HC_TRY_HIP(hc::apply_op<T>(pl.g, pl.segs, tb, qd, x, y, part, done, st, pl.cand, nrhs, hc::STATE_STRIDE_INT, rb,
+                             f32 ? op->desc_dev + 1 : op->desc_dev));
   return HC_OK;
 }
 
@@ -361,7 +222,7 @@ extern "C" {
 const char *
 hc_last_error(void)
 {
-  return g_last_error.c_str();
+  return hc::last_error();
 }
 
 int
@@ -437,16 +298,14 @@ hc_op_create(const hc_mesh_desc * mesh, int order, const double * nu_e, const do
   const int64_t nq = op->E * 12 * Q * Q * Q;
   int rc;
   if ((rc = op->alloc(op->d_nu, op->E)) || (rc = op->alloc(op->d_sig, op->E)) || (rc = op->alloc(op->qd64, nq)) ||
-      (rc = op->alloc(op->diag, op->N)) || (rc = op->alloc(op->dinv64, op->N)) ||
-      (rc = op->alloc(op->part64, 4 * op->N)))
+      (rc = op->alloc(op->diag, op->N)) || (rc = op->alloc(op->dinv64, op->N)) || (rc = op->alloc(op->desc_dev, 2)))
   {
     delete op;
     return rc;
   }
   if (op->precision & HC_PREC_FP32)
   {
-    if ((rc = op->alloc(op->qd32, nq)) || (rc = op->alloc(op->dinv32, op->N)) ||
-        (rc = op->alloc(op->part32, 4 * op->N)))
+    if ((rc = op->alloc(op->qd32, nq)) || (rc = op->alloc(op->dinv32, op->N)))
     {
       delete op;
       return rc;
@@ -539,10 +398,9 @@ hc_op_apply(hc_op_t op, const double * x, double * y, int nrhs, void * stream)
     return fail(HC_EINVAL, "hc_op_apply: null argument");
   if (!op->setup_done)
     return fail(HC_EINVAL, "hc_op_apply: hc_op_setup has not been called");
-  if (nrhs != 1)
-    return fail(HC_EINVAL, "hc_op_apply: nrhs > 1 is not supported by this entry point yet");
-  HC_TRY_HIP(apply64(op, x, y, nullptr, (hipStream_t)stream));
-  return HC_OK;
+  if (nrhs < 1)
+    return fail(HC_EINVAL, "hc_op_apply: nrhs must be >= 1");
+  return apply_t<double>(op, x, y, nrhs, nullptr, (hipStream_t)stream);
 }
 
 int
@@ -554,9 +412,17 @@ hc_op_apply_f32(hc_op_t op, const float * x, float * y, int nrhs, void * stream)
     return fail(HC_EINVAL, "hc_op_apply_f32: hc_op_setup has not been called");
   if (!op->qd32)
     return fail(HC_EINVAL, "hc_op_apply_f32: operator created without HC_PREC_FP32");
-  if (nrhs != 1)
-    return fail(HC_EINVAL, "hc_op_apply_f32: nrhs > 1 is not supported by this entry point yet");
-  HC_TRY_HIP(apply_t<float>(op, op->tb32, op->qd32, x, y, op->part32, nullptr, (hipStream_t)stream));
+  if (nrhs < 1)
+    return fail(HC_EINVAL, "hc_op_apply_f32: nrhs must be >= 1");
+  return apply_t<float>(op, x, y, nrhs, nullptr, (hipStream_t)stream);
+}
+
+int
+hc_op_set_rhs_block(hc_op_t op, int rhs_block)
+{
+  if (!op || rhs_block < 0)
+    return fail(HC_EINVAL, "hc_op_set_rhs_block: bad argument");
+  op->rhs_block = rhs_block;
   return HC_OK;
 }
 
@@ -571,126 +437,7 @@ hc_op_diagonal(hc_op_t op, double * d, void * stream)
   return HC_OK;
 }
 
-int
-hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, int maxit, int mode,
-             hc_solve_stats * stats, void * stream)
-{
-  if (!op || !b || !x)
-    return fail(HC_EINVAL, "hc_pcg_solve: null argument");
-  if (!op->setup_done)
-    return fail(HC_EINVAL, "hc_pcg_solve: hc_op_setup has not been called");
-  if (nrhs != 1)
-    return fail(HC_EINVAL, "hc_pcg_solve: nrhs > 1 goes through the batch interface");
-  if (!(rtol > 0) || maxit < 1)
-    return fail(HC_EINVAL, "hc_pcg_solve: rtol must be > 0 and maxit >= 1");
-  if (mode == HC_SOLVE_FP32_IR && !op->qd32)
-    return fail(HC_EINVAL, "hc_pcg_solve: FP32 refinement needs an operator created with HC_PREC_FP32");
-  hipStream_t st = (hipStream_t)stream;
-  int rc = ensure_solver_work(op, mode == HC_SOLVE_FP32_IR);
-  if (rc)
-    return rc;
-  hipEvent_t e0, e1;
-  HC_TRY_HIP(hipEventCreate(&e0));
-  HC_TRY_HIP(hipEventCreate(&e1));
-  HC_TRY_HIP(hipEventRecord(e0, st));
-  hc_solve_stats s{};
-  hc::PcgWork w{op->red, op->st_dev};
-  hc::PcgWork wo{op->red, op->st_dev + 1};
-  hc::PcgState fin{};
-  if (mode == HC_SOLVE_FP64)
-  {
-    // PCG; when the recursive residual meets rtol but the true FP64 residual
-    // does not (round-off drift near attainable accuracy), restart from the
-    // current x with the true residual (at most 4 restarts).
-    int total = 0;
-    for (int restart = 0; restart < 5 && total < maxit; restart++)
-    {
-      rc = run_pcg<double>(op, op->qd64, op->tb64, op->part64, op->dinv64, b, x, op->r64, op->p64, op->q64, false,
-                           rtol, maxit - total, st, fin);
-      if (rc)
-        return rc;
-      total += fin.iters;
-      s.status = fin.status;
-      if (fin.status != 0 || !fin.done)
-        break;
-      HC_TRY_HIP(apply64(op, x, op->q64, nullptr, st));
-      HC_TRY_HIP(hc::residual64(b, op->q64, op->r64, op->N, wo, true, st));
-      hc::PcgState chk{};
-      if ((rc = read_state(op, 1, st, chk)))
-        return rc;
-      if (chk.rr <= rtol * rtol * chk.bb)
-        break;
-      s.outer_iterations++;
-    }
-    s.iterations = total;
-  }
-  else if (mode == HC_SOLVE_FP32_IR)
-  {
-    // outer residual in FP64
-    HC_TRY_HIP(apply64(op, x, op->q64, nullptr, st));
-    HC_TRY_HIP(hc::residual64(b, op->q64, op->r64, op->N, wo, true, st));
-    hc::PcgState outer{};
-    if ((rc = read_state(op, 1, st, outer)))
-      return rc;
-    const double bnorm = std::sqrt(outer.bb);
-    const int max_outer = 30;
-    // inner tolerance: FP32 CG reliably reduces its residual by ~1e-4 to 1e-5
-    const double inner_rtol = 3e-5;
-    int total = 0, k = 0;
-    while (std::sqrt(outer.rr) > rtol * bnorm && k < max_outer && total < maxit && outer.status == 0)
-    {
-      HC_TRY_HIP(hc::to_f32_scaled(op->r64, op->b32, op->N, op->st_dev + 1, st));
-      hipLaunchKernelGGL(k_zero_f, dim3(1024), dim3(256), 0, st, op->x32, op->N);
-      const double want = std::max(inner_rtol, 0.5 * rtol * bnorm / std::sqrt(outer.rr));
-      rc = run_pcg<float>(op, op->qd32, op->tb32, op->part32, op->dinv32, op->b32, op->x32, op->r32, op->p32,
-                          op->q32, true, want, maxit - total, st, fin);
-      if (rc)
-        return rc;
-      total += fin.iters;
-      HC_TRY_HIP(hc::ir_update(x, op->x32, op->N, op->st_dev + 1, st));
-      HC_TRY_HIP(apply64(op, x, op->q64, nullptr, st));
-      HC_TRY_HIP(hc::residual64(b, op->q64, op->r64, op->N, wo, false, st));
-      if ((rc = read_state(op, 1, st, outer)))
-        return rc;
-      k++;
-      if (fin.status != 0)
-        break;
-    }
-    s.iterations = total;
-    s.outer_iterations = k;
-    s.status = outer.status ? outer.status : fin.status;
-  }
-  else
-    return fail(HC_EINVAL, "hc_pcg_solve: unknown mode");
-  // final true residual in FP64
-  HC_TRY_HIP(apply64(op, x, op->q64, nullptr, st));
-  HC_TRY_HIP(hc::residual64(b, op->q64, op->r64, op->N, wo, true, st));
-  HC_TRY_HIP(hipEventRecord(e1, st));
-  hc::PcgState outer{};
-  if ((rc = read_state(op, 1, st, outer)))
-    return rc;
-  float ms = 0;
-  HC_TRY_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  s.rel_res = outer.bb > 0 ? std::sqrt(outer.rr / outer.bb) : 0.0;
-  s.converged = (s.status == 0 && s.rel_res <= rtol * (1.0 + 1e-6)) ? 1 : 0;
-  s.solve_ms = ms;
-  if (s.status == 0 && !s.converged)
-    s.status = HC_ENOTCONV;
-  if (stats)
-    *stats = s;
-  if (s.status == HC_ENAN)
-    return fail(HC_ENAN, "hc_pcg_solve: NaN/Inf met in a reduction");
-  if (!s.converged)
-  {
-    char buf[160];
-    std::snprintf(buf, sizeof(buf), "hc_pcg_solve: not converged after %d iterations, true relative residual %.3e",
-                  s.iterations, s.rel_res);
-    return fail(HC_ENOTCONV, buf);
-  }
-  return HC_OK;
-}
+} // extern "C"
 
 namespace
 {
@@ -706,6 +453,8 @@ find_candidate(int p, int tx, int ty, int tz, int prefetch)
   return nullptr;
 }
 } // namespace
+
+extern "C" {
 
 int
 hc_tile_list(int order, int * out, int max)
@@ -740,7 +489,7 @@ hc_op_get_tile(hc_op_t op, int prec_bits, int * tx, int * ty, int * tz, int * pr
 {
   if (!op || (prec_bits != 64 && prec_bits != 32) || !tx || !ty || !tz || !prefetch)
     return fail(HC_EINVAL, "hc_op_get_tile: bad argument");
-  const ApplyPlan & pl = prec_bits == 64 ? op->plan64 : op->plan32;
+  const hc::ApplyPlan & pl = prec_bits == 64 ? op->plan64 : op->plan32;
   *tx = pl.cand->shape.tx;
   *ty = pl.cand->shape.ty;
   *tz = pl.cand->shape.tz;
@@ -758,7 +507,7 @@ hc_op_autotune(hc_op_t op, int reps, double * best_us, void * stream)
   if (reps < 1)
     reps = 5;
   hipStream_t st = (hipStream_t)stream;
-  int rc = ensure_solver_work(op, op->qd32 != nullptr);
+  int rc = op->ensure_ws(1, op->qd32 != nullptr);
   if (rc)
     return rc;
   const hc::TileCandidate * c;
@@ -766,28 +515,31 @@ hc_op_autotune(hc_op_t op, int reps, double * best_us, void * stream)
   hipEvent_t e0, e1;
   HC_TRY_HIP(hipEventCreate(&e0));
   HC_TRY_HIP(hipEventCreate(&e1));
-  HC_TRY_HIP(hc::fill_uniform(&op->g, 17, op->r64, op->N, st));
+  // input: a PEC-consistent random vector (r64 / r32 of the workspace)
+  HC_TRY_HIP(hc::fill_uniform(&op->g, 17, op->ws.r64, op->N, st));
   for (int prec = 64; prec >= 32; prec -= 32)
   {
     if (prec == 32 && !op->qd32)
       break;
     if (prec == 32)
-      HC_TRY_HIP(hc::cast_f32(op->r64, op->r32, op->N, st));
-    ApplyPlan & pl = prec == 64 ? op->plan64 : op->plan32;
+      HC_TRY_HIP(hc::cast_f32(op->ws.r64, op->ws.r32, op->N, st));
+    hc::ApplyPlan & pl = prec == 64 ? op->plan64 : op->plan32;
     float best = 1e30f;
     const hc::TileCandidate * bestc = pl.cand;
     for (int i = 0; i < n; i++)
     {
       set_plan(op, pl, &c[i]);
-      auto run = [&]() -> hipError_t {
-        return prec == 64 ? apply64(op, op->r64, op->q64, nullptr, st)
-                          : apply_t<float>(op, op->tb32, op->qd32, op->r32, op->q32, op->part32, nullptr, st);
+      auto run = [&]() -> int {
+        return prec == 64 ? apply_t<double>(op, op->ws.r64, op->ws.q64, 1, nullptr, st)
+                          : apply_t<float>(op, op->ws.r32, op->ws.q32, 1, nullptr, st);
       };
       for (int w = 0; w < 2; w++)
-        HC_TRY_HIP(run());
+        if ((rc = run()))
+          return rc;
       HC_TRY_HIP(hipEventRecord(e0, st));
       for (int r = 0; r < reps; r++)
-        HC_TRY_HIP(run());
+        if ((rc = run()))
+          return rc;
       HC_TRY_HIP(hipEventRecord(e1, st));
       HC_TRY_HIP(hipEventSynchronize(e1));
       float ms = 0;

@@ -44,17 +44,6 @@ struct Geom
   int64_t cd[3][3];// component grid dims [c][axis]
 };
 
-// Geometry of one instance inside a multi-instance batch.
-struct InstDesc
-{
-  Geom g;
-  int64_t xoff;     // offset of this instance's vectors
-  int64_t qoff;     // offset of its quadrature data
-  int64_t poff;     // offset of its boundary partials (4 N entries)
-  int tile0;        // first global tile id of the instance
-  int ntiles;
-};
-
 __host__ __device__ inline bool
 is_ess(const Geom & g, int c, int64_t i0, int64_t i1, int64_t i2)
 {

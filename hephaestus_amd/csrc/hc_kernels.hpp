@@ -22,9 +22,11 @@ struct PlaneSegs
   int64_t start[7];
 };
 
-// One operator instance of an apply launch.  Work items are (tile, rhs) with
-// the rhs index fastest; the items of instance i occupy [w0, w0 + ntiles*nrhs)
-// of the launch's flattened work range.  Vectors of rhs r start at r*N.
+// One operator instance of an apply launch.  Work items are (tile, rhs group)
+// with the group index fastest; a group is rb consecutive right-hand sides (the
+// single-instance kernel uses rb = 1).  The items of instance i occupy
+// [w0, w0 + ntiles*ceil(nrhs/rb)) of the launch's flattened work range.
+// Vectors of rhs r start at r*N.
 struct ApplyDesc
 {
   Geom g;
@@ -36,6 +38,7 @@ struct ApplyDesc
   const int * done;// per-rhs flags at done[r * done_stride], or null
   int done_stride;
   int nrhs;
+  int rb;          // right-hand sides per work item of the batched kernels
   int w0;          // first tile work item
   int ntiles;
   int pw0;         // first plane-assembly work item
@@ -87,8 +90,11 @@ hipError_t pa_setup(const Geom & g,
 hipError_t cast_f32(const double * a, float * b, int64_t n, hipStream_t st);
 hipError_t diagonal(const Geom & g, const Tables<double> & tb, const double * qd, double * d, hipStream_t st);
 
-// y = A x (PEC eliminated).  part: 4*N scratch for tile-face partials.  done:
-// optional device flag; when *done != 0 the kernels return at once.
+// y = A x (PEC eliminated) for nrhs right-hand sides (vector r at r*N).
+// part: 4*N*nrhs scratch for tile-face partials.  done: optional device flags
+// done[r*done_stride]; the kernels skip rhs r when its flag is set.  rb > 1:
+// the batched kernel with rb right-hand sides per work item; its descriptor
+// is staged in desc_dev (device, one ApplyDesc).
 template <typename T>
 hipError_t apply_op(const Geom & g,
                     const PlaneSegs & segs,
@@ -101,7 +107,11 @@ hipError_t apply_op(const Geom & g,
                     hipStream_t st,
                     const TileCandidate * cand = nullptr,
                     int nrhs = 1,
-                    int done_stride = 0);
+                    int done_stride = 0,
+                    int rb = 1,
+                    ApplyDesc * desc_dev = nullptr);
+// Right-hand sides per work item for a batch of nrhs on ntiles tiles.
+int default_rhs_block(int ntiles, int nrhs);
 
 // Fill the plane-assembly fields of a descriptor (segments, rows per rhs).
 void plane_rows(ApplyDesc & d);
@@ -117,48 +127,83 @@ template <typename T>
 hipError_t assemble_planes(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int npwork, hipStream_t st);
 
 // ---------------------------------------------------------------------------
-// Jacobi-PCG (hc_pcg.hip)
+// Jacobi-PCG over solve slots (hc_pcg.hip)
 // ---------------------------------------------------------------------------
-constexpr int RED_BLOCKS = 512; // fixed grid of the reduction kernels
+constexpr int RED_BLOCKS = 512; // most blocks per slot of the reduction kernels
 constexpr int RED_THREADS = 256;
 
-// Device-resident solver scalars of one right-hand side.
+// Blocks of a slot of length n (a function of n only: the summation order of a
+// slot does not depend on the rest of the batch).
+__host__ __device__ inline int
+red_blocks(int64_t n)
+{
+  const int64_t b = (n + 4 * RED_THREADS - 1) / (4 * RED_THREADS);
+  return b < 1 ? 1 : (b > RED_BLOCKS ? RED_BLOCKS : (int)b);
+}
+
+// Device-resident solver scalars of one slot (inner PCG or outer refinement).
 struct PcgState
 {
   double gam[2]; // (r, z) of the previous / current iteration (by parity)
   double bb;     // (b, b)
   double rr;     // (r, r) of the last iterate
-  double tol2;   // rtol^2
-  int done;      // 1 once converged (or NaN met)
-  int iters;     // iterations performed
+  double tol2;   // tolerance^2 relative to bb
+  int done;      // 1 once converged, out of iterations, or NaN met
+  int iters;     // inner: iterations of this PCG; outer: refinement steps
   int status;    // 0, or HC_ENAN
-  int pad;
+  int it0;       // inner: value of the iteration counter at the start
+  int maxit;     // inner: iteration cap
+  int inner;     // outer: inner iterations summed over the refinement steps
 };
+constexpr int STATE_STRIDE_INT = sizeof(PcgState) / sizeof(int);
 
-struct PcgWork
+// One right-hand side of one operator instance.  The inner vectors (ib..iq,
+// idinv) have the precision of the inner solve (FP64 solve: b, x, r64, p64,
+// q64 and the FP64 diagonal inverse; refinement: FP32 work vectors).
+struct SolveSlot
 {
-  double * part;    // RED_BLOCKS * 4 partial sums
-  PcgState * state; // device
+  int64_t n;
+  double rtol;
+  int maxit;
+  int pad;
+  const double * b; // FP64 right-hand side
+  double * x;       // FP64 solution
+  double * r64;     // FP64 outer residual
+  double * q64;     // FP64 A x
+  const void * ib;
+  void * ix;
+  void * ir;
+  void * ip;
+  void * iq;
+  const void * idinv;
+  PcgState * in;
+  PcgState * out;
+  double * part; // RED_BLOCKS * 4 partial sums
 };
 
-template <typename T>
-hipError_t pcg_init(const T * b, const T * q, const T * dinv, T * r, T * p, int64_t n, double rtol, PcgWork w,
-                    hipStream_t st);
-template <typename T>
-hipError_t pcg_dot_pq(const T * p, const T * q, int64_t n, PcgWork w, hipStream_t st);
-template <typename T>
-hipError_t pcg_update_xr(int it, T * x, const T * p, T * r, const T * q, const T * dinv, int64_t n, PcgWork w,
-                         hipStream_t st);
-template <typename T>
-hipError_t pcg_update_p(int it, T * p, const T * r, const T * dinv, int64_t n, PcgWork w, hipStream_t st);
+struct SlotTable
+{
+  const SolveSlot * slots; // device
+  int nslots;
+  int nb_max; // max red_blocks(n) over the slots
+};
 
-// r = b - Ax (FP64), state->rr = (r,r), state->bb = (b,b) when set_bb.
-hipError_t residual64(const double * b, const double * Ax, double * r, int64_t n, PcgWork w, bool set_bb,
-                      hipStream_t st);
-// r32 = (float)(r64 / sqrt(state->rr)).
-hipError_t to_f32_scaled(const double * r64, float * r32, int64_t n, const PcgState * s, hipStream_t st);
-// x += sqrt(state->rr) * (double)d32   (the scale of the matching to_f32_scaled)
-hipError_t ir_update(double * x, const float * d32, int64_t n, const PcgState * s, hipStream_t st);
+// r = b - A x (or b with x = 0 when zero_guess), p = D^-1 r, and the inner
+// states (tolerance: slot rtol, or the refinement rule when inner_floor > 0).
+// gate: slots whose outer state is done are skipped and marked done.
+template <typename T>
+hipError_t pcg_init(const SlotTable & t, const int * git, bool zero_guess, bool gate, double inner_floor,
+                    hipStream_t st);
+// One PCG iteration after q = A p; advances the counter *git.
+template <typename T>
+hipError_t pcg_iterate(const SlotTable & t, int * git, hipStream_t st);
+// r64 = b - q64 and the outer states (see k_residual_fin).
+hipError_t residual64(const SlotTable & t, bool set_bb, bool gate, bool accumulate, int max_outer, hipStream_t st);
+// ib = (float)(r64 / ||r64||) for slots whose outer state is not done.
+hipError_t to_f32_scaled(const SlotTable & t, hipStream_t st);
+// x += ||r64|| * (double)ix for slots whose outer state is not done.
+hipError_t ir_update(const SlotTable & t, hipStream_t st);
+hipError_t zero_int(int * v, int n, hipStream_t st);
 hipError_t invert_diag(const double * d, double * dinv64, float * dinv32, int64_t n, hipStream_t st);
 hipError_t fill_uniform(const Geom * g, uint64_t seed, double * v, int64_t n, hipStream_t st);
 

@@ -1,0 +1,259 @@
+// Library-internal state of an operator handle (hc_op_t) and of the batched
+// solve engine shared by the C ABI entry points.
+#pragma once
+
+#include "../../include/hc_hcurl.h"
+#include "hc_kernels.hpp"
+
+#include <string>
+#include <vector>
+
+namespace hc
+{
+
+int fail(int code, const std::string & msg);
+
+#define HC_TRY_HIP(expr)                                                                                        \
+  do                                                                                                             \
+  {                                                                                                              \
+    hipError_t _e = (expr);                                                                                      \
+    if (_e != hipSuccess)                                                                                        \
+      return ::hc::fail(HC_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));                             \
+  } while (0)
+
+// Device memory block owned by one object (one hipMalloc, carved into
+// 256-byte aligned pieces).
+struct DevBlock
+{
+  void * base = nullptr;
+  size_t bytes = 0;
+  DevBlock() = default;
+  DevBlock(const DevBlock &) = delete;
+  DevBlock & operator=(const DevBlock &) = delete;
+  ~DevBlock() { release(); }
+  void release()
+  {
+    if (base)
+      (void)hipFree(base);
+    base = nullptr;
+    bytes = 0;
+  }
+  int alloc(size_t sz)
+  {
+    release();
+    if (sz == 0)
+      sz = 256;
+    if (hipMalloc(&base, sz) != hipSuccess)
+    {
+      base = nullptr;
+      return fail(HC_ENOMEM, "hipMalloc of " + std::to_string(sz) + " bytes failed");
+    }
+    bytes = sz;
+    return HC_OK;
+  }
+};
+
+inline size_t
+align256(size_t v)
+{
+  return (v + 255) & ~(size_t)255;
+}
+
+// Work vectors of the solves of nrhs right-hand sides of one operator
+// (N DOFs each, rhs r at offset r*N), plus the tile-face partials of the
+// apply (4 N per rhs and precision).
+struct Workspace
+{
+  int64_t N = 0;
+  int nrhs = 0;
+  bool has32 = false;
+  double *r64 = nullptr, *q64 = nullptr, *p64 = nullptr, *part64 = nullptr;
+  float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *part32 = nullptr;
+
+  static size_t bytes_for(int64_t N, int nrhs, bool need32)
+  {
+    const size_t v64 = align256(sizeof(double) * (size_t)N * nrhs);
+    const size_t v32 = align256(sizeof(float) * (size_t)N * nrhs);
+    size_t b = 3 * v64 + 4 * v64;
+    if (need32)
+      b += 5 * v32 + 4 * v32;
+    return b;
+  }
+  // carve the pieces out of [base, base + bytes_for(...))
+  void carve(void * base, int64_t n, int k, bool need32)
+  {
+    N = n;
+    nrhs = k;
+    has32 = need32;
+    char * p = (char *)base;
+    const size_t v64 = align256(sizeof(double) * (size_t)N * nrhs);
+    const size_t v32 = align256(sizeof(float) * (size_t)N * nrhs);
+    r64 = (double *)p;
+    p += v64;
+    q64 = (double *)p;
+    p += v64;
+    p64 = (double *)p;
+    p += v64;
+    part64 = (double *)p;
+    p += 4 * v64;
+    if (need32)
+    {
+      b32 = (float *)p;
+      p += v32;
+      x32 = (float *)p;
+      p += v32;
+      r32 = (float *)p;
+      p += v32;
+      p32 = (float *)p;
+      p += v32;
+      q32 = (float *)p;
+      p += v32;
+      part32 = (float *)p;
+    }
+    else
+      b32 = x32 = r32 = p32 = q32 = part32 = nullptr;
+  }
+};
+
+// Apply kernel variant of one precision: geometry with the tile decomposition,
+// the tile-face assembly segments and the compiled candidate.
+struct ApplyPlan
+{
+  Geom g{};
+  PlaneSegs segs{};
+  const TileCandidate * cand = nullptr;
+};
+
+class Engine;
+
+} // namespace hc
+
+struct hc_op_s
+{
+  hc::Geom g{}; // base geometry (tile fields follow plan64)
+  hc::ApplyPlan plan64, plan32;
+  hc::Tables<double> tb64{};
+  hc::Tables<float> tb32{};
+  int p = 0;
+  int precision = 0;
+  int64_t E = 0, N = 0, n_ess = 0;
+  bool box = true;
+  double h[3] = {0, 0, 0};
+  std::vector<double> nu, sig;
+  bool coef_dirty = true;
+  bool setup_done = false;
+  double last_dt = 0.0;
+  int64_t bytes = 0;
+  // device
+  double * d_vert = nullptr;
+  double * d_nu = nullptr;
+  double * d_sig = nullptr;
+  double * qd64 = nullptr;
+  float * qd32 = nullptr;
+  double * diag = nullptr;
+  double * dinv64 = nullptr;
+  float * dinv32 = nullptr;
+  hc::ApplyDesc * desc_dev = nullptr; // staging of the batched apply descriptor
+  int rhs_block = 0;                    // right-hand sides per work item (0: automatic)
+  std::vector<void *> allocs;
+  // work vectors of hc_op_apply (nrhs > 1) and hc_pcg_solve, grown on demand
+  hc::DevBlock ws_mem;
+  hc::Workspace ws;
+  hc::Engine * engine = nullptr;
+
+  template <typename T>
+  int alloc(T *& ptr, int64_t count)
+  {
+    void * v = nullptr;
+    const size_t sz = sizeof(T) * (size_t)(count > 0 ? count : 1);
+    if (hipMalloc(&v, sz) != hipSuccess)
+      return hc::fail(HC_ENOMEM, "hipMalloc of " + std::to_string(sz) + " bytes failed");
+    ptr = (T *)v;
+    allocs.push_back(v);
+    bytes += (int64_t)sz;
+    return HC_OK;
+  }
+  // workspace for at least nrhs right-hand sides (and FP32 vectors if need32)
+  int ensure_ws(int nrhs, bool need32);
+  ~hc_op_s();
+};
+
+namespace hc
+{
+
+// Batched Jacobi-PCG / FP32+FP64 refinement over the right-hand sides of one
+// or more operator instances (see hc_solve.hip).
+struct EngineInst
+{
+  hc_op_t op;
+  int nrhs;
+  const double * b; // nrhs * N, rhs r at r*N
+  double * x;
+  double rtol;
+  int maxit;
+  const Workspace * ws;
+  int slot0;
+};
+
+struct SlotStats
+{
+  int inner;     // inner PCG iterations (summed over refinement steps / restarts)
+  int outer;     // refinement steps or residual-replacement restarts
+  int status;
+  int converged;
+  double rel_res; // final FP64 true relative residual
+};
+
+class Engine
+{
+public:
+  Engine() = default;
+  ~Engine();
+  Engine(const Engine &) = delete;
+  Engine & operator=(const Engine &) = delete;
+
+  // Replace the instance list; per order the tile variants cand64/cand32
+  // (nullptr: each instance's own plan) and the right-hand-side block of the
+  // batched apply (0: default).
+  int set(const std::vector<EngineInst> & inst, int mode, int rhs_block = 0, bool force_batched = false);
+  // Run the solve; fills per-slot statistics; *ms = device time.
+  int solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double * ms);
+  int nslots() const { return nslots_; }
+
+private:
+  struct ApplyGroup
+  {
+    int order;
+    bool batched;      // descriptor table (BATCH kernel) or by-value single instance
+    ApplyDesc single;  // by-value descriptor (batched == false)
+    const ApplyDesc * descs; // device
+    int ndesc, nwork, npwork;
+    const TileCandidate * cand;
+    const void * tables; // Tables<double> or Tables<float>
+    int prec;
+  };
+  enum Kind
+  {
+    K_PQ = 0,  // A p -> q in the inner precision, gated by the inner state
+    K_XQ_G,    // A x -> q64 (FP64), gated by the outer state
+    K_XQ_ALL,  // A x -> q64 (FP64), every slot
+    K_COUNT
+  };
+  int apply(Kind k, hipStream_t st);
+  int read_states(PcgState * dev, hipStream_t st);
+  bool all_done(const PcgState * h) const;
+
+  std::vector<EngineInst> inst_;
+  int nslots_ = 0;
+  int mode_ = 0;
+  DevBlock mem_;
+  SlotTable table_{};
+  PcgState * in_ = nullptr;
+  PcgState * out_ = nullptr;
+  int * git_ = nullptr;
+  PcgState * host_ = nullptr; // pinned, nslots states
+  int host_cap_ = 0;
+  std::vector<ApplyGroup> groups_[K_COUNT];
+};
+
+} // namespace hc

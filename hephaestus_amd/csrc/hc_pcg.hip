@@ -1,21 +1,33 @@
 // Jacobi-preconditioned CG kernels: the device side of DefaultJacobiPCGSolver
-// (src/solvers/hephaestus_solvers.hpp:34-57: HyprePCG + HypreDiagScale [ext]).
+// (src/solvers/hephaestus_solvers.hpp:34-57: HyprePCG + HypreDiagScale [ext]),
+// batched over solve slots.
 //
-// One iteration is: q = A p (operator kernels), then
+// A slot is one right-hand side of one operator instance (its own length n,
+// vectors, Jacobi diagonal, state and reduction partials), read from a device
+// table; the grid is (blocks per slot, slots).  One iteration is q = A p (the
+// operator kernels, gated by each slot's done flag), then
 //   pcg_dot_pq     partial (p, q) per block
 //   pcg_update_xr  alpha = gamma / (p,q);  x += alpha p;  r -= alpha q;
 //                  partials (r, D^-1 r), (r, r)          -- one pass over x,p,r,q,d
 //   pcg_update_p   beta = gamma_new / gamma;  p = D^-1 r + beta p
-// Every scalar stays on the device (PcgState); reductions are two-stage with a
-// fixed block grid and a fixed summation order, so results are bitwise
-// reproducible and independent of what else runs on the GPU.  Dot products are
-// accumulated in FP64 for FP32 vectors too.  Convergence sets state->done; all
-// later kernels of the solve return at once, so the host checks the flag only
-// every few iterations.
+// Every scalar stays on the device (PcgState).  Reductions are two-stage with
+// a block count fixed by n and a fixed summation order, so the result of a
+// slot is bitwise independent of the other slots of the launch.  Dot products
+// accumulate in FP64 for FP32 vectors too.  A converged slot sets its done
+// flag and every later kernel skips it, so the host checks the flags only
+// every few iterations and slots may join a running batch: the iteration
+// index of a slot is (device counter) - (its start), the counter being
+// advanced by pcg_dot_pq.
 #include "hc_kernels.hpp"
 
 namespace hc
 {
+
+__device__ inline int
+slot_blocks(int64_t n)
+{
+  return red_blocks(n);
+}
 
 // Block-wide sum of K values per thread (fixed tree order).
 template <int K>
@@ -43,17 +55,17 @@ block_sum(double (&v)[K], double * sh)
   __syncthreads();
 }
 
-// Reduce the RED_BLOCKS partials of slot k (stride 4) in a fixed order; every
-// block computes the same value.
+// Reduce the nb block partials of slots k0.. (stride 4) in a fixed order;
+// every block computes the same value.
 template <int K>
 __device__ inline void
-reduce_partials(const double * __restrict__ part, int k0, double (&out)[K], double * sh)
+reduce_partials(const double * __restrict__ part, int nb, int k0, double (&out)[K], double * sh)
 {
   double v[K];
 #pragma unroll
   for (int k = 0; k < K; k++)
     v[k] = 0.0;
-  for (int i = threadIdx.x; i < RED_BLOCKS; i += RED_THREADS)
+  for (int i = threadIdx.x; i < nb; i += RED_THREADS)
   {
 #pragma unroll
     for (int k = 0; k < K; k++)
@@ -71,17 +83,39 @@ bad(double v)
   return !(fabs(v) <= 1.7976931348623157e308);
 }
 
+// grid-stride loop over the entries of one slot handled by this block
+#define HC_SLOT_LOOP(i, n, nb)                                                                                    \
+  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < (n); i += (int64_t)(nb)*RED_THREADS)
+
+// ---------------------------------------------------------------------------
+// PCG
+// ---------------------------------------------------------------------------
+// r = b - q (or b when zero_guess, then x = 0);  p = D^-1 r;
+// partials (b,b), (r, D^-1 r), (r,r).  gate: skip slots whose outer solve is done.
 template <typename T>
 __global__ void __launch_bounds__(RED_THREADS)
-k_pcg_init(const T * __restrict__ b, const T * __restrict__ q, const T * __restrict__ dinv, T * __restrict__ r,
-           T * __restrict__ p, int64_t n, double * __restrict__ part)
+k_pcg_init(const SolveSlot * __restrict__ slots, int zero_guess, int gate)
 {
+  const SolveSlot & sl = slots[blockIdx.y];
+  const int nb = slot_blocks(sl.n);
+  if ((int)blockIdx.x >= nb || (gate && sl.out->done))
+    return;
+  const T * __restrict__ b = (const T *)sl.ib;
+  const T * __restrict__ q = (const T *)sl.iq;
+  const T * __restrict__ dinv = (const T *)sl.idinv;
+  T * __restrict__ x = (T *)sl.ix;
+  T * __restrict__ r = (T *)sl.ir;
+  T * __restrict__ p = (T *)sl.ip;
   __shared__ double sh[3 * RED_THREADS];
   double v[3] = {0, 0, 0};
-  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * RED_THREADS)
+  HC_SLOT_LOOP(i, sl.n, nb)
   {
     const double bi = (double)b[i];
-    const double ri = q ? bi - (double)q[i] : bi;
+    double ri = bi;
+    if (zero_guess)
+      x[i] = T(0);
+    else
+      ri = bi - (double)q[i];
     const double zi = (double)dinv[i] * ri;
     r[i] = (T)ri;
     p[i] = (T)zi;
@@ -92,64 +126,98 @@ k_pcg_init(const T * __restrict__ b, const T * __restrict__ q, const T * __restr
   block_sum<3>(v, sh);
   if (threadIdx.x == 0)
   {
-    part[blockIdx.x * 4 + 0] = v[0];
-    part[blockIdx.x * 4 + 1] = v[1];
-    part[blockIdx.x * 4 + 2] = v[2];
+    sl.part[blockIdx.x * 4 + 0] = v[0];
+    sl.part[blockIdx.x * 4 + 1] = v[1];
+    sl.part[blockIdx.x * 4 + 2] = v[2];
   }
 }
 
+// Inner state of each slot.  Tolerance: rtol of the slot, or, for the inner
+// solve of a refinement (inner_floor > 0), max(inner_floor, 0.5 rtol ||b||/||r||)
+// of the outer state.  A slot whose outer solve is done is marked done at once.
 __global__ void __launch_bounds__(RED_THREADS)
-k_pcg_init_fin(const double * __restrict__ part, PcgState * s, double tol2)
+k_pcg_init_fin(const SolveSlot * __restrict__ slots, const int * __restrict__ git, int gate, double inner_floor)
 {
+  const SolveSlot & sl = slots[blockIdx.x];
+  PcgState * s = sl.in;
+  const PcgState * o = sl.out;
+  if (gate && o->done)
+  {
+    if (threadIdx.x == 0)
+    {
+      s->done = 1;
+      s->iters = 0;
+    }
+    return;
+  }
   __shared__ double sh[3 * RED_THREADS];
-  double o[3];
-  reduce_partials<3>(part, 0, o, sh);
+  double v[3];
+  reduce_partials<3>(sl.part, slot_blocks(sl.n), 0, v, sh);
   if (threadIdx.x == 0)
   {
-    s->bb = o[0];
-    s->gam[0] = o[1];
+    double tol = sl.rtol;
+    if (inner_floor > 0)
+      tol = fmax(inner_floor, o->rr > 0 ? 0.5 * sl.rtol * sqrt(o->bb / o->rr) : 1.0);
+    s->bb = v[0];
+    s->gam[0] = v[1];
     s->gam[1] = 0.0;
-    s->rr = o[2];
-    s->tol2 = tol2;
+    s->rr = v[2];
+    s->tol2 = tol * tol;
     s->iters = 0;
+    s->it0 = *git;
+    // iteration budget left to the slot (sl.maxit over all inner solves)
+    s->maxit = sl.maxit - (gate ? o->inner : 0);
     s->status = 0;
-    const bool nan = bad(o[0]) || bad(o[1]) || bad(o[2]);
+    const bool nan = bad(v[0]) || bad(v[1]) || bad(v[2]);
     if (nan)
       s->status = -5;
-    s->done = (nan || o[2] <= tol2 * o[0]) ? 1 : 0;
+    s->done = (nan || v[2] <= s->tol2 * v[0] || s->maxit <= 0) ? 1 : 0;
   }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(RED_THREADS)
-k_pcg_dot_pq(const T * __restrict__ p, const T * __restrict__ q, int64_t n, double * __restrict__ part,
-             const PcgState * __restrict__ s)
+k_pcg_dot_pq(const SolveSlot * __restrict__ slots, int * git)
 {
-  if (s->done)
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+    *git = *git + 1; // read by the next kernels only
+  const SolveSlot & sl = slots[blockIdx.y];
+  const int nb = slot_blocks(sl.n);
+  if ((int)blockIdx.x >= nb || sl.in->done)
     return;
+  const T * __restrict__ p = (const T *)sl.ip;
+  const T * __restrict__ q = (const T *)sl.iq;
   __shared__ double sh[RED_THREADS];
   double v[1] = {0};
-  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * RED_THREADS)
-    v[0] += (double)p[i] * (double)q[i];
+  HC_SLOT_LOOP(i, sl.n, nb)
+  v[0] += (double)p[i] * (double)q[i];
   block_sum<1>(v, sh);
   if (threadIdx.x == 0)
-    part[blockIdx.x * 4 + 3] = v[0];
+    sl.part[blockIdx.x * 4 + 3] = v[0];
 }
 
 template <typename T>
 __global__ void __launch_bounds__(RED_THREADS)
-k_pcg_update_xr(int it, T * __restrict__ x, const T * __restrict__ p, T * __restrict__ r, const T * __restrict__ q,
-                const T * __restrict__ dinv, int64_t n, double * __restrict__ part, const PcgState * __restrict__ s)
+k_pcg_update_xr(const SolveSlot * __restrict__ slots, const int * __restrict__ git)
 {
-  if (s->done)
+  const SolveSlot & sl = slots[blockIdx.y];
+  const PcgState * s = sl.in;
+  const int nb = slot_blocks(sl.n);
+  if ((int)blockIdx.x >= nb || s->done)
     return;
+  const int it = *git - 1 - s->it0;
+  T * __restrict__ x = (T *)sl.ix;
+  T * __restrict__ r = (T *)sl.ir;
+  const T * __restrict__ p = (const T *)sl.ip;
+  const T * __restrict__ q = (const T *)sl.iq;
+  const T * __restrict__ dinv = (const T *)sl.idinv;
   __shared__ double sh[2 * RED_THREADS];
   double pq[1];
-  reduce_partials<1>(part, 3, pq, sh);
+  reduce_partials<1>(sl.part, nb, 3, pq, sh);
   const double alpha = s->gam[it & 1] / pq[0];
   const T a = (T)alpha;
   double v[2] = {0, 0};
-  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * RED_THREADS)
+  HC_SLOT_LOOP(i, sl.n, nb)
   {
     x[i] = x[i] + a * p[i];
     const T ri = r[i] - a * q[i];
@@ -161,25 +229,29 @@ k_pcg_update_xr(int it, T * __restrict__ x, const T * __restrict__ p, T * __rest
   block_sum<2>(v, sh);
   if (threadIdx.x == 0)
   {
-    part[blockIdx.x * 4 + 0] = v[0];
-    part[blockIdx.x * 4 + 1] = v[1];
+    sl.part[blockIdx.x * 4 + 0] = v[0];
+    sl.part[blockIdx.x * 4 + 1] = v[1];
   }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(RED_THREADS)
-k_pcg_update_p(int it, T * __restrict__ p, const T * __restrict__ r, const T * __restrict__ dinv, int64_t n,
-               const double * __restrict__ part, PcgState * s)
+k_pcg_update_p(const SolveSlot * __restrict__ slots, const int * __restrict__ git)
 {
-  if (s->done)
+  const SolveSlot & sl = slots[blockIdx.y];
+  PcgState * s = sl.in;
+  const int nb = slot_blocks(sl.n);
+  if ((int)blockIdx.x >= nb || s->done)
     return;
+  const int it = *git - 1 - s->it0;
   __shared__ double sh[2 * RED_THREADS];
   double o[2];
-  reduce_partials<2>(part, 0, o, sh);
+  reduce_partials<2>(sl.part, nb, 0, o, sh);
   const double gnew = o[0], rr = o[1];
   const double gold = s->gam[it & 1];
   const bool nan = bad(gnew) || bad(rr);
   const bool conv = nan || rr <= s->tol2 * s->bb;
+  const bool stop = conv || it + 1 >= s->maxit;
   if (blockIdx.x == 0 && threadIdx.x == 0)
   {
     s->gam[(it + 1) & 1] = gnew;
@@ -187,26 +259,36 @@ k_pcg_update_p(int it, T * __restrict__ p, const T * __restrict__ r, const T * _
     s->iters = it + 1;
     if (nan)
       s->status = -5;
-    if (conv)
+    if (stop)
       s->done = 1;
   }
-  if (conv)
+  if (stop)
     return;
+  T * __restrict__ p = (T *)sl.ip;
+  const T * __restrict__ r = (const T *)sl.ir;
+  const T * __restrict__ dinv = (const T *)sl.idinv;
   const T beta = (T)(gnew / gold);
-  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * RED_THREADS)
-    p[i] = dinv[i] * r[i] + beta * p[i];
+  HC_SLOT_LOOP(i, sl.n, nb)
+  p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
-// ----------------------------------------------------------------------------
-// FP64 refinement helpers
-// ----------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// FP64 outer residual and refinement
+// ---------------------------------------------------------------------------
+// r64 = b - q64 (q64 = A x), partials (r,r), (b,b)
 __global__ void __launch_bounds__(RED_THREADS)
-k_residual64(const double * __restrict__ b, const double * __restrict__ Ax, double * __restrict__ r, int64_t n,
-             double * __restrict__ part)
+k_residual64(const SolveSlot * __restrict__ slots, int gate)
 {
+  const SolveSlot & sl = slots[blockIdx.y];
+  const int nb = slot_blocks(sl.n);
+  if ((int)blockIdx.x >= nb || (gate && sl.out->done))
+    return;
+  const double * __restrict__ b = sl.b;
+  const double * __restrict__ Ax = sl.q64;
+  double * __restrict__ r = sl.r64;
   __shared__ double sh[2 * RED_THREADS];
   double v[2] = {0, 0};
-  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * RED_THREADS)
+  HC_SLOT_LOOP(i, sl.n, nb)
   {
     const double ri = b[i] - Ax[i];
     r[i] = ri;
@@ -216,49 +298,81 @@ k_residual64(const double * __restrict__ b, const double * __restrict__ Ax, doub
   block_sum<2>(v, sh);
   if (threadIdx.x == 0)
   {
-    part[blockIdx.x * 4 + 0] = v[0];
-    part[blockIdx.x * 4 + 1] = v[1];
+    sl.part[blockIdx.x * 4 + 0] = v[0];
+    sl.part[blockIdx.x * 4 + 1] = v[1];
   }
 }
 
+// Outer state: rr (and bb, a fresh sequence, when set_bb); done once
+// rr <= rtol^2 bb or after max_outer steps.  accumulate: add the inner
+// iterations of the slot's last inner solve to the outer count and step.
 __global__ void __launch_bounds__(RED_THREADS)
-k_residual_fin(const double * __restrict__ part, PcgState * s, int set_bb)
+k_residual_fin(const SolveSlot * __restrict__ slots, int set_bb, int gate, int accumulate, int max_outer)
 {
+  const SolveSlot & sl = slots[blockIdx.x];
+  PcgState * s = sl.out;
+  if (gate && s->done)
+    return;
   __shared__ double sh[2 * RED_THREADS];
   double o[2];
-  reduce_partials<2>(part, 0, o, sh);
+  reduce_partials<2>(sl.part, slot_blocks(sl.n), 0, o, sh);
   if (threadIdx.x == 0)
   {
     s->rr = o[0];
     if (set_bb)
     {
-      // start of a new outer sequence
       s->bb = o[1];
+      s->tol2 = sl.rtol * sl.rtol;
       s->status = 0;
-      s->done = 0;
       s->iters = 0;
+      s->inner = 0;
+    }
+    if (accumulate)
+    {
+      s->inner += sl.in->iters;
+      s->iters += 1;
+      if (sl.in->status)
+        s->status = sl.in->status;
     }
     if (bad(o[0]))
       s->status = -5;
+    s->done = (s->status != 0 || o[0] <= s->tol2 * s->bb || s->iters >= max_outer || s->inner >= sl.maxit) ? 1 : 0;
   }
 }
 
+// ib (float) = r64 / ||r64||
 __global__ void
-k_to_f32_scaled(const double * __restrict__ r64, float * __restrict__ r32, int64_t n, const PcgState * __restrict__ s)
+k_to_f32_scaled(const SolveSlot * __restrict__ slots)
 {
+  const SolveSlot & sl = slots[blockIdx.y];
+  const PcgState * s = sl.out;
+  if (s->done)
+    return;
   const double inv = s->rr > 0 ? 1.0 / sqrt(s->rr) : 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+  const double * __restrict__ r64 = sl.r64;
+  float * __restrict__ r32 = (float *)sl.ib;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sl.n; i += (int64_t)gridDim.x * blockDim.x)
     r32[i] = (float)(r64[i] * inv);
 }
 
+// x += ||r64|| * ix (float), the scale of the matching k_to_f32_scaled
 __global__ void
-k_ir_update(double * __restrict__ x, const float * __restrict__ d32, int64_t n, const PcgState * __restrict__ s)
+k_ir_update(const SolveSlot * __restrict__ slots)
 {
+  const SolveSlot & sl = slots[blockIdx.y];
+  const PcgState * s = sl.out;
+  if (s->done)
+    return;
   const double sc = sqrt(s->rr);
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+  double * __restrict__ x = sl.x;
+  const float * __restrict__ d32 = (const float *)sl.ix;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sl.n; i += (int64_t)gridDim.x * blockDim.x)
     x[i] += sc * (double)d32[i];
 }
 
+// ---------------------------------------------------------------------------
+// single-vector helpers
+// ---------------------------------------------------------------------------
 __global__ void
 k_invert_diag(const double * __restrict__ d, double * __restrict__ d64, float * __restrict__ d32, int64_t n)
 {
@@ -299,75 +413,78 @@ k_fill_uniform(Geom g, int use_g, uint64_t seed, double * __restrict__ v, int64_
   }
 }
 
+__global__ void
+k_zero_int(int * v, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    v[i] = 0;
+}
+
 // ----------------------------------------------------------------------------
 // launchers
 // ----------------------------------------------------------------------------
-#define HC_LAUNCH_RED(kern, ...)                                                                                  \
-  hipLaunchKernelGGL(kern, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, __VA_ARGS__)
-
-template <typename T>
-hipError_t
-pcg_init(const T * b, const T * q, const T * dinv, T * r, T * p, int64_t n, double rtol, PcgWork w, hipStream_t st)
+static inline dim3
+slot_grid(const SlotTable & t)
 {
-  HC_LAUNCH_RED(k_pcg_init<T>, b, q, dinv, r, p, n, w.part);
-  hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(RED_THREADS), 0, st, w.part, w.state, rtol * rtol);
-  return hipGetLastError();
+  return dim3(t.nb_max, t.nslots);
 }
 
 template <typename T>
 hipError_t
-pcg_dot_pq(const T * p, const T * q, int64_t n, PcgWork w, hipStream_t st)
+pcg_init(const SlotTable & t, const int * git, bool zero_guess, bool gate, double inner_floor, hipStream_t st)
 {
-  HC_LAUNCH_RED(k_pcg_dot_pq<T>, p, q, n, w.part, w.state);
+  hipLaunchKernelGGL(k_pcg_init<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, zero_guess ? 1 : 0,
+                     gate ? 1 : 0);
+  hipLaunchKernelGGL(k_pcg_init_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, git, gate ? 1 : 0,
+                     inner_floor);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t
-pcg_update_xr(int it, T * x, const T * p, T * r, const T * q, const T * dinv, int64_t n, PcgWork w, hipStream_t st)
+pcg_iterate(const SlotTable & t, int * git, hipStream_t st)
 {
-  HC_LAUNCH_RED(k_pcg_update_xr<T>, it, x, p, r, q, dinv, n, w.part, w.state);
+  hipLaunchKernelGGL(k_pcg_dot_pq<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, git);
+  hipLaunchKernelGGL(k_pcg_update_xr<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, (const int *)git);
+  hipLaunchKernelGGL(k_pcg_update_p<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, (const int *)git);
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t
-pcg_update_p(int it, T * p, const T * r, const T * dinv, int64_t n, PcgWork w, hipStream_t st)
-{
-  HC_LAUNCH_RED(k_pcg_update_p<T>, it, p, r, dinv, n, w.part, w.state);
-  return hipGetLastError();
-}
-
-#define HC_INST(T)                                                                                                 \
-  template hipError_t pcg_init<T>(const T *, const T *, const T *, T *, T *, int64_t, double, PcgWork,            \
-                                  hipStream_t);                                                                    \
-  template hipError_t pcg_dot_pq<T>(const T *, const T *, int64_t, PcgWork, hipStream_t);                         \
-  template hipError_t pcg_update_xr<T>(int, T *, const T *, T *, const T *, const T *, int64_t, PcgWork,          \
-                                       hipStream_t);                                                               \
-  template hipError_t pcg_update_p<T>(int, T *, const T *, const T *, int64_t, PcgWork, hipStream_t);
-HC_INST(double)
-HC_INST(float)
-#undef HC_INST
+template hipError_t pcg_init<double>(const SlotTable &, const int *, bool, bool, double, hipStream_t);
+template hipError_t pcg_init<float>(const SlotTable &, const int *, bool, bool, double, hipStream_t);
+template hipError_t pcg_iterate<double>(const SlotTable &, int *, hipStream_t);
+template hipError_t pcg_iterate<float>(const SlotTable &, int *, hipStream_t);
 
 hipError_t
-residual64(const double * b, const double * Ax, double * r, int64_t n, PcgWork w, bool set_bb, hipStream_t st)
+residual64(const SlotTable & t, bool set_bb, bool gate, bool accumulate, int max_outer, hipStream_t st)
 {
-  HC_LAUNCH_RED(k_residual64, b, Ax, r, n, w.part);
-  hipLaunchKernelGGL(k_residual_fin, dim3(1), dim3(RED_THREADS), 0, st, w.part, w.state, set_bb ? 1 : 0);
+  hipLaunchKernelGGL(k_residual64, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, gate ? 1 : 0);
+  hipLaunchKernelGGL(k_residual_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, set_bb ? 1 : 0,
+                     gate ? 1 : 0, accumulate ? 1 : 0, max_outer);
   return hipGetLastError();
 }
 
 hipError_t
-to_f32_scaled(const double * r64, float * r32, int64_t n, const PcgState * s, hipStream_t st)
+to_f32_scaled(const SlotTable & t, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_to_f32_scaled, dim3(1024), dim3(256), 0, st, r64, r32, n, s);
+  hipLaunchKernelGGL(k_to_f32_scaled, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
   return hipGetLastError();
 }
 
 hipError_t
-ir_update(double * x, const float * d32, int64_t n, const PcgState * s, hipStream_t st)
+ir_update(const SlotTable & t, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_ir_update, dim3(1024), dim3(256), 0, st, x, d32, n, s);
+  hipLaunchKernelGGL(k_ir_update, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
+  return hipGetLastError();
+}
+
+hipError_t
+zero_int(int * v, int n, hipStream_t st)
+{
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, st, v, n);
   return hipGetLastError();
 }
 

@@ -1,0 +1,594 @@
+// Batched solve engine: Jacobi-PCG (the device side of DefaultJacobiPCGSolver,
+// src/solvers/hephaestus_solvers.hpp:34-57) and FP32 PCG + FP64 iterative
+// refinement over the right-hand sides of one or more operator instances, and
+// the C ABI entry points built on it (hc_pcg_solve, hc_batch_*).
+//
+// Every right-hand side of every instance is one solve slot (hc_pcg.hip); the
+// operator applies of one iteration are one launch per ND order over a device
+// table of instances (hc_apply.hip), each gated by the slots' done flags.
+#include "hc_op.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+
+namespace hc
+{
+
+namespace
+{
+
+constexpr int MAX_OUTER_FP64 = 5;  // PCG runs (1 + residual-replacement restarts)
+constexpr int MAX_OUTER_IR = 30;   // refinement steps
+constexpr double INNER_FLOOR = 3e-5; // FP32 PCG reduces its residual reliably to ~1e-5
+
+void
+geom_for(const hc_op_s * op, const TileCandidate * c, Geom & g, PlaneSegs & segs)
+{
+  g = op->g;
+  g.T[0] = c->shape.tx;
+  g.T[1] = c->shape.ty;
+  g.T[2] = c->shape.tz;
+  for (int a = 0; a < 3; a++)
+    g.nt[a] = (g.n[a] + g.T[a] - 1) / g.T[a];
+  segs = make_plane_segs(g);
+}
+
+} // namespace
+
+Engine::~Engine()
+{
+  if (host_)
+    (void)hipHostFree(host_);
+}
+
+int
+Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool force_batched)
+{
+  inst_ = inst;
+  mode_ = mode;
+  const bool ir = (mode == HC_SOLVE_FP32_IR);
+  nslots_ = 0;
+  int nb_max = 1;
+  for (auto & e : inst_)
+  {
+    if (e.nrhs < 1)
+      return fail(HC_EINVAL, "solve: nrhs must be >= 1");
+    if (!e.op->setup_done)
+      return fail(HC_EINVAL, "solve: hc_op_setup has not been called on an operator of the batch");
+    if (ir && !e.op->qd32)
+      return fail(HC_EINVAL, "solve: FP32 refinement needs operators created with HC_PREC_FP32");
+    if (!e.ws || e.ws->nrhs < e.nrhs || (ir && !e.ws->has32))
+      return fail(HC_EINVAL, "solve: work vectors missing");
+    e.slot0 = nslots_;
+    nslots_ += e.nrhs;
+    nb_max = std::max(nb_max, red_blocks(e.op->N));
+  }
+  if (nslots_ == 0)
+    return fail(HC_EINVAL, "solve: empty batch");
+  // groups by order (one apply launch each)
+  std::vector<std::vector<int>> by_order(PMAX + 1);
+  for (int i = 0; i < (int)inst_.size(); i++)
+    by_order[inst_[i].op->p].push_back(i);
+  size_t ndesc_total = 0;
+  for (int p = 1; p <= PMAX; p++)
+    ndesc_total += by_order[p].size();
+  // device layout
+  const size_t o_slots = 0;
+  const size_t o_in = align256(o_slots + sizeof(SolveSlot) * nslots_);
+  const size_t o_out = align256(o_in + sizeof(PcgState) * nslots_);
+  const size_t o_part = align256(o_out + sizeof(PcgState) * nslots_);
+  const size_t o_git = align256(o_part + sizeof(double) * 4 * RED_BLOCKS * (size_t)nslots_);
+  const size_t o_desc = align256(o_git + sizeof(int));
+  const size_t total = o_desc + sizeof(ApplyDesc) * ndesc_total * K_COUNT;
+  if (mem_.bytes < total)
+  {
+    int rc = mem_.alloc(total);
+    if (rc)
+      return rc;
+  }
+  char * base = (char *)mem_.base;
+  SolveSlot * d_slots = (SolveSlot *)(base + o_slots);
+  in_ = (PcgState *)(base + o_in);
+  out_ = (PcgState *)(base + o_out);
+  double * d_part = (double *)(base + o_part);
+  git_ = (int *)(base + o_git);
+  ApplyDesc * d_desc = (ApplyDesc *)(base + o_desc);
+  // slots
+  std::vector<SolveSlot> hs(nslots_);
+  for (auto & e : inst_)
+  {
+    const hc_op_s * op = e.op;
+    const int64_t N = op->N;
+    const Workspace & w = *e.ws;
+    for (int r = 0; r < e.nrhs; r++)
+    {
+      SolveSlot & s = hs[e.slot0 + r];
+      std::memset(&s, 0, sizeof(s));
+      const int64_t o = (int64_t)r * N;
+      s.n = N;
+      s.rtol = e.rtol;
+      s.maxit = e.maxit;
+      s.b = e.b + o;
+      s.x = e.x + o;
+      s.r64 = w.r64 + o;
+      s.q64 = w.q64 + o;
+      if (ir)
+      {
+        s.ib = w.b32 + o;
+        s.ix = w.x32 + o;
+        s.ir = w.r32 + o;
+        s.ip = w.p32 + o;
+        s.iq = w.q32 + o;
+        s.idinv = op->dinv32;
+      }
+      else
+      {
+        s.ib = s.b;
+        s.ix = s.x;
+        s.ir = s.r64;
+        s.ip = w.p64 + o;
+        s.iq = s.q64;
+        s.idinv = op->dinv64;
+      }
+      s.in = in_ + e.slot0 + r;
+      s.out = out_ + e.slot0 + r;
+      s.part = d_part + (size_t)(e.slot0 + r) * 4 * RED_BLOCKS;
+    }
+  }
+  table_.slots = d_slots;
+  table_.nslots = nslots_;
+  table_.nb_max = nb_max;
+  // apply groups
+  std::vector<ApplyDesc> hd(ndesc_total * K_COUNT);
+  size_t dpos = 0;
+  for (int k = 0; k < K_COUNT; k++)
+  {
+    groups_[k].clear();
+    const bool inner = (k == K_PQ);
+    const bool prec32 = inner && ir;
+    for (int p = 1; p <= PMAX; p++)
+    {
+      const auto & ids = by_order[p];
+      if (ids.empty())
+        continue;
+      const hc_op_s * op0 = inst_[ids[0]].op;
+      ApplyGroup G{};
+      G.order = p;
+      G.prec = prec32 ? 32 : 64;
+      G.cand = prec32 ? op0->plan32.cand : op0->plan64.cand;
+      G.tables = prec32 ? (const void *)&op0->tb32 : (const void *)&op0->tb64;
+      G.batched = force_batched || ids.size() > 1 || inst_[ids[0]].nrhs > 1;
+      G.descs = d_desc + dpos;
+      G.ndesc = (int)ids.size();
+      int w0 = 0, pw0 = 0;
+      for (int i : ids)
+      {
+        const EngineInst & e = inst_[i];
+        const hc_op_s * op = e.op;
+        const Workspace & w = *e.ws;
+        ApplyDesc d{};
+        geom_for(op, G.cand, d.g, d.segs);
+        d.prows = (int)d.segs.start[d.segs.nseg];
+        d.ntiles = d.g.nt[0] * d.g.nt[1] * d.g.nt[2];
+        d.nrhs = e.nrhs;
+        d.rb = G.batched ? (rhs_block > 0 ? std::min(rhs_block, e.nrhs) : default_rhs_block(d.ntiles, e.nrhs)) : 1;
+        d.qd = prec32 ? (const void *)op->qd32 : (const void *)op->qd64;
+        if (k == K_PQ)
+        {
+          d.x = ir ? (const void *)w.p32 : (const void *)w.p64;
+          d.y = ir ? (void *)w.q32 : (void *)w.q64;
+          d.part = ir ? (void *)w.part32 : (void *)w.part64;
+          d.done = &in_[e.slot0].done;
+        }
+        else
+        {
+          d.x = e.x;
+          d.y = w.q64;
+          d.part = w.part64;
+          d.done = (k == K_XQ_G) ? &out_[e.slot0].done : nullptr;
+        }
+        d.done_stride = STATE_STRIDE_INT;
+        d.w0 = w0;
+        d.pw0 = pw0;
+        w0 += d.ntiles * ((d.nrhs + d.rb - 1) / d.rb);
+        pw0 += d.prows * d.nrhs;
+        if (!G.batched)
+          G.single = d;
+        hd[dpos++] = d;
+      }
+      G.nwork = w0;
+      G.npwork = pw0;
+      groups_[k].push_back(G);
+    }
+  }
+  HC_TRY_HIP(hipMemcpy(d_slots, hs.data(), sizeof(SolveSlot) * nslots_, hipMemcpyHostToDevice));
+  HC_TRY_HIP(hipMemcpy(d_desc, hd.data(), sizeof(ApplyDesc) * hd.size(), hipMemcpyHostToDevice));
+  if (host_cap_ < nslots_)
+  {
+    if (host_)
+      (void)hipHostFree(host_);
+    host_ = nullptr;
+    host_cap_ = 0;
+    if (hipHostMalloc((void **)&host_, sizeof(PcgState) * nslots_, hipHostMallocDefault) != hipSuccess)
+      return fail(HC_ENOMEM, "hipHostMalloc of the solver state failed");
+    host_cap_ = nslots_;
+  }
+  return HC_OK;
+}
+
+int
+Engine::apply(Kind k, hipStream_t st)
+{
+  for (const ApplyGroup & G : groups_[k])
+  {
+    hipError_t e;
+    if (G.batched)
+      e = G.prec == 64 ? apply_multi<double>(G.cand, *(const Tables<double> *)G.tables, G.descs, G.ndesc, G.nwork,
+                                             G.npwork, st)
+                       : apply_multi<float>(G.cand, *(const Tables<float> *)G.tables, G.descs, G.ndesc, G.nwork,
+                                            G.npwork, st);
+    else
+    {
+      e = G.cand->launch(G.single, nullptr, 1, G.nwork, G.tables, G.prec, st);
+      if (e == hipSuccess)
+        e = G.prec == 64 ? assemble_planes<double>(G.single, nullptr, 1, G.npwork, st)
+                         : assemble_planes<float>(G.single, nullptr, 1, G.npwork, st);
+    }
+    if (e != hipSuccess)
+      return fail(HC_EHIP, std::string("operator apply launch: ") + hipGetErrorString(e));
+  }
+  return HC_OK;
+}
+
+int
+Engine::read_states(PcgState * dev, hipStream_t st)
+{
+  HC_TRY_HIP(hipMemcpyAsync(host_, dev, sizeof(PcgState) * nslots_, hipMemcpyDeviceToHost, st));
+  HC_TRY_HIP(hipStreamSynchronize(st));
+  return HC_OK;
+}
+
+bool
+Engine::all_done(const PcgState * h) const
+{
+  for (int i = 0; i < nslots_; i++)
+    if (!h[i].done)
+      return false;
+  return true;
+}
+
+int
+Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double * ms)
+{
+  if (mode != mode_)
+    return fail(HC_EINVAL, "solve: mode differs from the one the batch was set up for");
+  const bool ir = (mode == HC_SOLVE_FP32_IR);
+  const int max_outer = ir ? MAX_OUTER_IR : MAX_OUTER_FP64;
+  int maxit_all = 0;
+  for (auto & e : inst_)
+    maxit_all = std::max(maxit_all, e.maxit);
+  hipEvent_t e0, e1;
+  HC_TRY_HIP(hipEventCreate(&e0));
+  HC_TRY_HIP(hipEventCreate(&e1));
+  HC_TRY_HIP(hipEventRecord(e0, st));
+  HC_TRY_HIP(zero_int(git_, 1, st));
+  int rc;
+  // initial FP64 residual; sets ||b|| and the outer tolerance
+  if ((rc = apply(K_XQ_ALL, st)))
+    return rc;
+  HC_TRY_HIP(residual64(table_, true, false, false, max_outer, st));
+  int guard = 0;
+  for (int outer = 0; outer < max_outer + 1; outer++)
+  {
+    if ((rc = read_states(out_, st)))
+      return rc;
+    if (all_done(host_))
+      break;
+    if (ir)
+    {
+      HC_TRY_HIP(to_f32_scaled(table_, st));
+      HC_TRY_HIP(pcg_init<float>(table_, git_, true, true, INNER_FLOOR, st));
+    }
+    else
+      HC_TRY_HIP(pcg_init<double>(table_, git_, false, true, 0.0, st));
+    // inner PCG: host checks the done flags every `chunk` iterations
+    int chunk = 8;
+    for (;;)
+    {
+      if ((rc = read_states(in_, st)))
+        return rc;
+      if (all_done(host_))
+        break;
+      if (guard > maxit_all + 64)
+        return fail(HC_EHIP, "solve: inner iterations exceeded the cap without every slot finishing");
+      for (int k = 0; k < chunk; k++)
+      {
+        if ((rc = apply(K_PQ, st)))
+          return rc;
+        if (ir)
+          HC_TRY_HIP(pcg_iterate<float>(table_, git_, st));
+        else
+          HC_TRY_HIP(pcg_iterate<double>(table_, git_, st));
+      }
+      guard += chunk;
+      chunk = std::min(chunk * 2, 64);
+    }
+    guard = 0;
+    if (ir)
+      HC_TRY_HIP(ir_update(table_, st));
+    if ((rc = apply(K_XQ_G, st)))
+      return rc;
+    HC_TRY_HIP(residual64(table_, false, true, true, max_outer, st));
+  }
+  // final true residual of every slot
+  if ((rc = apply(K_XQ_ALL, st)))
+    return rc;
+  HC_TRY_HIP(residual64(table_, false, false, false, max_outer, st));
+  HC_TRY_HIP(hipEventRecord(e1, st));
+  if ((rc = read_states(out_, st)))
+    return rc;
+  float t = 0;
+  HC_TRY_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms)
+    *ms = t;
+  stats.resize(nslots_);
+  for (int i = 0; i < nslots_; i++)
+  {
+    const PcgState & s = host_[i];
+    SlotStats & o = stats[i];
+    o.inner = s.inner;
+    o.outer = s.iters;
+    o.status = s.status;
+    o.rel_res = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
+    o.converged = (s.status == 0 && s.rr <= s.tol2 * s.bb * (1.0 + 1e-6)) ? 1 : 0;
+  }
+  return HC_OK;
+}
+
+} // namespace hc
+
+// ---------------------------------------------------------------------------
+// operator workspace
+// ---------------------------------------------------------------------------
+int
+hc_op_s::ensure_ws(int nrhs, bool need32)
+{
+  if (ws.nrhs >= nrhs && (ws.has32 || !need32) && ws_mem.base)
+    return HC_OK;
+  const int k = std::max(nrhs, ws.nrhs);
+  const bool n32 = need32 || ws.has32;
+  const size_t sz = hc::Workspace::bytes_for(N, k, n32);
+  bytes -= (int64_t)ws_mem.bytes;
+  ws = hc::Workspace{};
+  int rc = ws_mem.alloc(sz);
+  if (rc)
+    return rc;
+  bytes += (int64_t)sz;
+  ws.carve(ws_mem.base, N, k, n32);
+  return HC_OK;
+}
+
+hc_op_s::~hc_op_s()
+{
+  delete engine;
+  for (void * v : allocs)
+    (void)hipFree(v);
+}
+
+namespace
+{
+
+void
+fold_stats(const std::vector<hc::SlotStats> & s, int begin, int count, double ms, hc_solve_stats * out)
+{
+  hc_solve_stats r{};
+  r.converged = 1;
+  for (int i = begin; i < begin + count; i++)
+  {
+    r.iterations = std::max(r.iterations, s[i].inner);
+    r.outer_iterations = std::max(r.outer_iterations, s[i].outer);
+    r.rel_res = std::max(r.rel_res, s[i].rel_res);
+    if (!s[i].converged)
+      r.converged = 0;
+    if (s[i].status && !r.status)
+      r.status = s[i].status;
+  }
+  if (r.status == 0 && !r.converged)
+    r.status = HC_ENOTCONV;
+  r.solve_ms = ms;
+  *out = r;
+}
+
+int
+status_rc(const hc_solve_stats & s, const char * who)
+{
+  if (s.status == HC_ENAN)
+    return hc::fail(HC_ENAN, std::string(who) + ": NaN/Inf met in a reduction");
+  if (!s.converged)
+  {
+    char buf[200];
+    std::snprintf(buf, sizeof(buf), "%s: not converged after %d iterations, true relative residual %.3e", who,
+                  s.iterations, s.rel_res);
+    return hc::fail(HC_ENOTCONV, buf);
+  }
+  return HC_OK;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// batches of operator instances
+// ---------------------------------------------------------------------------
+struct hc_batch_s
+{
+  struct Entry
+  {
+    hc_op_t op;
+    const double * b;
+    double * x;
+    int nrhs;
+    double rtol;
+    int maxit;
+  };
+  std::vector<Entry> entries;
+  std::vector<std::unique_ptr<hc::DevBlock>> ws_mem;
+  std::vector<hc::Workspace> ws;
+  hc::Engine engine;
+  int rhs_block = 0;
+  bool dirty = true;
+  int mode = -1;
+  std::vector<hc::SlotStats> stats;
+};
+
+extern "C" {
+
+int
+hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, int maxit, int mode,
+             hc_solve_stats * stats, void * stream)
+{
+  if (!op || !b || !x)
+    return hc::fail(HC_EINVAL, "hc_pcg_solve: null argument");
+  if (!op->setup_done)
+    return hc::fail(HC_EINVAL, "hc_pcg_solve: hc_op_setup has not been called");
+  if (nrhs < 1)
+    return hc::fail(HC_EINVAL, "hc_pcg_solve: nrhs must be >= 1");
+  if (!(rtol > 0) || maxit < 1)
+    return hc::fail(HC_EINVAL, "hc_pcg_solve: rtol must be > 0 and maxit >= 1");
+  if (mode != HC_SOLVE_FP64 && mode != HC_SOLVE_FP32_IR)
+    return hc::fail(HC_EINVAL, "hc_pcg_solve: unknown mode");
+  if (mode == HC_SOLVE_FP32_IR && !op->qd32)
+    return hc::fail(HC_EINVAL, "hc_pcg_solve: FP32 refinement needs an operator created with HC_PREC_FP32");
+  int rc = op->ensure_ws(nrhs, mode == HC_SOLVE_FP32_IR);
+  if (rc)
+    return rc;
+  if (!op->engine)
+    op->engine = new hc::Engine();
+  std::vector<hc::EngineInst> inst{{op, nrhs, b, x, rtol, maxit, &op->ws, 0}};
+  if ((rc = op->engine->set(inst, mode, op->rhs_block)))
+    return rc;
+  std::vector<hc::SlotStats> s;
+  double ms = 0;
+  if ((rc = op->engine->solve(mode, (hipStream_t)stream, s, &ms)))
+    return rc;
+  hc_solve_stats out{};
+  fold_stats(s, 0, nrhs, ms, &out);
+  if (stats)
+    *stats = out;
+  return status_rc(out, "hc_pcg_solve");
+}
+
+int
+hc_batch_create(hc_batch_t * out)
+{
+  if (!out)
+    return hc::fail(HC_EINVAL, "hc_batch_create: null argument");
+  *out = new hc_batch_s();
+  return HC_OK;
+}
+
+int
+hc_batch_add(hc_batch_t bt, hc_op_t op, const double * b, double * x, int nrhs, double rtol, int maxit)
+{
+  if (!bt || !op || !b || !x)
+    return hc::fail(HC_EINVAL, "hc_batch_add: null argument");
+  if (nrhs < 1 || !(rtol > 0) || maxit < 1)
+    return hc::fail(HC_EINVAL, "hc_batch_add: nrhs >= 1, rtol > 0 and maxit >= 1 are required");
+  if (!op->setup_done)
+    return hc::fail(HC_EINVAL, "hc_batch_add: hc_op_setup has not been called");
+  bt->entries.push_back({op, b, x, nrhs, rtol, maxit});
+  bt->dirty = true;
+  return (int)bt->entries.size() - 1;
+}
+
+int
+hc_batch_set_rhs_block(hc_batch_t bt, int rb)
+{
+  if (!bt || rb < 0)
+    return hc::fail(HC_EINVAL, "hc_batch_set_rhs_block: bad argument");
+  bt->rhs_block = rb;
+  bt->dirty = true;
+  return HC_OK;
+}
+
+int
+hc_batch_solve(hc_batch_t bt, int mode, hc_solve_stats * stats, void * stream)
+{
+  if (!bt)
+    return hc::fail(HC_EINVAL, "hc_batch_solve: null handle");
+  if (bt->entries.empty())
+    return hc::fail(HC_EINVAL, "hc_batch_solve: empty batch");
+  if (mode != HC_SOLVE_FP64 && mode != HC_SOLVE_FP32_IR)
+    return hc::fail(HC_EINVAL, "hc_batch_solve: unknown mode");
+  const bool ir = mode == HC_SOLVE_FP32_IR;
+  int rc;
+  if (bt->dirty || bt->mode != mode)
+  {
+    // one workspace per entry
+    bt->ws.resize(bt->entries.size());
+    bt->ws_mem.resize(bt->entries.size());
+    for (size_t i = 0; i < bt->entries.size(); i++)
+    {
+      const auto & e = bt->entries[i];
+      hc::Workspace & w = bt->ws[i];
+      if (!bt->ws_mem[i])
+        bt->ws_mem[i].reset(new hc::DevBlock());
+      if (w.N != e.op->N || w.nrhs < e.nrhs || (ir && !w.has32))
+      {
+        const size_t sz = hc::Workspace::bytes_for(e.op->N, e.nrhs, ir);
+        if ((rc = bt->ws_mem[i]->alloc(sz)))
+          return rc;
+        w.carve(bt->ws_mem[i]->base, e.op->N, e.nrhs, ir);
+      }
+    }
+    std::vector<hc::EngineInst> inst;
+    for (size_t i = 0; i < bt->entries.size(); i++)
+    {
+      const auto & e = bt->entries[i];
+      inst.push_back({e.op, e.nrhs, e.b, e.x, e.rtol, e.maxit, &bt->ws[i], 0});
+    }
+    if ((rc = bt->engine.set(inst, mode, bt->rhs_block, true)))
+      return rc;
+    bt->dirty = false;
+    bt->mode = mode;
+  }
+  double ms = 0;
+  if ((rc = bt->engine.solve(mode, (hipStream_t)stream, bt->stats, &ms)))
+    return rc;
+  int pos = 0;
+  hc_solve_stats all{};
+  all.converged = 1;
+  for (size_t i = 0; i < bt->entries.size(); i++)
+  {
+    hc_solve_stats s{};
+    fold_stats(bt->stats, pos, bt->entries[i].nrhs, ms, &s);
+    pos += bt->entries[i].nrhs;
+    if (stats)
+      stats[i] = s;
+    all.iterations = std::max(all.iterations, s.iterations);
+    all.outer_iterations = std::max(all.outer_iterations, s.outer_iterations);
+    all.rel_res = std::max(all.rel_res, s.rel_res);
+    if (!s.converged)
+      all.converged = 0;
+    if (s.status && !all.status)
+      all.status = s.status;
+  }
+  all.solve_ms = ms;
+  return status_rc(all, "hc_batch_solve");
+}
+
+int
+hc_batch_destroy(hc_batch_t bt)
+{
+  if (!bt)
+    return hc::fail(HC_EINVAL, "hc_batch_destroy: null handle");
+  (void)hipDeviceSynchronize();
+  delete bt;
+  return HC_OK;
+}
+
+} // extern "C"

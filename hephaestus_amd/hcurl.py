@@ -35,7 +35,8 @@ EXPORTS = [
     "hc_op_create", "hc_op_set_coefficients", "hc_op_setup", "hc_op_ndofs", "hc_op_ess_count",
     "hc_op_device_bytes", "hc_op_apply", "hc_op_apply_f32", "hc_op_diagonal", "hc_pcg_solve",
     "hc_op_destroy", "hc_last_error", "hc_fill_uniform", "hc_op_set_tile", "hc_op_get_tile",
-    "hc_op_autotune", "hc_tile_list",
+    "hc_op_autotune", "hc_tile_list", "hc_op_set_rhs_block", "hc_batch_create", "hc_batch_add", "hc_batch_set_rhs_block",
+    "hc_batch_solve", "hc_batch_destroy",
 ]
 
 
@@ -94,6 +95,12 @@ def load():
                                    C.POINTER(C.c_int)]
     lib.hc_op_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_double), vp]
     lib.hc_tile_list.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_int]
+    lib.hc_op_set_rhs_block.argtypes = [vp, C.c_int]
+    lib.hc_batch_create.argtypes = [C.POINTER(vp)]
+    lib.hc_batch_add.argtypes = [vp, vp, vp, vp, C.c_int, C.c_double, C.c_int]
+    lib.hc_batch_set_rhs_block.argtypes = [vp, C.c_int]
+    lib.hc_batch_solve.argtypes = [vp, C.c_int, C.POINTER(SolveStats), vp]
+    lib.hc_batch_destroy.argtypes = [vp]
     _lib = lib
     return lib
 
@@ -112,6 +119,18 @@ def _stream(stream=None):
 
 def _ptr(t):
     return C.c_void_p(t.data_ptr())
+
+
+def _nrhs(v, n, dtype):
+    """Right-hand sides of a vector argument: shape (N,) or (nrhs, N), contiguous."""
+    import torch
+    if v.dtype != dtype or not v.is_cuda or not v.is_contiguous():
+        raise ValueError(f"expected a contiguous {dtype} device tensor")
+    if v.dim() == 1 and v.shape[0] == n:
+        return 1
+    if v.dim() == 2 and v.shape[1] == n:
+        return int(v.shape[0])
+    raise ValueError(f"expected shape ({n},) or (nrhs, {n}), got {tuple(v.shape)}")
 
 
 def tile_list(order):
@@ -172,20 +191,19 @@ class Operator:
         return int(_lib.hc_op_device_bytes(self._h))
 
     def apply(self, x, y=None, stream=None):
+        """y = A x; x of shape (N,) or (nrhs, N), float64."""
         import torch
-        assert x.dtype == torch.float64 and x.is_cuda and x.is_contiguous()
+        nrhs = _nrhs(x, self.ndofs, torch.float64)
         if y is None:
             y = torch.empty_like(x)
-        nrhs = 1 if x.dim() == 1 else x.shape[1]
         _check(_lib.hc_op_apply(self._h, _ptr(x), _ptr(y), nrhs, _stream(stream)), "hc_op_apply")
         return y
 
     def apply_f32(self, x, y=None, stream=None):
         import torch
-        assert x.dtype == torch.float32 and x.is_cuda and x.is_contiguous()
+        nrhs = _nrhs(x, self.ndofs, torch.float32)
         if y is None:
             y = torch.empty_like(x)
-        nrhs = 1 if x.dim() == 1 else x.shape[1]
         _check(_lib.hc_op_apply_f32(self._h, _ptr(x), _ptr(y), nrhs, _stream(stream)), "hc_op_apply_f32")
         return y
 
@@ -196,8 +214,13 @@ class Operator:
         return d
 
     def solve(self, b, x, rtol=1e-8, maxit=10000, mode=SOLVE_FP64, stream=None, raise_on_fail=True):
+        """Jacobi-PCG (mode SOLVE_FP64) or FP32 PCG + FP64 refinement on A x = b;
+        b, x of shape (N,) or (nrhs, N), x holds the initial guess."""
+        import torch
+        nrhs = _nrhs(b, self.ndofs, torch.float64)
+        if _nrhs(x, self.ndofs, torch.float64) != nrhs:
+            raise ValueError("b and x differ in their number of right-hand sides")
         st = SolveStats()
-        nrhs = 1 if b.dim() == 1 else b.shape[1]
         rc = _lib.hc_pcg_solve(self._h, _ptr(b), _ptr(x), nrhs, float(rtol), int(maxit), int(mode),
                                C.byref(st), _stream(stream))
         if rc != HC_OK and (raise_on_fail or rc not in (HC_ENOTCONV, HC_ENAN)):
@@ -205,6 +228,10 @@ class Operator:
         d = st.as_dict()
         d["rc"] = rc
         return d
+
+    def set_rhs_block(self, rb):
+        """Right-hand sides per work item of multi-RHS applies/solves (0 = automatic)."""
+        _check(_lib.hc_op_set_rhs_block(self._h, int(rb)), "hc_op_set_rhs_block")
 
     def set_tile(self, prec_bits, tx, ty, tz, prefetch=-1):
         _check(_lib.hc_op_set_tile(self._h, prec_bits, tx, ty, tz, prefetch), "hc_op_set_tile")
@@ -227,6 +254,52 @@ class Operator:
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             _lib.hc_op_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Solves over several operator instances advanced together (hc_batch_*).
+    Keeps references to the operators and vectors it was given."""
+
+    def __init__(self):
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.hc_batch_create(C.byref(h)), "hc_batch_create")
+        self._h = h
+        self._keep = []
+
+    def add(self, op, b, x, rtol=1e-8, maxit=10000):
+        import torch
+        nrhs = _nrhs(b, op.ndofs, torch.float64)
+        if _nrhs(x, op.ndofs, torch.float64) != nrhs:
+            raise ValueError("b and x differ in their number of right-hand sides")
+        rc = _lib.hc_batch_add(self._h, op._h, _ptr(b), _ptr(x), nrhs, float(rtol), int(maxit))
+        if rc < 0:
+            _check(rc, "hc_batch_add")
+        self._keep.append((op, b, x))
+        return rc
+
+    def set_rhs_block(self, rb):
+        _check(_lib.hc_batch_set_rhs_block(self._h, int(rb)), "hc_batch_set_rhs_block")
+
+    def solve(self, mode=SOLVE_FP32_IR, stream=None, raise_on_fail=True):
+        n = len(self._keep)
+        st = (SolveStats * max(n, 1))()
+        rc = _lib.hc_batch_solve(self._h, int(mode), st, _stream(stream))
+        if rc != HC_OK and (raise_on_fail or rc not in (HC_ENOTCONV, HC_ENAN)):
+            _check(rc, "hc_batch_solve")
+        out = [st[i].as_dict() for i in range(n)]
+        return rc, out
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.hc_batch_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

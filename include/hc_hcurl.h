@@ -43,7 +43,10 @@
  *     (open, Gauss-Legendre) and n_d*p+1 along the others (closed,
  *     Gauss-Lobatto); x-component first, each component x-fastest.  All edges
  *     and faces are oriented along +axes (no sign flips).
- *   - Multi-RHS vectors use the layout [dof][rhs] (rhs fastest).
+ *   - Multi-RHS vectors: nrhs vectors of N entries stored one after the other
+ *     (right-hand side r at offset r*N; column-major N x nrhs, leading
+ *     dimension N).  Each right-hand side's solve is independent of the others
+ *     in the batch: same iterations, same result bits.
  */
 #ifndef HC_HCURL_H
 #define HC_HCURL_H
@@ -80,6 +83,7 @@ extern "C" {
 #define HC_SOLVE_FP32_IR 1 /* FP32 Jacobi-PCG inner solves + FP64 refinement */
 
 typedef struct hc_op_s * hc_op_t;
+typedef struct hc_batch_s * hc_batch_t;
 
 typedef struct
 {
@@ -92,8 +96,10 @@ typedef struct
 
 typedef struct
 {
-  int iterations;       /* total inner PCG iterations (all RHS, max over RHS when batched) */
-  int outer_iterations; /* refinement steps (HC_SOLVE_FP32_IR), else 0 */
+  int iterations;       /* inner PCG iterations (summed over refinement steps or
+                           restarts; max over the right-hand sides) */
+  int outer_iterations; /* HC_SOLVE_FP32_IR: refinement steps; HC_SOLVE_FP64:
+                           PCG runs (1 + residual-replacement restarts) */
   int converged;        /* 1 if every RHS met rtol */
   int status;           /* HC_OK or HC_ENOTCONV / HC_ENAN */
   double rel_res;       /* max over RHS of the final FP64 true relative residual */
@@ -122,13 +128,18 @@ int hc_op_ess_count(hc_op_t op, int64_t * n_ess);
 /* Device bytes owned by the handle. */
 int64_t hc_op_device_bytes(hc_op_t op);
 
-/* y = A x (PEC eliminated).  x, y: device, layout [dof][rhs]. */
+/* y = A x (PEC eliminated).  x, y: device, nrhs vectors (rhs r at r*N).
+ * With nrhs > 1 a work item of the kernel applies the operator to a group of
+ * right-hand sides, reading each tile's quadrature data once per group. */
 int hc_op_apply(hc_op_t op, const double * x, double * y, int nrhs, void * stream);
 int hc_op_apply_f32(hc_op_t op, const float * x, float * y, int nrhs, void * stream);
+/* Right-hand sides per work item of the multi-RHS apply and solve (0 =
+ * automatic: the fewest groups that still give the GPU enough work items).
+ * Results do not depend on it. */
+int hc_op_set_rhs_block(hc_op_t op, int rhs_block);
 /* Element-tile shape of the apply kernel.  Each order has a set of compiled
  * tile variants: tile extent tx x ty x tz elements and a load mode (0:
- * quadrature data loaded where it is used, 1: at kernel entry, 2: persistent
- * workgroups that load tile i+1 while finishing tile i), plus 4 when each
+ * quadrature data loaded where it is used, 1: at kernel entry), plus 4 when each
  * pencil's quadrature points are split over two wave groups.  Results differ
  * between shapes by round-off only and are bitwise reproducible for a fixed
  * shape.  prec_bits: 64 or 32; mode -1 in hc_op_set_tile matches any mode. */
@@ -158,6 +169,27 @@ int hc_pcg_solve(hc_op_t op,
                  int mode,
                  hc_solve_stats * stats,
                  void * stream);
+
+/* Batches of solves over several operator instances (differing sizes and
+ * orders), advanced together: every iteration applies all instances of one
+ * order in one launch and updates all right-hand sides in one launch per
+ * vector kernel; converged right-hand sides drop out through device flags.
+ * Replaces a loop of ImplicitSolve calls
+ * (src/problem_operators/time_domain_equation_system_problem_operator.cpp:30-51)
+ * over independent problems; the reference has no batched counterpart.
+ * The tile variant of each order is that of the batch's first instance of that
+ * order (hc_op_set_tile / hc_op_autotune).  Each instance's results equal
+ * those of the same solve run alone with the same tile variant.
+ *
+ * hc_batch_add: b, x are device arrays of nrhs*N doubles of op (x = initial
+ * guess); returns the entry index (>= 0) or a negative status.
+ * hc_batch_solve: stats (optional) receives one entry per added instance. */
+int hc_batch_create(hc_batch_t * out);
+int hc_batch_add(hc_batch_t batch, hc_op_t op, const double * b, double * x, int nrhs, double rtol, int maxit);
+/* Right-hand sides per work item of the batched apply (0 = automatic). */
+int hc_batch_set_rhs_block(hc_batch_t batch, int rhs_block);
+int hc_batch_solve(hc_batch_t batch, int mode, hc_solve_stats * stats, void * stream);
+int hc_batch_destroy(hc_batch_t batch);
 
 int hc_op_destroy(hc_op_t op);
 const char * hc_last_error(void);

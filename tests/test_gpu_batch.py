@@ -1,0 +1,176 @@
+"""Multi-RHS applies/solves and batches of uneven instances (needs an MI355X).
+
+Bars: FP64 applies of every right-hand side agree with the oracle to relative
+L2 <= 1e-13 and with the single-RHS apply to <= 1e-15 (same arithmetic per
+right-hand side); every solve's true residual, recomputed by the CPU oracle,
+meets rtol; a solve's result inside a batch is bitwise identical to the same
+solve run alone through the batch interface (mix invariance).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def hc():
+    import hephaestus_amd as H
+    H.load()
+    return H
+
+
+@pytest.fixture(scope="module")
+def O():
+    from oracle import oracle as O
+    O.build()
+    return O
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _two_region(n, inner, outer):
+    nx, ny, nz = n
+    cx = (np.arange(nx) + 0.5) / nx
+    cy = (np.arange(ny) + 0.5) / ny
+    cz = (np.arange(nz) + 0.5) / nz
+    ins = ((cx[None, None, :] >= 0.25) & (cx[None, None, :] <= 0.75) & (cy[None, :, None] >= 0.25)
+           & (cy[None, :, None] <= 0.75) & (cz[:, None, None] >= 0.25) & (cz[:, None, None] <= 0.75))
+    return np.where(ins, inner, outer).ravel().astype(np.float64)
+
+
+def _rand_rhs(torch, op, seeds):
+    B = torch.empty((len(seeds), op.ndofs), dtype=torch.float64, device="cuda")
+    for i, s in enumerate(seeds):
+        op.fill_uniform(s, B[i])
+    return B
+
+
+@pytest.mark.parametrize("p,n,rb", [(1, (9, 6, 5), 3), (2, (10, 7, 6), 2), (3, (6, 5, 7), 4), (2, (5, 5, 5), 0)])
+def test_multi_rhs_apply(torch_cuda, hc, O, p, n, rb):
+    torch = torch_cuda
+    print(f"case multi-rhs apply p={p} n={n} rb={rb}", flush=True)
+    m = O.Mesh(*n)
+    sig = _two_region(n, 3.0, 0.5)
+    op = hc.Operator(n, p, 1.0, sig).setup(1.0)
+    op.set_rhs_block(rb)
+    X = _rand_rhs(torch, op, [11, 12, 13, 14, 15])
+    Y = op.apply(X)
+    qd = O.pa_setup(m, p, 1.0, sig)
+    ess = O.ess_mask(m, p)
+    Xh = X.cpu().numpy()
+    Yh = Y.cpu().numpy()
+    for r in range(X.shape[0]):
+        y1 = op.apply(X[r].contiguous()).cpu().numpy()
+        assert _rel(Yh[r], y1) <= 1e-15
+        assert _rel(Yh[r], O.pa_apply(m, p, qd, ess, Xh[r])) <= 1e-13
+    # the same right-hand sides again: bitwise reproducible
+    Y2 = op.apply(X)
+    assert torch.equal(Y, Y2)
+    # FP32 batched apply
+    X32 = X.float()
+    Y32 = op.apply_f32(X32).double().cpu().numpy()
+    for r in range(X.shape[0]):
+        assert _rel(Y32[r], Yh[r]) <= 2e-6
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32_ir"])
+def test_multi_rhs_solve(torch_cuda, hc, O, mode):
+    torch = torch_cuda
+    n, p = (8, 7, 6), 2
+    print(f"case multi-rhs solve {mode} n={n} p={p}", flush=True)
+    md = hc.SOLVE_FP64 if mode == "fp64" else hc.SOLVE_FP32_IR
+    m = O.Mesh(*n)
+    sig = _two_region(n, 100.0, 0.1)
+    op = hc.Operator(n, p, 1.0, sig).setup(1.0)
+    op.set_rhs_block(2)
+    B = _rand_rhs(torch, op, [21, 22, 23])
+    B = torch.cat([B, torch.zeros_like(B[:1])])  # a zero right-hand side converges at once
+    X = torch.zeros_like(B)
+    rtol = 1e-9
+    st = op.solve(B, X, rtol=rtol, maxit=5000, mode=md)
+    assert st["converged"] and st["rel_res"] <= rtol * (1 + 1e-9)
+    qd = O.pa_setup(m, p, 1.0, sig)
+    ess = O.ess_mask(m, p)
+    Xh, Bh = X.cpu().numpy(), B.cpu().numpy()
+    assert not Xh[3].any()
+    for r in range(3):
+        res = Bh[r] - O.pa_apply(m, p, qd, ess, Xh[r])
+        assert np.linalg.norm(res) <= rtol * np.linalg.norm(Bh[r]) * (1 + 1e-6)
+        # the same right-hand side solved alone
+        x1 = torch.zeros_like(B[r])
+        st1 = op.solve(B[r].contiguous(), x1, rtol=rtol, maxit=5000, mode=md)
+        assert st1["converged"]
+        assert _rel(Xh[r], x1.cpu().numpy()) <= 1e-12
+
+
+def _instances(hc, specs, prec=None):
+    ops = []
+    for (n, p, s_in, s_out) in specs:
+        sig = _two_region(n, s_in, s_out)
+        op = hc.Operator(n, p, 1.0, sig).setup(1.0)
+        ops.append((op, sig))
+    return ops
+
+
+SPECS = [((8, 8, 8), 1, 1e2, 1e-2), ((6, 9, 7), 2, 10.0, 1.0), ((5, 5, 6), 3, 1e3, 1e-1),
+         ((10, 6, 5), 1, 1.0, 1.0), ((7, 7, 7), 2, 1e2, 1e-2)]
+
+
+@pytest.mark.parametrize("mode", ["fp32_ir", "fp64"])
+def test_batch_uneven_mix_invariance(torch_cuda, hc, O, mode):
+    torch = torch_cuda
+    md = hc.SOLVE_FP64 if mode == "fp64" else hc.SOLVE_FP32_IR
+    print(f"case uneven batch {mode} ({len(SPECS)} instances)", flush=True)
+    ops = _instances(hc, SPECS)
+    rtol = 1e-8
+    bs, xs = [], []
+    for i, (op, _) in enumerate(ops):
+        b = torch.empty(op.ndofs, dtype=torch.float64, device="cuda")
+        op.fill_uniform(100 + i, b)
+        bs.append(b)
+        xs.append(torch.zeros_like(b))
+    bt = hc.Batch()
+    for (op, _), b, x in zip(ops, bs, xs):
+        bt.add(op, b, x, rtol=rtol, maxit=20000)
+    rc, stats = bt.solve(md)
+    assert rc == 0 and all(s["converged"] for s in stats)
+    for i, ((op, sig), b, x) in enumerate(zip(ops, bs, xs)):
+        n, p = SPECS[i][0], SPECS[i][1]
+        m = O.Mesh(*n)
+        qd = O.pa_setup(m, p, 1.0, sig)
+        ess = O.ess_mask(m, p)
+        bh, xh = b.cpu().numpy(), x.cpu().numpy()
+        res = bh - O.pa_apply(m, p, qd, ess, xh)
+        assert np.linalg.norm(res) <= rtol * np.linalg.norm(bh) * (1 + 1e-6), i
+        # alone through the batch interface: bitwise the same result
+        x1 = torch.zeros_like(b)
+        b1 = hc.Batch()
+        b1.add(op, b, x1, rtol=rtol, maxit=20000)
+        rc1, st1 = b1.solve(md)
+        assert rc1 == 0
+        assert torch.equal(x1, x), f"instance {i} differs alone vs in the batch"
+        assert st1[0]["iterations"] == stats[i]["iterations"]
+
+
+def test_batch_errors(torch_cuda, hc):
+    torch = torch_cuda
+    op = hc.Operator(4, 1, 1.0, 1.0, precision=hc.PREC_FP64).setup(1.0)
+    b = torch.ones(op.ndofs, dtype=torch.float64, device="cuda")
+    x = torch.zeros_like(b)
+    bt = hc.Batch()
+    bt.add(op, b, x)
+    with pytest.raises(hc.HcError):
+        bt.solve(hc.SOLVE_FP32_IR)  # operator has no FP32 copy
+    empty = hc.Batch()
+    with pytest.raises(hc.HcError):
+        empty.solve(hc.SOLVE_FP64)
