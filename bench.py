@@ -41,6 +41,20 @@ def apply_bytes(N, E, p, s_vec=8, s_q=8, nrhs=1):
     return s_vec * 2 * N * nrhs + s_q * 12 * E * Q ** 3
 
 
+def measured_traffic(n, p):
+    """HBM bytes per launch of the FP64 tile kernel from the committed rocprofv3
+    counter summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated
+    against a device copy of known size), or None when there is none."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"k_apply_tile_fp64_{n}_p{p}_hbm_counters.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
+
+
 def time_applies(torch, fn, steps, warmup):
     st = torch.cuda.current_stream()
     for _ in range(warmup):
@@ -82,6 +96,60 @@ def cpu_baseline(n, p, sig, budget_s):
             "ms_per_apply": t * 1e3}
 
 
+def solve_entry(st, nsolves=1):
+    return {"iterations": st["iterations"], "outer": st["outer_iterations"], "converged": bool(st["converged"]),
+            "true_rel_res": st["rel_res"], "solve_ms": round(st["solve_ms"], 3),
+            "solves_per_s": round(1000.0 * nsolves / st["solve_ms"], 4) if st["solve_ms"] > 0 else None}
+
+
+def solves_config2(torch, H, op, x, args):
+    """48^3 p=3 two-region, uniform RHS (seed 2): FP64 Jacobi-PCG vs FP32 PCG + FP64 refinement."""
+    b = torch.empty_like(x)
+    op.fill_uniform(2, b)
+    sol = {"rhs": "uniform [-1,1) per free DOF, seed 2", "rtol": 1e-8}
+    for name, mode in (("fp64", H.SOLVE_FP64), ("fp32_ir", H.SOLVE_FP32_IR)):
+        xs = torch.zeros_like(b)
+        st = op.solve(b, xs, rtol=1e-8, maxit=args.solve_maxit, mode=mode, raise_on_fail=False)
+        sol[name] = solve_entry(st)
+        print(f"[bench] config2 solve {name}: {sol[name]}", file=sys.stderr, flush=True)
+    return sol
+
+
+def solves_config3(torch, H, args):
+    """32^3 p=2 two-region, 32 coil right-hand sides (seeds 3..34): plain FP64
+    Jacobi-PCG one right-hand side at a time vs one batched FP32 PCG + FP64
+    refinement solve of all 32, both to a true relative residual of 1e-8."""
+    n, p, nr = 32, 2, 32
+    op = H.Operator(n, p, 1.0, two_region(n)).setup(1.0)
+    op.autotune(reps=5)
+    N = op.ndofs
+    B = torch.empty((nr, N), dtype=torch.float64, device="cuda")
+    for r in range(nr):
+        op.rhs_coil(3 + r, B[r])
+    res = {"workload": f"{n}^3 p={p} two-region 1e4/1e-6, {nr} coil RHS (J = curl psi, seeds 3..{2 + nr})",
+           "rtol": 1e-8}
+    # plain FP64 path: one solve per right-hand side (a sample of them)
+    k = min(args.c3_fp64_sample, nr)
+    tot_ms, its, worst = 0.0, [], 0.0
+    for r in range(k):
+        xs = torch.zeros(N, dtype=torch.float64, device="cuda")
+        st = op.solve(B[r].contiguous(), xs, rtol=1e-8, maxit=args.solve_maxit, mode=H.SOLVE_FP64,
+                      raise_on_fail=False)
+        tot_ms += st["solve_ms"]
+        its.append(st["iterations"])
+        worst = max(worst, st["rel_res"])
+    res["fp64_single"] = {"sample_rhs": k, "iterations_mean": float(np.mean(its)), "max_true_rel_res": worst,
+                          "solves_per_s": round(1000.0 * k / tot_ms, 4)}
+    X = torch.zeros_like(B)
+    st = op.solve(B, X, rtol=1e-8, maxit=args.solve_maxit, mode=H.SOLVE_FP32_IR, raise_on_fail=False)
+    res["fp32_ir_batched"] = solve_entry(st, nr)
+    res["speedup_vs_fp64_single"] = round(res["fp32_ir_batched"]["solves_per_s"] /
+                                          res["fp64_single"]["solves_per_s"], 3)
+    print(f"[bench] config3: {res}", file=sys.stderr, flush=True)
+    del op
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +161,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--solve-maxit", type=int, default=20000)
+    ap.add_argument("--c3-fp64-sample", type=int, default=8)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a 1 GiB device copy (reference kernel for the HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -124,6 +195,17 @@ def main():
     achieved = B / (ms * 1e-3) / 1e9
     print(f"[bench] FP64 apply {ms * 1e3:.1f} us  {gdofs:.1f} GDOF/s  {achieved:.0f} GB/s",
           file=sys.stderr, flush=True)
+    # per-kernel durations (HIP events around each launch, on the launch stream)
+    prof = op.profile_apply(x, y, reps=max(args.steps, 5))
+    tile_ach = B / (prof["tile_us"] * 1e-6) / 1e9
+    print(f"[bench] FP64 tile kernel {prof['tile_us']:.1f} us ({tile_ach:.0f} GB/s algorithmic), "
+          f"tile-face assembly {prof['planes_us']:.1f} us", file=sys.stderr, flush=True)
+    if args.copy_calib:
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        c = torch.empty_like(a)
+        for _ in range(3):
+            c.copy_(a)
+        torch.cuda.synchronize()
     # FP32 apply
     x32 = x.float()
     y32 = torch.empty_like(x32)
@@ -149,29 +231,19 @@ def main():
                                f"sigma/dt two-region 1e4/1e-6, PEC all faces",
                    "n": n, "order": p, "ndofs": N, "elements": E,
                    "tile_fp64": list(tune["tile64"]), "tile_fp32": list(tune["tile32"])},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_apply_tile + k_assemble_planes (one apply)",
-                     "bytes_per_launch": B},
+        "roofline": {"bound": "hbm", "achieved": round(tile_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(tile_ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, p),
+                     "kernel": "k_apply_tile (FP64)", "kernel_us": round(prof["tile_us"], 2),
+                     "algorithmic_bytes": B,
+                     "whole_apply": {"us": round(ms * 1e3, 2), "GB_s": round(achieved, 1),
+                                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                                     "assembly_kernel_us": round(prof["planes_us"], 2)}},
         "fp32_apply": {"ms": round(ms32, 5), "GDOF_s": round(N / ms32 / 1e6, 3),
                        "GB_s": round(B32 / ms32 / 1e6, 1)},
     }
     if not args.no_solve:
-        b = torch.empty_like(x)
-        op.fill_uniform(2, b)
-        sol = {}
-        for name, mode in (("fp64", H.SOLVE_FP64), ("fp32_ir", H.SOLVE_FP32_IR)):
-            xs = torch.zeros_like(b)
-            t0 = time.perf_counter()
-            st = op.solve(b, xs, rtol=1e-8, maxit=args.solve_maxit, mode=mode, raise_on_fail=False)
-            torch.cuda.synchronize()
-            wall = time.perf_counter() - t0
-            sol[name] = {"iterations": st["iterations"], "outer": st["outer_iterations"],
-                         "converged": bool(st["converged"]), "true_rel_res": st["rel_res"],
-                         "solve_ms": round(st["solve_ms"], 3), "wall_s": round(wall, 3),
-                         "solves_per_s": round(1000.0 / st["solve_ms"], 4) if st["solve_ms"] > 0 else None}
-            print(f"[bench] solve {name}: {sol[name]}", file=sys.stderr, flush=True)
-        out["solves"] = sol
+        out["solves"] = {"config2": solves_config2(torch, H, op, x, args),
+                         "config3": solves_config3(torch, H, args)}
     if not args.no_cpu:
         del op
         torch.cuda.empty_cache()

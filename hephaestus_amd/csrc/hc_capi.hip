@@ -274,7 +274,10 @@ hc_op_create(const hc_mesh_desc * mesh, int order, const double * nu_e, const do
   op->sig.assign(sigma_e, sigma_e + op->E);
   op->box = (mesh->vert == nullptr);
   for (int a = 0; a < 3; a++)
+  {
     op->h[a] = mesh->len[a] / g.n[a];
+    op->x0[a] = mesh->x0[a];
+  }
   // PEC count (host, structured)
   int64_t ness = 0;
   for (int c = 0; c < 3; c++)
@@ -566,6 +569,103 @@ hc_op_destroy(hc_op_t op)
     return fail(HC_EINVAL, "hc_op_destroy: null handle");
   (void)hipDeviceSynchronize();
   delete op;
+  return HC_OK;
+}
+
+int
+hc_op_profile_apply(hc_op_t op, int prec_bits, const void * x, void * y, int reps, double * out_us, void * stream)
+{
+  if (!op || !x || !y || !out_us || reps < 1 || (prec_bits != 64 && prec_bits != 32))
+    return fail(HC_EINVAL, "hc_op_profile_apply: bad argument");
+  if (!op->setup_done)
+    return fail(HC_EINVAL, "hc_op_profile_apply: hc_op_setup has not been called");
+  if (prec_bits == 32 && !op->qd32)
+    return fail(HC_EINVAL, "hc_op_profile_apply: operator created without HC_PREC_FP32");
+  const bool f32 = prec_bits == 32;
+  int rc = op->ensure_ws(1, f32);
+  if (rc)
+    return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const hc::ApplyPlan & pl = f32 ? op->plan32 : op->plan64;
+  hc::ApplyDesc d{};
+  d.g = pl.g;
+  d.segs = pl.segs;
+  d.prows = (int)pl.segs.start[pl.segs.nseg];
+  d.qd = f32 ? (const void *)op->qd32 : (const void *)op->qd64;
+  d.x = x;
+  d.y = y;
+  d.part = f32 ? (void *)op->ws.part32 : (void *)op->ws.part64;
+  d.nrhs = 1;
+  d.rb = 1;
+  d.ntiles = pl.g.nt[0] * pl.g.nt[1] * pl.g.nt[2];
+  const void * tb = f32 ? (const void *)&op->tb32 : (const void *)&op->tb64;
+  std::vector<hipEvent_t> ev(3 * reps);
+  for (auto & e : ev)
+    HC_TRY_HIP(hipEventCreate(&e));
+  for (int r = 0; r < reps; r++)
+  {
+    HC_TRY_HIP(hipEventRecord(ev[3 * r], st));
+    HC_TRY_HIP(pl.cand->launch(d, nullptr, 1, d.ntiles, tb, prec_bits, st));
+    HC_TRY_HIP(hipEventRecord(ev[3 * r + 1], st));
+    HC_TRY_HIP(f32 ? hc::assemble_planes<float>(d, nullptr, 1, d.prows, st)
+                   : hc::assemble_planes<double>(d, nullptr, 1, d.prows, st));
+    HC_TRY_HIP(hipEventRecord(ev[3 * r + 2], st));
+  }
+  HC_TRY_HIP(hipEventSynchronize(ev[3 * reps - 1]));
+  double t_tile = 0, t_plane = 0;
+  for (int r = 0; r < reps; r++)
+  {
+    float a = 0, b = 0;
+    HC_TRY_HIP(hipEventElapsedTime(&a, ev[3 * r], ev[3 * r + 1]));
+    HC_TRY_HIP(hipEventElapsedTime(&b, ev[3 * r + 1], ev[3 * r + 2]));
+    t_tile += a;
+    t_plane += b;
+  }
+  for (auto & e : ev)
+    (void)hipEventDestroy(e);
+  out_us[0] = 1000.0 * t_tile / reps;
+  out_us[1] = 1000.0 * t_plane / reps;
+  return HC_OK;
+}
+
+int
+hc_op_rhs_coil(hc_op_t op, uint64_t seed, double * b, void * stream)
+{
+  if (!op || !b)
+    return fail(HC_EINVAL, "hc_op_rhs_coil: null argument");
+  hc::CoilParams cp{};
+  uint64_t k = 0;
+  auto u01 = [&]() {
+    uint64_t z = seed + (++k) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  for (int j = 0; j < 4; j++)
+  {
+    for (int r = 0; r < 3; r++)
+      cp.a[j][r] = 2.0 * u01() - 1.0;
+    for (int r = 0; r < 3; r++)
+      cp.c[j][r] = 0.3 + 0.4 * u01();
+    cp.s[j] = 0.05 + 0.1 * u01();
+  }
+  const int p = op->p, Qr = p + 2;
+  double on[hc::PMAX], ow[hc::PMAX], cn[hc::PMAX + 1];
+  gauss_legendre01(p, on, ow);
+  gauss_lobatto01(p + 1, cn);
+  gauss_legendre01(Qr, cp.qp, cp.qw);
+  for (int q = 0; q < Qr; q++)
+  {
+    double v[hc::PMAX + 1];
+    lagrange(p, on, cp.qp[q], v, nullptr);
+    for (int j = 0; j < p; j++)
+      cp.Bo[q * p + j] = v[j];
+    lagrange(p + 1, cn, cp.qp[q], v, nullptr);
+    for (int j = 0; j <= p; j++)
+      cp.Bc[q * (p + 1) + j] = v[j];
+  }
+  HC_TRY_HIP(hc::rhs_coil(op->g, cp, op->d_vert, op->h, op->x0, b, (hipStream_t)stream));
   return HC_OK;
 }
 

@@ -386,4 +386,169 @@ diagonal(const Geom & g, const Tables<double> & tb, const double * qd, double * 
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Coil-like right-hand side b_i = int J . phi_i, J = curl psi, psi a sum of
+// four Gaussian bumps a_k exp(-|x - c_k|^2 / (2 s_k^2)); Gauss rule with
+// p + 2 points per direction.  Elements of one parity colour share no DOF, so
+// each colour's launch adds its element vectors without conflicts; colours in
+// a fixed order make the result deterministic.  PEC DOFs are zeroed after.
+// ---------------------------------------------------------------------------
+__global__ void
+k_rhs_coil(Geom g, CoilParams cp, const double * __restrict__ vert, double hx, double hy, double hz, double ox,
+           double oy, double oz, int col, double * __restrict__ b)
+{
+  const int p = g.p, O = p, D = p + 1, Qr = p + 2;
+  const int cx = col & 1, cy = (col >> 1) & 1, cz = (col >> 2) & 1;
+  const int64_t hxn = (g.n[0] - cx + 1) / 2, hyn = (g.n[1] - cy + 1) / 2, hzn = (g.n[2] - cz + 1) / 2;
+  const int64_t si = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (si >= hxn * hyn * hzn)
+    return;
+  const int64_t ex = cx + 2 * (si % hxn), ey = cy + 2 * ((si / hxn) % hyn), ez = cz + 2 * (si / (hxn * hyn));
+  double be[3 * PMAX * (PMAX + 1) * (PMAX + 1)];
+  const int nd = 3 * p * (p + 1) * (p + 1);
+  for (int i = 0; i < nd; i++)
+    be[i] = 0.0;
+  const int64_t sy = g.n[0] + 1, sz = (int64_t)(g.n[0] + 1) * (g.n[1] + 1);
+  for (int qz = 0; qz < Qr; qz++)
+    for (int qy = 0; qy < Qr; qy++)
+      for (int qx = 0; qx < Qr; qx++)
+      {
+        const double ta = cp.qp[qx], tb = cp.qp[qy], tc = cp.qp[qz];
+        double X[3], J[9];
+        if (!vert)
+        {
+          X[0] = ox + (ex + ta) * hx;
+          X[1] = oy + (ey + tb) * hy;
+          X[2] = oz + (ez + tc) * hz;
+          J[0] = hx;
+          J[1] = J[2] = J[3] = 0;
+          J[4] = hy;
+          J[5] = J[6] = J[7] = 0;
+          J[8] = hz;
+        }
+        else
+        {
+          const double L[2][3] = {{1 - ta, 1 - tb, 1 - tc}, {ta, tb, tc}};
+          const double dL[2] = {-1.0, 1.0};
+          for (int i = 0; i < 9; i++)
+            J[i] = 0;
+          X[0] = X[1] = X[2] = 0;
+          for (int kc = 0; kc < 2; kc++)
+            for (int kb = 0; kb < 2; kb++)
+              for (int ka = 0; ka < 2; ka++)
+              {
+                const double * V = vert + 3 * ((ex + ka) + (ey + kb) * sy + (ez + kc) * sz);
+                const double w0 = L[ka][0] * L[kb][1] * L[kc][2];
+                const double g0 = dL[ka] * L[kb][1] * L[kc][2];
+                const double g1 = L[ka][0] * dL[kb] * L[kc][2];
+                const double g2 = L[ka][0] * L[kb][1] * dL[kc];
+                for (int r = 0; r < 3; r++)
+                {
+                  X[r] += V[r] * w0;
+                  J[r * 3 + 0] += V[r] * g0;
+                  J[r * 3 + 1] += V[r] * g1;
+                  J[r * 3 + 2] += V[r] * g2;
+                }
+              }
+        }
+        const double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                           J[2] * (J[3] * J[7] - J[4] * J[6]);
+        double Jc[3] = {0, 0, 0};
+        for (int k = 0; k < 4; k++)
+        {
+          const double d0 = X[0] - cp.c[k][0], d1 = X[1] - cp.c[k][1], d2 = X[2] - cp.c[k][2];
+          const double s2 = cp.s[k] * cp.s[k];
+          const double e = exp(-(d0 * d0 + d1 * d1 + d2 * d2) / (2 * s2));
+          const double gr0 = -d0 / s2 * e, gr1 = -d1 / s2 * e, gr2 = -d2 / s2 * e;
+          // curl(e a) = grad e x a
+          Jc[0] += gr1 * cp.a[k][2] - gr2 * cp.a[k][1];
+          Jc[1] += gr2 * cp.a[k][0] - gr0 * cp.a[k][2];
+          Jc[2] += gr0 * cp.a[k][1] - gr1 * cp.a[k][0];
+        }
+        // pulled back: J^{-1} Jc |det| w = adj(J) Jc sign(det) w
+        double adj[9];
+        adj[0] = J[4] * J[8] - J[5] * J[7];
+        adj[1] = -(J[1] * J[8] - J[2] * J[7]);
+        adj[2] = J[1] * J[5] - J[2] * J[4];
+        adj[3] = -(J[3] * J[8] - J[5] * J[6]);
+        adj[4] = J[0] * J[8] - J[2] * J[6];
+        adj[5] = -(J[0] * J[5] - J[2] * J[3]);
+        adj[6] = J[3] * J[7] - J[4] * J[6];
+        adj[7] = -(J[0] * J[7] - J[1] * J[6]);
+        adj[8] = J[0] * J[4] - J[1] * J[3];
+        const double w = cp.qw[qx] * cp.qw[qy] * cp.qw[qz] * (det >= 0 ? 1.0 : -1.0);
+        double f[3];
+        for (int r = 0; r < 3; r++)
+          f[r] = w * (adj[r * 3 + 0] * Jc[0] + adj[r * 3 + 1] * Jc[1] + adj[r * 3 + 2] * Jc[2]);
+        int l = 0;
+        for (int c = 0; c < 3; c++)
+        {
+          const int n0 = (c == 0) ? O : D, n1 = (c == 1) ? O : D, n2 = (c == 2) ? O : D;
+          for (int dz = 0; dz < n2; dz++)
+          {
+            const double vz = (c == 2) ? cp.Bo[qz * O + dz] : cp.Bc[qz * D + dz];
+            for (int dy = 0; dy < n1; dy++)
+            {
+              const double vy = (c == 1) ? cp.Bo[qy * O + dy] : cp.Bc[qy * D + dy];
+              for (int dx = 0; dx < n0; dx++, l++)
+              {
+                const double vx = (c == 0) ? cp.Bo[qx * O + dx] : cp.Bc[qx * D + dx];
+                be[l] += f[c] * vx * vy * vz;
+              }
+            }
+          }
+        }
+      }
+  int l = 0;
+  for (int c = 0; c < 3; c++)
+  {
+    const int n0 = (c == 0) ? O : D, n1 = (c == 1) ? O : D, n2 = (c == 2) ? O : D;
+    for (int dz = 0; dz < n2; dz++)
+      for (int dy = 0; dy < n1; dy++)
+        for (int dx = 0; dx < n0; dx++, l++)
+        {
+          const int64_t gid = g.off[c] + ((ez * p + dz) * g.cd[c][1] + (ey * p + dy)) * g.cd[c][0] + ex * p + dx;
+          b[gid] += be[l];
+        }
+  }
+}
+
+__global__ void
+k_fill_zero(double * __restrict__ v, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = 0.0;
+}
+
+__global__ void
+k_zero_ess(Geom g, double * __restrict__ v)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.off[3]; i += (int64_t)gridDim.x * blockDim.x)
+  {
+    int c;
+    int64_t a0, a1, a2;
+    dof_coords(g, i, c, a0, a1, a2);
+    if (is_ess(g, c, a0, a1, a2))
+      v[i] = 0.0;
+  }
+}
+
+hipError_t
+rhs_coil(const Geom & g, const CoilParams & cp, const double * vert, const double h[3], const double x0[3],
+         double * b, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_fill_zero, dim3(1024), dim3(256), 0, st, b, g.off[3]);
+  for (int col = 0; col < 8; col++)
+  {
+    const int cx = col & 1, cy = (col >> 1) & 1, cz = (col >> 2) & 1;
+    const int64_t nsub = (int64_t)((g.n[0] - cx + 1) / 2) * ((g.n[1] - cy + 1) / 2) * ((g.n[2] - cz + 1) / 2);
+    if (nsub == 0)
+      continue;
+    hipLaunchKernelGGL(k_rhs_coil, dim3(div_up(nsub, 64)), dim3(64), 0, st, g, cp, vert, h[0], h[1], h[2], x0[0],
+                       x0[1], x0[2], col, b);
+  }
+  hipLaunchKernelGGL(k_zero_ess, dim3(1024), dim3(256), 0, st, g, b);
+  return hipGetLastError();
+}
+
 } // namespace hc

@@ -88,6 +88,18 @@ hipError_t pa_setup(const Geom & g,
                     double * qd,
                     hipStream_t st);
 hipError_t cast_f32(const double * a, float * b, int64_t n, hipStream_t st);
+
+// Coil-like current J = curl psi, psi = sum_k a_k exp(-|x - c_k|^2 / (2 s_k^2)),
+// with its Gauss rule (p + 2 points) and the ND basis at those points.
+struct CoilParams
+{
+  double a[4][3], c[4][3], s[4];
+  double qp[QMAXC + 1], qw[QMAXC + 1];
+  double Bo[(QMAXC + 1) * PMAX], Bc[(QMAXC + 1) * (PMAX + 1)];
+};
+// b_i = int J . phi_i (zero on PEC DOFs)
+hipError_t rhs_coil(const Geom & g, const CoilParams & cp, const double * vert, const double h[3],
+                    const double x0[3], double * b, hipStream_t st);
 hipError_t diagonal(const Geom & g, const Tables<double> & tb, const double * qd, double * d, hipStream_t st);
 
 // y = A x (PEC eliminated) for nrhs right-hand sides (vector r at r*N).

@@ -139,6 +139,7 @@ struct hc_op_s
   int64_t E = 0, N = 0, n_ess = 0;
   bool box = true;
   double h[3] = {0, 0, 0};
+  double x0[3] = {0, 0, 0};
   std::vector<double> nu, sig;
   bool coef_dirty = true;
   bool setup_done = false;

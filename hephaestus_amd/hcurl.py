@@ -36,7 +36,7 @@ EXPORTS = [
     "hc_op_device_bytes", "hc_op_apply", "hc_op_apply_f32", "hc_op_diagonal", "hc_pcg_solve",
     "hc_op_destroy", "hc_last_error", "hc_fill_uniform", "hc_op_set_tile", "hc_op_get_tile",
     "hc_op_autotune", "hc_tile_list", "hc_op_set_rhs_block", "hc_batch_create", "hc_batch_add", "hc_batch_set_rhs_block",
-    "hc_batch_solve", "hc_batch_destroy",
+    "hc_batch_solve", "hc_batch_destroy", "hc_op_rhs_coil", "hc_op_profile_apply",
 ]
 
 
@@ -101,6 +101,8 @@ def load():
     lib.hc_batch_set_rhs_block.argtypes = [vp, C.c_int]
     lib.hc_batch_solve.argtypes = [vp, C.c_int, C.POINTER(SolveStats), vp]
     lib.hc_batch_destroy.argtypes = [vp]
+    lib.hc_op_rhs_coil.argtypes = [vp, C.c_uint64, vp, vp]
+    lib.hc_op_profile_apply.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.POINTER(C.c_double), vp]
     _lib = lib
     return lib
 
@@ -250,6 +252,26 @@ class Operator:
         _check(_lib.hc_fill_uniform(self._h, C.c_uint64(seed), _ptr(v), v.numel(), _stream(stream)),
                "hc_fill_uniform")
         return v
+
+    def profile_apply(self, x, y, reps=20, stream=None):
+        """Mean per-launch durations (us) of the tile kernel and of the tile-face
+        assembly kernel over `reps` applies, timed with HIP events on the stream."""
+        import torch
+        bits = 64 if x.dtype == torch.float64 else 32
+        _nrhs(x, self.ndofs, x.dtype)
+        out = (C.c_double * 2)()
+        _check(_lib.hc_op_profile_apply(self._h, bits, _ptr(x), _ptr(y), int(reps), out, _stream(stream)),
+               "hc_op_profile_apply")
+        return {"tile_us": out[0], "planes_us": out[1]}
+
+    def rhs_coil(self, seed, b=None, stream=None):
+        """Coil-like divergence-free current right-hand side (seeded)."""
+        import torch
+        if b is None:
+            b = torch.empty(self.ndofs, dtype=torch.float64, device="cuda")
+        _nrhs(b, self.ndofs, torch.float64)
+        _check(_lib.hc_op_rhs_coil(self._h, C.c_uint64(seed), _ptr(b), _stream(stream)), "hc_op_rhs_coil")
+        return b
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -198,6 +198,20 @@ const char * hc_last_error(void);
  * zeroed on the operator's PEC DOFs when op != NULL. */
 int hc_fill_uniform(hc_op_t op, uint64_t seed, double * v, int64_t n, void * stream);
 
+/* Timing of `reps` single-RHS applies y = A x (prec_bits 64 or 32, x/y of
+ * that precision) with HIP events around each kernel on `stream`: out_us[0] =
+ * mean duration of the tile kernel, out_us[1] = of the tile-face assembly
+ * kernel.  No reference counterpart (measurement entry point). */
+int hc_op_profile_apply(hc_op_t op, int prec_bits, const void * x, void * y, int reps, double * out_us,
+                        void * stream);
+/* Coil-like right-hand side b_i = int J . phi_i of a divergence-free current
+ * J = curl psi, psi = sum_{k<4} a_k exp(-|x - c_k|^2 / (2 s_k^2)) with
+ * a_k in U[-1,1]^3, c_k in U[0.3,0.7]^3, s_k in U[0.05,0.15] drawn in that
+ * order from splitmix64(seed); Gauss rule with p+2 points per direction; zero
+ * on PEC DOFs.  b: device, N doubles.  Plays the role of the coil source
+ * terms of the reference's examples (DivFreeSource, src/sources/div_free_source.cpp). */
+int hc_op_rhs_coil(hc_op_t op, uint64_t seed, double * b, void * stream);
+
 #ifdef __cplusplus
 }
 #endif

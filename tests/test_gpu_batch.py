@@ -174,3 +174,23 @@ def test_batch_errors(torch_cuda, hc):
     empty = hc.Batch()
     with pytest.raises(hc.HcError):
         empty.solve(hc.SOLVE_FP64)
+
+
+@pytest.mark.parametrize("p,trilinear", [(1, False), (2, True), (3, False), (3, True)])
+def test_rhs_coil_vs_oracle(torch_cuda, hc, O, p, trilinear):
+    n = (6, 5, 7)
+    print(f"case coil rhs p={p} trilinear={trilinear}", flush=True)
+    x0, length = (-0.1, 0.05, 0.0), (1.2, 0.9, 1.1)
+    m = O.Mesh(*n, x0=x0, length=length)
+    vert = None
+    if trilinear:
+        rng = np.random.default_rng(5)
+        vert = m.box_vertices()
+        vert = vert + 0.15 * rng.uniform(-1, 1, vert.shape) * np.array(length) / np.array(n)
+        m = O.Mesh(*n, vert=vert)
+    ess = O.ess_mask(m, p)
+    b_ref = O.rhs_coil(m, p, 7, ess)
+    op = hc.Operator(n, p, 1.0, 1.0, vert=vert, x0=x0, length=length)
+    b = op.rhs_coil(7).cpu().numpy()
+    assert np.linalg.norm(b_ref) > 0
+    assert _rel(b, b_ref) <= 1e-13
