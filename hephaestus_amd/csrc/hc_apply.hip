@@ -70,7 +70,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   const int rb = BATCH ? ad.rb : 1;
   const int ngrp = BATCH ? (ad.nrhs + rb - 1) / rb : ad.nrhs;
   const int r_begin = (wl % ngrp) * rb;
-  const int r_end = min(ad.nrhs, r_begin + rb);
+  const int r_end = BATCH ? min(ad.nrhs, r_begin + rb) : r_begin + 1;
   const int tcur = wl / ngrp;
   if (!BATCH && ad.done && ad.done[r_begin * ad.done_stride])
     return;
@@ -406,11 +406,22 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   // ======================= quadrature points =======================
   if (MODE == 0)
     load_qf(e0, ne);
+  // term weights (batched kernel only; the single-instance kernel applies A)
+  const T msc = BATCH ? (T)ad.mscale : T(1), csc = BATCH ? (T)ad.cscale : T(1);
 #pragma unroll
   for (int j = 0; j < QH; j++)
   {
-    const T u0 = U[0][j], u1 = U[1][j], u2 = U[2][j];
-    const T c0 = Cu[0][j], c1 = Cu[1][j], c2 = Cu[2][j];
+    T u0 = U[0][j], u1 = U[1][j], u2 = U[2][j];
+    T c0 = Cu[0][j], c1 = Cu[1][j], c2 = Cu[2][j];
+    if constexpr (BATCH)
+    {
+      u0 *= msc;
+      u1 *= msc;
+      u2 *= msc;
+      c0 *= csc;
+      c1 *= csc;
+      c2 *= csc;
+    }
     U[0][j] = qf[j][0] * u0 + qf[j][1] * u1 + qf[j][2] * u2;
     U[1][j] = qf[j][1] * u0 + qf[j][3] * u1 + qf[j][4] * u2;
     U[2][j] = qf[j][2] * u0 + qf[j][4] * u1 + qf[j][5] * u2;
@@ -882,6 +893,38 @@ apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T *
     return e;
   return assemble_planes<T>(d, nullptr, 1, d.prows * nrhs, st);
 }
+
+template <typename T>
+hipError_t
+apply_scaled(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T * qd, const T * x, T * y,
+             T * part, double mscale, double cscale, const TileCandidate * cand, ApplyDesc * desc_dev,
+             hipStream_t st)
+{
+  ApplyDesc d{};
+  d.g = g;
+  d.segs = segs;
+  d.prows = (int)segs.start[segs.nseg];
+  d.qd = qd;
+  d.x = x;
+  d.y = y;
+  d.part = part;
+  d.nrhs = 1;
+  d.rb = 1;
+  d.ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+  d.mscale = mscale;
+  d.cscale = cscale;
+  // term weights are honoured by the batched kernel (descriptor in device memory)
+  hipError_t e = hipMemcpyAsync(desc_dev, &d, sizeof(d), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess)
+    return e;
+  e = cand->launch(d, desc_dev, 1, d.ntiles, &tb, sizeof(T) == 8 ? 64 : 32, st);
+  if (e != hipSuccess)
+    return e;
+  return assemble_planes<T>(d, nullptr, 1, d.prows, st);
+}
+template hipError_t apply_scaled<double>(const Geom &, const PlaneSegs &, const Tables<double> &, const double *,
+                                         const double *, double *, double *, double, double,
+                                         const TileCandidate *, ApplyDesc *, hipStream_t);
 
 int
 default_rhs_block(int ntiles, int nrhs)

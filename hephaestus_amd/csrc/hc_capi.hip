@@ -421,6 +421,22 @@ hc_op_apply_f32(hc_op_t op, const float * x, float * y, int nrhs, void * stream)
 }
 
 int
+hc_op_apply_curl(hc_op_t op, const double * x, double * y, void * stream)
+{
+  if (!op || !x || !y)
+    return fail(HC_EINVAL, "hc_op_apply_curl: null argument");
+  if (!op->setup_done)
+    return fail(HC_EINVAL, "hc_op_apply_curl: hc_op_setup has not been called");
+  int rc = op->ensure_ws(1, false);
+  if (rc)
+    return rc;
+  // the quadrature data hold dt * nu: scale the curl term back by 1/dt
+  HC_TRY_HIP(hc::apply_scaled<double>(op->plan64.g, op->plan64.segs, op->tb64, op->qd64, x, y, op->ws.part64, 0.0,
+                                      1.0 / op->last_dt, op->plan64.cand, op->desc_dev, (hipStream_t)stream));
+  return HC_OK;
+}
+
+int
 hc_op_set_rhs_block(hc_op_t op, int rhs_block)
 {
   if (!op || rhs_block < 0)
@@ -666,6 +682,24 @@ hc_op_rhs_coil(hc_op_t op, uint64_t seed, double * b, void * stream)
       cp.Bc[q * (p + 1) + j] = v[j];
   }
   HC_TRY_HIP(hc::rhs_coil(op->g, cp, op->d_vert, op->h, op->x0, b, (hipStream_t)stream));
+  return HC_OK;
+}
+
+int
+hc_axpby(int64_t n, double a, const double * x, double b, double * y, void * stream)
+{
+  if (n < 0 || (n > 0 && (!x || !y)))
+    return fail(HC_EINVAL, "hc_axpby: bad argument");
+  HC_TRY_HIP(hc::axpby(n, a, x, b, y, (hipStream_t)stream));
+  return HC_OK;
+}
+
+int
+hc_op_zero_ess(hc_op_t op, double * v, void * stream)
+{
+  if (!op || !v)
+    return fail(HC_EINVAL, "hc_op_zero_ess: null argument");
+  HC_TRY_HIP(hc::zero_ess(op->g, v, (hipStream_t)stream));
   return HC_OK;
 }
 

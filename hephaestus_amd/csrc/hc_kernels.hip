@@ -551,4 +551,28 @@ rhs_coil(const Geom & g, const CoilParams & cp, const double * vert, const doubl
   return hipGetLastError();
 }
 
+__global__ void
+k_axpby(int64_t n, double a, const double * __restrict__ x, double b, double * __restrict__ y)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + (b == 0.0 ? 0.0 : b * y[i]);
+}
+
+hipError_t
+axpby(int64_t n, double a, const double * x, double b, double * y, hipStream_t st)
+{
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_axpby, dim3((unsigned)std::min<int64_t>(div_up(n, 256), 4096)), dim3(256), 0, st, n, a, x,
+                     b, y);
+  return hipGetLastError();
+}
+
+hipError_t
+zero_ess(const Geom & g, double * v, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_zero_ess, dim3(1024), dim3(256), 0, st, g, v);
+  return hipGetLastError();
+}
+
 } // namespace hc

@@ -43,6 +43,8 @@ struct ApplyDesc
   int ntiles;
   int pw0;         // first plane-assembly work item
   int prows;       // plane rows per rhs
+  double mscale = 1.0; // weight of the mass term (0: curl-curl part only)
+  double cscale = 1.0; // weight of the curl-curl term
 };
 
 // Locate the instance of a flattened work item: largest i with key(i) <= w.
@@ -97,6 +99,10 @@ struct CoilParams
   double qp[QMAXC + 1], qw[QMAXC + 1];
   double Bo[(QMAXC + 1) * PMAX], Bc[(QMAXC + 1) * (PMAX + 1)];
 };
+// y = a x + b y (b == 0: y = a x)
+hipError_t axpby(int64_t n, double a, const double * x, double b, double * y, hipStream_t st);
+// v_i = 0 on the PEC DOFs of g
+hipError_t zero_ess(const Geom & g, double * v, hipStream_t st);
 // b_i = int J . phi_i (zero on PEC DOFs)
 hipError_t rhs_coil(const Geom & g, const CoilParams & cp, const double * vert, const double h[3],
                     const double x0[3], double * b, hipStream_t st);
@@ -124,6 +130,13 @@ hipError_t apply_op(const Geom & g,
                     ApplyDesc * desc_dev = nullptr);
 // Right-hand sides per work item for a batch of nrhs on ntiles tiles.
 int default_rhs_block(int ntiles, int nrhs);
+// y = (mscale M + cscale K') x (K' = the curl part of the quadrature data,
+// i.e. dt K) with the batched kernel; its descriptor is staged in desc_dev.
+// part: 4 N scratch.
+template <typename T>
+hipError_t apply_scaled(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T * qd, const T * x,
+                        T * y, T * part, double mscale, double cscale, const TileCandidate * cand,
+                        ApplyDesc * desc_dev, hipStream_t st);
 
 // Fill the plane-assembly fields of a descriptor (segments, rows per rhs).
 void plane_rows(ApplyDesc & d);

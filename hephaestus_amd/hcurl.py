@@ -36,7 +36,7 @@ EXPORTS = [
     "hc_op_device_bytes", "hc_op_apply", "hc_op_apply_f32", "hc_op_diagonal", "hc_pcg_solve",
     "hc_op_destroy", "hc_last_error", "hc_fill_uniform", "hc_op_set_tile", "hc_op_get_tile",
     "hc_op_autotune", "hc_tile_list", "hc_op_set_rhs_block", "hc_batch_create", "hc_batch_add", "hc_batch_set_rhs_block",
-    "hc_batch_solve", "hc_batch_destroy", "hc_op_rhs_coil", "hc_op_profile_apply",
+    "hc_batch_solve", "hc_batch_destroy", "hc_op_rhs_coil", "hc_op_profile_apply", "hc_op_apply_curl", "hc_axpby", "hc_op_zero_ess",
 ]
 
 
@@ -102,6 +102,9 @@ def load():
     lib.hc_batch_solve.argtypes = [vp, C.c_int, C.POINTER(SolveStats), vp]
     lib.hc_batch_destroy.argtypes = [vp]
     lib.hc_op_rhs_coil.argtypes = [vp, C.c_uint64, vp, vp]
+    lib.hc_op_apply_curl.argtypes = [vp, vp, vp, vp]
+    lib.hc_axpby.argtypes = [C.c_int64, C.c_double, vp, C.c_double, vp, vp]
+    lib.hc_op_zero_ess.argtypes = [vp, vp, vp]
     lib.hc_op_profile_apply.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.POINTER(C.c_double), vp]
     _lib = lib
     return lib
@@ -199,6 +202,16 @@ class Operator:
         if y is None:
             y = torch.empty_like(x)
         _check(_lib.hc_op_apply(self._h, _ptr(x), _ptr(y), nrhs, _stream(stream)), "hc_op_apply")
+        return y
+
+    def apply_curl(self, x, y=None, stream=None):
+        """y = K x, the curl-curl part alone (nu curl u, curl w); x of shape (N,)."""
+        import torch
+        if _nrhs(x, self.ndofs, torch.float64) != 1:
+            raise ValueError("apply_curl takes one vector")
+        if y is None:
+            y = torch.empty_like(x)
+        _check(_lib.hc_op_apply_curl(self._h, _ptr(x), _ptr(y), _stream(stream)), "hc_op_apply_curl")
         return y
 
     def apply_f32(self, x, y=None, stream=None):

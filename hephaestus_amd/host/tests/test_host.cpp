@@ -1,0 +1,190 @@
+// Tests of the host C++ layer.
+//   test_host cpu                         framework logic, no device needed
+//   test_host gpu <dir> <n> <p> <steps> <dt> <precision>
+//                                         backward-Euler steps of the H(curl)
+//                                         curl-curl system on the device; writes
+//                                         u after each step to <dir>/u_<k>.bin
+#include "hephaestus/equation_system.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+
+using namespace hephaestus;
+
+static int g_fail = 0;
+#define EXPECT(cond)                                                                                             \
+  do                                                                                                             \
+  {                                                                                                              \
+    if (!(cond))                                                                                                 \
+    {                                                                                                            \
+      std::fprintf(stderr, "FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);                                     \
+      g_fail++;                                                                                                  \
+    }                                                                                                            \
+  } while (0)
+
+template <typename F>
+static bool
+throws(F f)
+{
+  try
+  {
+    f();
+  }
+  catch (const Error &)
+  {
+    return true;
+  }
+  return false;
+}
+
+static void
+test_cpu()
+{
+  // InputParameters
+  InputParameters p;
+  p.SetParam("CoefficientName", std::string("sigma"));
+  p.SetParam("MaxIter", 50);
+  EXPECT(p.GetParam<std::string>("CoefficientName") == "sigma");
+  EXPECT(p.GetParam<int>("MaxIter") == 50);
+  EXPECT(p.GetOptionalParam<double>("Tolerance", 1e-9) == 1e-9);
+  EXPECT(throws([&] { p.GetParam<std::string>("Missing"); }));
+  EXPECT(throws([&] { p.GetParam<double>("MaxIter"); })); // wrong type
+  // NamedFieldsMap
+  NamedFieldsMap<ConstantCoefficient> m;
+  m.Register("a", std::make_shared<ConstantCoefficient>(2.0));
+  EXPECT(m.Has("a") && !m.Has("b") && m.Get("a")->constant == 2.0);
+  EXPECT(throws([&] { m.Get("b"); }));
+  // coefficients on a two-region mesh
+  StructuredHexMesh mesh = StructuredHexMesh::TwoRegionCube(8);
+  int inner = 0;
+  for (int a : mesh.attributes)
+    inner += (a == 2);
+  EXPECT(inner == 4 * 4 * 4);
+  Coefficients coefs;
+  coefs._scalars.Register("sigma", std::make_shared<PWConstCoefficient>(std::vector<double>{1e-6, 1e4}));
+  coefs._scalars.Register("nu", std::make_shared<ConstantCoefficient>(1.0));
+  auto sv = coefs._scalars.Get("sigma")->ElementValues(mesh);
+  int64_t e_in = (4 * 8 + 4) * 8 + 4, e_out = 0;
+  EXPECT(sv[e_in] == 1e4 && sv[e_out] == 1e-6);
+  ConstantCoefficient dt(0.25);
+  TransformedCoefficient dtnu(&dt, coefs._scalars.Get("nu"), prodFunc);
+  EXPECT(dtnu.Eval(mesh, 3) == 0.25);
+  TransformedCoefficient frac(coefs._scalars.Get("nu"), &dt, fracFunc);
+  EXPECT(frac.Eval(mesh, 3) == 4.0);
+  // FE space sizes: N = 3 (np)(np+1)^2 on an n^3 cube
+  FESpace fes(&mesh, "ND_3D_P2");
+  EXPECT(fes.Order() == 2 && fes.GetTrueVSize() == 3 * 16 * 17 * 17);
+  EXPECT(throws([&] { FESpace bad(&mesh, "H1_3D_P1"); }));
+  EXPECT(throws([&] { FESpace bad(&mesh, "ND_3D_P7"); }));
+  // boundary conditions
+  BCMap bcs;
+  bcs.Register("pec_x", std::make_shared<VectorDirichletBC>("pec_x", HC_FACE_XMIN | HC_FACE_XMAX));
+  bcs.Register("pec_yz", std::make_shared<VectorDirichletBC>("pec_yz", HC_FACE_ALL & ~(HC_FACE_XMIN | HC_FACE_XMAX)));
+  EXPECT(bcs.EssentialFaces() == HC_FACE_ALL);
+  // kernel registration of the curl-curl formulation
+  InputParameters esp;
+  esp.SetParam("HCurlVarName", std::string("h"));
+  esp.SetParam("AlphaCoefName", std::string("nu"));
+  esp.SetParam("BetaCoefName", std::string("sigma"));
+  CurlCurlEquationSystem es(esp);
+  es.AddKernels();
+  EXPECT(es._test_var_names.size() == 1 && es._test_var_names[0] == "dh_dt");
+  EXPECT(es._trial_var_names.size() == 1 && es._trial_var_names[0] == "h");
+  EXPECT(es._trial_var_time_derivative_names.size() == 1);
+  EXPECT(throws([&] {
+    InputParameters bad;
+    CurlCurlKernel k(bad); // CoefficientName missing
+  }));
+  // without a device, allocation fails with an Error (not an abort)
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    EXPECT(throws([&] { DeviceVector v(16); }));
+  std::printf("cpu tests: %s\n", g_fail ? "FAILED" : "ok");
+}
+
+static void
+write_vec(const std::string & path, const std::vector<double> & v)
+{
+  std::ofstream f(path, std::ios::binary);
+  f.write((const char *)v.data(), sizeof(double) * v.size());
+}
+
+static void
+test_gpu(const std::string & dir, int n, int order, int steps, double dt, const std::string & precision)
+{
+  StructuredHexMesh mesh = StructuredHexMesh::TwoRegionCube(n);
+  FESpaces fespaces;
+  fespaces.Register("HCurl", std::make_shared<FESpace>(&mesh, "ND_3D_P" + std::to_string(order)));
+  GridFunctions gfs;
+  gfs.Register("h", std::make_shared<GridFunction>(fespaces.Get("HCurl")));
+  gfs.Register("dh_dt", std::make_shared<GridFunction>(fespaces.Get("HCurl")));
+  Coefficients coefs;
+  coefs._scalars.Register("nu", std::make_shared<PWConstCoefficient>(std::vector<double>{1.0, 0.5}));
+  coefs._scalars.Register("sigma", std::make_shared<PWConstCoefficient>(std::vector<double>{1e-2, 1e2}));
+  BCMap bcs;
+  bcs.Register("pec", std::make_shared<VectorDirichletBC>("pec", HC_FACE_ALL));
+  Sources sources;
+  sources.Register("coil", std::make_shared<CoilSource>(11));
+  InputParameters esp;
+  esp.SetParam("HCurlVarName", std::string("h"));
+  esp.SetParam("AlphaCoefName", std::string("nu"));
+  esp.SetParam("BetaCoefName", std::string("sigma"));
+  CurlCurlEquationSystem es(esp);
+  es.AddKernels();
+  es.Init(gfs, fespaces, bcs, coefs);
+  InputParameters solver;
+  solver.SetParam("Tolerance", 1e-12);
+  solver.SetParam("MaxIter", 20000);
+  solver.SetParam("Precision", precision);
+  TimeDomainEquationSystemProblemOperator op(es, gfs, bcs, coefs, sources, solver);
+  for (int k = 0; k < steps; k++)
+  {
+    const hc_solve_stats & st = op.Step(dt, "h");
+    std::printf("step %d: iterations %d outer %d rel_res %.3e converged %d\n", k, st.iterations,
+                st.outer_iterations, st.rel_res, st.converged);
+    EXPECT(st.converged == 1);
+    write_vec(dir + "/u_" + std::to_string(k) + ".bin", gfs.Get("h")->Download());
+  }
+  // dt change re-assembles (dt rule), an unchanged dt does not
+  BilinearForm & blf = es.GetGradient();
+  es.SetTimeStep(dt);
+  blf.Assemble();
+  EXPECT(blf.LastAssembleWork() == 0);
+  es.SetTimeStep(dt * 0.5);
+  EXPECT(blf.LastAssembleWork() == 1);
+  std::printf("gpu tests: %s\n", g_fail ? "FAILED" : "ok");
+}
+
+int
+main(int argc, char ** argv)
+{
+  if (argc < 2)
+  {
+    std::fprintf(stderr, "usage: test_host cpu | gpu <dir> <n> <p> <steps> <dt> <precision>\n");
+    return 2;
+  }
+  try
+  {
+    if (!std::strcmp(argv[1], "cpu"))
+      test_cpu();
+    else if (!std::strcmp(argv[1], "gpu") && argc >= 8)
+      test_gpu(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), std::atof(argv[6]), argv[7]);
+    else
+    {
+      std::fprintf(stderr, "bad arguments\n");
+      return 2;
+    }
+  }
+  catch (const Error & e)
+  {
+    std::fprintf(stderr, "hephaestus::Error: %s\n", e.what());
+    return 1;
+  }
+  return g_fail ? 1 : 0;
+}

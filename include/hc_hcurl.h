@@ -133,6 +133,12 @@ int64_t hc_op_device_bytes(hc_op_t op);
  * right-hand sides, reading each tile's quadrature data once per group. */
 int hc_op_apply(hc_op_t op, const double * x, double * y, int nrhs, void * stream);
 int hc_op_apply_f32(hc_op_t op, const float * x, float * y, int nrhs, void * stream);
+/* y = K x with K the curl-curl part alone, (nu curl u, curl w), same PEC
+ * convention as hc_op_apply (y_ess = x_ess).  The operator of
+ * WeakCurlCurlKernel (src/kernels/weak_curl_curl_kernel.cpp:22-31: the
+ * right-hand side term -K u_n of a backward-Euler step), applied from the
+ * quadrature data of hc_op_setup (no second assembly). */
+int hc_op_apply_curl(hc_op_t op, const double * x, double * y, void * stream);
 /* Right-hand sides per work item of the multi-RHS apply and solve (0 =
  * automatic: the fewest groups that still give the GPU enough work items).
  * Results do not depend on it. */
@@ -204,6 +210,13 @@ int hc_fill_uniform(hc_op_t op, uint64_t seed, double * v, int64_t n, void * str
  * kernel.  No reference counterpart (measurement entry point). */
 int hc_op_profile_apply(hc_op_t op, int prec_bits, const void * x, void * y, int reps, double * out_us,
                         void * stream);
+/* Device vector helpers of the host layer.
+ * hc_axpby: y = a x + b y (n doubles; b == 0 ignores y's old contents).
+ * hc_op_zero_ess: v_i = 0 on the operator's PEC DOFs -- the right-hand side
+ * part of FormLinearSystem with zero tangential boundary values
+ * (src/equation_systems/equation_system.cpp:132-190, B_ess = x_bc = 0). */
+int hc_axpby(int64_t n, double a, const double * x, double b, double * y, void * stream);
+int hc_op_zero_ess(hc_op_t op, double * v, void * stream);
 /* Coil-like right-hand side b_i = int J . phi_i of a divergence-free current
  * J = curl psi, psi = sum_{k<4} a_k exp(-|x - c_k|^2 / (2 s_k^2)) with
  * a_k in U[-1,1]^3, c_k in U[0.3,0.7]^3, s_k in U[0.05,0.15] drawn in that
