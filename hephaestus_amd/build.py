@@ -19,10 +19,10 @@ LIB = os.path.join(HERE, "libhcurl.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["hc_kernels.hip", "hc_apply.hip", "hc_pcg.hip", "hc_capi.hip", "hc_solve.hip"]
+SOURCES = ["hc_kernels.hip", "hc_apply.hip", "hc_pcg.hip", "hc_capi.hip", "hc_solve.hip", "hc_serve.hip"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-munsafe-fp-atomics", "-Wall", "-Wno-unused-result", "-Wno-unused-function"]
+         "-munsafe-fp-atomics", "-pthread", "-Wall", "-Wno-unused-result", "-Wno-unused-function"]
 
 
 def _compile(src: str) -> str:
@@ -39,7 +39,7 @@ def build(verbose: bool = True) -> str:
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp,
+    cmd = [HIPCC, "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", *objs, "-o", tmp,
            "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

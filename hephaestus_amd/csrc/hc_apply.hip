@@ -72,7 +72,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   const int r_begin = (wl % ngrp) * rb;
   const int r_end = BATCH ? min(ad.nrhs, r_begin + rb) : r_begin + 1;
   const int tcur = wl / ngrp;
-  if (!BATCH && ad.done && ad.done[r_begin * ad.done_stride])
+  if (!BATCH && ad.done && ad.done[r_begin * ad.done_stride] != ad.gate)
     return;
   const Geom & g = ad.g;
   const T * __restrict__ qd = (const T *)ad.qd;
@@ -201,7 +201,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
 
   for (int rhs = r_begin; rhs < r_end; rhs++)
   {
-  if (BATCH && ad.done && ad.done[rhs * ad.done_stride])
+  if (BATCH && ad.done && ad.done[rhs * ad.done_stride] != ad.gate)
     continue;
   x = (const T *)ad.x + (int64_t)rhs * g.off[3];
   y = (T *)ad.y + (int64_t)rhs * g.off[3];

@@ -152,7 +152,7 @@ k_assemble_planes(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc)
   const int local = w - d.pw0;
   const int rhs = local / d.prows;
   const int row0 = local - rhs * d.prows;
-  if (d.done && d.done[rhs * d.done_stride])
+  if (d.done && d.done[rhs * d.done_stride] != d.gate)
     return;
   const Geom & g = d.g;
   int s = 0;

@@ -35,8 +35,9 @@ struct ApplyDesc
   const void * x;
   void * y;
   void * part;     // 4 N per rhs
-  const int * done;// per-rhs flags at done[r * done_stride], or null
-  int done_stride;
+  const int * done;// per-rhs flags at done[r * done_stride], or null: rhs r
+  int done_stride; // is applied only while its flag equals `gate`
+  int gate = 0;
   int nrhs;
   int rb;          // right-hand sides per work item of the batched kernels
   int w0;          // first tile work item
@@ -173,14 +174,29 @@ struct PcgState
   double bb;     // (b, b)
   double rr;     // (r, r) of the last iterate
   double tol2;   // tolerance^2 relative to bb
-  int done;      // 1 once converged, out of iterations, or NaN met
+  int done;      // inner: 1 once converged, out of iterations, or NaN met
   int iters;     // inner: iterations of this PCG; outer: refinement steps
   int status;    // 0, or HC_ENAN
   int it0;       // inner: value of the iteration counter at the start
   int maxit;     // inner: iteration cap
   int inner;     // outer: inner iterations summed over the refinement steps
+  int phase;     // outer: PH_* below
+  int first;     // outer: 1 while the residual of the initial guess is due
 };
 constexpr int STATE_STRIDE_INT = sizeof(PcgState) / sizeof(int);
+
+// Phases of a solve slot (outer state).  Each round of the engine moves every
+// slot through: PH_FIRST/finished inner -> PH_RESID (FP64 residual of x, the
+// refinement update first when an FP32 inner solve finished) -> PH_DONE or
+// PH_INIT (inner PCG start) -> PH_INNER (iterations until the inner done flag).
+enum SlotPhase
+{
+  PH_FIRST = 1,
+  PH_RESID = 2,
+  PH_INIT = 3,
+  PH_INNER = 4,
+  PH_DONE = 5
+};
 
 // One right-hand side of one operator instance.  The inner vectors (ib..iq,
 // idinv) have the precision of the inner solve (FP64 solve: b, x, r64, p64,
@@ -213,21 +229,26 @@ struct SlotTable
   int nb_max; // max red_blocks(n) over the slots
 };
 
-// r = b - A x (or b with x = 0 when zero_guess), p = D^-1 r, and the inner
-// states (tolerance: slot rtol, or the refinement rule when inner_floor > 0).
-// gate: slots whose outer state is done are skipped and marked done.
+// Phase moves at the start of a round (one thread per slot).
+hipError_t phase_prep(const SlotTable & t, hipStream_t st);
+// x += ||r64|| * (double)ix for slots in PH_RESID after an FP32 inner solve.
+hipError_t ir_update(const SlotTable & t, hipStream_t st);
+// r64 = b - q64 for slots in PH_RESID; outer state update and the move to
+// PH_DONE or PH_INIT.
+hipError_t residual64(const SlotTable & t, int max_outer, hipStream_t st);
+// ib = (float)(r64 / ||r64||) for slots in PH_INIT (refinement).
+hipError_t to_f32_scaled(const SlotTable & t, hipStream_t st);
+// Inner PCG start for slots in PH_INIT: r = b - A x (or b with x = 0 when
+// zero_guess), p = D^-1 r; tolerance: slot rtol, or the refinement rule when
+// inner_floor > 0; moves them to PH_INNER.
 template <typename T>
-hipError_t pcg_init(const SlotTable & t, const int * git, bool zero_guess, bool gate, double inner_floor,
-                    hipStream_t st);
-// One PCG iteration after q = A p; advances the counter *git.
+hipError_t pcg_init(const SlotTable & t, const int * git, bool zero_guess, double inner_floor, hipStream_t st);
+// One PCG iteration after q = A p for slots whose inner done flag is clear;
+// advances the counter *git.
 template <typename T>
 hipError_t pcg_iterate(const SlotTable & t, int * git, hipStream_t st);
-// r64 = b - q64 and the outer states (see k_residual_fin).
-hipError_t residual64(const SlotTable & t, bool set_bb, bool gate, bool accumulate, int max_outer, hipStream_t st);
-// ib = (float)(r64 / ||r64||) for slots whose outer state is not done.
-hipError_t to_f32_scaled(const SlotTable & t, hipStream_t st);
-// x += ||r64|| * (double)ix for slots whose outer state is not done.
-hipError_t ir_update(const SlotTable & t, hipStream_t st);
+// Copies the outer states of the slots into dst[0..nslots).
+hipError_t gather_states(const SlotTable & t, PcgState * dst, hipStream_t st);
 hipError_t zero_int(int * v, int n, hipStream_t st);
 hipError_t invert_diag(const double * d, double * dinv64, float * dinv32, int64_t n, hipStream_t st);
 hipError_t fill_uniform(const Geom * g, uint64_t seed, double * v, int64_t n, hipStream_t st);

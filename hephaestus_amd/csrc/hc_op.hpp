@@ -194,16 +194,24 @@ struct EngineInst
   int maxit;
   const Workspace * ws;
   int slot0;
+  // per-rhs device states and reduction partials (nrhs each); null: owned by
+  // the engine (then they do not survive a later set())
+  PcgState * in = nullptr;
+  PcgState * out = nullptr;
+  double * part = nullptr; // nrhs * 4 * RED_BLOCKS
 };
 
 struct SlotStats
 {
   int inner;     // inner PCG iterations (summed over refinement steps / restarts)
-  int outer;     // refinement steps or residual-replacement restarts
+  int outer;     // refinement steps or PCG runs
   int status;
   int converged;
   double rel_res; // final FP64 true relative residual
 };
+
+// Initial values of the states of a slot that starts a solve.
+void initial_states(PcgState & in, PcgState & out);
 
 class Engine
 {
@@ -213,13 +221,26 @@ public:
   Engine(const Engine &) = delete;
   Engine & operator=(const Engine &) = delete;
 
-  // Replace the instance list; per order the tile variants cand64/cand32
-  // (nullptr: each instance's own plan) and the right-hand-side block of the
-  // batched apply (0: default).
-  int set(const std::vector<EngineInst> & inst, int mode, int rhs_block = 0, bool force_batched = false);
-  // Run the solve; fills per-slot statistics; *ms = device time.
+  // Replace the instance list and rebuild the device tables (no work of the
+  // engine may be in flight).  Per order the
+  // tile variant is cands64/cands32[order] when given, else the plan of the
+  // first instance of that order.  rhs_block: right-hand sides per work item
+  // of the batched apply (0: default).
+  int set(const std::vector<EngineInst> & inst, int mode, int rhs_block = 0, bool force_batched = false,
+          const TileCandidate * const * cands64 = nullptr, const TileCandidate * const * cands32 = nullptr);
+  // One round: phase moves, FP64 residuals and refinement updates, inner
+  // starts, then `iters` PCG iterations of every slot in an inner solve.
+  int round(int iters, hipStream_t st);
+  // Copies the outer states of all slots to host_states() (synchronises).
+  int gather(hipStream_t st);
+  const PcgState * host_states() const { return host_; }
+  // Zero the iteration counter (before the first round of a fresh batch).
+  int reset_counter(hipStream_t st);
+  // Batch solve from fresh states to completion; per-slot statistics.
   int solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double * ms);
   int nslots() const { return nslots_; }
+  int max_outer() const;
+  static SlotStats stats_of(const PcgState & out);
 
 private:
   struct ApplyGroup
@@ -235,24 +256,20 @@ private:
   };
   enum Kind
   {
-    K_PQ = 0,  // A p -> q in the inner precision, gated by the inner state
-    K_XQ_G,    // A x -> q64 (FP64), gated by the outer state
-    K_XQ_ALL,  // A x -> q64 (FP64), every slot
+    K_PQ = 0, // A p -> q in the inner precision, slots with a running inner solve
+    K_XQ,     // A x -> q64 (FP64), slots in PH_RESID
     K_COUNT
   };
   int apply(Kind k, hipStream_t st);
-  int read_states(PcgState * dev, hipStream_t st);
-  bool all_done(const PcgState * h) const;
 
   std::vector<EngineInst> inst_;
   int nslots_ = 0;
   int mode_ = 0;
   DevBlock mem_;
   SlotTable table_{};
-  PcgState * in_ = nullptr;
-  PcgState * out_ = nullptr;
   int * git_ = nullptr;
-  PcgState * host_ = nullptr; // pinned, nslots states
+  PcgState * gathered_ = nullptr; // device, nslots
+  PcgState * host_ = nullptr;     // pinned, nslots
   int host_cap_ = 0;
   std::vector<ApplyGroup> groups_[K_COUNT];
 };

@@ -1,11 +1,11 @@
 // Jacobi-preconditioned CG kernels: the device side of DefaultJacobiPCGSolver
 // (src/solvers/hephaestus_solvers.hpp:34-57: HyprePCG + HypreDiagScale [ext]),
-// batched over solve slots.
+// and the FP64 outer residual / refinement steps, batched over solve slots.
 //
 // A slot is one right-hand side of one operator instance (its own length n,
-// vectors, Jacobi diagonal, state and reduction partials), read from a device
-// table; the grid is (blocks per slot, slots).  One iteration is q = A p (the
-// operator kernels, gated by each slot's done flag), then
+// vectors, Jacobi diagonal, states and reduction partials), read from a device
+// table; the grid is (blocks per slot, slots).  One PCG iteration is q = A p
+// (the operator kernels, gated by each slot's inner done flag), then
 //   pcg_dot_pq     partial (p, q) per block
 //   pcg_update_xr  alpha = gamma / (p,q);  x += alpha p;  r -= alpha q;
 //                  partials (r, D^-1 r), (r, r)          -- one pass over x,p,r,q,d
@@ -13,11 +13,11 @@
 // Every scalar stays on the device (PcgState).  Reductions are two-stage with
 // a block count fixed by n and a fixed summation order, so the result of a
 // slot is bitwise independent of the other slots of the launch.  Dot products
-// accumulate in FP64 for FP32 vectors too.  A converged slot sets its done
-// flag and every later kernel skips it, so the host checks the flags only
-// every few iterations and slots may join a running batch: the iteration
-// index of a slot is (device counter) - (its start), the counter being
-// advanced by pcg_dot_pq.
+// accumulate in FP64 for FP32 vectors too.  Each slot carries a phase (outer
+// state) that gates the outer kernels, and the inner done flag gates the
+// iteration kernels, so slots at different stages -- and slots that join a
+// running batch -- share every launch; the iteration index of a slot is
+// (device counter) - (its start), the counter being advanced by pcg_dot_pq.
 #include "hc_kernels.hpp"
 
 namespace hc
@@ -91,14 +91,14 @@ bad(double v)
 // PCG
 // ---------------------------------------------------------------------------
 // r = b - q (or b when zero_guess, then x = 0);  p = D^-1 r;
-// partials (b,b), (r, D^-1 r), (r,r).  gate: skip slots whose outer solve is done.
+// partials (b,b), (r, D^-1 r), (r,r); slots in PH_INIT.
 template <typename T>
 __global__ void __launch_bounds__(RED_THREADS)
-k_pcg_init(const SolveSlot * __restrict__ slots, int zero_guess, int gate)
+k_pcg_init(const SolveSlot * __restrict__ slots, int zero_guess)
 {
   const SolveSlot & sl = slots[blockIdx.y];
   const int nb = slot_blocks(sl.n);
-  if ((int)blockIdx.x >= nb || (gate && sl.out->done))
+  if ((int)blockIdx.x >= nb || sl.out->phase != PH_INIT)
     return;
   const T * __restrict__ b = (const T *)sl.ib;
   const T * __restrict__ q = (const T *)sl.iq;
@@ -132,24 +132,18 @@ k_pcg_init(const SolveSlot * __restrict__ slots, int zero_guess, int gate)
   }
 }
 
-// Inner state of each slot.  Tolerance: rtol of the slot, or, for the inner
-// solve of a refinement (inner_floor > 0), max(inner_floor, 0.5 rtol ||b||/||r||)
-// of the outer state.  A slot whose outer solve is done is marked done at once.
+// Inner state of each slot in PH_INIT, which moves to PH_INNER.  Tolerance:
+// rtol of the slot, or, for the inner solve of a refinement (inner_floor > 0),
+// max(inner_floor, 0.5 rtol ||b|| / ||r||) of the outer state.  Iteration
+// budget: the slot's maxit less the inner iterations it has used.
 __global__ void __launch_bounds__(RED_THREADS)
-k_pcg_init_fin(const SolveSlot * __restrict__ slots, const int * __restrict__ git, int gate, double inner_floor)
+k_pcg_init_fin(const SolveSlot * __restrict__ slots, const int * __restrict__ git, double inner_floor)
 {
   const SolveSlot & sl = slots[blockIdx.x];
   PcgState * s = sl.in;
-  const PcgState * o = sl.out;
-  if (gate && o->done)
-  {
-    if (threadIdx.x == 0)
-    {
-      s->done = 1;
-      s->iters = 0;
-    }
+  PcgState * o = sl.out;
+  if (o->phase != PH_INIT)
     return;
-  }
   __shared__ double sh[3 * RED_THREADS];
   double v[3];
   reduce_partials<3>(sl.part, slot_blocks(sl.n), 0, v, sh);
@@ -165,13 +159,13 @@ k_pcg_init_fin(const SolveSlot * __restrict__ slots, const int * __restrict__ gi
     s->tol2 = tol * tol;
     s->iters = 0;
     s->it0 = *git;
-    // iteration budget left to the slot (sl.maxit over all inner solves)
-    s->maxit = sl.maxit - (gate ? o->inner : 0);
+    s->maxit = sl.maxit - o->inner;
     s->status = 0;
     const bool nan = bad(v[0]) || bad(v[1]) || bad(v[2]);
     if (nan)
       s->status = -5;
     s->done = (nan || v[2] <= s->tol2 * v[0] || s->maxit <= 0) ? 1 : 0;
+    o->phase = PH_INNER;
   }
 }
 
@@ -273,15 +267,52 @@ k_pcg_update_p(const SolveSlot * __restrict__ slots, const int * __restrict__ gi
 }
 
 // ---------------------------------------------------------------------------
-// FP64 outer residual and refinement
+// phases, FP64 outer residual and refinement
 // ---------------------------------------------------------------------------
-// r64 = b - q64 (q64 = A x), partials (r,r), (b,b)
+// PH_FIRST -> PH_RESID (residual of the initial guess); PH_INNER with a
+// finished inner solve -> PH_RESID.  One thread per slot.
+__global__ void
+k_phase_prep(const SolveSlot * __restrict__ slots, int nslots)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots)
+    return;
+  PcgState * o = slots[i].out;
+  if (o->phase == PH_FIRST)
+  {
+    o->phase = PH_RESID;
+    o->first = 1;
+  }
+  else if (o->phase == PH_INNER && slots[i].in->done)
+  {
+    o->phase = PH_RESID;
+    o->first = 0;
+  }
+}
+
+// x += ||r64|| * ix (float): the refinement update after an FP32 inner solve,
+// the scale of the k_to_f32_scaled that made its right-hand side
+__global__ void
+k_ir_update(const SolveSlot * __restrict__ slots)
+{
+  const SolveSlot & sl = slots[blockIdx.y];
+  const PcgState * s = sl.out;
+  if (s->phase != PH_RESID || s->first)
+    return;
+  const double sc = sqrt(s->rr);
+  double * __restrict__ x = sl.x;
+  const float * __restrict__ d32 = (const float *)sl.ix;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sl.n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] += sc * (double)d32[i];
+}
+
+// r64 = b - q64 (q64 = A x), partials (r,r), (b,b); slots in PH_RESID
 __global__ void __launch_bounds__(RED_THREADS)
-k_residual64(const SolveSlot * __restrict__ slots, int gate)
+k_residual64(const SolveSlot * __restrict__ slots)
 {
   const SolveSlot & sl = slots[blockIdx.y];
   const int nb = slot_blocks(sl.n);
-  if ((int)blockIdx.x >= nb || (gate && sl.out->done))
+  if ((int)blockIdx.x >= nb || sl.out->phase != PH_RESID)
     return;
   const double * __restrict__ b = sl.b;
   const double * __restrict__ Ax = sl.q64;
@@ -303,15 +334,16 @@ k_residual64(const SolveSlot * __restrict__ slots, int gate)
   }
 }
 
-// Outer state: rr (and bb, a fresh sequence, when set_bb); done once
-// rr <= rtol^2 bb or after max_outer steps.  accumulate: add the inner
-// iterations of the slot's last inner solve to the outer count and step.
+// Outer state of slots in PH_RESID: rr (and bb on the first residual);
+// otherwise the inner iterations of the finished inner solve are added and the
+// step counted.  Then PH_DONE once rr <= rtol^2 bb, or on a NaN, after
+// max_outer steps or when the iteration budget is spent; PH_INIT otherwise.
 __global__ void __launch_bounds__(RED_THREADS)
-k_residual_fin(const SolveSlot * __restrict__ slots, int set_bb, int gate, int accumulate, int max_outer)
+k_residual_fin(const SolveSlot * __restrict__ slots, int max_outer)
 {
   const SolveSlot & sl = slots[blockIdx.x];
   PcgState * s = sl.out;
-  if (gate && s->done)
+  if (s->phase != PH_RESID)
     return;
   __shared__ double sh[2 * RED_THREADS];
   double o[2];
@@ -319,15 +351,16 @@ k_residual_fin(const SolveSlot * __restrict__ slots, int set_bb, int gate, int a
   if (threadIdx.x == 0)
   {
     s->rr = o[0];
-    if (set_bb)
+    if (s->first)
     {
       s->bb = o[1];
       s->tol2 = sl.rtol * sl.rtol;
       s->status = 0;
       s->iters = 0;
       s->inner = 0;
+      s->first = 0;
     }
-    if (accumulate)
+    else
     {
       s->inner += sl.in->iters;
       s->iters += 1;
@@ -336,17 +369,20 @@ k_residual_fin(const SolveSlot * __restrict__ slots, int set_bb, int gate, int a
     }
     if (bad(o[0]))
       s->status = -5;
-    s->done = (s->status != 0 || o[0] <= s->tol2 * s->bb || s->iters >= max_outer || s->inner >= sl.maxit) ? 1 : 0;
+    const bool stop =
+        s->status != 0 || o[0] <= s->tol2 * s->bb || s->iters >= max_outer || s->inner >= sl.maxit;
+    s->done = stop ? 1 : 0;
+    s->phase = stop ? PH_DONE : PH_INIT;
   }
 }
 
-// ib (float) = r64 / ||r64||
+// ib (float) = r64 / ||r64|| for slots in PH_INIT
 __global__ void
 k_to_f32_scaled(const SolveSlot * __restrict__ slots)
 {
   const SolveSlot & sl = slots[blockIdx.y];
   const PcgState * s = sl.out;
-  if (s->done)
+  if (s->phase != PH_INIT)
     return;
   const double inv = s->rr > 0 ? 1.0 / sqrt(s->rr) : 0.0;
   const double * __restrict__ r64 = sl.r64;
@@ -355,19 +391,12 @@ k_to_f32_scaled(const SolveSlot * __restrict__ slots)
     r32[i] = (float)(r64[i] * inv);
 }
 
-// x += ||r64|| * ix (float), the scale of the matching k_to_f32_scaled
 __global__ void
-k_ir_update(const SolveSlot * __restrict__ slots)
+k_gather_states(const SolveSlot * __restrict__ slots, int nslots, PcgState * __restrict__ dst)
 {
-  const SolveSlot & sl = slots[blockIdx.y];
-  const PcgState * s = sl.out;
-  if (s->done)
-    return;
-  const double sc = sqrt(s->rr);
-  double * __restrict__ x = sl.x;
-  const float * __restrict__ d32 = (const float *)sl.ix;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sl.n; i += (int64_t)gridDim.x * blockDim.x)
-    x[i] += sc * (double)d32[i];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nslots)
+    dst[i] = *slots[i].out;
 }
 
 // ---------------------------------------------------------------------------
@@ -432,12 +461,10 @@ slot_grid(const SlotTable & t)
 
 template <typename T>
 hipError_t
-pcg_init(const SlotTable & t, const int * git, bool zero_guess, bool gate, double inner_floor, hipStream_t st)
+pcg_init(const SlotTable & t, const int * git, bool zero_guess, double inner_floor, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_pcg_init<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, zero_guess ? 1 : 0,
-                     gate ? 1 : 0);
-  hipLaunchKernelGGL(k_pcg_init_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, git, gate ? 1 : 0,
-                     inner_floor);
+  hipLaunchKernelGGL(k_pcg_init<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, zero_guess ? 1 : 0);
+  hipLaunchKernelGGL(k_pcg_init_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, git, inner_floor);
   return hipGetLastError();
 }
 
@@ -451,17 +478,23 @@ pcg_iterate(const SlotTable & t, int * git, hipStream_t st)
   return hipGetLastError();
 }
 
-template hipError_t pcg_init<double>(const SlotTable &, const int *, bool, bool, double, hipStream_t);
-template hipError_t pcg_init<float>(const SlotTable &, const int *, bool, bool, double, hipStream_t);
+template hipError_t pcg_init<double>(const SlotTable &, const int *, bool, double, hipStream_t);
+template hipError_t pcg_init<float>(const SlotTable &, const int *, bool, double, hipStream_t);
 template hipError_t pcg_iterate<double>(const SlotTable &, int *, hipStream_t);
 template hipError_t pcg_iterate<float>(const SlotTable &, int *, hipStream_t);
 
 hipError_t
-residual64(const SlotTable & t, bool set_bb, bool gate, bool accumulate, int max_outer, hipStream_t st)
+phase_prep(const SlotTable & t, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_residual64, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, gate ? 1 : 0);
-  hipLaunchKernelGGL(k_residual_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, set_bb ? 1 : 0,
-                     gate ? 1 : 0, accumulate ? 1 : 0, max_outer);
+  hipLaunchKernelGGL(k_phase_prep, dim3((t.nslots + 255) / 256), dim3(256), 0, st, t.slots, t.nslots);
+  return hipGetLastError();
+}
+
+hipError_t
+residual64(const SlotTable & t, int max_outer, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_residual64, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
+  hipLaunchKernelGGL(k_residual_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, max_outer);
   return hipGetLastError();
 }
 
@@ -476,6 +509,13 @@ hipError_t
 ir_update(const SlotTable & t, hipStream_t st)
 {
   hipLaunchKernelGGL(k_ir_update, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
+  return hipGetLastError();
+}
+
+hipError_t
+gather_states(const SlotTable & t, PcgState * dst, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_gather_states, dim3((t.nslots + 255) / 256), dim3(256), 0, st, t.slots, t.nslots, dst);
   return hipGetLastError();
 }
 

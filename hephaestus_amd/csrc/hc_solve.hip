@@ -38,6 +38,15 @@ geom_for(const hc_op_s * op, const TileCandidate * c, Geom & g, PlaneSegs & segs
 
 } // namespace
 
+void
+initial_states(PcgState & in, PcgState & out)
+{
+  std::memset(&in, 0, sizeof(in));
+  std::memset(&out, 0, sizeof(out));
+  in.done = 1; // no inner solve running
+  out.phase = PH_FIRST;
+}
+
 Engine::~Engine()
 {
   if (host_)
@@ -45,13 +54,21 @@ Engine::~Engine()
 }
 
 int
-Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool force_batched)
+Engine::max_outer() const
+{
+  return mode_ == HC_SOLVE_FP32_IR ? MAX_OUTER_IR : MAX_OUTER_FP64;
+}
+
+int
+Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool force_batched,
+            const TileCandidate * const * cands64, const TileCandidate * const * cands32)
 {
   inst_ = inst;
   mode_ = mode;
   const bool ir = (mode == HC_SOLVE_FP32_IR);
   nslots_ = 0;
   int nb_max = 1;
+  int owned = 0; // slots whose states the engine owns
   for (auto & e : inst_)
   {
     if (e.nrhs < 1)
@@ -64,25 +81,31 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
       return fail(HC_EINVAL, "solve: work vectors missing");
     e.slot0 = nslots_;
     nslots_ += e.nrhs;
+    if (!e.in)
+      owned += e.nrhs;
     nb_max = std::max(nb_max, red_blocks(e.op->N));
   }
   if (nslots_ == 0)
     return fail(HC_EINVAL, "solve: empty batch");
-  // groups by order (one apply launch each)
   std::vector<std::vector<int>> by_order(PMAX + 1);
   for (int i = 0; i < (int)inst_.size(); i++)
     by_order[inst_[i].op->p].push_back(i);
   size_t ndesc_total = 0;
   for (int p = 1; p <= PMAX; p++)
     ndesc_total += by_order[p].size();
-  // device layout
+  // device layout: slot table, gathered states, counter, apply tables, owned states
   const size_t o_slots = 0;
-  const size_t o_in = align256(o_slots + sizeof(SolveSlot) * nslots_);
-  const size_t o_out = align256(o_in + sizeof(PcgState) * nslots_);
-  const size_t o_part = align256(o_out + sizeof(PcgState) * nslots_);
-  const size_t o_git = align256(o_part + sizeof(double) * 4 * RED_BLOCKS * (size_t)nslots_);
+  const size_t o_gath = align256(o_slots + sizeof(SolveSlot) * nslots_);
+  const size_t o_git = align256(o_gath + sizeof(PcgState) * nslots_);
   const size_t o_desc = align256(o_git + sizeof(int));
-  const size_t total = o_desc + sizeof(ApplyDesc) * ndesc_total * K_COUNT;
+  const size_t o_in = align256(o_desc + sizeof(ApplyDesc) * ndesc_total * K_COUNT);
+  const size_t o_out = align256(o_in + sizeof(PcgState) * owned);
+  const size_t o_part = align256(o_out + sizeof(PcgState) * owned);
+  const size_t total = o_part + sizeof(double) * 4 * RED_BLOCKS * (size_t)owned;
+  // the iteration counter survives a rebuild (slots may be mid-solve)
+  int git_host = 0;
+  if (git_)
+    HC_TRY_HIP(hipMemcpy(&git_host, git_, sizeof(int), hipMemcpyDeviceToHost));
   if (mem_.bytes < total)
   {
     int rc = mem_.alloc(total);
@@ -91,11 +114,21 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
   }
   char * base = (char *)mem_.base;
   SolveSlot * d_slots = (SolveSlot *)(base + o_slots);
-  in_ = (PcgState *)(base + o_in);
-  out_ = (PcgState *)(base + o_out);
-  double * d_part = (double *)(base + o_part);
+  gathered_ = (PcgState *)(base + o_gath);
   git_ = (int *)(base + o_git);
   ApplyDesc * d_desc = (ApplyDesc *)(base + o_desc);
+  PcgState * own_in = (PcgState *)(base + o_in);
+  PcgState * own_out = (PcgState *)(base + o_out);
+  double * own_part = (double *)(base + o_part);
+  int opos = 0;
+  for (auto & e : inst_)
+    if (!e.in)
+    {
+      e.in = own_in + opos;
+      e.out = own_out + opos;
+      e.part = own_part + (size_t)opos * 4 * RED_BLOCKS;
+      opos += e.nrhs;
+    }
   // slots
   std::vector<SolveSlot> hs(nslots_);
   for (auto & e : inst_)
@@ -133,9 +166,9 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
         s.iq = s.q64;
         s.idinv = op->dinv64;
       }
-      s.in = in_ + e.slot0 + r;
-      s.out = out_ + e.slot0 + r;
-      s.part = d_part + (size_t)(e.slot0 + r) * 4 * RED_BLOCKS;
+      s.in = e.in + r;
+      s.out = e.out + r;
+      s.part = e.part + (size_t)r * 4 * RED_BLOCKS;
     }
   }
   table_.slots = d_slots;
@@ -147,8 +180,7 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
   for (int k = 0; k < K_COUNT; k++)
   {
     groups_[k].clear();
-    const bool inner = (k == K_PQ);
-    const bool prec32 = inner && ir;
+    const bool prec32 = (k == K_PQ) && ir;
     for (int p = 1; p <= PMAX; p++)
     {
       const auto & ids = by_order[p];
@@ -158,7 +190,8 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
       ApplyGroup G{};
       G.order = p;
       G.prec = prec32 ? 32 : 64;
-      G.cand = prec32 ? op0->plan32.cand : op0->plan64.cand;
+      const TileCandidate * over = prec32 ? (cands32 ? cands32[p] : nullptr) : (cands64 ? cands64[p] : nullptr);
+      G.cand = over ? over : (prec32 ? op0->plan32.cand : op0->plan64.cand);
       G.tables = prec32 ? (const void *)&op0->tb32 : (const void *)&op0->tb64;
       G.batched = force_batched || ids.size() > 1 || inst_[ids[0]].nrhs > 1;
       G.descs = d_desc + dpos;
@@ -181,14 +214,16 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
           d.x = ir ? (const void *)w.p32 : (const void *)w.p64;
           d.y = ir ? (void *)w.q32 : (void *)w.q64;
           d.part = ir ? (void *)w.part32 : (void *)w.part64;
-          d.done = &in_[e.slot0].done;
+          d.done = &e.in->done;
+          d.gate = 0;
         }
         else
         {
           d.x = e.x;
           d.y = w.q64;
           d.part = w.part64;
-          d.done = (k == K_XQ_G) ? &out_[e.slot0].done : nullptr;
+          d.done = &e.out->phase;
+          d.gate = PH_RESID;
         }
         d.done_stride = STATE_STRIDE_INT;
         d.w0 = w0;
@@ -206,6 +241,7 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
   }
   HC_TRY_HIP(hipMemcpy(d_slots, hs.data(), sizeof(SolveSlot) * nslots_, hipMemcpyHostToDevice));
   HC_TRY_HIP(hipMemcpy(d_desc, hd.data(), sizeof(ApplyDesc) * hd.size(), hipMemcpyHostToDevice));
+  HC_TRY_HIP(hipMemcpy(git_, &git_host, sizeof(int), hipMemcpyHostToDevice));
   if (host_cap_ < nslots_)
   {
     if (host_)
@@ -244,20 +280,61 @@ Engine::apply(Kind k, hipStream_t st)
 }
 
 int
-Engine::read_states(PcgState * dev, hipStream_t st)
+Engine::reset_counter(hipStream_t st)
 {
-  HC_TRY_HIP(hipMemcpyAsync(host_, dev, sizeof(PcgState) * nslots_, hipMemcpyDeviceToHost, st));
+  HC_TRY_HIP(zero_int(git_, 1, st));
+  return HC_OK;
+}
+
+int
+Engine::round(int iters, hipStream_t st)
+{
+  const bool ir = (mode_ == HC_SOLVE_FP32_IR);
+  int rc;
+  HC_TRY_HIP(phase_prep(table_, st));
+  if (ir)
+    HC_TRY_HIP(ir_update(table_, st));
+  if ((rc = apply(K_XQ, st)))
+    return rc;
+  HC_TRY_HIP(residual64(table_, max_outer(), st));
+  if (ir)
+  {
+    HC_TRY_HIP(to_f32_scaled(table_, st));
+    HC_TRY_HIP(pcg_init<float>(table_, git_, true, INNER_FLOOR, st));
+  }
+  else
+    HC_TRY_HIP(pcg_init<double>(table_, git_, false, 0.0, st));
+  for (int k = 0; k < iters; k++)
+  {
+    if ((rc = apply(K_PQ, st)))
+      return rc;
+    if (ir)
+      HC_TRY_HIP(pcg_iterate<float>(table_, git_, st));
+    else
+      HC_TRY_HIP(pcg_iterate<double>(table_, git_, st));
+  }
+  return HC_OK;
+}
+
+int
+Engine::gather(hipStream_t st)
+{
+  HC_TRY_HIP(gather_states(table_, gathered_, st));
+  HC_TRY_HIP(hipMemcpyAsync(host_, gathered_, sizeof(PcgState) * nslots_, hipMemcpyDeviceToHost, st));
   HC_TRY_HIP(hipStreamSynchronize(st));
   return HC_OK;
 }
 
-bool
-Engine::all_done(const PcgState * h) const
+SlotStats
+Engine::stats_of(const PcgState & s)
 {
-  for (int i = 0; i < nslots_; i++)
-    if (!h[i].done)
-      return false;
-  return true;
+  SlotStats o{};
+  o.inner = s.inner;
+  o.outer = s.iters;
+  o.status = s.status;
+  o.rel_res = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
+  o.converged = (s.status == 0 && s.rr <= s.tol2 * s.bb * (1.0 + 1e-6)) ? 1 : 0;
+  return o;
 }
 
 int
@@ -265,71 +342,46 @@ Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double *
 {
   if (mode != mode_)
     return fail(HC_EINVAL, "solve: mode differs from the one the batch was set up for");
-  const bool ir = (mode == HC_SOLVE_FP32_IR);
-  const int max_outer = ir ? MAX_OUTER_IR : MAX_OUTER_FP64;
   int maxit_all = 0;
   for (auto & e : inst_)
     maxit_all = std::max(maxit_all, e.maxit);
+  // fresh states for every slot
+  std::vector<PcgState> hin(nslots_), hout(nslots_);
+  for (int i = 0; i < nslots_; i++)
+    initial_states(hin[i], hout[i]);
+  for (auto & e : inst_)
+  {
+    HC_TRY_HIP(hipMemcpyAsync(e.in, hin.data() + e.slot0, sizeof(PcgState) * e.nrhs, hipMemcpyHostToDevice, st));
+    HC_TRY_HIP(hipMemcpyAsync(e.out, hout.data() + e.slot0, sizeof(PcgState) * e.nrhs, hipMemcpyHostToDevice, st));
+  }
   hipEvent_t e0, e1;
   HC_TRY_HIP(hipEventCreate(&e0));
   HC_TRY_HIP(hipEventCreate(&e1));
   HC_TRY_HIP(hipEventRecord(e0, st));
-  HC_TRY_HIP(zero_int(git_, 1, st));
   int rc;
-  // initial FP64 residual; sets ||b|| and the outer tolerance
-  if ((rc = apply(K_XQ_ALL, st)))
+  if ((rc = reset_counter(st)))
     return rc;
-  HC_TRY_HIP(residual64(table_, true, false, false, max_outer, st));
-  int guard = 0;
-  for (int outer = 0; outer < max_outer + 1; outer++)
+  // rounds until every slot is done; the iteration count per round grows so
+  // that the host looks at the states rarely once solves are long
+  int chunk = 8, rounds = 0;
+  const int max_rounds = max_outer() * (maxit_all / 8 + 4) + 8;
+  for (;;)
   {
-    if ((rc = read_states(out_, st)))
+    if ((rc = round(chunk, st)))
       return rc;
-    if (all_done(host_))
+    if ((rc = gather(st)))
+      return rc;
+    bool all = true;
+    for (int i = 0; i < nslots_ && all; i++)
+      all = host_[i].phase == PH_DONE;
+    if (all)
       break;
-    if (ir)
-    {
-      HC_TRY_HIP(to_f32_scaled(table_, st));
-      HC_TRY_HIP(pcg_init<float>(table_, git_, true, true, INNER_FLOOR, st));
-    }
-    else
-      HC_TRY_HIP(pcg_init<double>(table_, git_, false, true, 0.0, st));
-    // inner PCG: host checks the done flags every `chunk` iterations
-    int chunk = 8;
-    for (;;)
-    {
-      if ((rc = read_states(in_, st)))
-        return rc;
-      if (all_done(host_))
-        break;
-      if (guard > maxit_all + 64)
-        return fail(HC_EHIP, "solve: inner iterations exceeded the cap without every slot finishing");
-      for (int k = 0; k < chunk; k++)
-      {
-        if ((rc = apply(K_PQ, st)))
-          return rc;
-        if (ir)
-          HC_TRY_HIP(pcg_iterate<float>(table_, git_, st));
-        else
-          HC_TRY_HIP(pcg_iterate<double>(table_, git_, st));
-      }
-      guard += chunk;
-      chunk = std::min(chunk * 2, 64);
-    }
-    guard = 0;
-    if (ir)
-      HC_TRY_HIP(ir_update(table_, st));
-    if ((rc = apply(K_XQ_G, st)))
-      return rc;
-    HC_TRY_HIP(residual64(table_, false, true, true, max_outer, st));
+    if (++rounds > max_rounds)
+      return fail(HC_EHIP, "solve: rounds exceeded the bound without every slot finishing");
+    chunk = std::min(chunk * 2, 64);
   }
-  // final true residual of every slot
-  if ((rc = apply(K_XQ_ALL, st)))
-    return rc;
-  HC_TRY_HIP(residual64(table_, false, false, false, max_outer, st));
   HC_TRY_HIP(hipEventRecord(e1, st));
-  if ((rc = read_states(out_, st)))
-    return rc;
+  HC_TRY_HIP(hipEventSynchronize(e1));
   float t = 0;
   HC_TRY_HIP(hipEventElapsedTime(&t, e0, e1));
   (void)hipEventDestroy(e0);
@@ -338,15 +390,7 @@ Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double *
     *ms = t;
   stats.resize(nslots_);
   for (int i = 0; i < nslots_; i++)
-  {
-    const PcgState & s = host_[i];
-    SlotStats & o = stats[i];
-    o.inner = s.inner;
-    o.outer = s.iters;
-    o.status = s.status;
-    o.rel_res = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
-    o.converged = (s.status == 0 && s.rr <= s.tol2 * s.bb * (1.0 + 1e-6)) ? 1 : 0;
-  }
+    stats[i] = stats_of(host_[i]);
   return HC_OK;
 }
 

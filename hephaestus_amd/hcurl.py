@@ -37,6 +37,8 @@ EXPORTS = [
     "hc_op_destroy", "hc_last_error", "hc_fill_uniform", "hc_op_set_tile", "hc_op_get_tile",
     "hc_op_autotune", "hc_tile_list", "hc_op_set_rhs_block", "hc_batch_create", "hc_batch_add", "hc_batch_set_rhs_block",
     "hc_batch_solve", "hc_batch_destroy", "hc_op_rhs_coil", "hc_op_profile_apply", "hc_op_apply_curl", "hc_axpby", "hc_op_zero_ess",
+    "hc_server_request_bytes", "hc_server_create", "hc_server_set_tile", "hc_server_submit", "hc_server_poll",
+    "hc_server_info", "hc_server_destroy",
 ]
 
 
@@ -51,6 +53,29 @@ class MeshDesc(C.Structure):
                 ("vert", C.POINTER(C.c_double)),
                 ("x0", C.c_double * 3), ("len", C.c_double * 3),
                 ("pec_faces", C.c_int)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_int), ("mode", C.c_int),
+                ("rtol", C.c_double), ("maxit", C.c_int), ("round_iters", C.c_int)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("status", C.c_int), ("converged", C.c_int), ("iterations", C.c_int),
+                ("outer_iterations", C.c_int), ("rel_res", C.c_double), ("latency_ms", C.c_double),
+                ("queue_ms", C.c_double), ("arrival_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rounds", C.c_int64),
+                ("pending", C.c_int64), ("active", C.c_int64), ("peak_active", C.c_int64),
+                ("budget_bytes", C.c_int64), ("peak_bytes", C.c_int64), ("elapsed_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class SolveStats(C.Structure):
@@ -105,6 +130,14 @@ def load():
     lib.hc_op_apply_curl.argtypes = [vp, vp, vp, vp]
     lib.hc_axpby.argtypes = [C.c_int64, C.c_double, vp, C.c_double, vp, vp]
     lib.hc_op_zero_ess.argtypes = [vp, vp, vp]
+    lib.hc_server_request_bytes.argtypes = [C.c_int64, C.c_int]
+    lib.hc_server_request_bytes.restype = C.c_int64
+    lib.hc_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(vp)]
+    lib.hc_server_set_tile.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.hc_server_submit.argtypes = [vp, vp, vp, vp, C.c_double, C.c_uint64, C.c_double]
+    lib.hc_server_poll.argtypes = [vp, C.POINTER(RequestResult), C.c_int, C.c_int]
+    lib.hc_server_info.argtypes = [vp, C.POINTER(ServerStats)]
+    lib.hc_server_destroy.argtypes = [vp]
     lib.hc_op_profile_apply.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.POINTER(C.c_double), vp]
     _lib = lib
     return lib
@@ -335,6 +368,62 @@ class Batch:
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             _lib.hc_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching server of backward-Euler step solves (hc_server_*).
+    A request (op, b_src, u, dt) computes u <- u + dt k with
+    (dt K + M) k = b_src - K u on the operator set up with that dt."""
+
+    def __init__(self, budget_bytes, mode=SOLVE_FP32_IR, rtol=1e-8, maxit=20000, max_active=0, round_iters=16):
+        lib = load()
+        cfg = ServerConfig(int(budget_bytes), int(max_active), int(mode), float(rtol), int(maxit), int(round_iters))
+        h = C.c_void_p()
+        _check(lib.hc_server_create(C.byref(cfg), C.byref(h)), "hc_server_create")
+        self._h = h
+        self.mode = mode
+        self._keep = {}
+
+    @staticmethod
+    def request_bytes(ndofs, mode=SOLVE_FP32_IR):
+        return int(load().hc_server_request_bytes(int(ndofs), int(mode)))
+
+    def set_tile(self, order, prec_bits, tx, ty, tz, mode=-1):
+        _check(_lib.hc_server_set_tile(self._h, order, prec_bits, tx, ty, tz, mode), "hc_server_set_tile")
+
+    def submit(self, op, b_src, u, dt, tag, arrival_ms=-1.0):
+        import torch
+        _nrhs(b_src, op.ndofs, torch.float64)
+        _nrhs(u, op.ndofs, torch.float64)
+        _check(_lib.hc_server_submit(self._h, op._h, _ptr(b_src), _ptr(u), float(dt), C.c_uint64(tag),
+                                     float(arrival_ms)), "hc_server_submit")
+        self._keep[int(tag)] = (op, b_src, u)
+
+    def poll(self, max_results=256, wait_ms=100):
+        buf = (RequestResult * max_results)()
+        n = _lib.hc_server_poll(self._h, buf, int(max_results), int(wait_ms))
+        if n < 0:
+            _check(n, "hc_server_poll")
+        out = [buf[i].as_dict() for i in range(n)]
+        for r in out:
+            self._keep.pop(int(r["tag"]), None)
+        return out
+
+    def info(self):
+        st = ServerStats()
+        _check(_lib.hc_server_info(self._h, C.byref(st)), "hc_server_info")
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.hc_server_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -197,6 +197,65 @@ int hc_batch_set_rhs_block(hc_batch_t batch, int rhs_block);
 int hc_batch_solve(hc_batch_t batch, int mode, hc_solve_stats * stats, void * stream);
 int hc_batch_destroy(hc_batch_t batch);
 
+/* Continuous-batching server of backward-Euler time-step solves
+ * (TimeDomainEquationSystemProblemOperator::ImplicitSolve + the
+ * BackwardEulerSolver update, time_domain_equation_system_problem_operator.cpp:30-51):
+ * for a request (op, b_src, u) with A = dt K_nu + M_sigma set up by
+ * hc_op_setup(op, dt):  f = b_src - K_nu u,  A k = f,  u <- u + dt k.
+ * Operators stay resident (owned by the caller, used by the server alone while
+ * its requests are in flight).  A scheduler thread admits requests in arrival
+ * order as soon as their work vectors fit in the device-memory budget and
+ * fewer than max_active are running, advances all admitted requests together
+ * (one launch per ND order per operator apply), and retires each as soon as it
+ * converges; a request's result does not depend on what else is running.
+ * The reference serves one problem per process and has no counterpart. */
+typedef struct hc_server_s * hc_server_t;
+typedef struct
+{
+  int64_t hbm_budget_bytes; /* device memory for the work vectors of admitted requests */
+  int max_active;           /* most requests running at once (0 = no limit) */
+  int mode;                 /* HC_SOLVE_FP64 or HC_SOLVE_FP32_IR */
+  double rtol;              /* true relative residual of each solve */
+  int maxit;                /* inner PCG iterations per request */
+  int round_iters;          /* PCG iterations between scheduling decisions (0 = 16) */
+} hc_server_config;
+typedef struct
+{
+  uint64_t tag;
+  int status;           /* HC_OK, HC_ENOTCONV or HC_ENAN */
+  int converged;
+  int iterations;       /* inner PCG iterations */
+  int outer_iterations; /* refinement steps (FP32+IR) or PCG runs (FP64) */
+  double rel_res;       /* final FP64 true relative residual of A k = f */
+  double latency_ms;    /* arrival -> result */
+  double queue_ms;      /* arrival -> admission */
+  double arrival_ms;    /* arrival, from the server's creation */
+} hc_request_result;
+typedef struct
+{
+  int64_t submitted, completed, rounds, pending, active;
+  int64_t peak_active;
+  int64_t budget_bytes, peak_bytes;
+  double elapsed_ms;
+} hc_server_stats;
+/* Work-vector bytes one request of an operator with N DOFs takes from the
+ * budget (mode HC_SOLVE_FP64 or HC_SOLVE_FP32_IR). */
+int64_t hc_server_request_bytes(int64_t ndofs, int mode);
+int hc_server_create(const hc_server_config * cfg, hc_server_t * out);
+/* Tile variant used for all operators of an order (before any submit). */
+int hc_server_set_tile(hc_server_t srv, int order, int prec_bits, int tx, int ty, int tz, int mode);
+/* b_src, u: device, N doubles, valid until the request's result is polled.
+ * arrival_ms >= 0: the request arrives that many ms after hc_server_create
+ * (scheduled arrivals); < 0: now.  HC_EBUSY if it can never fit the budget. */
+int hc_server_submit(hc_server_t srv, hc_op_t op, const double * b_src, double * u, double dt, uint64_t tag,
+                     double arrival_ms);
+/* Completed requests (up to max), waiting up to wait_ms for the first;
+ * returns the number written or a negative status. */
+int hc_server_poll(hc_server_t srv, hc_request_result * out, int max, int wait_ms);
+int hc_server_info(hc_server_t srv, hc_server_stats * stats);
+/* Finishes every submitted request, then releases the server. */
+int hc_server_destroy(hc_server_t srv);
+
 int hc_op_destroy(hc_op_t op);
 const char * hc_last_error(void);
 
