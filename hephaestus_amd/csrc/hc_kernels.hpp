@@ -181,7 +181,16 @@ struct PcgState
   int maxit;     // inner: iteration cap
   int inner;     // outer: inner iterations summed over the refinement steps
   int phase;     // outer: PH_* below
-  int first;     // outer: 1 while the residual of the initial guess is due
+  int first;     // outer: 1 while the residual of the initial guess is due;
+                 // inner (reliable mode): 1 when paused for a replacement
+  double rrmax;  // inner: largest (r, r) since the last residual replacement;
+                 // outer: (r, r) of the previous FP64 residual
+  int counted;   // inner: iterations already added to the outer count
+  int prec64;    // outer: 1 when the slot iterates in FP64, 0 in FP32
+  int run;       // inner: 0 idle, 1 FP32 iterations running, 2 FP64 running
+  int stall;     // outer: FP32 replacements in a row without progress
+  int revert;    // outer: 1 while the last FP32 increment is being undone
+  int pad;
 };
 constexpr int STATE_STRIDE_INT = sizeof(PcgState) / sizeof(int);
 
@@ -201,22 +210,27 @@ enum SlotPhase
 // One right-hand side of one operator instance.  The inner vectors (ib..iq,
 // idinv) have the precision of the inner solve (FP64 solve: b, x, r64, p64,
 // q64 and the FP64 diagonal inverse; refinement: FP32 work vectors).
+// One right-hand side of one operator instance.  FP64 PCG iterates on
+// (b, x, r64, p64, q64, dinv64); FP32 PCG (reliable updates) on (r32, x32,
+// p32, q32, dinv32) with x and the residual replaced in FP64.
 struct SolveSlot
 {
   int64_t n;
   double rtol;
   int maxit;
   int pad;
-  const double * b; // FP64 right-hand side
-  double * x;       // FP64 solution
-  double * r64;     // FP64 outer residual
-  double * q64;     // FP64 A x
-  const void * ib;
-  void * ix;
-  void * ir;
-  void * ip;
-  void * iq;
-  const void * idinv;
+  double delta2;   // FP32: replacement when (r, r) < delta2 * rrmax
+  const double * b;
+  double * x;
+  double * r64;    // FP64 residual
+  double * p64;
+  double * q64;    // FP64 A p, or A x for the residual
+  const double * dinv64;
+  float * r32;
+  float * x32;     // FP32 iterate since the last replacement
+  float * p32;
+  float * q32;
+  const float * dinv32;
   PcgState * in;
   PcgState * out;
   double * part; // RED_BLOCKS * 4 partial sums
@@ -231,22 +245,24 @@ struct SlotTable
 
 // Phase moves at the start of a round (one thread per slot).
 hipError_t phase_prep(const SlotTable & t, hipStream_t st);
-// x += ||r64|| * (double)ix for slots in PH_RESID after an FP32 inner solve.
+// x += (double) x32 for slots in PH_RESID after FP32 iterations (reliable mode).
 hipError_t ir_update(const SlotTable & t, hipStream_t st);
 // r64 = b - q64 for slots in PH_RESID; outer state update and the move to
-// PH_DONE or PH_INIT.
+// PH_DONE or PH_INIT (or back to PH_RESID after undoing an FP32 increment
+// that increased the residual).
 hipError_t residual64(const SlotTable & t, int max_outer, hipStream_t st);
-// ib = (float)(r64 / ||r64||) for slots in PH_INIT (refinement).
-hipError_t to_f32_scaled(const SlotTable & t, hipStream_t st);
-// Inner PCG start for slots in PH_INIT: r = b - A x (or b with x = 0 when
-// zero_guess), p = D^-1 r; tolerance: slot rtol, or the refinement rule when
-// inner_floor > 0; moves them to PH_INNER.
+// The iteration counters git_pair[0] (FP32 iterations) and git_pair[1] (FP64)
+// give each slot its iteration index: counter - value at the slot's start.
+// FP32 PCG (re)start for FP32 slots in PH_INIT: r32 = (float) r64, x32 = 0,
+// p = D^-1 r32 unless paused for a replacement (direction kept).
+hipError_t pcg_init_reliable(const SlotTable & t, const int * git_pair, hipStream_t st);
+// FP64 PCG start for FP64 slots in PH_INIT: r = b - A x (q64 = A x), p =
+// D^-1 r; moves them to PH_INNER.
+hipError_t pcg_init64(const SlotTable & t, const int * git_pair, hipStream_t st);
+// One PCG iteration after q = A p for the slots running in precision T;
+// advances the counter of T.
 template <typename T>
-hipError_t pcg_init(const SlotTable & t, const int * git, bool zero_guess, double inner_floor, hipStream_t st);
-// One PCG iteration after q = A p for slots whose inner done flag is clear;
-// advances the counter *git.
-template <typename T>
-hipError_t pcg_iterate(const SlotTable & t, int * git, hipStream_t st);
+hipError_t pcg_iterate(const SlotTable & t, int * git_pair, hipStream_t st);
 // Copies the outer states of the slots into dst[0..nslots).
 hipError_t gather_states(const SlotTable & t, PcgState * dst, hipStream_t st);
 hipError_t zero_int(int * v, int n, hipStream_t st);

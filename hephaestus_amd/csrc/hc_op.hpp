@@ -208,10 +208,11 @@ struct SlotStats
   int status;
   int converged;
   double rel_res; // final FP64 true relative residual
+  int fp64;       // finished in FP64
 };
 
-// Initial values of the states of a slot that starts a solve.
-void initial_states(PcgState & in, PcgState & out);
+// Initial values of the states of a slot that starts a solve in `mode`.
+void initial_states(PcgState & in, PcgState & out, int mode);
 
 class Engine
 {
@@ -256,8 +257,9 @@ private:
   };
   enum Kind
   {
-    K_PQ = 0, // A p -> q in the inner precision, slots with a running inner solve
-    K_XQ,     // A x -> q64 (FP64), slots in PH_RESID
+    K_PQ32 = 0, // A p -> q in FP32, slots running FP32 iterations
+    K_PQ64,     // A p -> q in FP64, slots running FP64 iterations
+    K_XQ,       // A x -> q64 (FP64), slots in PH_RESID
     K_COUNT
   };
   int apply(Kind k, hipStream_t st);
@@ -267,10 +269,11 @@ private:
   int mode_ = 0;
   DevBlock mem_;
   SlotTable table_{};
-  int * git_ = nullptr;
+  int * git_ = nullptr; // iteration counters of the FP32 and FP64 groups
   PcgState * gathered_ = nullptr; // device, nslots
   PcgState * host_ = nullptr;     // pinned, nslots
   int host_cap_ = 0;
+  bool any32_ = false, any64_ = true; // precisions with running slots
   std::vector<ApplyGroup> groups_[K_COUNT];
 };
 

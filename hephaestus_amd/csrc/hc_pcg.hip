@@ -77,6 +77,10 @@ reduce_partials(const double * __restrict__ part, int nb, int k0, double (&out)[
     out[k] = v[k];
 }
 
+// FP32 PCG: FP64 residual replacement at least every this many iterations (a
+// stagnating FP32 recursion is found within that many)
+constexpr int RELIABLE_MAX_GAP = 256;
+
 __device__ inline bool
 bad(double v)
 {
@@ -87,38 +91,58 @@ bad(double v)
 #define HC_SLOT_LOOP(i, n, nb)                                                                                    \
   for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < (n); i += (int64_t)(nb)*RED_THREADS)
 
+// Vectors of a slot in the precision of T.
+template <typename T>
+struct SlotVecs;
+template <>
+struct SlotVecs<double>
+{
+  static constexpr int RUN = 2;
+  const SolveSlot & s;
+  __device__ double * x() const { return s.x; }
+  __device__ double * r() const { return s.r64; }
+  __device__ double * p() const { return s.p64; }
+  __device__ double * q() const { return s.q64; }
+  __device__ const double * dinv() const { return s.dinv64; }
+};
+template <>
+struct SlotVecs<float>
+{
+  static constexpr int RUN = 1;
+  const SolveSlot & s;
+  __device__ float * x() const { return s.x32; }
+  __device__ float * r() const { return s.r32; }
+  __device__ float * p() const { return s.p32; }
+  __device__ float * q() const { return s.q32; }
+  __device__ const float * dinv() const { return s.dinv32; }
+};
+
 // ---------------------------------------------------------------------------
 // PCG
 // ---------------------------------------------------------------------------
-// r = b - q (or b when zero_guess, then x = 0);  p = D^-1 r;
-// partials (b,b), (r, D^-1 r), (r,r); slots in PH_INIT.
-template <typename T>
+// FP64 start: r = b - q64 (q64 = A x of this round), p = D^-1 r; partials
+// (b,b), (r, D^-1 r), (r,r).  FP64 slots in PH_INIT.
 __global__ void __launch_bounds__(RED_THREADS)
-k_pcg_init(const SolveSlot * __restrict__ slots, int zero_guess)
+k_pcg_init64(const SolveSlot * __restrict__ slots)
 {
   const SolveSlot & sl = slots[blockIdx.y];
   const int nb = slot_blocks(sl.n);
-  if ((int)blockIdx.x >= nb || sl.out->phase != PH_INIT)
+  if ((int)blockIdx.x >= nb || sl.out->phase != PH_INIT || !sl.out->prec64)
     return;
-  const T * __restrict__ b = (const T *)sl.ib;
-  const T * __restrict__ q = (const T *)sl.iq;
-  const T * __restrict__ dinv = (const T *)sl.idinv;
-  T * __restrict__ x = (T *)sl.ix;
-  T * __restrict__ r = (T *)sl.ir;
-  T * __restrict__ p = (T *)sl.ip;
+  const double * __restrict__ b = sl.b;
+  const double * __restrict__ q = sl.q64;
+  const double * __restrict__ dinv = sl.dinv64;
+  double * __restrict__ r = sl.r64;
+  double * __restrict__ p = sl.p64;
   __shared__ double sh[3 * RED_THREADS];
   double v[3] = {0, 0, 0};
   HC_SLOT_LOOP(i, sl.n, nb)
   {
-    const double bi = (double)b[i];
-    double ri = bi;
-    if (zero_guess)
-      x[i] = T(0);
-    else
-      ri = bi - (double)q[i];
-    const double zi = (double)dinv[i] * ri;
-    r[i] = (T)ri;
-    p[i] = (T)zi;
+    const double bi = b[i];
+    const double ri = bi - q[i];
+    const double zi = dinv[i] * ri;
+    r[i] = ri;
+    p[i] = zi;
     v[0] += bi * bi;
     v[1] += ri * zi;
     v[2] += ri * ri;
@@ -132,39 +156,38 @@ k_pcg_init(const SolveSlot * __restrict__ slots, int zero_guess)
   }
 }
 
-// Inner state of each slot in PH_INIT, which moves to PH_INNER.  Tolerance:
-// rtol of the slot, or, for the inner solve of a refinement (inner_floor > 0),
-// max(inner_floor, 0.5 rtol ||b|| / ||r||) of the outer state.  Iteration
-// budget: the slot's maxit less the inner iterations it has used.
+// Inner state of FP64 slots in PH_INIT, which move to PH_INNER.  Tolerance:
+// the slot's rtol; iteration budget: maxit less the inner iterations used.
 __global__ void __launch_bounds__(RED_THREADS)
-k_pcg_init_fin(const SolveSlot * __restrict__ slots, const int * __restrict__ git, double inner_floor)
+k_pcg_init64_fin(const SolveSlot * __restrict__ slots, const int * __restrict__ git)
 {
   const SolveSlot & sl = slots[blockIdx.x];
   PcgState * s = sl.in;
   PcgState * o = sl.out;
-  if (o->phase != PH_INIT)
+  if (o->phase != PH_INIT || !o->prec64)
     return;
   __shared__ double sh[3 * RED_THREADS];
   double v[3];
   reduce_partials<3>(sl.part, slot_blocks(sl.n), 0, v, sh);
   if (threadIdx.x == 0)
   {
-    double tol = sl.rtol;
-    if (inner_floor > 0)
-      tol = fmax(inner_floor, o->rr > 0 ? 0.5 * sl.rtol * sqrt(o->bb / o->rr) : 1.0);
     s->bb = v[0];
     s->gam[0] = v[1];
     s->gam[1] = 0.0;
     s->rr = v[2];
-    s->tol2 = tol * tol;
+    s->rrmax = v[2];
+    s->tol2 = sl.rtol * sl.rtol;
     s->iters = 0;
+    s->counted = 0;
     s->it0 = *git;
     s->maxit = sl.maxit - o->inner;
     s->status = 0;
+    s->first = 0;
     const bool nan = bad(v[0]) || bad(v[1]) || bad(v[2]);
     if (nan)
       s->status = -5;
     s->done = (nan || v[2] <= s->tol2 * v[0] || s->maxit <= 0) ? 1 : 0;
+    s->run = s->done ? 0 : 2;
     o->phase = PH_INNER;
   }
 }
@@ -177,10 +200,11 @@ k_pcg_dot_pq(const SolveSlot * __restrict__ slots, int * git)
     *git = *git + 1; // read by the next kernels only
   const SolveSlot & sl = slots[blockIdx.y];
   const int nb = slot_blocks(sl.n);
-  if ((int)blockIdx.x >= nb || sl.in->done)
+  const SlotVecs<T> V{sl};
+  if ((int)blockIdx.x >= nb || sl.in->run != SlotVecs<T>::RUN)
     return;
-  const T * __restrict__ p = (const T *)sl.ip;
-  const T * __restrict__ q = (const T *)sl.iq;
+  const T * __restrict__ p = V.p();
+  const T * __restrict__ q = V.q();
   __shared__ double sh[RED_THREADS];
   double v[1] = {0};
   HC_SLOT_LOOP(i, sl.n, nb)
@@ -197,14 +221,15 @@ k_pcg_update_xr(const SolveSlot * __restrict__ slots, const int * __restrict__ g
   const SolveSlot & sl = slots[blockIdx.y];
   const PcgState * s = sl.in;
   const int nb = slot_blocks(sl.n);
-  if ((int)blockIdx.x >= nb || s->done)
+  const SlotVecs<T> V{sl};
+  if ((int)blockIdx.x >= nb || s->run != SlotVecs<T>::RUN)
     return;
   const int it = *git - 1 - s->it0;
-  T * __restrict__ x = (T *)sl.ix;
-  T * __restrict__ r = (T *)sl.ir;
-  const T * __restrict__ p = (const T *)sl.ip;
-  const T * __restrict__ q = (const T *)sl.iq;
-  const T * __restrict__ dinv = (const T *)sl.idinv;
+  T * __restrict__ x = V.x();
+  T * __restrict__ r = V.r();
+  const T * __restrict__ p = V.p();
+  const T * __restrict__ q = V.q();
+  const T * __restrict__ dinv = V.dinv();
   __shared__ double sh[2 * RED_THREADS];
   double pq[1];
   reduce_partials<1>(sl.part, nb, 3, pq, sh);
@@ -235,7 +260,8 @@ k_pcg_update_p(const SolveSlot * __restrict__ slots, const int * __restrict__ gi
   const SolveSlot & sl = slots[blockIdx.y];
   PcgState * s = sl.in;
   const int nb = slot_blocks(sl.n);
-  if ((int)blockIdx.x >= nb || s->done)
+  const SlotVecs<T> V{sl};
+  if ((int)blockIdx.x >= nb || s->run != SlotVecs<T>::RUN)
     return;
   const int it = *git - 1 - s->it0;
   __shared__ double sh[2 * RED_THREADS];
@@ -251,16 +277,21 @@ k_pcg_update_p(const SolveSlot * __restrict__ slots, const int * __restrict__ gi
     s->gam[(it + 1) & 1] = gnew;
     s->rr = rr;
     s->iters = it + 1;
+    if (rr > s->rrmax)
+      s->rrmax = rr; // FP32: k_phase_prep compares against it
     if (nan)
       s->status = -5;
     if (stop)
+    {
       s->done = 1;
+      s->run = 0; // later blocks of this launch may see it: they stop too
+    }
   }
   if (stop)
     return;
-  T * __restrict__ p = (T *)sl.ip;
-  const T * __restrict__ r = (const T *)sl.ir;
-  const T * __restrict__ dinv = (const T *)sl.idinv;
+  T * __restrict__ p = V.p();
+  const T * __restrict__ r = V.r();
+  const T * __restrict__ dinv = V.dinv();
   const T beta = (T)(gnew / gold);
   HC_SLOT_LOOP(i, sl.n, nb)
   p[i] = dinv[i] * r[i] + beta * p[i];
@@ -270,40 +301,51 @@ k_pcg_update_p(const SolveSlot * __restrict__ slots, const int * __restrict__ gi
 // phases, FP64 outer residual and refinement
 // ---------------------------------------------------------------------------
 // PH_FIRST -> PH_RESID (residual of the initial guess); PH_INNER with a
-// finished inner solve -> PH_RESID.  One thread per slot.
+// finished inner solve -> PH_RESID; an FP32 PCG whose recursive residual fell
+// by delta since the last replacement, or that ran RELIABLE_MAX_GAP iterations
+// since, -> PH_RESID too (its iterate is folded into x and the residual
+// recomputed in FP64; the search direction is kept).  One thread per slot.
 __global__ void
 k_phase_prep(const SolveSlot * __restrict__ slots, int nslots)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslots)
     return;
-  PcgState * o = slots[i].out;
+  const SolveSlot & sl = slots[i];
+  PcgState * o = sl.out;
+  PcgState * in = sl.in;
   if (o->phase == PH_FIRST)
   {
     o->phase = PH_RESID;
     o->first = 1;
   }
-  else if (o->phase == PH_INNER && slots[i].in->done)
+  else if (o->phase == PH_RESID && o->revert)
+    o->revert = 0; // x was restored last round: its residual is recomputed now
+  else if (o->phase == PH_INNER &&
+           (in->run == 0 ||
+            (!o->prec64 && (in->rr < sl.delta2 * in->rrmax || in->iters - in->counted >= RELIABLE_MAX_GAP))))
   {
+    in->first = in->run != 0 ? 1 : 0; // 1: the search direction stays valid
+    in->run = 0;
+    in->done = 1;
     o->phase = PH_RESID;
     o->first = 0;
   }
 }
 
-// x += ||r64|| * ix (float): the refinement update after an FP32 inner solve,
-// the scale of the k_to_f32_scaled that made its right-hand side
+// x += x32: the FP32 iterate since the last replacement folded into the FP64
+// solution (FP32 slots entering PH_RESID after iterations)
 __global__ void
 k_ir_update(const SolveSlot * __restrict__ slots)
 {
   const SolveSlot & sl = slots[blockIdx.y];
   const PcgState * s = sl.out;
-  if (s->phase != PH_RESID || s->first)
+  if (s->phase != PH_RESID || s->first || s->prec64)
     return;
-  const double sc = sqrt(s->rr);
   double * __restrict__ x = sl.x;
-  const float * __restrict__ d32 = (const float *)sl.ix;
+  const float * __restrict__ d32 = sl.x32;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sl.n; i += (int64_t)gridDim.x * blockDim.x)
-    x[i] += sc * (double)d32[i];
+    x[i] += (double)d32[i];
 }
 
 // r64 = b - q64 (q64 = A x), partials (r,r), (b,b); slots in PH_RESID
@@ -335,9 +377,12 @@ k_residual64(const SolveSlot * __restrict__ slots)
 }
 
 // Outer state of slots in PH_RESID: rr (and bb on the first residual);
-// otherwise the inner iterations of the finished inner solve are added and the
-// step counted.  Then PH_DONE once rr <= rtol^2 bb, or on a NaN, after
-// max_outer steps or when the iteration budget is spent; PH_INIT otherwise.
+// otherwise the iterations of the paused or finished PCG are counted and the
+// step too.  An FP32 slot whose FP64 residual did not halve over two
+// replacements in a row, or met a NaN, continues in FP64 (the FP32 operator
+// cannot resolve its near-null space).  Then PH_DONE once rr <= rtol^2 bb, on
+// a NaN in FP64, after max_outer steps or when the iteration budget is spent;
+// PH_INIT otherwise.
 __global__ void __launch_bounds__(RED_THREADS)
 k_residual_fin(const SolveSlot * __restrict__ slots, int max_outer)
 {
@@ -350,7 +395,8 @@ k_residual_fin(const SolveSlot * __restrict__ slots, int max_outer)
   reduce_partials<2>(sl.part, slot_blocks(sl.n), 0, o, sh);
   if (threadIdx.x == 0)
   {
-    s->rr = o[0];
+    const double rr = o[0];
+    s->rr = rr;
     if (s->first)
     {
       s->bb = o[1];
@@ -359,36 +405,139 @@ k_residual_fin(const SolveSlot * __restrict__ slots, int max_outer)
       s->iters = 0;
       s->inner = 0;
       s->first = 0;
+      s->stall = 0;
     }
     else
     {
-      s->inner += sl.in->iters;
+      PcgState * in = sl.in;
+      s->inner += in->iters - in->counted;
+      in->counted = in->iters;
       s->iters += 1;
-      if (sl.in->status)
-        s->status = sl.in->status;
+      if (!s->prec64)
+      {
+        if (in->status || !(rr <= s->rrmax))
+        {
+          // the FP32 increment made things worse (or overflowed): undo it
+          // and continue in FP64 from the previous x
+          s->prec64 = 1;
+          s->revert = 1;
+          in->status = 0;
+          in->first = 0;
+          s->phase = PH_RESID; // its residual is recomputed next round
+          s->done = 0;
+          return;
+        }
+        s->stall = (rr > 0.25 * s->rrmax) ? s->stall + 1 : 0;
+        if (s->stall >= 2)
+        {
+          s->prec64 = 1; // slow progress: continue in FP64 from this x
+          in->first = 0;
+        }
+      }
+      else if (in->status)
+        s->status = in->status;
     }
-    if (bad(o[0]))
+    s->rrmax = rr;
+    if (bad(rr))
       s->status = -5;
-    const bool stop =
-        s->status != 0 || o[0] <= s->tol2 * s->bb || s->iters >= max_outer || s->inner >= sl.maxit;
+    const bool stop = s->status != 0 || rr <= s->tol2 * s->bb || s->iters >= max_outer || s->inner >= sl.maxit;
     s->done = stop ? 1 : 0;
     s->phase = stop ? PH_DONE : PH_INIT;
   }
 }
 
-// ib (float) = r64 / ||r64|| for slots in PH_INIT
+// x -= x32 for slots that reject their last FP32 increment
 __global__ void
-k_to_f32_scaled(const SolveSlot * __restrict__ slots)
+k_ir_revert(const SolveSlot * __restrict__ slots)
 {
   const SolveSlot & sl = slots[blockIdx.y];
-  const PcgState * s = sl.out;
-  if (s->phase != PH_INIT)
+  if (!sl.out->revert || sl.out->phase != PH_RESID)
     return;
-  const double inv = s->rr > 0 ? 1.0 / sqrt(s->rr) : 0.0;
-  const double * __restrict__ r64 = sl.r64;
-  float * __restrict__ r32 = (float *)sl.ib;
+  double * __restrict__ x = sl.x;
+  const float * __restrict__ d32 = sl.x32;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sl.n; i += (int64_t)gridDim.x * blockDim.x)
-    r32[i] = (float)(r64[i] * inv);
+    x[i] -= (double)d32[i];
+}
+
+// FP32 PCG (re)start of FP32 slots in PH_INIT: r32 = (float) r64 (the FP64
+// true residual), x32 = 0; p = D^-1 r32 unless the PCG paused for a
+// replacement (then the search direction is kept).  Partials (r32, D^-1 r32),
+// (r32, r32).
+__global__ void __launch_bounds__(RED_THREADS)
+k_pcg_init_rel(const SolveSlot * __restrict__ slots)
+{
+  const SolveSlot & sl = slots[blockIdx.y];
+  const int nb = slot_blocks(sl.n);
+  const PcgState * o = sl.out;
+  if ((int)blockIdx.x >= nb || o->phase != PH_INIT || o->prec64)
+    return;
+  const bool fresh = !(o->iters > 0 && sl.in->first);
+  const double * __restrict__ r64 = sl.r64;
+  const float * __restrict__ dinv = sl.dinv32;
+  float * __restrict__ x = sl.x32;
+  float * __restrict__ r = sl.r32;
+  float * __restrict__ p = sl.p32;
+  __shared__ double sh[2 * RED_THREADS];
+  double v[2] = {0, 0};
+  HC_SLOT_LOOP(i, sl.n, nb)
+  {
+    const float ri = (float)r64[i];
+    r[i] = ri;
+    x[i] = 0.0f;
+    const float zi = dinv[i] * ri;
+    if (fresh)
+      p[i] = zi;
+    v[0] += (double)ri * (double)zi;
+    v[1] += (double)ri * (double)ri;
+  }
+  block_sum<2>(v, sh);
+  if (threadIdx.x == 0)
+  {
+    sl.part[blockIdx.x * 4 + 0] = v[0];
+    sl.part[blockIdx.x * 4 + 1] = v[1];
+  }
+}
+
+// Inner state of FP32 slots in PH_INIT, which move to PH_INNER.  The
+// iteration index continues across replacements, so (r, z) goes to the parity
+// slot the next iteration reads.
+__global__ void __launch_bounds__(RED_THREADS)
+k_pcg_init_rel_fin(const SolveSlot * __restrict__ slots, const int * __restrict__ git)
+{
+  const SolveSlot & sl = slots[blockIdx.x];
+  PcgState * s = sl.in;
+  PcgState * o = sl.out;
+  if (o->phase != PH_INIT || o->prec64)
+    return;
+  __shared__ double sh[2 * RED_THREADS];
+  double v[2];
+  reduce_partials<2>(sl.part, slot_blocks(sl.n), 0, v, sh);
+  if (threadIdx.x == 0)
+  {
+    const bool fresh = !(o->iters > 0 && s->first);
+    const int g = *git;
+    if (fresh)
+    {
+      s->it0 = g;
+      s->iters = 0;
+      s->counted = 0;
+      s->gam[0] = s->gam[1] = 0.0;
+      s->status = 0;
+      s->maxit = sl.maxit - o->inner;
+    }
+    s->first = 0;
+    s->gam[(g - s->it0) & 1] = v[0];
+    s->rr = v[1];
+    s->rrmax = v[1];
+    s->bb = o->bb;
+    s->tol2 = sl.rtol * sl.rtol;
+    const bool nan = bad(v[0]) || bad(v[1]);
+    if (nan)
+      s->status = -5;
+    s->done = (nan || v[1] <= s->tol2 * s->bb || s->iters >= s->maxit) ? 1 : 0;
+    s->run = s->done ? 0 : 1;
+    o->phase = PH_INNER;
+  }
 }
 
 __global__ void
@@ -459,27 +608,27 @@ slot_grid(const SlotTable & t)
   return dim3(t.nb_max, t.nslots);
 }
 
-template <typename T>
 hipError_t
-pcg_init(const SlotTable & t, const int * git, bool zero_guess, double inner_floor, hipStream_t st)
+pcg_init64(const SlotTable & t, const int * git_pair, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_pcg_init<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, zero_guess ? 1 : 0);
-  hipLaunchKernelGGL(k_pcg_init_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, git, inner_floor);
+  hipLaunchKernelGGL(k_pcg_init64, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
+  hipLaunchKernelGGL(k_pcg_init64_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, git_pair + 1);
   return hipGetLastError();
 }
 
+// The iteration counter of precision T: git[0] for FP32, git[1] for FP64
+// (each advances once per iteration of its precision group).
 template <typename T>
 hipError_t
-pcg_iterate(const SlotTable & t, int * git, hipStream_t st)
+pcg_iterate(const SlotTable & t, int * git_pair, hipStream_t st)
 {
+  int * git = git_pair + (sizeof(T) == 4 ? 0 : 1);
   hipLaunchKernelGGL(k_pcg_dot_pq<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, git);
   hipLaunchKernelGGL(k_pcg_update_xr<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, (const int *)git);
   hipLaunchKernelGGL(k_pcg_update_p<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, (const int *)git);
   return hipGetLastError();
 }
 
-template hipError_t pcg_init<double>(const SlotTable &, const int *, bool, double, hipStream_t);
-template hipError_t pcg_init<float>(const SlotTable &, const int *, bool, double, hipStream_t);
 template hipError_t pcg_iterate<double>(const SlotTable &, int *, hipStream_t);
 template hipError_t pcg_iterate<float>(const SlotTable &, int *, hipStream_t);
 
@@ -495,13 +644,15 @@ residual64(const SlotTable & t, int max_outer, hipStream_t st)
 {
   hipLaunchKernelGGL(k_residual64, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
   hipLaunchKernelGGL(k_residual_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, max_outer);
+  hipLaunchKernelGGL(k_ir_revert, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
   return hipGetLastError();
 }
 
 hipError_t
-to_f32_scaled(const SlotTable & t, hipStream_t st)
+pcg_init_reliable(const SlotTable & t, const int * git_pair, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_to_f32_scaled, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
+  hipLaunchKernelGGL(k_pcg_init_rel, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
+  hipLaunchKernelGGL(k_pcg_init_rel_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, git_pair);
   return hipGetLastError();
 }
 

@@ -173,7 +173,7 @@ hc_server_s::start_request(Request * r)
   HC_TRY_HIP(hc::zero_ess(op->g, r->f, st));
   HC_TRY_HIP(hc::axpby(N, 0.0, r->k, 0.0, r->k, st));
   hc::PcgState hin, hout;
-  hc::initial_states(hin, hout);
+  hc::initial_states(hin, hout, cfg.mode);
   HC_TRY_HIP(hipMemcpyAsync(r->in, &hin, sizeof(hin), hipMemcpyHostToDevice, st));
   HC_TRY_HIP(hipMemcpyAsync(r->out, &hout, sizeof(hout), hipMemcpyHostToDevice, st));
   HC_TRY_HIP(hipStreamSynchronize(st)); // the host state copies above are from the stack
@@ -313,6 +313,7 @@ hc_server_s::loop()
       o.iterations = s.inner;
       o.outer_iterations = s.outer;
       o.rel_res = s.rel_res;
+      o.fp64_fallback = (cfg.mode == HC_SOLVE_FP32_IR) ? s.fp64 : 0;
       res.push_back(o);
     }
     if (hipStreamSynchronize(st) != hipSuccess)

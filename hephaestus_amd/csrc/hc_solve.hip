@@ -21,8 +21,9 @@ namespace
 {
 
 constexpr int MAX_OUTER_FP64 = 5;  // PCG runs (1 + residual-replacement restarts)
-constexpr int MAX_OUTER_IR = 30;   // refinement steps
-constexpr double INNER_FLOOR = 3e-5; // FP32 PCG reduces its residual reliably to ~1e-5
+constexpr int MAX_OUTER_IR = 400;  // FP64 residual replacements of the FP32 PCG
+// FP32 PCG: an FP64 residual replacement each time ||r|| fell by 10x
+constexpr double RELIABLE_DELTA2 = 1e-2;
 
 void
 geom_for(const hc_op_s * op, const TileCandidate * c, Geom & g, PlaneSegs & segs)
@@ -39,12 +40,13 @@ geom_for(const hc_op_s * op, const TileCandidate * c, Geom & g, PlaneSegs & segs
 } // namespace
 
 void
-initial_states(PcgState & in, PcgState & out)
+initial_states(PcgState & in, PcgState & out, int mode)
 {
   std::memset(&in, 0, sizeof(in));
   std::memset(&out, 0, sizeof(out));
   in.done = 1; // no inner solve running
   out.phase = PH_FIRST;
+  out.prec64 = (mode == HC_SOLVE_FP32_IR) ? 0 : 1;
 }
 
 Engine::~Engine()
@@ -97,15 +99,15 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
   const size_t o_slots = 0;
   const size_t o_gath = align256(o_slots + sizeof(SolveSlot) * nslots_);
   const size_t o_git = align256(o_gath + sizeof(PcgState) * nslots_);
-  const size_t o_desc = align256(o_git + sizeof(int));
+  const size_t o_desc = align256(o_git + 2 * sizeof(int));
   const size_t o_in = align256(o_desc + sizeof(ApplyDesc) * ndesc_total * K_COUNT);
   const size_t o_out = align256(o_in + sizeof(PcgState) * owned);
   const size_t o_part = align256(o_out + sizeof(PcgState) * owned);
   const size_t total = o_part + sizeof(double) * 4 * RED_BLOCKS * (size_t)owned;
   // the iteration counter survives a rebuild (slots may be mid-solve)
-  int git_host = 0;
+  int git_host[2] = {0, 0};
   if (git_)
-    HC_TRY_HIP(hipMemcpy(&git_host, git_, sizeof(int), hipMemcpyDeviceToHost));
+    HC_TRY_HIP(hipMemcpy(git_host, git_, sizeof(git_host), hipMemcpyDeviceToHost));
   if (mem_.bytes < total)
   {
     int rc = mem_.alloc(total);
@@ -144,27 +146,20 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
       s.n = N;
       s.rtol = e.rtol;
       s.maxit = e.maxit;
+      s.delta2 = ir ? RELIABLE_DELTA2 : 0.0;
       s.b = e.b + o;
       s.x = e.x + o;
       s.r64 = w.r64 + o;
+      s.p64 = w.p64 + o;
       s.q64 = w.q64 + o;
+      s.dinv64 = op->dinv64;
       if (ir)
       {
-        s.ib = w.b32 + o;
-        s.ix = w.x32 + o;
-        s.ir = w.r32 + o;
-        s.ip = w.p32 + o;
-        s.iq = w.q32 + o;
-        s.idinv = op->dinv32;
-      }
-      else
-      {
-        s.ib = s.b;
-        s.ix = s.x;
-        s.ir = s.r64;
-        s.ip = w.p64 + o;
-        s.iq = s.q64;
-        s.idinv = op->dinv64;
+        s.r32 = w.r32 + o;
+        s.x32 = w.x32 + o;
+        s.p32 = w.p32 + o;
+        s.q32 = w.q32 + o;
+        s.dinv32 = op->dinv32;
       }
       s.in = e.in + r;
       s.out = e.out + r;
@@ -174,13 +169,20 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
   table_.slots = d_slots;
   table_.nslots = nslots_;
   table_.nb_max = nb_max;
+  // new slots start in the mode's precision
+  if (ir)
+    any32_ = true;
+  else
+    any64_ = true;
   // apply groups
   std::vector<ApplyDesc> hd(ndesc_total * K_COUNT);
   size_t dpos = 0;
   for (int k = 0; k < K_COUNT; k++)
   {
     groups_[k].clear();
-    const bool prec32 = (k == K_PQ) && ir;
+    if (k == K_PQ32 && !ir)
+      continue;
+    const bool prec32 = (k == K_PQ32);
     for (int p = 1; p <= PMAX; p++)
     {
       const auto & ids = by_order[p];
@@ -209,13 +211,21 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
         d.nrhs = e.nrhs;
         d.rb = G.batched ? (rhs_block > 0 ? std::min(rhs_block, e.nrhs) : default_rhs_block(d.ntiles, e.nrhs)) : 1;
         d.qd = prec32 ? (const void *)op->qd32 : (const void *)op->qd64;
-        if (k == K_PQ)
+        if (k == K_PQ32)
         {
-          d.x = ir ? (const void *)w.p32 : (const void *)w.p64;
-          d.y = ir ? (void *)w.q32 : (void *)w.q64;
-          d.part = ir ? (void *)w.part32 : (void *)w.part64;
-          d.done = &e.in->done;
-          d.gate = 0;
+          d.x = w.p32;
+          d.y = w.q32;
+          d.part = w.part32;
+          d.done = &e.in->run;
+          d.gate = 1;
+        }
+        else if (k == K_PQ64)
+        {
+          d.x = w.p64;
+          d.y = w.q64;
+          d.part = w.part64;
+          d.done = &e.in->run;
+          d.gate = 2;
         }
         else
         {
@@ -241,7 +251,7 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
   }
   HC_TRY_HIP(hipMemcpy(d_slots, hs.data(), sizeof(SolveSlot) * nslots_, hipMemcpyHostToDevice));
   HC_TRY_HIP(hipMemcpy(d_desc, hd.data(), sizeof(ApplyDesc) * hd.size(), hipMemcpyHostToDevice));
-  HC_TRY_HIP(hipMemcpy(git_, &git_host, sizeof(int), hipMemcpyHostToDevice));
+  HC_TRY_HIP(hipMemcpy(git_, git_host, sizeof(git_host), hipMemcpyHostToDevice));
   if (host_cap_ < nslots_)
   {
     if (host_)
@@ -282,7 +292,7 @@ Engine::apply(Kind k, hipStream_t st)
 int
 Engine::reset_counter(hipStream_t st)
 {
-  HC_TRY_HIP(zero_int(git_, 1, st));
+  HC_TRY_HIP(zero_int(git_, 2, st));
   return HC_OK;
 }
 
@@ -298,20 +308,25 @@ Engine::round(int iters, hipStream_t st)
     return rc;
   HC_TRY_HIP(residual64(table_, max_outer(), st));
   if (ir)
-  {
-    HC_TRY_HIP(to_f32_scaled(table_, st));
-    HC_TRY_HIP(pcg_init<float>(table_, git_, true, INNER_FLOOR, st));
-  }
-  else
-    HC_TRY_HIP(pcg_init<double>(table_, git_, false, 0.0, st));
+    HC_TRY_HIP(pcg_init_reliable(table_, git_, st));
+  HC_TRY_HIP(pcg_init64(table_, git_, st));
+  // the precisions that have slots running, as of the last gathered states
+  // (a slot that switched to FP64 in this round starts iterating next round)
+  const bool run32 = ir && any32_, run64 = !ir || any64_;
   for (int k = 0; k < iters; k++)
   {
-    if ((rc = apply(K_PQ, st)))
-      return rc;
-    if (ir)
+    if (run32)
+    {
+      if ((rc = apply(K_PQ32, st)))
+        return rc;
       HC_TRY_HIP(pcg_iterate<float>(table_, git_, st));
-    else
+    }
+    if (run64)
+    {
+      if ((rc = apply(K_PQ64, st)))
+        return rc;
       HC_TRY_HIP(pcg_iterate<double>(table_, git_, st));
+    }
   }
   return HC_OK;
 }
@@ -322,6 +337,10 @@ Engine::gather(hipStream_t st)
   HC_TRY_HIP(gather_states(table_, gathered_, st));
   HC_TRY_HIP(hipMemcpyAsync(host_, gathered_, sizeof(PcgState) * nslots_, hipMemcpyDeviceToHost, st));
   HC_TRY_HIP(hipStreamSynchronize(st));
+  any32_ = any64_ = false;
+  for (int i = 0; i < nslots_; i++)
+    if (host_[i].phase != PH_DONE)
+      (host_[i].prec64 ? any64_ : any32_) = true;
   return HC_OK;
 }
 
@@ -334,6 +353,7 @@ Engine::stats_of(const PcgState & s)
   o.status = s.status;
   o.rel_res = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
   o.converged = (s.status == 0 && s.rr <= s.tol2 * s.bb * (1.0 + 1e-6)) ? 1 : 0;
+  o.fp64 = s.prec64;
   return o;
 }
 
@@ -348,7 +368,9 @@ Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double *
   // fresh states for every slot
   std::vector<PcgState> hin(nslots_), hout(nslots_);
   for (int i = 0; i < nslots_; i++)
-    initial_states(hin[i], hout[i]);
+    initial_states(hin[i], hout[i], mode);
+  any32_ = (mode == HC_SOLVE_FP32_IR);
+  any64_ = !any32_;
   for (auto & e : inst_)
   {
     HC_TRY_HIP(hipMemcpyAsync(e.in, hin.data() + e.slot0, sizeof(PcgState) * e.nrhs, hipMemcpyHostToDevice, st));
@@ -428,7 +450,7 @@ namespace
 {
 
 void
-fold_stats(const std::vector<hc::SlotStats> & s, int begin, int count, double ms, hc_solve_stats * out)
+fold_stats(const std::vector<hc::SlotStats> & s, int begin, int count, double ms, int mode, hc_solve_stats * out)
 {
   hc_solve_stats r{};
   r.converged = 1;
@@ -441,6 +463,8 @@ fold_stats(const std::vector<hc::SlotStats> & s, int begin, int count, double ms
       r.converged = 0;
     if (s[i].status && !r.status)
       r.status = s[i].status;
+    if (mode == HC_SOLVE_FP32_IR)
+      r.fp64_fallbacks += s[i].fp64;
   }
   if (r.status == 0 && !r.converged)
     r.status = HC_ENOTCONV;
@@ -520,7 +544,7 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
   if ((rc = op->engine->solve(mode, (hipStream_t)stream, s, &ms)))
     return rc;
   hc_solve_stats out{};
-  fold_stats(s, 0, nrhs, ms, &out);
+  fold_stats(s, 0, nrhs, ms, mode, &out);
   if (stats)
     *stats = out;
   return status_rc(out, "hc_pcg_solve");
@@ -609,7 +633,7 @@ hc_batch_solve(hc_batch_t bt, int mode, hc_solve_stats * stats, void * stream)
   for (size_t i = 0; i < bt->entries.size(); i++)
   {
     hc_solve_stats s{};
-    fold_stats(bt->stats, pos, bt->entries[i].nrhs, ms, &s);
+    fold_stats(bt->stats, pos, bt->entries[i].nrhs, ms, mode, &s);
     pos += bt->entries[i].nrhs;
     if (stats)
       stats[i] = s;
@@ -620,6 +644,7 @@ hc_batch_solve(hc_batch_t bt, int mode, hc_solve_stats * stats, void * stream)
       all.converged = 0;
     if (s.status && !all.status)
       all.status = s.status;
+    all.fp64_fallbacks += s.fp64_fallbacks;
   }
   all.solve_ms = ms;
   return status_rc(all, "hc_batch_solve");

@@ -63,7 +63,8 @@ class ServerConfig(C.Structure):
 class RequestResult(C.Structure):
     _fields_ = [("tag", C.c_uint64), ("status", C.c_int), ("converged", C.c_int), ("iterations", C.c_int),
                 ("outer_iterations", C.c_int), ("rel_res", C.c_double), ("latency_ms", C.c_double),
-                ("queue_ms", C.c_double), ("arrival_ms", C.c_double)]
+                ("queue_ms", C.c_double), ("arrival_ms", C.c_double), ("fp64_fallback", C.c_int),
+                ("pad", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -80,7 +81,8 @@ class ServerStats(C.Structure):
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("outer_iterations", C.c_int), ("converged", C.c_int),
-                ("status", C.c_int), ("rel_res", C.c_double), ("solve_ms", C.c_double)]
+                ("status", C.c_int), ("rel_res", C.c_double), ("solve_ms", C.c_double),
+                ("fp64_fallbacks", C.c_int), ("pad", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

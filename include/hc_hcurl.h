@@ -104,6 +104,9 @@ typedef struct
   int status;           /* HC_OK or HC_ENOTCONV / HC_ENAN */
   double rel_res;       /* max over RHS of the final FP64 true relative residual */
   double solve_ms;      /* device time of the solve (hipEvents) */
+  int fp64_fallbacks;   /* HC_SOLVE_FP32_IR: right-hand sides that finished in
+                           FP64 (their FP32 iteration stopped making progress) */
+  int pad;
 } hc_solve_stats;
 
 /* Create an operator on a structured hex mesh.  nu_e / sigma_e: per-element
@@ -230,6 +233,8 @@ typedef struct
   double latency_ms;    /* arrival -> result */
   double queue_ms;      /* arrival -> admission */
   double arrival_ms;    /* arrival, from the server's creation */
+  int fp64_fallback;    /* 1 if an FP32+IR solve finished in FP64 */
+  int pad;
 } hc_request_result;
 typedef struct
 {

@@ -1,8 +1,9 @@
 """Continuous-batching server of backward-Euler steps (needs an MI355X).
 
 Each request computes u <- u + dt k with (dt K + M) k = b_coil - K u.  Checks:
-one request against the oracle's step with a sparse direct solve (relative L2
-<= 1e-7 at rtol 1e-10: the PCG bound times the conditioning); every request of
+one request against the oracle's step: the served k meets rtol 1e-10 on the
+oracle's assembled operator and u agrees with a sparse direct solve to relative
+L2 <= 1e-6 (rtol times the conditioning of this system); every request of
 a mix of uneven instances (orders 1-3, sizes 5-9) is bitwise identical to the
 same request served alone (mix invariance); an HBM budget that holds only two
 requests at a time bounds the admitted set and still completes every request;
@@ -141,5 +142,9 @@ def test_server_step_vs_oracle(torch_cuda, hc):
     ug = u.cpu().numpy()
     free = ~ess.astype(bool)
     err = np.linalg.norm(ug[free] - u_ref[free]) / np.linalg.norm(u_ref[free])
-    print(f"server step vs oracle: relative L2 {err:.3e}")
-    assert err <= 1e-7
+    # the served k = (u1 - u0) / dt meets rtol on the oracle's own operator
+    k = (ug - u0) / DT
+    res = np.linalg.norm(f - A @ k) / np.linalg.norm(f)
+    print(f"server step vs oracle: relative L2 {err:.3e}, oracle residual of k {res:.3e}")
+    assert res <= 1e-10 * 1.05
+    assert err <= 1e-6
