@@ -267,7 +267,12 @@ private:
   std::vector<EngineInst> inst_;
   int nslots_ = 0;
   int mode_ = 0;
+  int rhs_block_ = 0;
+  bool force_batched_ = false;
+  const TileCandidate * cands64_[PMAX + 1] = {};
+  const TileCandidate * cands32_[PMAX + 1] = {};
   DevBlock mem_;
+  DevBlock own_; // states of instances that came without their own
   SlotTable table_{};
   int * git_ = nullptr; // iteration counters of the FP32 and FP64 groups
   PcgState * gathered_ = nullptr; // device, nslots

@@ -378,11 +378,12 @@ k_residual64(const SolveSlot * __restrict__ slots)
 
 // Outer state of slots in PH_RESID: rr (and bb on the first residual);
 // otherwise the iterations of the paused or finished PCG are counted and the
-// step too.  An FP32 slot whose FP64 residual did not halve over two
-// replacements in a row, or met a NaN, continues in FP64 (the FP32 operator
-// cannot resolve its near-null space).  Then PH_DONE once rr <= rtol^2 bb, on
-// a NaN in FP64, after max_outer steps or when the iteration budget is spent;
-// PH_INIT otherwise.
+// step too.  An FP32 slot continues in FP64 when its true residual norm exceeds
+// the recursive one by sqrt(10) at a replacement, when two replacements in a
+// row did not halve it, or -- after undoing the last FP32 increment -- when
+// the true residual grew or a NaN appeared.  Then PH_DONE once
+// rr <= rtol^2 bb, on a NaN, after max_outer steps or when the iteration
+// budget is spent; PH_INIT otherwise.
 __global__ void __launch_bounds__(RED_THREADS)
 k_residual_fin(const SolveSlot * __restrict__ slots, int max_outer)
 {
@@ -417,7 +418,7 @@ k_residual_fin(const SolveSlot * __restrict__ slots, int max_outer)
       {
         if (in->status || !(rr <= s->rrmax))
         {
-          // the FP32 increment made things worse (or overflowed): undo it
+          // the FP32 increment blew the residual up (or overflowed): undo it
           // and continue in FP64 from the previous x
           s->prec64 = 1;
           s->revert = 1;
@@ -427,10 +428,14 @@ k_residual_fin(const SolveSlot * __restrict__ slots, int max_outer)
           s->done = 0;
           return;
         }
+        // continue in FP64 from this x when the FP32 recursion no longer
+        // tracks the true residual (the FP32 operator cannot resolve what is
+        // left of it), or after two replacements in a row that did not halve
+        // the residual norm
         s->stall = (rr > 0.25 * s->rrmax) ? s->stall + 1 : 0;
-        if (s->stall >= 2)
+        if (rr > 10.0 * in->rr || s->stall >= 2)
         {
-          s->prec64 = 1; // slow progress: continue in FP64 from this x
+          s->prec64 = 1;
           in->first = 0;
         }
       }

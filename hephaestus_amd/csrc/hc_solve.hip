@@ -95,15 +95,24 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
   size_t ndesc_total = 0;
   for (int p = 1; p <= PMAX; p++)
     ndesc_total += by_order[p].size();
-  // device layout: slot table, gathered states, counter, apply tables, owned states
+  // device layout: slot table, gathered states, counters, apply tables; the
+  // states the engine owns live in own_ (kept when a later set() passes them
+  // back, as the compaction of a running batch does)
   const size_t o_slots = 0;
   const size_t o_gath = align256(o_slots + sizeof(SolveSlot) * nslots_);
   const size_t o_git = align256(o_gath + sizeof(PcgState) * nslots_);
   const size_t o_desc = align256(o_git + 2 * sizeof(int));
-  const size_t o_in = align256(o_desc + sizeof(ApplyDesc) * ndesc_total * K_COUNT);
-  const size_t o_out = align256(o_in + sizeof(PcgState) * owned);
-  const size_t o_part = align256(o_out + sizeof(PcgState) * owned);
-  const size_t total = o_part + sizeof(double) * 4 * RED_BLOCKS * (size_t)owned;
+  const size_t total = o_desc + sizeof(ApplyDesc) * ndesc_total * K_COUNT;
+  const size_t own_in = 0;
+  const size_t own_out = align256(own_in + sizeof(PcgState) * owned);
+  const size_t own_part = align256(own_out + sizeof(PcgState) * owned);
+  const size_t own_total = own_part + sizeof(double) * 4 * RED_BLOCKS * (size_t)owned;
+  if (owned > 0 && own_.bytes < own_total)
+  {
+    int rc = own_.alloc(own_total);
+    if (rc)
+      return rc;
+  }
   // the iteration counter survives a rebuild (slots may be mid-solve)
   int git_host[2] = {0, 0};
   if (git_)
@@ -119,18 +128,23 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
   gathered_ = (PcgState *)(base + o_gath);
   git_ = (int *)(base + o_git);
   ApplyDesc * d_desc = (ApplyDesc *)(base + o_desc);
-  PcgState * own_in = (PcgState *)(base + o_in);
-  PcgState * own_out = (PcgState *)(base + o_out);
-  double * own_part = (double *)(base + o_part);
   int opos = 0;
   for (auto & e : inst_)
     if (!e.in)
     {
-      e.in = own_in + opos;
-      e.out = own_out + opos;
-      e.part = own_part + (size_t)opos * 4 * RED_BLOCKS;
+      char * ob = (char *)own_.base;
+      e.in = (PcgState *)(ob + own_in) + opos;
+      e.out = (PcgState *)(ob + own_out) + opos;
+      e.part = (double *)(ob + own_part) + (size_t)opos * 4 * RED_BLOCKS;
       opos += e.nrhs;
     }
+  rhs_block_ = rhs_block;
+  force_batched_ = force_batched;
+  for (int p = 0; p <= PMAX; p++)
+  {
+    cands64_[p] = cands64 ? cands64[p] : nullptr;
+    cands32_[p] = cands32 ? cands32[p] : nullptr;
+  }
   // slots
   std::vector<SolveSlot> hs(nslots_);
   for (auto & e : inst_)
@@ -384,7 +398,14 @@ Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double *
   if ((rc = reset_counter(st)))
     return rc;
   // rounds until every slot is done; the iteration count per round grows so
-  // that the host looks at the states rarely once solves are long
+  // that the host looks at the states rarely once solves are long.  Instances
+  // whose slots are all done leave the tables once a quarter of those still
+  // running has finished, so a long tail does not launch work for them.
+  const std::vector<EngineInst> all = inst_;
+  std::vector<int> orig(all.size()); // original index of each current instance
+  for (size_t i = 0; i < all.size(); i++)
+    orig[i] = (int)i;
+  stats.assign(nslots_, SlotStats{});
   int chunk = 8, rounds = 0;
   const int max_rounds = max_outer() * (maxit_all / 8 + 4) + 8;
   for (;;)
@@ -393,14 +414,40 @@ Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double *
       return rc;
     if ((rc = gather(st)))
       return rc;
-    bool all = true;
-    for (int i = 0; i < nslots_ && all; i++)
-      all = host_[i].phase == PH_DONE;
-    if (all)
+    std::vector<EngineInst> keep;
+    std::vector<int> keep_orig;
+    for (size_t j = 0; j < inst_.size(); j++)
+    {
+      const EngineInst & e = inst_[j];
+      bool done = true;
+      for (int r = 0; r < e.nrhs && done; r++)
+        done = host_[e.slot0 + r].phase == PH_DONE;
+      if (done)
+        for (int r = 0; r < e.nrhs; r++)
+          stats[all[orig[j]].slot0 + r] = stats_of(host_[e.slot0 + r]);
+      else
+      {
+        keep.push_back(e);
+        keep_orig.push_back(orig[j]);
+      }
+    }
+    if (keep.empty())
       break;
     if (++rounds > max_rounds)
       return fail(HC_EHIP, "solve: rounds exceeded the bound without every slot finishing");
     chunk = std::min(chunk * 2, 64);
+    if ((int)(inst_.size() - keep.size()) * 4 >= (int)inst_.size())
+    {
+      const TileCandidate * c64[PMAX + 1], *c32[PMAX + 1];
+      for (int p = 0; p <= PMAX; p++)
+      {
+        c64[p] = cands64_[p];
+        c32[p] = cands32_[p];
+      }
+      if ((rc = set(keep, mode_, rhs_block_, force_batched_, c64, c32)))
+        return rc;
+      orig.swap(keep_orig);
+    }
   }
   HC_TRY_HIP(hipEventRecord(e1, st));
   HC_TRY_HIP(hipEventSynchronize(e1));
@@ -410,10 +457,14 @@ Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double *
   (void)hipEventDestroy(e1);
   if (ms)
     *ms = t;
-  stats.resize(nslots_);
-  for (int i = 0; i < nslots_; i++)
-    stats[i] = stats_of(host_[i]);
-  return HC_OK;
+  // the full instance list again (the caller may solve the same batch again)
+  const TileCandidate * c64[PMAX + 1], *c32[PMAX + 1];
+  for (int p = 0; p <= PMAX; p++)
+  {
+    c64[p] = cands64_[p];
+    c32[p] = cands32_[p];
+  }
+  return set(all, mode_, rhs_block_, force_batched_, c64, c32);
 }
 
 } // namespace hc
