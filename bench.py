@@ -103,15 +103,27 @@ def solve_entry(st, nsolves=1):
 
 
 def solves_config2(torch, H, op, x, args):
-    """48^3 p=3 two-region, uniform RHS (seed 2): FP64 Jacobi-PCG vs FP32 PCG + FP64 refinement."""
-    b = torch.empty_like(x)
-    op.fill_uniform(2, b)
-    sol = {"rhs": "uniform [-1,1) per free DOF, seed 2", "rtol": 1e-8}
-    for name, mode in (("fp64", H.SOLVE_FP64), ("fp32_ir", H.SOLVE_FP32_IR)):
-        xs = torch.zeros_like(b)
-        st = op.solve(b, xs, rtol=1e-8, maxit=args.solve_maxit, mode=mode, raise_on_fail=False)
-        sol[name] = solve_entry(st)
-        print(f"[bench] config2 solve {name}: {sol[name]}", file=sys.stderr, flush=True)
+    """48^3 p=3 two-region: FP64 Jacobi-PCG vs FP32 PCG + FP64 refinement to a
+    true relative residual of 1e-8, with a coil current right-hand side
+    (divergence-free, seed 2) and, second, with a uniform random right-hand
+    side (seed 2) whose large gradient content Jacobi-PCG resolves slowly at
+    this sigma contrast (capped at --c2-uniform-maxit iterations)."""
+    sol = {"rtol": 1e-8}
+    for rhs, maxit in (("coil", args.solve_maxit), ("uniform", args.c2_uniform_maxit)):
+        b = torch.empty_like(x)
+        if rhs == "coil":
+            op.rhs_coil(2, b)
+        else:
+            op.fill_uniform(2, b)
+        entry = {"rhs": "coil current J = curl psi, seed 2" if rhs == "coil"
+                 else "uniform [-1,1) per free DOF, seed 2", "maxit": maxit}
+        for name, mode in (("fp64", H.SOLVE_FP64), ("fp32_ir", H.SOLVE_FP32_IR)):
+            xs = torch.zeros_like(b)
+            st = op.solve(b, xs, rtol=1e-8, maxit=maxit, mode=mode, raise_on_fail=False)
+            entry[name] = solve_entry(st)
+            entry[name]["fp64_continuations"] = st["fp64_fallbacks"] if mode == H.SOLVE_FP32_IR else 0
+            print(f"[bench] config2 {rhs} solve {name}: {entry[name]}", file=sys.stderr, flush=True)
+        sol[rhs] = entry
     return sol
 
 
@@ -150,6 +162,111 @@ def solves_config3(torch, H, args):
     return res
 
 
+def _splitmix_u01(seed, i):
+    z = (seed + (i + 1) * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return ((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0)
+
+
+def config4_specs(count=64):
+    """Uneven instances: cube edge n ~ U{8..40}, order p ~ U{1,2,3}, sigma/dt of
+    the inner region [0.25,0.75]^3 and of the outer region each 10^U[-6,4];
+    instance i draws from splitmix64 with seed 4 + i."""
+    out = []
+    for i in range(count):
+        s = 4 + i
+        n = 8 + int(_splitmix_u01(s, 0) * 33)
+        p = 1 + int(_splitmix_u01(s, 1) * 3)
+        s_in = 10.0 ** (-6 + 10 * _splitmix_u01(s, 2))
+        s_out = 10.0 ** (-6 + 10 * _splitmix_u01(s, 3))
+        out.append((n, p, s_in, s_out))
+    return out
+
+
+def build_config4(torch, H, count=64):
+    ops, bs = [], []
+    for i, (n, p, s_in, s_out) in enumerate(config4_specs(count)):
+        op = H.Operator(n, p, 1.0, two_region(n, s_in, s_out)).setup(1.0)
+        ops.append(op)
+        bs.append(op.rhs_coil(1000 + i))
+    torch.cuda.synchronize()
+    return ops, bs
+
+
+def solves_config4(torch, H, ops, bs, args):
+    """64 uneven instances, coil right-hand sides, rtol 1e-8: one batched
+    FP32 PCG + FP64 refinement solve (FP64 continuation per instance where FP32
+    stalls) against the plain FP64 path solving the instances one at a time."""
+    res = {"workload": f"{len(ops)} cubes n~U{{8..40}}, p~U{{1,2,3}}, two regions sigma/dt=10^U[-6,4] "
+                       f"(seed 4+i), coil RHS (seed 1000+i); {sum(o.ndofs for o in ops)} DOFs in total",
+           "rtol": 1e-8}
+    xs = [torch.zeros_like(b) for b in bs]
+    bt = H.Batch()
+    for op, b, x in zip(ops, bs, xs):
+        bt.add(op, b, x, rtol=1e-8, maxit=args.solve_maxit)
+    rc, st = bt.solve(H.SOLVE_FP32_IR, raise_on_fail=False)
+    ms = st[0]["solve_ms"]
+    res["fp32_ir_batched"] = {"solve_ms": round(ms, 1), "solves_per_s": round(1000.0 * len(ops) / ms, 3),
+                              "converged": int(sum(s["converged"] for s in st)),
+                              "fp64_continuations": int(sum(s["fp64_fallbacks"] for s in st)),
+                              "max_true_rel_res": max(s["rel_res"] for s in st),
+                              "iterations_mean": float(np.mean([s["iterations"] for s in st]))}
+    del bt
+    tot, its, conv, worst = 0.0, [], 0, 0.0
+    for op, b in zip(ops, bs):
+        x = torch.zeros_like(b)
+        s1 = op.solve(b, x, rtol=1e-8, maxit=args.solve_maxit, mode=H.SOLVE_FP64, raise_on_fail=False)
+        tot += s1["solve_ms"]
+        its.append(s1["iterations"])
+        conv += int(s1["converged"])
+        worst = max(worst, s1["rel_res"])
+    res["fp64_single"] = {"solve_ms_total": round(tot, 1), "solves_per_s": round(1000.0 * len(ops) / tot, 3),
+                          "converged": conv, "max_true_rel_res": worst, "iterations_mean": float(np.mean(its))}
+    res["speedup_vs_fp64_single"] = round(res["fp32_ir_batched"]["solves_per_s"] /
+                                          res["fp64_single"]["solves_per_s"], 3)
+    print(f"[bench] config4: {res}", file=sys.stderr, flush=True)
+    return res
+
+
+def serve_config5(torch, H, ops, bs, args, fp64_single_sps):
+    """Poisson arrivals (mean 200 req/s, seed 5) of backward-Euler step solves
+    drawn uniformly from the config-4 operators (resident), FP32 PCG + FP64
+    refinement to 1e-8, continuous batching under an HBM budget."""
+    rng = np.random.default_rng(5)
+    nreq = args.serve_requests
+    gaps = rng.exponential(1.0 / 200.0, nreq)
+    arrivals_ms = np.cumsum(gaps) * 1000.0
+    pick = rng.integers(0, len(ops), nreq)
+    us = [torch.zeros_like(bs[k]) for k in pick]
+    srv = H.Server(args.serve_budget_gb * (1 << 30), mode=H.SOLVE_FP32_IR, rtol=1e-8,
+                   maxit=args.solve_maxit, round_iters=16)
+    t0 = time.perf_counter()
+    for i in range(nreq):
+        srv.submit(ops[pick[i]], bs[pick[i]], us[i], 1.0, tag=i, arrival_ms=float(arrivals_ms[i]))
+    out = []
+    while len(out) < nreq:
+        out.extend(srv.poll(wait_ms=1000))
+    wall = time.perf_counter() - t0
+    info = srv.info()
+    srv.close()
+    lat = np.array([r["latency_ms"] for r in out])
+    span_ms = max(r["arrival_ms"] + r["latency_ms"] for r in out)
+    sps = nreq / (span_ms / 1000.0)
+    res = {"workload": f"{nreq} requests, Poisson arrivals mean 200 req/s (seed 5), operators drawn from the "
+                       f"config-4 mix, u_n = 0, rtol 1e-8, FP32 PCG + FP64 refinement",
+           "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+           "mean_queue_ms": round(float(np.mean([r["queue_ms"] for r in out])), 2),
+           "solves_per_s": round(sps, 3), "converged": int(sum(r["converged"] for r in out)),
+           "fp64_continuations": int(sum(r["fp64_fallback"] for r in out)),
+           "max_true_rel_res": max(r["rel_res"] for r in out), "peak_active": info["peak_active"],
+           "budget_gb": args.serve_budget_gb, "peak_budget_use_gb": round(info["peak_bytes"] / 2 ** 30, 2),
+           "rounds": info["rounds"], "wall_s": round(wall, 2),
+           "speedup_vs_fp64_single": round(sps / fp64_single_sps, 3) if fp64_single_sps else None}
+    print(f"[bench] config5: {res}", file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +279,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--solve-maxit", type=int, default=20000)
     ap.add_argument("--c3-fp64-sample", type=int, default=8)
+    ap.add_argument("--c2-uniform-maxit", type=int, default=5000)
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven batch and serving workloads")
+    ap.add_argument("--serve-requests", type=int, default=128)
+    ap.add_argument("--serve-budget-gb", type=int, default=16)
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for the HBM counters)")
     args = ap.parse_args()
@@ -244,8 +365,17 @@ def main():
     if not args.no_solve:
         out["solves"] = {"config2": solves_config2(torch, H, op, x, args),
                          "config3": solves_config3(torch, H, args)}
+        if not args.no_batch:
+            del op
+            torch.cuda.empty_cache()
+            ops4, bs4 = build_config4(torch, H)
+            c4 = solves_config4(torch, H, ops4, bs4, args)
+            out["solves"]["config4"] = c4
+            out["serving"] = serve_config5(torch, H, ops4, bs4, args, c4["fp64_single"]["solves_per_s"])
+            del ops4, bs4
+            torch.cuda.empty_cache()
     if not args.no_cpu:
-        del op
+        op = None
         torch.cuda.empty_cache()
         cb = cpu_baseline(n, p, sig, args.cpu_budget)
         out["cpu_baseline"] = cb
