@@ -742,7 +742,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
     const int Tl[3] = {TX * P, TY * P, TZ * P};
     int gi[3], dims[3];
     bool ok = true, ess = false, shared = false, owner = true;
-    int slot = 0, bit = 1;
+    int slot = 0, bit = 1, alo = 3;
 #pragma unroll
     for (int a = 0; a < 3; a++)
     {
@@ -756,6 +756,8 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
       const bool lo = (l[a] == 0 && tpos[a] > 0);
       const bool hi = (l[a] == Tl[a] && tpos[a] < g.nt[a] - 1);
       shared = shared || lo || hi;
+      if ((lo || hi) && alo == 3)
+        alo = a; // lowest axis of an interior tile plane through the DOF
       if (hi)
       {
         owner = false;
@@ -773,7 +775,19 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
         y[gid] = x[gid];
     }
     else if (shared)
-      part[slot * N + gid] = sOut[i];
+    {
+      // compact entry of the segment (c, alo): row (plane k, position along
+      // the second in-plane axis), entry along the first in-plane axis
+      const int ga = (alo == 0) ? gi[0] : ((alo == 1) ? gi[1] : gi[2]);
+      const int ta = (alo == 0) ? TX * P : ((alo == 1) ? TY * P : TZ * P);
+      const int j0 = (alo == 0) ? gi[1] : gi[0];
+      const int j1 = (alo == 2) ? gi[1] : gi[2];
+      const int n0 = (alo == 0) ? dims[1] : dims[0];
+      const int n1 = (alo == 2) ? dims[1] : dims[2];
+      const int64_t eo = ad.segs.eoff[c][alo];
+      const int64_t pid = eo + ((int64_t)(ga / ta - 1) * n1 + j1) * n0 + j0;
+      part[slot * N + pid] = sOut[i];
+    }
     else
       y[gid] = sOut[i];
   }

@@ -171,6 +171,7 @@ k_assemble_planes(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc)
   const int a2 = 3 - c - a; // the other closed axis of component c
   const int step2 = g.T[a2] * g.p;
   const int dlo = (c == 0) ? 1 : 0;
+  const int64_t ebase = d.segs.eoff[c][a] + (int64_t)row * n0;
   for (int j0 = threadIdx.x; j0 < n0; j0 += blockDim.x)
   {
     int gi[3];
@@ -195,13 +196,14 @@ k_assemble_planes(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc)
     // slot bits: bit 0 = lower-numbered closed axis, bit 1 = the other
     const bool on_lo = (a == dlo) ? true : on2;
     const bool on_hi = (a == dlo) ? on2 : true;
-    T v = part[gid];
+    const int64_t pid = ebase + j0; // compact entry (contiguous in j0)
+    T v = part[pid];
     if (on_lo)
-      v += part[N + gid];
+      v += part[N + pid];
     if (on_hi)
-      v += part[2 * N + gid];
+      v += part[2 * N + pid];
     if (on_lo && on_hi)
-      v += part[3 * N + gid];
+      v += part[3 * N + pid];
     y[gid] = v;
   }
 }
@@ -323,15 +325,19 @@ make_plane_segs(const Geom & g)
   PlaneSegs s{};
   s.nseg = 0;
   s.start[0] = 0;
+  int64_t e = 0;
   for (int c = 0; c < 3; c++)
     for (int a = 0; a < 3; a++)
     {
       if (a == c || g.nt[a] <= 1)
         continue;
+      const int b0 = (a == 0) ? 1 : 0;
       const int b1 = (a == 2) ? 1 : 2;
       const int64_t rows = (int64_t)(g.nt[a] - 1) * g.cd[c][b1];
       s.comp[s.nseg] = c;
       s.axis[s.nseg] = a;
+      s.eoff[c][a] = e;
+      e += rows * g.cd[c][b0];
       s.start[s.nseg + 1] = s.start[s.nseg] + rows;
       s.nseg++;
     }

@@ -12,14 +12,19 @@ struct TileShape
   int tx, ty, tz, nthr;
 };
 
-// Segments (component, plane axis) of the tile-face assembly; start[] is the
-// prefix of plane rows.
+// Segments (component c, plane axis a) of the tile-face assembly: the DOFs of
+// component c on the interior tile planes normal to axis a, stored compactly
+// as rows (plane k, position along the second in-plane axis) of entries along
+// the first in-plane axis.  start[] is the prefix of rows, eoff[c][a] the first
+// entry of segment (c, a) in the compact partial arrays.  A DOF on planes of
+// two axes belongs to the segment of the lower-numbered axis.
 struct PlaneSegs
 {
   int nseg;
   int comp[6];
   int axis[6];
   int64_t start[7];
+  int64_t eoff[3][3];
 };
 
 // One operator instance of an apply launch.  Work items are (tile, rhs group)
@@ -34,7 +39,7 @@ struct ApplyDesc
   const void * qd; // quadrature data (element type of the launch)
   const void * x;
   void * y;
-  void * part;     // 4 N per rhs
+  void * part;     // 4 N per rhs: slot s of compact plane entry j at s*N + j
   const int * done;// per-rhs flags at done[r * done_stride], or null: rhs r
   int done_stride; // is applied only while its flag equals `gate`
   int gate = 0;
