@@ -43,7 +43,21 @@ struct Tile
   static constexpr int pad_to(int v, int m, int r) { return ((v - r + m - 1) / m) * m + r; }
   static constexpr int SA = pad_to(5 * LA, 16, 4);
   static constexpr int SB = pad_to(3 * LB, 16, 4);
+  // wave-synchronous stages: when the (p+1)^2 pencils, stage items and
+  // transposed-stage items of an element always fall in one wave (64 divisible
+  // by (p+1)^2, whole waves of pencils, no split of the qz range), every LDS
+  // exchange between the forward and backward stages stays inside a wave and
+  // needs no workgroup barrier -- only the row walkers and the write-out do.
+  static constexpr bool WS = (SP == 1) && (64 % QQ == 0) && (NP % 64 == 0);
 };
+
+// Order the LDS traffic of one wave (all lanes of a wavefront run in lockstep,
+// so completing its own LDS operations is enough).
+__device__ inline void
+wave_lds_sync()
+{
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+}
 
 // BATCH = false: one instance, described by the by-value argument d0 (its
 // fields stay in scalar registers), one rhs per work item.  BATCH = true: the
@@ -59,6 +73,13 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   constexpr int O = C::O, D = C::D, Q = C::Q, QQ = C::QQ;
   constexpr int SQX = C::SQX, LA = C::LA, SDZ = C::SDZ, LB = C::LB, SA = C::SA, SB = C::SB;
   constexpr int NT = C::NT, TE = C::TE;
+  // barrier between stages whose LDS exchanges may cross waves
+  auto stage_sync = [&]() {
+    if constexpr (C::WS)
+      wave_lds_sync();
+    else
+      __syncthreads();
+  };
   // ---- work item: (instance, tile, rhs group), group fastest; a group is
   // rb consecutive right-hand sides (rb = 1 unless BATCH) ----
   const int w = blockIdx.x;
@@ -257,7 +278,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
         }
       }
     }
-    __syncthreads();
+    stage_sync();
     // ---- stage y: A -> B (contract dy -> qy) ----
     {
       constexpr int NJ = TE * Q * D;
@@ -341,7 +362,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
         }
       }
     }
-    __syncthreads();
+    stage_sync();
     // ---- stage z: B -> registers (contract dz -> qz), own qz range ----
     if (pact)
     {
@@ -491,7 +512,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
       if (pact && ph == h)
         ztr(true);
     }
-    __syncthreads();
+    stage_sync();
     // ---- y^T: S -> T (A region), contract qy -> dy ----
     {
       constexpr int NJ = TE * Q * D;
@@ -573,12 +594,14 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
         }
       }
     }
-    __syncthreads();
+    stage_sync();
     // (the barrier above also orders this y^T before the next z^T overwrites B)
   };
   backward(std::integral_constant<int, 0>{});
   backward(std::integral_constant<int, 1>{});
   backward(std::integral_constant<int, 2>{});
+  if constexpr (C::WS)
+    __syncthreads(); // the row walkers read the T arrays of other waves' elements
 
   // ======================= row walkers =======================
   // One wave per component walks the tile's DOF rows of that component along
@@ -826,16 +849,20 @@ launch_one(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, 
 #define HC_CAND(P, TX, TY, TZ, M, SP)                                                                            \
   {P, {TX, TY, TZ, Tile<P, TX, TY, TZ, SP>::NT}, M + 4 * (SP - 1), &launch_one<P, TX, TY, TZ, M, SP>}
 
+#define HC_QP(P, TX, TY, TZ) {P, {TX, TY, TZ, 0}, MODE_QP, &launch_qp<P, TX, TY, TZ>}
+
 static const TileCandidate g_cands[] = {
     // first entry of each order = default before tuning
     HC_CAND(1, 4, 4, 4, 1, 2), HC_CAND(1, 4, 4, 4, 1, 1), HC_CAND(1, 8, 4, 2, 1, 1), HC_CAND(1, 8, 4, 2, 0, 2),
-    HC_CAND(1, 8, 8, 1, 0, 1),
+    HC_CAND(1, 8, 8, 1, 0, 1), HC_QP(1, 4, 4, 4), HC_QP(1, 8, 4, 2),
     HC_CAND(2, 4, 2, 2, 1, 1), HC_CAND(2, 4, 2, 2, 1, 2), HC_CAND(2, 4, 2, 2, 0, 2), HC_CAND(2, 4, 4, 2, 1, 1),
-    HC_CAND(2, 4, 4, 2, 0, 2), HC_CAND(2, 8, 2, 2, 0, 1),
+    HC_CAND(2, 4, 4, 2, 0, 2), HC_CAND(2, 8, 2, 2, 0, 1), HC_QP(2, 4, 2, 2), HC_QP(2, 4, 4, 2),
     HC_CAND(3, 4, 2, 2, 1, 2), HC_CAND(3, 4, 2, 2, 1, 1), HC_CAND(3, 4, 2, 2, 0, 2), HC_CAND(3, 4, 4, 1, 1, 2),
-    HC_CAND(3, 2, 2, 4, 1, 2), HC_CAND(3, 4, 2, 2, 0, 1),
+    HC_CAND(3, 2, 2, 4, 1, 2), HC_CAND(3, 4, 2, 2, 0, 1), HC_CAND(3, 2, 2, 2, 1, 1), HC_CAND(3, 2, 2, 2, 0, 1),
+    HC_CAND(3, 4, 2, 1, 1, 1), HC_QP(3, 4, 2, 2), HC_QP(3, 2, 2, 2), HC_QP(3, 4, 2, 1),
 };
 #undef HC_CAND
+#undef HC_QP
 
 int
 tile_candidates(int p, const TileCandidate ** out)

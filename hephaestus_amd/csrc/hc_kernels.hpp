@@ -78,9 +78,14 @@ struct TileCandidate
 {
   int p;
   TileShape shape;
-  int mode; // 0: qdata loaded at the quadrature stage, 1: at entry, 2: persistent pipelined
+  int mode; // 0: qdata loaded at the quadrature stage, 1: at entry; +4: split qz range; 8: qp kernel
   TileLaunch launch;
 };
+// The one-lane-per-quadrature-point tile kernel (hc_apply_qp.hip), mode 8.
+template <int P, int TX, int TY, int TZ>
+hipError_t launch_qp(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, const void * tables,
+                     int prec_bits, hipStream_t st);
+constexpr int MODE_QP = 8;
 // Candidates of order p (contiguous array); returns their count.
 int tile_candidates(int p, const TileCandidate ** out);
 TileShape default_tile(int p);
