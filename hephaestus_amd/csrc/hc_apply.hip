@@ -47,7 +47,7 @@ struct Tile
   // transposed-stage items of an element always fall in one wave (64 divisible
   // by (p+1)^2, whole waves of pencils, no split of the qz range), every LDS
   // exchange between the forward and backward stages stays inside a wave and
-  // needs no workgroup barrier -- only the row walkers and the write-out do.
+  // needs no workgroup barrier -- only the x^T rows and the write-out do.
   static constexpr bool WS = (SP == 1) && (64 % QQ == 0) && (NP % 64 == 0);
 };
 
@@ -82,14 +82,14 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   };
   // ---- work item: (instance, tile, rhs group), group fastest; a group is
   // rb consecutive right-hand sides (rb = 1 unless BATCH) ----
-  const int w = blockIdx.x;
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
   int di = 0;
   if constexpr (BATCH)
     di = __builtin_amdgcn_readfirstlane(find_instance(w, ndesc, [&](int i) { return descs[i].w0; }));
   const ApplyDesc & ad = BATCH ? descs[di] : d0;
-  const int wl = w - ad.w0;
   const int rb = BATCH ? ad.rb : 1;
   const int ngrp = BATCH ? (ad.nrhs + rb - 1) / rb : ad.nrhs;
+  const int wl = w - ad.w0;
   const int r_begin = (wl % ngrp) * rb;
   const int r_end = BATCH ? min(ad.nrhs, r_begin + rb) : r_begin + 1;
   const int tcur = wl / ngrp;
@@ -217,8 +217,20 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   };
   // MODE 0: quadrature data loaded at the quadrature-point stage (fewest
   // registers); MODE 1: at entry, hidden behind the forward contractions.
+  // Vector-memory loads complete in issue order (one counter): for a single
+  // rhs the x rows go first, so stage x waits for them alone and the
+  // quadrature data stays in flight through the forward contractions.
+  bool x_ready = false;
   if (MODE >= 1)
+  {
+    if constexpr (!BATCH)
+    {
+      x = (const T *)ad.x + (int64_t)r_begin * g.off[3];
+      load_x(e0, ne);
+      x_ready = true;
+    }
     load_qf(e0, ne);
+  }
 
   for (int rhs = r_begin; rhs < r_end; rhs++)
   {
@@ -227,7 +239,9 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   x = (const T *)ad.x + (int64_t)rhs * g.off[3];
   y = (T *)ad.y + (int64_t)rhs * g.off[3];
   part = (T *)ad.part + (int64_t)rhs * 4 * g.off[3];
-  load_x(e0, ne);
+  if (!x_ready)
+    load_x(e0, ne);
+  x_ready = false;
   T U[3][QH], Cu[3][QH];
 #pragma unroll
   for (int j = 0; j < QH; j++)
@@ -423,6 +437,11 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   forward(std::integral_constant<int, 0>{});
   forward(std::integral_constant<int, 1>{});
   forward(std::integral_constant<int, 2>{});
+  // With SP > 1 the groups sharing a pencil sit in different waves: all of
+  // them must have read the pencil's B entries (stage z of the last
+  // component) before z^T below overwrites them.
+  if constexpr (SP > 1)
+    __syncthreads();
 
   // ======================= quadrature points =======================
   if (MODE == 0)
@@ -601,184 +620,155 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   backward(std::integral_constant<int, 1>{});
   backward(std::integral_constant<int, 2>{});
   if constexpr (C::WS)
-    __syncthreads(); // the row walkers read the T arrays of other waves' elements
+    __syncthreads(); // the x^T rows read the T arrays of other waves' elements
 
-  // ======================= row walkers =======================
-  // One wave per component walks the tile's DOF rows of that component along
-  // x: it sums the T tensors of the (at most four) elements sharing the row in
-  // a fixed order (lower element first) and contracts along x.  For the
-  // components with a closed x axis the DOF on an element x-face inside the
-  // tile is (value of column lx-1 at dx = P) + (value of column lx at dx = 0),
-  // carried in a register.  Results go to an LDS box (B region, free now) that
-  // all threads then write out with x-contiguous stores.
+  // ======================= x^T rows =======================
+  // Item (component, DOF row (iy, iz) of the tile box, element column lx):
+  // sums the T tensors of the (at most four) elements sharing the row in a
+  // fixed order (lower element first), contracts along x and stores the
+  // column's DOFs in an LDS box (B region, free now).  For the components
+  // with a closed x axis the DOF on an element x-face inside the tile is
+  // (column lx-1 at dx = P) + (column lx at dx = 0): column lx-1 stores its
+  // dx = P value in the box, column lx keeps its dx = 0 value in sC, and a
+  // second pass adds the two (a sum of two terms, so the order is immaterial).
   constexpr int BX0 = TX * P, BX1 = TY * P + 1, BX2 = TZ * P + 1; // comp x box
   constexpr int BY0 = TX * P + 1, BY1 = TY * P, BY2 = TZ * P + 1; // comp y box
   constexpr int BZ0 = TX * P + 1, BZ1 = TY * P + 1, BZ2 = TZ * P; // comp z box
   constexpr int OB1 = BX0 * BX1 * BX2, OB2 = OB1 + BY0 * BY1 * BY2, OBN = OB2 + BZ0 * BZ1 * BZ2;
-  static_assert(OBN <= TE * SB, "output box must fit in the B region");
+  constexpr int NR0 = BX1 * BX2 * TX, NR1 = NR0 + BY1 * BY2 * TX, NRN = NR1 + BZ1 * BZ2 * TX;
+  static_assert(OBN + NRN <= TE * SB, "output box and x-face values must fit in the B region");
   T * const sOut = sB;
-  auto rows = [&](auto cc, int lane) {
+  T * const sC = sB + OBN;
+  auto xrow = [&](auto cc, int it, int j) {
     constexpr int c = decltype(cc)::value;
     constexpr int RY = (c == 1) ? TY * P : TY * P + 1;
-    constexpr int RZ = (c == 2) ? TZ * P : TZ * P + 1;
     constexpr int RX = (c == 0) ? TX * P : TX * P + 1;
+    const int lx = j % TX, r = j / TX;
+    if (lx >= ne[0])
+      return;
+    const int iy = r % RY, iz = r / RY;
+    const int ehy = iy / P, lhy = iy % P, ehz = iz / P, lhz = iz % P;
+    const int ey[2] = {ehy - 1, ehy}, dyc[2] = {P, lhy};
+    const int ez[2] = {ehz - 1, ehz}, dzc[2] = {P, lhz};
+    const bool vy[2] = {c != 1 && lhy == 0 && ehy > 0, c == 1 || ehy < ne[1]};
+    const bool vz[2] = {c != 2 && lhz == 0 && ehz > 0, c == 2 || ehz < ne[2]};
+    if (!(vy[0] || vy[1]) || !(vz[0] || vz[1]))
+      return;
     const T * TA = sA + (c == 0 ? 0 : (c == 1 ? LA : 3 * LA));
-    T * const OUT = sOut + (c == 0 ? 0 : (c == 1 ? OB1 : OB2));
-    for (int r = lane; r < RY * RZ; r += 64)
+    T * const orow = sOut + (c == 0 ? 0 : (c == 1 ? OB1 : OB2)) + r * RX + lx * P;
+    if (c == 0)
     {
-      const int iy = r % RY, iz = r / RY;
-      const int ehy = iy / P, lhy = iy % P, ehz = iz / P, lhz = iz % P;
-      const int ey[2] = {ehy - 1, ehy}, dyc[2] = {P, lhy};
-      const int ez[2] = {ehz - 1, ehz}, dzc[2] = {P, lhz};
-      const bool vy[2] = {c != 1 && lhy == 0 && ehy > 0, c == 1 || ehy < ne[1]};
-      const bool vz[2] = {c != 2 && lhz == 0 && ehz > 0, c == 2 || ehz < ne[2]};
-      if (!(vy[0] || vy[1]) || !(vz[0] || vz[1]))
-        continue;
-      T * const orow = OUT + (iz * RY + iy) * RX;
-      if (c == 0)
-      {
+      T ts[Q];
 #pragma unroll
-        for (int lx = 0; lx < TX; lx++)
+      for (int q = 0; q < Q; q++)
+        ts[q] = T(0);
+#pragma unroll
+      for (int kz = 0; kz < 2; kz++)
+#pragma unroll
+        for (int ky = 0; ky < 2; ky++)
         {
-          if (lx >= ne[0])
-            break;
-          T ts[Q];
+          if (!vz[kz] || !vy[ky])
+            continue;
+          const T * T1 = TA + ((ez[kz] * TY + ey[ky]) * TX + lx) * SA + dzc[kz] * D + dyc[ky];
 #pragma unroll
           for (int q = 0; q < Q; q++)
-            ts[q] = T(0);
-#pragma unroll
-          for (int kz = 0; kz < 2; kz++)
-#pragma unroll
-            for (int ky = 0; ky < 2; ky++)
-            {
-              if (!vz[kz] || !vy[ky])
-                continue;
-              const T * T1 = TA + ((ez[kz] * TY + ey[ky]) * TX + lx) * SA + dzc[kz] * D + dyc[ky];
-#pragma unroll
-              for (int q = 0; q < Q; q++)
-                ts[q] += T1[q * SQX];
-            }
-#pragma unroll
-          for (int d = 0; d < O; d++)
-          {
-            T v = 0;
-#pragma unroll
-            for (int q = 0; q < Q; q++)
-              v += tb.Bo[q * O + d] * ts[q];
-            orow[lx * P + d] = v;
-          }
+            ts[q] += T1[q * SQX];
         }
+#pragma unroll
+      for (int d = 0; d < O; d++)
+      {
+        T v = 0;
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+          v += tb.Bo[q * O + d] * ts[q];
+        orow[d] = v;
       }
-      else
-      {
-        T carry = 0;
+    }
+    else
+    {
+      T ta[Q], tb2[Q];
 #pragma unroll
-        for (int lx = 0; lx < TX; lx++)
+      for (int q = 0; q < Q; q++)
+        ta[q] = tb2[q] = T(0);
+#pragma unroll
+      for (int kz = 0; kz < 2; kz++)
+#pragma unroll
+        for (int ky = 0; ky < 2; ky++)
         {
-          if (lx >= ne[0])
-            break;
-          T ta[Q], tb2[Q];
+          if (!vz[kz] || !vy[ky])
+            continue;
+          const T * T1 = TA + ((ez[kz] * TY + ey[ky]) * TX + lx) * SA + dzc[kz] * D + dyc[ky];
 #pragma unroll
           for (int q = 0; q < Q; q++)
-            ta[q] = tb2[q] = T(0);
-#pragma unroll
-          for (int kz = 0; kz < 2; kz++)
-#pragma unroll
-            for (int ky = 0; ky < 2; ky++)
-            {
-              if (!vz[kz] || !vy[ky])
-                continue;
-              const T * T1 = TA + ((ez[kz] * TY + ey[ky]) * TX + lx) * SA + dzc[kz] * D + dyc[ky];
-#pragma unroll
-              for (int q = 0; q < Q; q++)
-              {
-                ta[q] += T1[q * SQX];
-                tb2[q] += T1[LA + q * SQX];
-              }
-            }
-          const bool last = (lx == ne[0] - 1);
-#pragma unroll
-          for (int d = 0; d <= P; d++)
           {
-            T v = 0;
-#pragma unroll
-            for (int q = 0; q < Q; q++)
-              v += tb.Bc[q * D + d] * ta[q] + tb.Gc[q * D + d] * tb2[q];
-            if (d == 0 && lx > 0)
-              v = carry + v;
-            if (d == P)
-            {
-              carry = v;
-              if (!last)
-                continue;
-            }
-            orow[lx * P + d] = v;
+            ta[q] += T1[q * SQX];
+            tb2[q] += T1[LA + q * SQX];
           }
         }
+#pragma unroll
+      for (int d = 0; d <= P; d++)
+      {
+        T v = 0;
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+          v += tb.Bc[q * D + d] * ta[q] + tb.Gc[q * D + d] * tb2[q];
+        if (d == 0 && lx > 0)
+          sC[it] = v;
+        else
+          orow[d] = v;
       }
     }
   };
+  for (int it = tid; it < NRN; it += NT)
   {
-    // component cc is walked by wave (cc mod number of waves)
-    constexpr int NW = NT / 64;
-    const int grp = tid >> 6, lane = tid & 63;
-    if (0 % NW == grp)
-      rows(std::integral_constant<int, 0>{}, lane);
-    if (1 % NW == grp)
-      rows(std::integral_constant<int, 1>{}, lane);
-    if (2 % NW == grp)
-      rows(std::integral_constant<int, 2>{}, lane);
+    if (it < NR0)
+      xrow(std::integral_constant<int, 0>{}, it, it);
+    else if (it < NR1)
+      xrow(std::integral_constant<int, 1>{}, it, it - NR0);
+    else
+      xrow(std::integral_constant<int, 2>{}, it, it - NR1);
+  }
+  __syncthreads();
+  // second pass: x-faces between the tile's element columns (closed-x components);
+  // rows outside the active extent hold unused values and are never written out
+  for (int it = NR0 + tid; it < NRN; it += NT)
+  {
+    const bool c1 = it < NR1;
+    const int j = c1 ? it - NR0 : it - NR1;
+    const int lx = j % TX, r = j / TX;
+    if (lx == 0 || lx >= ne[0])
+      continue;
+    T * const o = sOut + (c1 ? OB1 : OB2) + r * (TX * P + 1) + lx * P;
+    *o = *o + sC[it];
   }
   __syncthreads();
   // ---- write-out: interior DOFs to y, DOFs on a face shared with another
   // tile to the partials (slot bit0 = lower-numbered closed axis, bit1 = the
   // other; set when this tile holds the face as its upper face), PEC DOFs copy
   // x (written by the tile that holds them as a lower face) ----
-  for (int i = tid; i < OBN; i += NT)
-  {
-    int c, r, b0, b1;
-    if (i < OB1)
-    {
-      c = 0;
-      r = i;
-      b0 = BX0;
-      b1 = BX1;
-    }
-    else if (i < OB2)
-    {
-      c = 1;
-      r = i - OB1;
-      b0 = BY0;
-      b1 = BY1;
-    }
-    else
-    {
-      c = 2;
-      r = i - OB2;
-      b0 = BZ0;
-      b1 = BZ1;
-    }
+  auto wout = [&](auto cc, int i, int r) {
+    constexpr int c = decltype(cc)::value;
+    constexpr int B0 = (c == 0) ? TX * P : TX * P + 1;
+    constexpr int B1 = (c == 1) ? TY * P : TY * P + 1;
+    constexpr int Tl[3] = {TX * P, TY * P, TZ * P};
     int l[3];
-    l[0] = r % b0;
-    l[1] = (r / b0) % b1;
-    l[2] = r / (b0 * b1);
-    const int Tl[3] = {TX * P, TY * P, TZ * P};
+    l[0] = r % B0;
+    l[1] = (r / B0) % B1;
+    l[2] = r / (B0 * B1);
     int gi[3], dims[3];
-    bool ok = true, ess = false, shared = false, owner = true;
+    bool ok = true, ess = false, owner = true;
     int slot = 0, bit = 1, alo = 3;
 #pragma unroll
     for (int a = 0; a < 3; a++)
     {
       gi[a] = e0[a] * P + l[a];
-      const int dim = g.n[a] * P + (a != c ? 1 : 0);
-      dims[a] = dim;
-      ok = ok && gi[a] < dim;
+      dims[a] = g.n[a] * P + (a != c ? 1 : 0);
+      ok = ok && gi[a] < dims[a];
       if (a == c)
         continue;
-      ess = ess || (gi[a] == 0 && ((pec >> (2 * a)) & 1)) || (gi[a] == dim - 1 && ((pec >> (2 * a + 1)) & 1));
+      ess = ess || (gi[a] == 0 && ((pec >> (2 * a)) & 1)) || (gi[a] == dims[a] - 1 && ((pec >> (2 * a + 1)) & 1));
       const bool lo = (l[a] == 0 && tpos[a] > 0);
       const bool hi = (l[a] == Tl[a] && tpos[a] < g.nt[a] - 1);
-      shared = shared || lo || hi;
       if ((lo || hi) && alo == 3)
         alo = a; // lowest axis of an interior tile plane through the DOF
       if (hi)
@@ -789,30 +779,40 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
       bit <<= 1;
     }
     if (!ok)
-      continue;
-    const int64_t offc = (c == 0) ? g.off[0] : ((c == 1) ? g.off[1] : g.off[2]);
-    const int64_t gid = offc + ((int64_t)gi[2] * dims[1] + gi[1]) * dims[0] + gi[0];
+      return;
+    // index within the component (each component holds fewer than 2^32 DOFs)
+    const uint32_t li = ((uint32_t)gi[2] * (uint32_t)dims[1] + (uint32_t)gi[1]) * (uint32_t)dims[0] + (uint32_t)gi[0];
+    const int64_t gid = g.off[c] + (int64_t)li;
     if (ess)
     {
       if (owner)
         y[gid] = x[gid];
     }
-    else if (shared)
+    else if (alo != 3)
     {
       // compact entry of the segment (c, alo): row (plane k, position along
-      // the second in-plane axis), entry along the first in-plane axis
-      const int ga = (alo == 0) ? gi[0] : ((alo == 1) ? gi[1] : gi[2]);
-      const int ta = (alo == 0) ? TX * P : ((alo == 1) ? TY * P : TZ * P);
+      // the second in-plane axis), entry along the first in-plane axis; the
+      // plane through the DOF is the tile's lower (l = 0) or upper face
+      const int k = ((alo == 0) ? tpos[0] + (l[0] != 0) : ((alo == 1) ? tpos[1] + (l[1] != 0) : tpos[2] + (l[2] != 0))) - 1;
       const int j0 = (alo == 0) ? gi[1] : gi[0];
       const int j1 = (alo == 2) ? gi[1] : gi[2];
       const int n0 = (alo == 0) ? dims[1] : dims[0];
       const int n1 = (alo == 2) ? dims[1] : dims[2];
-      const int64_t eo = ad.segs.eoff[c][alo];
-      const int64_t pid = eo + ((int64_t)(ga / ta - 1) * n1 + j1) * n0 + j0;
+      const int64_t eo = (alo == 0) ? ad.segs.eoff[c][0] : ((alo == 1) ? ad.segs.eoff[c][1] : ad.segs.eoff[c][2]);
+      const int64_t pid = eo + ((int64_t)k * n1 + j1) * n0 + j0;
       part[slot * N + pid] = sOut[i];
     }
     else
       y[gid] = sOut[i];
+  };
+  for (int i = tid; i < OBN; i += NT)
+  {
+    if (i < OB1)
+      wout(std::integral_constant<int, 0>{}, i, i);
+    else if (i < OB2)
+      wout(std::integral_constant<int, 1>{}, i, i - OB1);
+    else
+      wout(std::integral_constant<int, 2>{}, i, i - OB2);
   }
   if (BATCH && rhs + 1 < r_end)
     __syncthreads(); // the next rhs overwrites the LDS stages

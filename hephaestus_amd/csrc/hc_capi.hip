@@ -236,6 +236,12 @@ hc_op_create(const hc_mesh_desc * mesh, int order, const double * nu_e, const do
     return fail(HC_EINVAL, "hc_op_create: order must be 1.." + std::to_string(hc::PMAX));
   if (mesh->nx < 1 || mesh->ny < 1 || mesh->nz < 1)
     return fail(HC_EINVAL, "hc_op_create: mesh needs at least one element per direction");
+  {
+    // the kernels index DOFs within one component with 32-bit integers
+    const int64_t m[3] = {(int64_t)mesh->nx * order + 1, (int64_t)mesh->ny * order + 1, (int64_t)mesh->nz * order + 1};
+    if (m[0] * m[1] * m[2] >= (int64_t(1) << 31))
+      return fail(HC_EINVAL, "hc_op_create: mesh too large (each vector component must hold fewer than 2^31 DOFs)");
+  }
   if (!(precision & (HC_PREC_FP64 | HC_PREC_FP32)))
     precision = HC_PREC_FP64;
   hc_op_t op = new hc_op_s();

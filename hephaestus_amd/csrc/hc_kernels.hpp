@@ -70,6 +70,20 @@ find_instance(int w, int ndesc, F key)
   return lo;
 }
 
+// Workgroups are dispatched to the GPU's 8 XCDs round-robin (blockIdx mod 8),
+// each XCD with its own L2.  This renumbering gives every XCD a contiguous
+// range of work items, so neighbouring tiles (which read the same x halo rows
+// and write parts of the same cache lines of y) run on one XCD at about the
+// same time and meet in its L2.  A bijection of [0, nb).
+constexpr int NXCD = 8;
+__device__ inline int
+xcd_remap(int b, int nb)
+{
+  const int q = nb / NXCD, r = nb % NXCD;
+  const int k = b % NXCD, i = b / NXCD;
+  return k * q + (k < r ? k : r) + i;
+}
+
 // A compiled tile variant of the apply kernel (shape + load mode).
 // descs == nullptr: the single instance d0; otherwise ndesc instances.
 using TileLaunch = hipError_t (*)(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork,

@@ -160,6 +160,60 @@ def test_every_tile_variant(torch_cuda, hc, oracle, p):
     assert info["tile64"] in [t for t in tiles]
 
 
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_many_tiles_multi_rhs(torch_cuda, hc, oracle, p):
+    """Every tile variant on a mesh of several hundred tiles (more workgroups
+    than the XCD renumbering's 8 groups times the CUs), three right-hand sides
+    in one launch over (tile, rhs) work items: oracle parity, and the single
+    rhs apply equals its row of the multi-rhs apply bitwise."""
+    torch = torch_cuda
+    from hephaestus_amd import hcurl
+    n = {1: (40, 36, 30), 2: (30, 26, 22), 3: (24, 22, 20)}[p]
+    print(f"case many tiles p={p} n={n}", flush=True)
+    rng = np.random.default_rng(500 + p)
+    m = oracle.Mesh(*n)
+    E = m.nelem
+    nu, sig = rng.uniform(0.5, 2.0, E), 10.0 ** rng.uniform(-3, 3, E)
+    qd = oracle.pa_setup(m, p, nu, sig)
+    ess = oracle.ess_mask(m, p)
+    N = oracle.ndofs(m, p)
+    X = np.stack([oracle.rand_uniform(31 + r, N) for r in range(3)])
+    Y = np.stack([oracle.pa_apply(m, p, qd, ess, X[r]) for r in range(3)])
+    op = hc.Operator(n, p, nu, sig).setup(1.0)
+    op.set_rhs_block(1)
+    xd = _dev(torch, X)
+    for (tx, ty, tz, pf) in hcurl.tile_list(p):
+        op.set_tile(64, tx, ty, tz, pf)
+        y = op.apply(xd)
+        for r in range(3):
+            assert _rel(y[r].cpu().numpy(), Y[r]) <= 1e-13, (tx, ty, tz, pf, r)
+        assert torch.equal(op.apply(xd[1]), y[1]), (tx, ty, tz, pf)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_tile_variants_repeatable(torch_cuda, hc, p):
+    """Every tile variant gives bitwise identical results over repeated applies
+    on a mesh of several hundred tiles (a race between the waves of a
+    workgroup shows up as an occasional difference)."""
+    torch = torch_cuda
+    from hephaestus_amd import hcurl
+    n = {1: (40, 36, 30), 2: (30, 26, 22), 3: (24, 22, 20)}[p]
+    print(f"case repeatable tiles p={p} n={n}", flush=True)
+    rng = np.random.default_rng(600 + p)
+    E = n[0] * n[1] * n[2]
+    op = hc.Operator(n, p, rng.uniform(0.5, 2.0, E), 10.0 ** rng.uniform(-3, 3, E)).setup(1.0)
+    x = torch.empty(op.ndofs, dtype=torch.float64, device="cuda")
+    op.fill_uniform(41, x)
+    x32 = x.float()
+    for t in hcurl.tile_list(p):
+        op.set_tile(64, *t)
+        op.set_tile(32, *t)
+        y0, z0 = op.apply(x), op.apply_f32(x32)
+        for _ in range(8):
+            assert torch.equal(op.apply(x), y0), t
+            assert torch.equal(op.apply_f32(x32), z0), t
+
+
 def test_setup_dt_rule(torch_cuda, hc, oracle):
     """hc_op_setup re-assembles only when dt changes by more than 1e-12 dt
     (TimeDependentEquationSystem::SetTimeStep, equation_system.cpp:413-426)."""
