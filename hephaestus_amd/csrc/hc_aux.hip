@@ -1,0 +1,160 @@
+// Auxiliary-space (gradient) correction: node diagonal of G^T A G from the
+// quadrature data, and the standalone preconditioner apply (the batched PCG
+// kernels use the same device functions, hc_aux.hpp).
+#include "hc_aux.hpp"
+#include "hc_kernels.hpp"
+
+namespace hc
+{
+
+AuxGeom
+make_aux_geom(const Geom & g, const double * dg1d)
+{
+  AuxGeom a{};
+  a.p = g.p;
+  int64_t nn = 1;
+  for (int d = 0; d < 3; d++)
+  {
+    a.n[d] = g.n[d];
+    a.M[d] = g.n[d] * g.p + 1;
+    nn *= a.M[d];
+  }
+  a.NN = (int)nn;
+  for (int c = 0; c < 3; c++)
+    for (int d = 0; d < 3; d++)
+      a.cd[c][d] = (int)g.cd[c][d];
+  for (int c = 0; c < 4; c++)
+    a.off[c] = g.off[c];
+  for (int i = 0; i < g.p * (g.p + 1); i++)
+    a.dg[i] = dg1d[i];
+  return a;
+}
+
+// Dg at the nodes of the elements of one colour (ex, ey, ez parities = bits
+// of col): elements of one colour share no node, so each adds its element
+// term to its nodes without conflicts and the colours run in a fixed order.
+// One workgroup per element, one thread per element node (a fastest):
+//   sum_q grad phi_n(q)^T Qm(q) grad phi_n(q)
+// with Qm the symmetric mass factors 0..5 of the point (the operator's sigma
+// term; the curl term vanishes on gradients) and grad phi_n in reference
+// coordinates (the ND basis' own).
+template <int P>
+__global__ void __launch_bounds__(64)
+k_aux_diag(Geom g, Tables<double> tb, const double * __restrict__ qd, int col, double * __restrict__ dg)
+{
+  constexpr int D = P + 1, Q = P + 1, QQ = Q * Q;
+  const int hx = (g.n[0] - (col & 1) + 1) / 2, hy = (g.n[1] - ((col >> 1) & 1) + 1) / 2;
+  const int b = blockIdx.x;
+  const int ex = 2 * (b % hx) + (col & 1);
+  const int ey = 2 * ((b / hx) % hy) + ((col >> 1) & 1);
+  const int ez = 2 * (b / (hx * hy)) + ((col >> 2) & 1);
+  if (ez >= g.n[2] || threadIdx.x >= D * D * D)
+    return;
+  const int a = threadIdx.x % D, bb = (threadIdx.x / D) % D, c = threadIdx.x / (D * D);
+  const int64_t e = ((int64_t)ez * g.n[1] + ey) * g.n[0] + ex;
+  const double * qe = qd + e * (12 * Q * QQ);
+  double s = 0.0;
+  for (int qz = 0; qz < Q; qz++)
+    for (int qy = 0; qy < Q; qy++)
+      for (int qx = 0; qx < Q; qx++)
+      {
+        const double gx = tb.Gc[qx * D + a] * tb.Bc[qy * D + bb] * tb.Bc[qz * D + c];
+        const double gy = tb.Bc[qx * D + a] * tb.Gc[qy * D + bb] * tb.Bc[qz * D + c];
+        const double gz = tb.Bc[qx * D + a] * tb.Bc[qy * D + bb] * tb.Gc[qz * D + c];
+        const double * f = qe + qz * 12 * QQ + qy * Q + qx;
+        const double m0 = f[0], m1 = f[QQ], m2 = f[2 * QQ], m3 = f[3 * QQ], m4 = f[4 * QQ], m5 = f[5 * QQ];
+        s += gx * (m0 * gx + m1 * gy + m2 * gz) + gy * (m1 * gx + m3 * gy + m4 * gz) +
+             gz * (m2 * gx + m4 * gy + m5 * gz);
+      }
+  const int M0 = g.n[0] * P + 1, M1 = g.n[1] * P + 1;
+  const int64_t node = ((int64_t)(ez * P + c) * M1 + (ey * P + bb)) * M0 + (ex * P + a);
+  dg[node] += s;
+}
+
+__global__ void
+k_aux_fill_zero(double * __restrict__ v, int n)
+{
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    v[i] = 0.0;
+}
+
+// Dg^-1 at the nodes (dg and g64 may be the same array: each entry is read,
+// then written, by one thread).
+__global__ void
+k_aux_invert(AuxGeom a, int pec, const double * dg, double * g64, float * __restrict__ g32)
+{
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < a.NN; k += gridDim.x * blockDim.x)
+  {
+    const int idx[3] = {k % a.M[0], (k / a.M[0]) % a.M[1], k / (a.M[0] * a.M[1])};
+    bool ess = false;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+      ess = ess || (idx[d] == 0 && ((pec >> (2 * d)) & 1)) || (idx[d] == a.M[d] - 1 && ((pec >> (2 * d + 1)) & 1));
+    const double v = dg[k];
+    const double inv = (ess || !(v > 0.0)) ? 0.0 : 1.0 / v;
+    g64[k] = inv;
+    if (g32)
+      g32[k] = (float)inv;
+  }
+}
+
+hipError_t
+aux_diag_inv(const Geom & g, const AuxGeom & a, const Tables<double> & tb, const double * qd, double * dg_scratch,
+             double * ginv64, float * ginv32, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_aux_fill_zero, dim3(1024), dim3(256), 0, st, dg_scratch, a.NN);
+  for (int col = 0; col < 8; col++)
+  {
+    const int hx = (g.n[0] - (col & 1) + 1) / 2, hy = (g.n[1] - ((col >> 1) & 1) + 1) / 2;
+    const int hz = (g.n[2] - ((col >> 2) & 1) + 1) / 2;
+    const int nb = hx * hy * hz;
+    if (nb <= 0)
+      continue;
+    switch (g.p)
+    {
+    case 1:
+      hipLaunchKernelGGL(k_aux_diag<1>, dim3(nb), dim3(64), 0, st, g, tb, qd, col, dg_scratch);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_aux_diag<2>, dim3(nb), dim3(64), 0, st, g, tb, qd, col, dg_scratch);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_aux_diag<3>, dim3(nb), dim3(64), 0, st, g, tb, qd, col, dg_scratch);
+      break;
+    default:
+      return hipErrorInvalidValue;
+    }
+  }
+  hipLaunchKernelGGL(k_aux_invert, dim3(1024), dim3(256), 0, st, a, g.pec, (const double *)dg_scratch, ginv64,
+                     ginv32);
+  return hipGetLastError();
+}
+
+// Standalone z = B r: t = Dg^-1 G^T r at the nodes, then z = D^-1 r + G t.
+__global__ void
+k_aux_nodes_single(AuxGeom a, const double * __restrict__ ginv, const double * __restrict__ r,
+                   double * __restrict__ t)
+{
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < a.NN; k += gridDim.x * blockDim.x)
+    t[k] = ginv[k] * aux_gt(a, r, k);
+}
+
+__global__ void
+k_aux_z_single(AuxGeom a, const double * __restrict__ dinv, const double * __restrict__ r,
+               const double * __restrict__ t, double * __restrict__ z)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.off[3];
+       i += (int64_t)gridDim.x * blockDim.x)
+    z[i] = dinv[i] * r[i] + aux_g(a, t, i);
+}
+
+hipError_t
+aux_precond_apply(const AuxGeom & a, const double * dinv, const double * ginv, const double * r, double * t,
+                  double * z, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_aux_nodes_single, dim3(2048), dim3(256), 0, st, a, ginv, r, t);
+  hipLaunchKernelGGL(k_aux_z_single, dim3(4096), dim3(256), 0, st, a, dinv, r, (const double *)t, z);
+  return hipGetLastError();
+}
+
+} // namespace hc

@@ -1,0 +1,122 @@
+// Discrete gradient of the structured ND_p space and the auxiliary-space
+// (gradient) correction of the Jacobi preconditioner.
+//
+// H1 space: continuous Q_p on the closed (Gauss-Lobatto) points, one node per
+// point of the (n0 p + 1) x (n1 p + 1) x (n2 p + 1) grid, x fastest.  The
+// discrete gradient G maps nodal values to ND DOFs exactly (the ND space holds
+// the gradients of Q_p): the DOF of component c at (open point i of element e
+// along c, closed indices j, k along the other axes) is
+//     sum_m Dg[i][m] phi(e p + m along c, j, k),   Dg[i][m] = L_m'(xi_i),
+// L_m the closed-point Lagrange basis, xi_i the open (Gauss) points, both on
+// the reference interval (0, 1) of the ND basis.  So K G = 0 and
+// G^T A G = G^T M_sigma G is the sigma-weighted H1 stiffness matrix.
+//
+// The preconditioner is the additive two-space form
+//     B r = D^-1 r + G Dg^-1 G^T r,   Dg = diag(G^T A G),
+// (Hiptmair-Xu with point smoothers in both spaces): SPD, and the gradient
+// fields -- the near-null space of the curl-curl term, whose eigenvalues are
+// set by sigma alone -- are handled in the space where they are smooth.
+// Nodes on PEC faces are essential (Dg^-1 = 0 there).  The reference's
+// counterpart for this operator is HypreAMS (hcurl_formulation.cpp:69-79),
+// which adds multigrid on both auxiliary spaces; this is its one-level form.
+#pragma once
+
+#include "hc_common.hpp"
+
+namespace hc
+{
+
+struct AuxGeom
+{
+  int n[3];                       // elements per direction
+  int p;                          // order
+  int M[3];                       // nodes per direction (n p + 1)
+  int cd[3][3];                   // ND component grid dims [c][axis]
+  int64_t off[4];                 // ND component offsets, off[3] = N
+  int NN;                         // nodes
+  double dg[PMAX * (PMAX + 1)];   // Dg[i][m] at i (p + 1) + m
+};
+
+// (G^T r) at node k.
+template <typename T>
+__device__ inline double
+aux_gt(const AuxGeom & a, const T * __restrict__ r, int k)
+{
+  const int p = a.p, D = p + 1;
+  const int I = k % a.M[0], J = (k / a.M[0]) % a.M[1], K = k / (a.M[0] * a.M[1]);
+  const int idx[3] = {I, J, K};
+  double g = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+  {
+    const int ic = idx[c];
+    // DOF line of component c through the node: stride along c and its base
+    int64_t base;
+    int stride;
+    if (c == 0)
+    {
+      base = a.off[0] + ((int64_t)K * a.cd[0][1] + J) * a.cd[0][0];
+      stride = 1;
+    }
+    else if (c == 1)
+    {
+      base = a.off[1] + (int64_t)K * a.cd[1][1] * a.cd[1][0] + I;
+      stride = a.cd[1][0];
+    }
+    else
+    {
+      base = a.off[2] + (int64_t)J * a.cd[2][0] + I;
+      stride = a.cd[2][0] * a.cd[2][1];
+    }
+    // the (at most two) elements along c holding the node: local index m
+#pragma unroll
+    for (int side = 0; side < 2; side++)
+    {
+      const int e = side == 0 ? ic / p : ic / p - 1;
+      const int m = side == 0 ? ic % p : p;
+      const bool ok = side == 0 ? (e < a.n[c]) : (ic % p == 0 && ic > 0);
+      if (!ok)
+        continue;
+      const T * rl = r + base + (int64_t)(e * p) * stride;
+      for (int i = 0; i < p; i++)
+        g += a.dg[i * D + m] * (double)rl[(int64_t)i * stride];
+    }
+  }
+  return g;
+}
+
+// (G t) at ND DOF i.
+template <typename T>
+__device__ inline double
+aux_g(const AuxGeom & a, const T * __restrict__ t, int64_t i)
+{
+  const int p = a.p, D = p + 1;
+  const int c = (i < a.off[1]) ? 0 : ((i < a.off[2]) ? 1 : 2);
+  const int li = (int)(i - a.off[c]);
+  const int d0 = a.cd[c][0], d1 = a.cd[c][1];
+  const int i0 = li % d0, rest = li / d0;
+  const int i1 = rest % d1, i2 = rest / d1;
+  const int ic = (c == 0) ? i0 : ((c == 1) ? i1 : i2);
+  const int e = ic / p, l = ic % p;
+  int node[3] = {i0, i1, i2};
+  node[c] = e * p;
+  const int stride = (c == 0) ? 1 : ((c == 1) ? a.M[0] : a.M[0] * a.M[1]);
+  const T * tl = t + ((int64_t)node[2] * a.M[1] + node[1]) * a.M[0] + node[0];
+  double v = 0.0;
+  for (int m = 0; m <= p; m++)
+    v += a.dg[l * D + m] * (double)tl[(int64_t)m * stride];
+  return v;
+}
+
+// Host: geometry of the auxiliary space of g, with the 1D derivative matrix dg1d.
+AuxGeom make_aux_geom(const Geom & g, const double * dg1d);
+// Node diagonal inverse Dg^-1 (0 on PEC nodes and where Dg = 0) from the
+// mass factors of the quadrature data qd (FP64) of the operator; dg_scratch
+// (NN doubles) may be ginv64 itself.
+hipError_t aux_diag_inv(const Geom & g, const AuxGeom & a, const Tables<double> & tb, const double * qd,
+                        double * dg_scratch, double * ginv64, float * ginv32, hipStream_t st);
+// z = D^-1 r + G Dg^-1 G^T r for one vector (t: node scratch, NN entries).
+hipError_t aux_precond_apply(const AuxGeom & a, const double * dinv, const double * ginv, const double * r, double * t,
+                             double * z, hipStream_t st);
+
+} // namespace hc

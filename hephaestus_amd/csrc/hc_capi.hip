@@ -152,6 +152,23 @@ make_tables(int p, hc::Tables<double> & t)
   }
 }
 
+// Dg[i][m] = L_m'(xi_i): derivative of the closed-point Lagrange basis at the
+// open points (the 1D factor of the discrete gradient, hc_aux.hpp).
+void
+make_grad_1d(int p, double * dg)
+{
+  double on[hc::PMAX], ow[hc::PMAX], cn[hc::PMAX + 1];
+  gauss_legendre01(p, on, ow);
+  gauss_lobatto01(p + 1, cn);
+  for (int i = 0; i < p; i++)
+  {
+    double v[hc::PMAX + 1], d[hc::PMAX + 1];
+    lagrange(p + 1, cn, on[i], v, d);
+    for (int m = 0; m <= p; m++)
+      dg[i * (p + 1) + m] = d[m];
+  }
+}
+
 template <typename A>
 void
 cast_tables(const hc::Tables<double> & a, hc::Tables<A> & b)
@@ -312,9 +329,20 @@ hc_op_create(const hc_mesh_desc * mesh, int order, const double * nu_e, const do
     delete op;
     return rc;
   }
+  {
+    double dg1d[hc::PMAX * (hc::PMAX + 1)];
+    make_grad_1d(order, dg1d);
+    op->ag = hc::make_aux_geom(g, dg1d);
+    if ((rc = op->alloc(op->ginv64, op->ag.NN)))
+    {
+      delete op;
+      return rc;
+    }
+  }
   if (op->precision & HC_PREC_FP32)
   {
-    if ((rc = op->alloc(op->qd32, nq)) || (rc = op->alloc(op->dinv32, op->N)))
+    if ((rc = op->alloc(op->qd32, nq)) || (rc = op->alloc(op->dinv32, op->N)) ||
+        (rc = op->alloc(op->ginv32, op->ag.NN)))
     {
       delete op;
       return rc;
@@ -372,6 +400,8 @@ hc_op_setup(hc_op_t op, double dt, void * stream)
     HC_TRY_HIP(hc::cast_f32(op->qd64, op->qd32, op->E * 12 * Q * Q * Q, st));
   HC_TRY_HIP(hc::diagonal(op->g, op->tb64, op->qd64, op->diag, st));
   HC_TRY_HIP(hc::invert_diag(op->diag, op->dinv64, op->dinv32, op->N, st));
+  // node diagonal of the gradient-space correction (computed in place)
+  HC_TRY_HIP(hc::aux_diag_inv(op->g, op->ag, op->tb64, op->qd64, op->ginv64, op->ginv64, op->ginv32, st));
   HC_TRY_HIP(hipStreamSynchronize(st));
   op->setup_done = true;
   op->coef_dirty = false;
@@ -448,6 +478,28 @@ hc_op_set_rhs_block(hc_op_t op, int rhs_block)
   if (!op || rhs_block < 0)
     return fail(HC_EINVAL, "hc_op_set_rhs_block: bad argument");
   op->rhs_block = rhs_block;
+  return HC_OK;
+}
+
+int
+hc_op_precond_apply(hc_op_t op, int mode, const double * r, double * z, void * stream)
+{
+  if (!op || !r || !z)
+    return fail(HC_EINVAL, "hc_op_precond_apply: null argument");
+  if (!op->setup_done)
+    return fail(HC_EINVAL, "hc_op_precond_apply: hc_op_setup has not been called");
+  if ((mode & ~(HC_SOLVE_AUX | 0xF)) || (hc::prec_mode(mode) != HC_SOLVE_FP64 && hc::prec_mode(mode) != HC_SOLVE_FP32_IR))
+    return fail(HC_EINVAL, "hc_op_precond_apply: unknown mode");
+  hipStream_t st = (hipStream_t)stream;
+  if (!(mode & HC_SOLVE_AUX))
+  {
+    HC_TRY_HIP(hc::mul(op->N, op->dinv64, r, z, st));
+    return HC_OK;
+  }
+  int rc = op->ensure_ws(1, false);
+  if (rc)
+    return rc;
+  HC_TRY_HIP(hc::aux_precond_apply(op->ag, op->dinv64, op->ginv64, r, op->ws.t64, z, st));
   return HC_OK;
 }
 

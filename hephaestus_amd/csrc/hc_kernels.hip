@@ -574,6 +574,22 @@ axpby(int64_t n, double a, const double * x, double b, double * y, hipStream_t s
   return hipGetLastError();
 }
 
+__global__ void
+k_mul(int64_t n, const double * __restrict__ d, const double * __restrict__ x, double * __restrict__ y)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = d[i] * x[i];
+}
+
+hipError_t
+mul(int64_t n, const double * d, const double * x, double * y, hipStream_t st)
+{
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_mul, dim3((unsigned)std::min<int64_t>(div_up(n, 256), 4096)), dim3(256), 0, st, n, d, x, y);
+  return hipGetLastError();
+}
+
 hipError_t
 zero_ess(const Geom & g, double * v, hipStream_t st)
 {

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "hc_common.hpp"
+#include "hc_aux.hpp"
 
 namespace hc
 {
@@ -126,6 +127,8 @@ struct CoilParams
 };
 // y = a x + b y (b == 0: y = a x)
 hipError_t axpby(int64_t n, double a, const double * x, double b, double * y, hipStream_t st);
+// y = d * x (entrywise)
+hipError_t mul(int64_t n, const double * d, const double * x, double * y, hipStream_t st);
 // v_i = 0 on the PEC DOFs of g
 hipError_t zero_ess(const Geom & g, double * v, hipStream_t st);
 // b_i = int J . phi_i (zero on PEC DOFs)
@@ -258,6 +261,15 @@ struct SolveSlot
   PcgState * in;
   PcgState * out;
   double * part; // RED_BLOCKS * 4 partial sums
+  // gradient-space correction of the preconditioner (aux != 0):
+  // z = D^-1 r + G Dg^-1 G^T r (hc_aux.hpp); t = Dg^-1 G^T r at the nodes
+  int aux;
+  int pad2;
+  const double * ginv64;
+  const float * ginv32;
+  double * t64;
+  float * t32;
+  AuxGeom ag;
 };
 
 struct SlotTable
@@ -265,6 +277,7 @@ struct SlotTable
   const SolveSlot * slots; // device
   int nslots;
   int nb_max; // max red_blocks(n) over the slots
+  int any_aux; // some slot uses the gradient-space correction
 };
 
 // Phase moves at the start of a round (one thread per slot).

@@ -53,6 +53,13 @@ struct DevBlock
   }
 };
 
+// Precision part of a solve mode (HC_SOLVE_FP64 / HC_SOLVE_FP32_IR).
+inline int
+prec_mode(int mode)
+{
+  return mode & ~HC_SOLVE_AUX;
+}
+
 inline size_t
 align256(size_t v)
 {
@@ -60,29 +67,39 @@ align256(size_t v)
 }
 
 // Work vectors of the solves of nrhs right-hand sides of one operator
-// (N DOFs each, rhs r at offset r*N), plus the tile-face partials of the
-// apply (4 N per rhs and precision).
+// (N DOFs each, rhs r at offset r*N), the tile-face partials of the apply
+// (4 N per rhs and precision) and the node vectors of the gradient-space
+// correction (nn_cap(N) per rhs and precision, node vector r at r*nn_cap(N)).
 struct Workspace
 {
   int64_t N = 0;
+  int64_t NNC = 0;
   int nrhs = 0;
   bool has32 = false;
-  double *r64 = nullptr, *q64 = nullptr, *p64 = nullptr, *part64 = nullptr;
-  float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *part32 = nullptr;
+  double *r64 = nullptr, *q64 = nullptr, *p64 = nullptr, *part64 = nullptr, *t64 = nullptr;
+  float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *part32 = nullptr,
+        *t32 = nullptr;
 
+  // An upper bound of the H1 node count of any structured ND space with N
+  // DOFs: with L_a = n_a p >= 1, N = sum_c L_c prod_{a != c} (L_a + 1)
+  // >= 3/2 prod_a (L_a + 1).
+  static int64_t nn_cap(int64_t N) { return 2 * N / 3 + 1; }
   static size_t bytes_for(int64_t N, int nrhs, bool need32)
   {
     const size_t v64 = align256(sizeof(double) * (size_t)N * nrhs);
     const size_t v32 = align256(sizeof(float) * (size_t)N * nrhs);
-    size_t b = 3 * v64 + 4 * v64;
+    const size_t t64 = align256(sizeof(double) * (size_t)nn_cap(N) * nrhs);
+    const size_t t32 = align256(sizeof(float) * (size_t)nn_cap(N) * nrhs);
+    size_t b = 3 * v64 + 4 * v64 + t64;
     if (need32)
-      b += 5 * v32 + 4 * v32;
+      b += 5 * v32 + 4 * v32 + t32;
     return b;
   }
   // carve the pieces out of [base, base + bytes_for(...))
   void carve(void * base, int64_t n, int k, bool need32)
   {
     N = n;
+    NNC = nn_cap(n);
     nrhs = k;
     has32 = need32;
     char * p = (char *)base;
@@ -96,6 +113,8 @@ struct Workspace
     p += v64;
     part64 = (double *)p;
     p += 4 * v64;
+    t64 = (double *)p;
+    p += align256(sizeof(double) * (size_t)NNC * nrhs);
     if (need32)
     {
       b32 = (float *)p;
@@ -109,9 +128,11 @@ struct Workspace
       q32 = (float *)p;
       p += v32;
       part32 = (float *)p;
+      p += 4 * v32;
+      t32 = (float *)p;
     }
     else
-      b32 = x32 = r32 = p32 = q32 = part32 = nullptr;
+      b32 = x32 = r32 = p32 = q32 = part32 = t32 = nullptr;
   }
 };
 
@@ -154,6 +175,10 @@ struct hc_op_s
   double * diag = nullptr;
   double * dinv64 = nullptr;
   float * dinv32 = nullptr;
+  // gradient-space correction: node diagonal inverse (computed at setup)
+  hc::AuxGeom ag{};
+  double * ginv64 = nullptr;
+  float * ginv32 = nullptr;
   hc::ApplyDesc * desc_dev = nullptr; // staging of the batched apply descriptor
   int rhs_block = 0;                    // right-hand sides per work item (0: automatic)
   std::vector<void *> allocs;
@@ -227,6 +252,7 @@ public:
   // tile variant is cands64/cands32[order] when given, else the plan of the
   // first instance of that order.  rhs_block: right-hand sides per work item
   // of the batched apply (0: default).
+  // mode: HC_SOLVE_FP64 or HC_SOLVE_FP32_IR, optionally | HC_SOLVE_AUX
   int set(const std::vector<EngineInst> & inst, int mode, int rhs_block = 0, bool force_batched = false,
           const TileCandidate * const * cands64 = nullptr, const TileCandidate * const * cands32 = nullptr);
   // One round: phase moves, FP64 residuals and refinement updates, inner
@@ -266,7 +292,8 @@ private:
 
   std::vector<EngineInst> inst_;
   int nslots_ = 0;
-  int mode_ = 0;
+  int mode_ = 0;     // precision mode (without HC_SOLVE_AUX)
+  bool aux_ = false; // gradient-space correction
   int rhs_block_ = 0;
   bool force_batched_ = false;
   const TileCandidate * cands64_[PMAX + 1] = {};

@@ -9,7 +9,11 @@
 //   pcg_dot_pq     partial (p, q) per block
 //   pcg_update_xr  alpha = gamma / (p,q);  x += alpha p;  r -= alpha q;
 //                  partials (r, D^-1 r), (r, r)          -- one pass over x,p,r,q,d
-//   pcg_update_p   beta = gamma_new / gamma;  p = D^-1 r + beta p
+//   pcg_update_p   beta = gamma_new / gamma;  p = z + beta p
+// with z = D^-1 r (Jacobi), or z = D^-1 r + G Dg^-1 G^T r for slots with the
+// gradient-space correction (hc_aux.hpp): then aux_nodes runs between the two
+// updates, t = Dg^-1 G^T r at the nodes and the partial (G^T r, t) that
+// completes (r, z), and pcg_update_p adds (G t) while it forms p.
 // Every scalar stays on the device (PcgState).  Reductions are two-stage with
 // a block count fixed by n and a fixed summation order, so the result of a
 // slot is bitwise independent of the other slots of the launch.  Dot products
@@ -104,6 +108,8 @@ struct SlotVecs<double>
   __device__ double * p() const { return s.p64; }
   __device__ double * q() const { return s.q64; }
   __device__ const double * dinv() const { return s.dinv64; }
+  __device__ double * t() const { return s.t64; }
+  __device__ const double * ginv() const { return s.ginv64; }
 };
 template <>
 struct SlotVecs<float>
@@ -115,6 +121,8 @@ struct SlotVecs<float>
   __device__ float * p() const { return s.p32; }
   __device__ float * q() const { return s.q32; }
   __device__ const float * dinv() const { return s.dinv32; }
+  __device__ float * t() const { return s.t32; }
+  __device__ const float * ginv() const { return s.ginv32; }
 };
 
 // ---------------------------------------------------------------------------
@@ -142,7 +150,8 @@ k_pcg_init64(const SolveSlot * __restrict__ slots)
     const double ri = bi - q[i];
     const double zi = dinv[i] * ri;
     r[i] = ri;
-    p[i] = zi;
+    if (!sl.aux)
+      p[i] = zi; // else set by aux_pset once t is known
     v[0] += bi * bi;
     v[1] += ri * zi;
     v[2] += ri * ri;
@@ -166,9 +175,11 @@ k_pcg_init64_fin(const SolveSlot * __restrict__ slots, const int * __restrict__ 
   PcgState * o = sl.out;
   if (o->phase != PH_INIT || !o->prec64)
     return;
-  __shared__ double sh[3 * RED_THREADS];
-  double v[3];
-  reduce_partials<3>(sl.part, slot_blocks(sl.n), 0, v, sh);
+  __shared__ double sh[4 * RED_THREADS];
+  double v[4];
+  reduce_partials<4>(sl.part, slot_blocks(sl.n), 0, v, sh);
+  if (sl.aux)
+    v[1] += v[3]; // (r, G Dg^-1 G^T r) from aux_nodes
   if (threadIdx.x == 0)
   {
     s->bb = v[0];
@@ -264,10 +275,10 @@ k_pcg_update_p(const SolveSlot * __restrict__ slots, const int * __restrict__ gi
   if ((int)blockIdx.x >= nb || s->run != SlotVecs<T>::RUN)
     return;
   const int it = *git - 1 - s->it0;
-  __shared__ double sh[2 * RED_THREADS];
-  double o[2];
-  reduce_partials<2>(sl.part, nb, 0, o, sh);
-  const double gnew = o[0], rr = o[1];
+  __shared__ double sh[3 * RED_THREADS];
+  double o[3];
+  reduce_partials<3>(sl.part, nb, 0, o, sh);
+  const double gnew = sl.aux ? o[0] + o[2] : o[0], rr = o[1];
   const double gold = s->gam[it & 1];
   const bool nan = bad(gnew) || bad(rr);
   const bool conv = nan || rr <= s->tol2 * s->bb;
@@ -293,8 +304,78 @@ k_pcg_update_p(const SolveSlot * __restrict__ slots, const int * __restrict__ gi
   const T * __restrict__ r = V.r();
   const T * __restrict__ dinv = V.dinv();
   const T beta = (T)(gnew / gold);
+  if (sl.aux)
+  {
+    const T * __restrict__ t = V.t();
+    HC_SLOT_LOOP(i, sl.n, nb)
+    p[i] = (dinv[i] * r[i] + (T)aux_g(sl.ag, t, i)) + beta * p[i];
+  }
+  else
+  {
+    HC_SLOT_LOOP(i, sl.n, nb)
+    p[i] = dinv[i] * r[i] + beta * p[i];
+  }
+}
+
+// Gradient-space part of z = B r for the slots with aux set: t = Dg^-1 G^T r
+// at the nodes and the block partials of (G^T r, t) = (r, G t) into part[kp].
+// which = 0: iterating slots of precision T (kp = 2, beside the (r, D^-1 r)
+// and (r, r) partials of pcg_update_xr); 1: FP64 slots starting (PH_INIT);
+// 2: FP32 slots starting (kp = 3 for both).
+template <typename T>
+__global__ void __launch_bounds__(RED_THREADS)
+k_aux_nodes(const SolveSlot * __restrict__ slots, int which)
+{
+  const SolveSlot & sl = slots[blockIdx.y];
+  const int nb = slot_blocks(sl.n);
+  if ((int)blockIdx.x >= nb || !sl.aux)
+    return;
+  if (which == 0 && sl.in->run != SlotVecs<T>::RUN)
+    return;
+  if (which == 1 && (sl.out->phase != PH_INIT || !sl.out->prec64))
+    return;
+  if (which == 2 && (sl.out->phase != PH_INIT || sl.out->prec64))
+    return;
+  const SlotVecs<T> V{sl};
+  const T * __restrict__ r = V.r();
+  const T * __restrict__ ginv = V.ginv();
+  T * __restrict__ t = V.t();
+  __shared__ double sh[RED_THREADS];
+  double v[1] = {0};
+  for (int k = blockIdx.x * RED_THREADS + threadIdx.x; k < sl.ag.NN; k += nb * RED_THREADS)
+  {
+    const double g = aux_gt(sl.ag, r, k);
+    const T tk = (T)((double)ginv[k] * g);
+    t[k] = tk;
+    v[0] += g * (double)tk;
+  }
+  block_sum<1>(v, sh);
+  if (threadIdx.x == 0)
+    sl.part[blockIdx.x * 4 + (which == 0 ? 2 : 3)] = v[0];
+}
+
+// First search direction p = D^-1 r + G t of starting aux slots (which as in
+// k_aux_nodes; FP32 slots only on a fresh start, a paused PCG keeps p).
+template <typename T>
+__global__ void __launch_bounds__(RED_THREADS)
+k_aux_pset(const SolveSlot * __restrict__ slots, int which)
+{
+  const SolveSlot & sl = slots[blockIdx.y];
+  const int nb = slot_blocks(sl.n);
+  const PcgState * o = sl.out;
+  if ((int)blockIdx.x >= nb || !sl.aux || o->phase != PH_INIT)
+    return;
+  if ((which == 1) != (o->prec64 != 0))
+    return;
+  if (which == 2 && o->iters > 0 && sl.in->first)
+    return;
+  const SlotVecs<T> V{sl};
+  const T * __restrict__ r = V.r();
+  const T * __restrict__ dinv = V.dinv();
+  const T * __restrict__ t = V.t();
+  T * __restrict__ p = V.p();
   HC_SLOT_LOOP(i, sl.n, nb)
-  p[i] = dinv[i] * r[i] + beta * p[i];
+  p[i] = dinv[i] * r[i] + (T)aux_g(sl.ag, t, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -490,8 +571,8 @@ k_pcg_init_rel(const SolveSlot * __restrict__ slots)
     r[i] = ri;
     x[i] = 0.0f;
     const float zi = dinv[i] * ri;
-    if (fresh)
-      p[i] = zi;
+    if (fresh && !sl.aux)
+      p[i] = zi; // aux: set by aux_pset once t is known
     v[0] += (double)ri * (double)zi;
     v[1] += (double)ri * (double)ri;
   }
@@ -514,9 +595,11 @@ k_pcg_init_rel_fin(const SolveSlot * __restrict__ slots, const int * __restrict_
   PcgState * o = sl.out;
   if (o->phase != PH_INIT || o->prec64)
     return;
-  __shared__ double sh[2 * RED_THREADS];
-  double v[2];
-  reduce_partials<2>(sl.part, slot_blocks(sl.n), 0, v, sh);
+  __shared__ double sh[4 * RED_THREADS];
+  double v[4];
+  reduce_partials<4>(sl.part, slot_blocks(sl.n), 0, v, sh);
+  if (sl.aux)
+    v[0] += v[3]; // (r, G Dg^-1 G^T r) from aux_nodes
   if (threadIdx.x == 0)
   {
     const bool fresh = !(o->iters > 0 && s->first);
@@ -617,6 +700,11 @@ hipError_t
 pcg_init64(const SlotTable & t, const int * git_pair, hipStream_t st)
 {
   hipLaunchKernelGGL(k_pcg_init64, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
+  if (t.any_aux)
+  {
+    hipLaunchKernelGGL(k_aux_nodes<double>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, 1);
+    hipLaunchKernelGGL(k_aux_pset<double>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, 1);
+  }
   hipLaunchKernelGGL(k_pcg_init64_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, git_pair + 1);
   return hipGetLastError();
 }
@@ -630,6 +718,8 @@ pcg_iterate(const SlotTable & t, int * git_pair, hipStream_t st)
   int * git = git_pair + (sizeof(T) == 4 ? 0 : 1);
   hipLaunchKernelGGL(k_pcg_dot_pq<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, git);
   hipLaunchKernelGGL(k_pcg_update_xr<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, (const int *)git);
+  if (t.any_aux)
+    hipLaunchKernelGGL(k_aux_nodes<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, 0);
   hipLaunchKernelGGL(k_pcg_update_p<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, (const int *)git);
   return hipGetLastError();
 }
@@ -657,6 +747,11 @@ hipError_t
 pcg_init_reliable(const SlotTable & t, const int * git_pair, hipStream_t st)
 {
   hipLaunchKernelGGL(k_pcg_init_rel, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
+  if (t.any_aux)
+  {
+    hipLaunchKernelGGL(k_aux_nodes<float>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, 2);
+    hipLaunchKernelGGL(k_aux_pset<float>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, 2);
+  }
   hipLaunchKernelGGL(k_pcg_init_rel_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, git_pair);
   return hipGetLastError();
 }

@@ -151,7 +151,7 @@ hc_server_s::start_request(Request * r)
 {
   hc_op_t op = r->op;
   const int64_t N = op->N;
-  const bool ir = cfg.mode == HC_SOLVE_FP32_IR;
+  const bool ir = hc::prec_mode(cfg.mode) == HC_SOLVE_FP32_IR;
   char * p = (char *)mem.base + r->off;
   const size_t wsb = hc::align256(hc::Workspace::bytes_for(N, 1, ir));
   r->ws.carve(p, N, 1, ir);
@@ -183,7 +183,7 @@ hc_server_s::start_request(Request * r)
 int
 hc_server_s::admit(std::vector<Request *> & fresh)
 {
-  const bool ir = cfg.mode == HC_SOLVE_FP32_IR;
+  const bool ir = hc::prec_mode(cfg.mode) == HC_SOLVE_FP32_IR;
   std::unique_lock<std::mutex> lk(mu);
   const auto now = Clock::now();
   while (!pending.empty())
@@ -313,7 +313,7 @@ hc_server_s::loop()
       o.iterations = s.inner;
       o.outer_iterations = s.outer;
       o.rel_res = s.rel_res;
-      o.fp64_fallback = (cfg.mode == HC_SOLVE_FP32_IR) ? s.fp64 : 0;
+      o.fp64_fallback = (hc::prec_mode(cfg.mode) == HC_SOLVE_FP32_IR) ? s.fp64 : 0;
       res.push_back(o);
     }
     if (hipStreamSynchronize(st) != hipSuccess)
@@ -371,7 +371,7 @@ extern "C" {
 int64_t
 hc_server_request_bytes(int64_t ndofs, int mode)
 {
-  return ndofs > 0 ? (int64_t)request_bytes(ndofs, mode == HC_SOLVE_FP32_IR) : -1;
+  return ndofs > 0 ? (int64_t)request_bytes(ndofs, hc::prec_mode(mode) == HC_SOLVE_FP32_IR) : -1;
 }
 
 int
@@ -380,7 +380,8 @@ hc_server_create(const hc_server_config * cfg, hc_server_t * out)
   if (!cfg || !out)
     return hc::fail(HC_EINVAL, "hc_server_create: null argument");
   if (cfg->hbm_budget_bytes <= 0 || !(cfg->rtol > 0) || cfg->maxit < 1 ||
-      (cfg->mode != HC_SOLVE_FP64 && cfg->mode != HC_SOLVE_FP32_IR))
+      (hc::prec_mode(cfg->mode) != HC_SOLVE_FP64 && hc::prec_mode(cfg->mode) != HC_SOLVE_FP32_IR) ||
+      (cfg->mode & ~(HC_SOLVE_AUX | 0xF)))
     return hc::fail(HC_EINVAL, "hc_server_create: budget > 0, rtol > 0, maxit >= 1 and a solve mode are required");
   hc_server_t s = new hc_server_s();
   s->cfg = *cfg;
@@ -438,7 +439,7 @@ hc_server_submit(hc_server_t s, hc_op_t op, const double * b_src, double * u, do
     return hc::fail(HC_EINVAL, "hc_server_submit: null argument");
   if (!op->setup_done || std::fabs(dt - op->last_dt) > 1e-12 * dt)
     return hc::fail(HC_EINVAL, "hc_server_submit: the operator must be set up with this time step");
-  const bool ir = s->cfg.mode == HC_SOLVE_FP32_IR;
+  const bool ir = hc::prec_mode(s->cfg.mode) == HC_SOLVE_FP32_IR;
   if (ir && !op->qd32)
     return hc::fail(HC_EINVAL, "hc_server_submit: FP32 refinement needs an operator created with HC_PREC_FP32");
   if (request_bytes(op->N, ir) > (size_t)s->cfg.hbm_budget_bytes)

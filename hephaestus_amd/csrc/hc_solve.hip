@@ -46,7 +46,7 @@ initial_states(PcgState & in, PcgState & out, int mode)
   std::memset(&out, 0, sizeof(out));
   in.done = 1; // no inner solve running
   out.phase = PH_FIRST;
-  out.prec64 = (mode == HC_SOLVE_FP32_IR) ? 0 : 1;
+  out.prec64 = (prec_mode(mode) == HC_SOLVE_FP32_IR) ? 0 : 1;
 }
 
 Engine::~Engine()
@@ -66,8 +66,9 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
             const TileCandidate * const * cands64, const TileCandidate * const * cands32)
 {
   inst_ = inst;
-  mode_ = mode;
-  const bool ir = (mode == HC_SOLVE_FP32_IR);
+  mode_ = prec_mode(mode);
+  aux_ = (mode & HC_SOLVE_AUX) != 0;
+  const bool ir = (mode_ == HC_SOLVE_FP32_IR);
   nslots_ = 0;
   int nb_max = 1;
   int owned = 0; // slots whose states the engine owns
@@ -81,6 +82,8 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
       return fail(HC_EINVAL, "solve: FP32 refinement needs operators created with HC_PREC_FP32");
     if (!e.ws || e.ws->nrhs < e.nrhs || (ir && !e.ws->has32))
       return fail(HC_EINVAL, "solve: work vectors missing");
+    if (aux_ && (!e.op->ginv64 || (ir && !e.op->ginv32)))
+      return fail(HC_EINVAL, "solve: gradient-space data missing (hc_op_setup has not run)");
     e.slot0 = nslots_;
     nslots_ += e.nrhs;
     if (!e.in)
@@ -178,11 +181,24 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
       s.in = e.in + r;
       s.out = e.out + r;
       s.part = e.part + (size_t)r * 4 * RED_BLOCKS;
+      if (aux_)
+      {
+        s.aux = 1;
+        s.ag = op->ag;
+        s.ginv64 = op->ginv64;
+        s.t64 = w.t64 + (size_t)r * w.NNC;
+        if (ir)
+        {
+          s.ginv32 = op->ginv32;
+          s.t32 = w.t32 + (size_t)r * w.NNC;
+        }
+      }
     }
   }
   table_.slots = d_slots;
   table_.nslots = nslots_;
   table_.nb_max = nb_max;
+  table_.any_aux = aux_ ? 1 : 0;
   // new slots start in the mode's precision
   if (ir)
     any32_ = true;
@@ -374,7 +390,7 @@ Engine::stats_of(const PcgState & s)
 int
 Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double * ms)
 {
-  if (mode != mode_)
+  if (prec_mode(mode) != mode_ || ((mode & HC_SOLVE_AUX) != 0) != aux_)
     return fail(HC_EINVAL, "solve: mode differs from the one the batch was set up for");
   int maxit_all = 0;
   for (auto & e : inst_)
@@ -383,7 +399,7 @@ Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double *
   std::vector<PcgState> hin(nslots_), hout(nslots_);
   for (int i = 0; i < nslots_; i++)
     initial_states(hin[i], hout[i], mode);
-  any32_ = (mode == HC_SOLVE_FP32_IR);
+  any32_ = (mode_ == HC_SOLVE_FP32_IR);
   any64_ = !any32_;
   for (auto & e : inst_)
   {
@@ -444,7 +460,7 @@ Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double *
         c64[p] = cands64_[p];
         c32[p] = cands32_[p];
       }
-      if ((rc = set(keep, mode_, rhs_block_, force_batched_, c64, c32)))
+      if ((rc = set(keep, mode_ | (aux_ ? HC_SOLVE_AUX : 0), rhs_block_, force_batched_, c64, c32)))
         return rc;
       orig.swap(keep_orig);
     }
@@ -464,7 +480,7 @@ Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double *
     c64[p] = cands64_[p];
     c32[p] = cands32_[p];
   }
-  return set(all, mode_, rhs_block_, force_batched_, c64, c32);
+  return set(all, mode_ | (aux_ ? HC_SOLVE_AUX : 0), rhs_block_, force_batched_, c64, c32);
 }
 
 } // namespace hc
@@ -514,7 +530,7 @@ fold_stats(const std::vector<hc::SlotStats> & s, int begin, int count, double ms
       r.converged = 0;
     if (s[i].status && !r.status)
       r.status = s[i].status;
-    if (mode == HC_SOLVE_FP32_IR)
+    if (hc::prec_mode(mode) == HC_SOLVE_FP32_IR)
       r.fp64_fallbacks += s[i].fp64;
   }
   if (r.status == 0 && !r.converged)
@@ -578,11 +594,12 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
     return hc::fail(HC_EINVAL, "hc_pcg_solve: nrhs must be >= 1");
   if (!(rtol > 0) || maxit < 1)
     return hc::fail(HC_EINVAL, "hc_pcg_solve: rtol must be > 0 and maxit >= 1");
-  if (mode != HC_SOLVE_FP64 && mode != HC_SOLVE_FP32_IR)
+  const int pm = hc::prec_mode(mode);
+  if ((pm != HC_SOLVE_FP64 && pm != HC_SOLVE_FP32_IR) || (mode & ~(HC_SOLVE_AUX | 0xF)))
     return hc::fail(HC_EINVAL, "hc_pcg_solve: unknown mode");
-  if (mode == HC_SOLVE_FP32_IR && !op->qd32)
+  if (pm == HC_SOLVE_FP32_IR && !op->qd32)
     return hc::fail(HC_EINVAL, "hc_pcg_solve: FP32 refinement needs an operator created with HC_PREC_FP32");
-  int rc = op->ensure_ws(nrhs, mode == HC_SOLVE_FP32_IR);
+  int rc = op->ensure_ws(nrhs, pm == HC_SOLVE_FP32_IR);
   if (rc)
     return rc;
   if (!op->engine)
@@ -641,9 +658,10 @@ hc_batch_solve(hc_batch_t bt, int mode, hc_solve_stats * stats, void * stream)
     return hc::fail(HC_EINVAL, "hc_batch_solve: null handle");
   if (bt->entries.empty())
     return hc::fail(HC_EINVAL, "hc_batch_solve: empty batch");
-  if (mode != HC_SOLVE_FP64 && mode != HC_SOLVE_FP32_IR)
+  const int pm = hc::prec_mode(mode);
+  if ((pm != HC_SOLVE_FP64 && pm != HC_SOLVE_FP32_IR) || (mode & ~(HC_SOLVE_AUX | 0xF)))
     return hc::fail(HC_EINVAL, "hc_batch_solve: unknown mode");
-  const bool ir = mode == HC_SOLVE_FP32_IR;
+  const bool ir = pm == HC_SOLVE_FP32_IR;
   int rc;
   if (bt->dirty || bt->mode != mode)
   {

@@ -29,11 +29,12 @@ PREC_FP64 = 0x1
 PREC_FP32 = 0x2
 SOLVE_FP64 = 0
 SOLVE_FP32_IR = 1
+SOLVE_AUX = 0x10  # flag: Jacobi + discrete-gradient auxiliary-space correction
 
 # Entry points declared in include/hc_hcurl.h (checked by the CPU test-suite).
 EXPORTS = [
     "hc_op_create", "hc_op_set_coefficients", "hc_op_setup", "hc_op_ndofs", "hc_op_ess_count",
-    "hc_op_device_bytes", "hc_op_apply", "hc_op_apply_f32", "hc_op_diagonal", "hc_pcg_solve",
+    "hc_op_device_bytes", "hc_op_apply", "hc_op_apply_f32", "hc_op_diagonal", "hc_op_precond_apply", "hc_pcg_solve",
     "hc_op_destroy", "hc_last_error", "hc_fill_uniform", "hc_op_set_tile", "hc_op_get_tile",
     "hc_op_autotune", "hc_tile_list", "hc_op_set_rhs_block", "hc_batch_create", "hc_batch_add", "hc_batch_set_rhs_block",
     "hc_batch_solve", "hc_batch_destroy", "hc_op_rhs_coil", "hc_op_profile_apply", "hc_op_apply_curl", "hc_axpby", "hc_op_zero_ess",
@@ -113,6 +114,7 @@ def load():
     lib.hc_op_apply.argtypes = [vp, vp, vp, C.c_int, vp]
     lib.hc_op_apply_f32.argtypes = [vp, vp, vp, C.c_int, vp]
     lib.hc_op_diagonal.argtypes = [vp, vp, vp]
+    lib.hc_op_precond_apply.argtypes = [vp, C.c_int, vp, vp, vp]
     lib.hc_pcg_solve.argtypes = [vp, vp, vp, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(SolveStats), vp]
     lib.hc_op_destroy.argtypes = [vp]
     lib.hc_last_error.restype = C.c_char_p
@@ -263,9 +265,20 @@ class Operator:
         _check(_lib.hc_op_diagonal(self._h, _ptr(d), _stream(stream)), "hc_op_diagonal")
         return d
 
+    def precond_apply(self, r, mode=SOLVE_FP64 | SOLVE_AUX, z=None, stream=None):
+        """z = B r with the preconditioner of a solve mode (Jacobi, or with
+        SOLVE_AUX Jacobi + the gradient-space correction)."""
+        import torch
+        _nrhs(r, self.ndofs, torch.float64)
+        if z is None:
+            z = torch.empty_like(r)
+        _check(_lib.hc_op_precond_apply(self._h, int(mode), _ptr(r), _ptr(z), _stream(stream)), "hc_op_precond_apply")
+        return z
+
     def solve(self, b, x, rtol=1e-8, maxit=10000, mode=SOLVE_FP64, stream=None, raise_on_fail=True):
         """Jacobi-PCG (mode SOLVE_FP64) or FP32 PCG + FP64 refinement on A x = b;
-        b, x of shape (N,) or (nrhs, N), x holds the initial guess."""
+        mode | SOLVE_AUX adds the gradient-space correction to the Jacobi
+        preconditioner.  b, x of shape (N,) or (nrhs, N), x holds the initial guess."""
         import torch
         nrhs = _nrhs(b, self.ndofs, torch.float64)
         if _nrhs(x, self.ndofs, torch.float64) != nrhs:

@@ -81,6 +81,12 @@ extern "C" {
 /* solve modes of hc_pcg_solve */
 #define HC_SOLVE_FP64 0    /* plain FP64 Jacobi-PCG */
 #define HC_SOLVE_FP32_IR 1 /* FP32 Jacobi-PCG inner solves + FP64 refinement */
+/* flag, or-ed into either mode: precondition with Jacobi plus the
+ * discrete-gradient auxiliary-space correction B = D^-1 + G Dg^-1 G^T
+ * (Dg = diag(G^T A G) on the H1 nodes; the one-level form of the HypreAMS
+ * preconditioner that HCurlFormulation::ConstructJacobianPreconditioner sets,
+ * src/formulations/HCurl/hcurl_formulation.cpp:69-79) instead of Jacobi alone */
+#define HC_SOLVE_AUX 0x10
 
 typedef struct hc_op_s * hc_op_t;
 typedef struct hc_batch_s * hc_batch_t;
@@ -130,6 +136,13 @@ int64_t hc_op_ndofs(hc_op_t op);
 int hc_op_ess_count(hc_op_t op, int64_t * n_ess);
 /* Device bytes owned by the handle. */
 int64_t hc_op_device_bytes(hc_op_t op);
+
+/* z = B r with the preconditioner of a solve mode: B = D^-1 (Jacobi, the
+ * HypreDiagScale of DefaultJacobiPCGSolver, hephaestus_solvers.hpp:52), or,
+ * with HC_SOLVE_AUX in mode, B = D^-1 + G Dg^-1 G^T (the gradient-space
+ * correction; one-level form of HypreAMS, hcurl_formulation.cpp:69-79).
+ * r, z: N doubles on the device.  Needs hc_op_setup. */
+int hc_op_precond_apply(hc_op_t op, int mode, const double * r, double * z, void * stream);
 
 /* y = A x (PEC eliminated).  x, y: device, nrhs vectors (rhs r at r*N).
  * With nrhs > 1 a work item of the kernel applies the operator to a group of

@@ -126,10 +126,12 @@ SPECS = [((8, 8, 8), 1, 1e2, 1e-2), ((6, 9, 7), 2, 10.0, 1.0), ((5, 5, 6), 3, 1e
          ((10, 6, 5), 1, 1.0, 1.0), ((7, 7, 7), 2, 1e2, 1e-2)]
 
 
-@pytest.mark.parametrize("mode", ["fp32_ir", "fp64"])
+@pytest.mark.parametrize("mode", ["fp32_ir", "fp64", "fp64_aux", "fp32_ir_aux"])
 def test_batch_uneven_mix_invariance(torch_cuda, hc, O, mode):
     torch = torch_cuda
-    md = hc.SOLVE_FP64 if mode == "fp64" else hc.SOLVE_FP32_IR
+    md = hc.SOLVE_FP64 if mode.startswith("fp64") else hc.SOLVE_FP32_IR
+    if mode.endswith("_aux"):
+        md |= hc.SOLVE_AUX
     print(f"case uneven batch {mode} ({len(SPECS)} instances)", flush=True)
     ops = _instances(hc, SPECS)
     rtol = 1e-8
