@@ -107,7 +107,8 @@ def solves_config2(torch, H, op, x, args):
     true relative residual of 1e-8, with a coil current right-hand side
     (divergence-free, seed 2) and, second, with a uniform random right-hand
     side (seed 2) whose large gradient content Jacobi-PCG resolves slowly at
-    this sigma contrast (capped at --c2-uniform-maxit iterations)."""
+    this sigma contrast (capped at --c2-uniform-maxit iterations).  Each also
+    with the gradient-space correction of the preconditioner (_aux)."""
     sol = {"rtol": 1e-8}
     for rhs, maxit in (("coil", args.solve_maxit), ("uniform", args.c2_uniform_maxit)):
         b = torch.empty_like(x)
@@ -117,11 +118,13 @@ def solves_config2(torch, H, op, x, args):
             op.fill_uniform(2, b)
         entry = {"rhs": "coil current J = curl psi, seed 2" if rhs == "coil"
                  else "uniform [-1,1) per free DOF, seed 2", "maxit": maxit}
-        for name, mode in (("fp64", H.SOLVE_FP64), ("fp32_ir", H.SOLVE_FP32_IR)):
+        for name, mode in (("fp64", H.SOLVE_FP64), ("fp32_ir", H.SOLVE_FP32_IR),
+                           ("fp64_aux", H.SOLVE_FP64 | H.SOLVE_AUX), ("fp32_ir_aux", H.SOLVE_FP32_IR | H.SOLVE_AUX)):
             xs = torch.zeros_like(b)
-            st = op.solve(b, xs, rtol=1e-8, maxit=maxit, mode=mode, raise_on_fail=False)
+            st = op.solve(b, xs, rtol=1e-8, maxit=args.solve_maxit if name.endswith("aux") else maxit, mode=mode,
+                          raise_on_fail=False)
             entry[name] = solve_entry(st)
-            entry[name]["fp64_continuations"] = st["fp64_fallbacks"] if mode == H.SOLVE_FP32_IR else 0
+            entry[name]["fp64_continuations"] = st["fp64_fallbacks"] if (mode & 0xF) == H.SOLVE_FP32_IR else 0
             print(f"[bench] config2 {rhs} solve {name}: {entry[name]}", file=sys.stderr, flush=True)
         sol[rhs] = entry
     return sol
@@ -130,7 +133,8 @@ def solves_config2(torch, H, op, x, args):
 def solves_config3(torch, H, args):
     """32^3 p=2 two-region, 32 coil right-hand sides (seeds 3..34): plain FP64
     Jacobi-PCG one right-hand side at a time vs one batched FP32 PCG + FP64
-    refinement solve of all 32, both to a true relative residual of 1e-8."""
+    refinement solve of all 32, both to a true relative residual of 1e-8; each
+    also with the gradient-space correction of the preconditioner (_aux)."""
     n, p, nr = 32, 2, 32
     op = H.Operator(n, p, 1.0, two_region(n)).setup(1.0)
     op.autotune(reps=5)
@@ -142,21 +146,25 @@ def solves_config3(torch, H, args):
            "rtol": 1e-8}
     # plain FP64 path: one solve per right-hand side (a sample of them)
     k = min(args.c3_fp64_sample, nr)
-    tot_ms, its, worst = 0.0, [], 0.0
-    for r in range(k):
-        xs = torch.zeros(N, dtype=torch.float64, device="cuda")
-        st = op.solve(B[r].contiguous(), xs, rtol=1e-8, maxit=args.solve_maxit, mode=H.SOLVE_FP64,
-                      raise_on_fail=False)
-        tot_ms += st["solve_ms"]
-        its.append(st["iterations"])
-        worst = max(worst, st["rel_res"])
-    res["fp64_single"] = {"sample_rhs": k, "iterations_mean": float(np.mean(its)), "max_true_rel_res": worst,
-                          "solves_per_s": round(1000.0 * k / tot_ms, 4)}
-    X = torch.zeros_like(B)
-    st = op.solve(B, X, rtol=1e-8, maxit=args.solve_maxit, mode=H.SOLVE_FP32_IR, raise_on_fail=False)
-    res["fp32_ir_batched"] = solve_entry(st, nr)
-    res["speedup_vs_fp64_single"] = round(res["fp32_ir_batched"]["solves_per_s"] /
-                                          res["fp64_single"]["solves_per_s"], 3)
+    for name, mode in (("fp64_single", H.SOLVE_FP64), ("fp64_aux_single", H.SOLVE_FP64 | H.SOLVE_AUX)):
+        tot_ms, its, worst = 0.0, [], 0.0
+        for r in range(k):
+            xs = torch.zeros(N, dtype=torch.float64, device="cuda")
+            st = op.solve(B[r].contiguous(), xs, rtol=1e-8, maxit=args.solve_maxit, mode=mode, raise_on_fail=False)
+            tot_ms += st["solve_ms"]
+            its.append(st["iterations"])
+            worst = max(worst, st["rel_res"])
+        res[name] = {"sample_rhs": k, "iterations_mean": float(np.mean(its)), "max_true_rel_res": worst,
+                     "solves_per_s": round(1000.0 * k / tot_ms, 4)}
+    for name, mode in (("fp32_ir_batched", H.SOLVE_FP32_IR), ("fp32_ir_aux_batched", H.SOLVE_FP32_IR | H.SOLVE_AUX)):
+        X = torch.zeros_like(B)
+        st = op.solve(B, X, rtol=1e-8, maxit=args.solve_maxit, mode=mode, raise_on_fail=False)
+        res[name] = solve_entry(st, nr)
+    base = res["fp64_single"]["solves_per_s"]
+    res["speedup_vs_fp64_single"] = round(res["fp32_ir_batched"]["solves_per_s"] / base, 3)
+    res["aux_speedup_vs_fp64_single"] = round(res["fp32_ir_aux_batched"]["solves_per_s"] / base, 3)
+    res["aux_speedup_vs_fp64_aux_single"] = round(res["fp32_ir_aux_batched"]["solves_per_s"] /
+                                                  res["fp64_aux_single"]["solves_per_s"], 3)
     print(f"[bench] config3: {res}", file=sys.stderr, flush=True)
     del op
     return res
@@ -197,49 +205,56 @@ def build_config4(torch, H, count=64):
 def solves_config4(torch, H, ops, bs, args):
     """64 uneven instances, coil right-hand sides, rtol 1e-8: one batched
     FP32 PCG + FP64 refinement solve (FP64 continuation per instance where FP32
-    stalls) against the plain FP64 path solving the instances one at a time."""
+    stalls) against the plain FP64 path solving the instances one at a time;
+    each also with the gradient-space correction of the preconditioner (_aux)."""
     res = {"workload": f"{len(ops)} cubes n~U{{8..40}}, p~U{{1,2,3}}, two regions sigma/dt=10^U[-6,4] "
                        f"(seed 4+i), coil RHS (seed 1000+i); {sum(o.ndofs for o in ops)} DOFs in total",
            "rtol": 1e-8}
-    xs = [torch.zeros_like(b) for b in bs]
-    bt = H.Batch()
-    for op, b, x in zip(ops, bs, xs):
-        bt.add(op, b, x, rtol=1e-8, maxit=args.solve_maxit)
-    rc, st = bt.solve(H.SOLVE_FP32_IR, raise_on_fail=False)
-    ms = st[0]["solve_ms"]
-    res["fp32_ir_batched"] = {"solve_ms": round(ms, 1), "solves_per_s": round(1000.0 * len(ops) / ms, 3),
-                              "converged": int(sum(s["converged"] for s in st)),
-                              "fp64_continuations": int(sum(s["fp64_fallbacks"] for s in st)),
-                              "max_true_rel_res": max(s["rel_res"] for s in st),
-                              "iterations_mean": float(np.mean([s["iterations"] for s in st]))}
-    del bt
-    tot, its, conv, worst = 0.0, [], 0, 0.0
-    for op, b in zip(ops, bs):
-        x = torch.zeros_like(b)
-        s1 = op.solve(b, x, rtol=1e-8, maxit=args.solve_maxit, mode=H.SOLVE_FP64, raise_on_fail=False)
-        tot += s1["solve_ms"]
-        its.append(s1["iterations"])
-        conv += int(s1["converged"])
-        worst = max(worst, s1["rel_res"])
-    res["fp64_single"] = {"solve_ms_total": round(tot, 1), "solves_per_s": round(1000.0 * len(ops) / tot, 3),
-                          "converged": conv, "max_true_rel_res": worst, "iterations_mean": float(np.mean(its))}
-    res["speedup_vs_fp64_single"] = round(res["fp32_ir_batched"]["solves_per_s"] /
-                                          res["fp64_single"]["solves_per_s"], 3)
+    for name, mode in (("fp32_ir_batched", H.SOLVE_FP32_IR), ("fp32_ir_aux_batched", H.SOLVE_FP32_IR | H.SOLVE_AUX)):
+        xs = [torch.zeros_like(b) for b in bs]
+        bt = H.Batch()
+        for op, b, x in zip(ops, bs, xs):
+            bt.add(op, b, x, rtol=1e-8, maxit=args.solve_maxit)
+        rc, st = bt.solve(mode, raise_on_fail=False)
+        ms = st[0]["solve_ms"]
+        res[name] = {"solve_ms": round(ms, 1), "solves_per_s": round(1000.0 * len(ops) / ms, 3),
+                     "converged": int(sum(s["converged"] for s in st)),
+                     "fp64_continuations": int(sum(s["fp64_fallbacks"] for s in st)),
+                     "max_true_rel_res": max(s["rel_res"] for s in st),
+                     "iterations_mean": float(np.mean([s["iterations"] for s in st]))}
+        del bt
+    for name, mode in (("fp64_single", H.SOLVE_FP64), ("fp64_aux_single", H.SOLVE_FP64 | H.SOLVE_AUX)):
+        tot, its, conv, worst = 0.0, [], 0, 0.0
+        for op, b in zip(ops, bs):
+            x = torch.zeros_like(b)
+            s1 = op.solve(b, x, rtol=1e-8, maxit=args.solve_maxit, mode=mode, raise_on_fail=False)
+            tot += s1["solve_ms"]
+            its.append(s1["iterations"])
+            conv += int(s1["converged"])
+            worst = max(worst, s1["rel_res"])
+        res[name] = {"solve_ms_total": round(tot, 1), "solves_per_s": round(1000.0 * len(ops) / tot, 3),
+                     "converged": conv, "max_true_rel_res": worst, "iterations_mean": float(np.mean(its))}
+    base = res["fp64_single"]["solves_per_s"]
+    res["speedup_vs_fp64_single"] = round(res["fp32_ir_batched"]["solves_per_s"] / base, 3)
+    res["aux_speedup_vs_fp64_single"] = round(res["fp32_ir_aux_batched"]["solves_per_s"] / base, 3)
+    res["aux_speedup_vs_fp64_aux_single"] = round(res["fp32_ir_aux_batched"]["solves_per_s"] /
+                                                  res["fp64_aux_single"]["solves_per_s"], 3)
     print(f"[bench] config4: {res}", file=sys.stderr, flush=True)
     return res
 
 
-def serve_config5(torch, H, ops, bs, args, fp64_single_sps):
+def serve_config5(torch, H, ops, bs, args, fp64_single_sps, aux=True):
     """Poisson arrivals (mean 200 req/s, seed 5) of backward-Euler step solves
     drawn uniformly from the config-4 operators (resident), FP32 PCG + FP64
-    refinement to 1e-8, continuous batching under an HBM budget."""
+    refinement to 1e-8 (preconditioner: Jacobi with the gradient-space
+    correction when aux, else Jacobi), continuous batching under an HBM budget."""
     rng = np.random.default_rng(5)
     nreq = args.serve_requests
     gaps = rng.exponential(1.0 / 200.0, nreq)
     arrivals_ms = np.cumsum(gaps) * 1000.0
     pick = rng.integers(0, len(ops), nreq)
     us = [torch.zeros_like(bs[k]) for k in pick]
-    srv = H.Server(args.serve_budget_gb * (1 << 30), mode=H.SOLVE_FP32_IR, rtol=1e-8,
+    srv = H.Server(args.serve_budget_gb * (1 << 30), mode=H.SOLVE_FP32_IR | (H.SOLVE_AUX if aux else 0), rtol=1e-8,
                    maxit=args.solve_maxit, round_iters=16)
     t0 = time.perf_counter()
     for i in range(nreq):
@@ -254,7 +269,8 @@ def serve_config5(torch, H, ops, bs, args, fp64_single_sps):
     span_ms = max(r["arrival_ms"] + r["latency_ms"] for r in out)
     sps = nreq / (span_ms / 1000.0)
     res = {"workload": f"{nreq} requests, Poisson arrivals mean 200 req/s (seed 5), operators drawn from the "
-                       f"config-4 mix, u_n = 0, rtol 1e-8, FP32 PCG + FP64 refinement",
+                       f"config-4 mix, u_n = 0, rtol 1e-8, FP32 PCG + FP64 refinement, "
+                       f"{'Jacobi + gradient-space correction' if aux else 'Jacobi'} preconditioner",
            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
            "mean_queue_ms": round(float(np.mean([r["queue_ms"] for r in out])), 2),
            "solves_per_s": round(sps, 3), "converged": int(sum(r["converged"] for r in out)),
@@ -263,7 +279,7 @@ def serve_config5(torch, H, ops, bs, args, fp64_single_sps):
            "budget_gb": args.serve_budget_gb, "peak_budget_use_gb": round(info["peak_bytes"] / 2 ** 30, 2),
            "rounds": info["rounds"], "wall_s": round(wall, 2),
            "speedup_vs_fp64_single": round(sps / fp64_single_sps, 3) if fp64_single_sps else None}
-    print(f"[bench] config5: {res}", file=sys.stderr, flush=True)
+    print(f"[bench] config5{' aux' if aux else ''}: {res}", file=sys.stderr, flush=True)
     return res
 
 
@@ -371,7 +387,9 @@ def main():
             ops4, bs4 = build_config4(torch, H)
             c4 = solves_config4(torch, H, ops4, bs4, args)
             out["solves"]["config4"] = c4
-            out["serving"] = serve_config5(torch, H, ops4, bs4, args, c4["fp64_single"]["solves_per_s"])
+            out["serving"] = serve_config5(torch, H, ops4, bs4, args, c4["fp64_single"]["solves_per_s"], aux=True)
+            out["serving_jacobi"] = serve_config5(torch, H, ops4, bs4, args, c4["fp64_single"]["solves_per_s"],
+                                                  aux=False)
             del ops4, bs4
             torch.cuda.empty_cache()
     if not args.no_cpu:

@@ -27,6 +27,9 @@ make_aux_geom(const Geom & g, const double * dg1d)
     a.off[c] = g.off[c];
   for (int i = 0; i < g.p * (g.p + 1); i++)
     a.dg[i] = dg1d[i];
+  for (int c = 0; c < 3; c++)
+    for (int d = 0; d < 2; d++)
+      a.rcd[c][d] = 1.0 / (double)a.cd[c][d];
   return a;
 }
 
@@ -135,8 +138,11 @@ __global__ void
 k_aux_nodes_single(AuxGeom a, const double * __restrict__ ginv, const double * __restrict__ r,
                    double * __restrict__ t)
 {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < a.NN; k += gridDim.x * blockDim.x)
-    t[k] = ginv[k] * aux_gt(a, r, k);
+  const int S = gridDim.x * blockDim.x;
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  NodeWalk nw(a, k, S);
+  for (; k < a.NN; k += S, nw.step(a))
+    t[k] = ginv[k] * aux_gt(a, r, nw.I, nw.J, nw.K);
 }
 
 __global__ void

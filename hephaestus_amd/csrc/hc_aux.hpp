@@ -35,15 +35,34 @@ struct AuxGeom
   int64_t off[4];                 // ND component offsets, off[3] = N
   int NN;                         // nodes
   double dg[PMAX * (PMAX + 1)];   // Dg[i][m] at i (p + 1) + m
+  double rcd[3][2];               // 1 / cd[c][0], 1 / cd[c][1]
 };
 
-// (G^T r) at node k.
+// q = n / d, r = n % d for 0 <= n < 2^31, 1 <= d, with rd = 1 / d: the FP64
+// product is within one of the quotient, one correction step makes it exact.
+__device__ inline void
+fast_divmod(int n, int d, double rd, int & q, int & r)
+{
+  q = (int)((double)n * rd);
+  r = n - q * d;
+  if (r < 0)
+  {
+    q -= 1;
+    r += d;
+  }
+  else if (r >= d)
+  {
+    q += 1;
+    r -= d;
+  }
+}
+
+// (G^T r) at node (I, J, K).
 template <typename T>
 __device__ inline double
-aux_gt(const AuxGeom & a, const T * __restrict__ r, int k)
+aux_gt(const AuxGeom & a, const T * __restrict__ r, int I, int J, int K)
 {
   const int p = a.p, D = p + 1;
-  const int I = k % a.M[0], J = (k / a.M[0]) % a.M[1], K = k / (a.M[0] * a.M[1]);
   const int idx[3] = {I, J, K};
   double g = 0.0;
 #pragma unroll
@@ -85,6 +104,41 @@ aux_gt(const AuxGeom & a, const T * __restrict__ r, int k)
   return g;
 }
 
+// Node index walker of a grid-stride loop (start k0, step S): keeps (I, J, K)
+// of k up to date with additions instead of divisions.
+struct NodeWalk
+{
+  int I, J, K, sI, sJ, sK;
+  __device__ NodeWalk(const AuxGeom & a, int k0, int S)
+  {
+    const int m01 = a.M[0] * a.M[1];
+    I = k0 % a.M[0];
+    J = (k0 / a.M[0]) % a.M[1];
+    K = k0 / m01;
+    sI = S % a.M[0];
+    sJ = (S / a.M[0]) % a.M[1];
+    sK = S / m01;
+  }
+  __device__ void step(const AuxGeom & a)
+  {
+    I += sI;
+    int cy = 0;
+    if (I >= a.M[0])
+    {
+      I -= a.M[0];
+      cy = 1;
+    }
+    J += sJ + cy;
+    cy = 0;
+    if (J >= a.M[1])
+    {
+      J -= a.M[1];
+      cy = 1;
+    }
+    K += sK + cy;
+  }
+};
+
 // (G t) at ND DOF i.
 template <typename T>
 __device__ inline double
@@ -93,9 +147,9 @@ aux_g(const AuxGeom & a, const T * __restrict__ t, int64_t i)
   const int p = a.p, D = p + 1;
   const int c = (i < a.off[1]) ? 0 : ((i < a.off[2]) ? 1 : 2);
   const int li = (int)(i - a.off[c]);
-  const int d0 = a.cd[c][0], d1 = a.cd[c][1];
-  const int i0 = li % d0, rest = li / d0;
-  const int i1 = rest % d1, i2 = rest / d1;
+  int i0, rest, i1, i2;
+  fast_divmod(li, a.cd[c][0], a.rcd[c][0], rest, i0);
+  fast_divmod(rest, a.cd[c][1], a.rcd[c][1], i2, i1);
   const int ic = (c == 0) ? i0 : ((c == 1) ? i1 : i2);
   const int e = ic / p, l = ic % p;
   int node[3] = {i0, i1, i2};

@@ -182,7 +182,7 @@ hipError_t assemble_planes(const ApplyDesc & d0, const ApplyDesc * descs, int nd
 // ---------------------------------------------------------------------------
 // Jacobi-PCG over solve slots (hc_pcg.hip)
 // ---------------------------------------------------------------------------
-constexpr int RED_BLOCKS = 512; // most blocks per slot of the reduction kernels
+constexpr int RED_BLOCKS = 2048; // most blocks per slot of the reduction kernels
 constexpr int RED_THREADS = 256;
 
 // Blocks of a slot of length n (a function of n only: the summation order of a

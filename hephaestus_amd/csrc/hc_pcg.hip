@@ -342,9 +342,12 @@ k_aux_nodes(const SolveSlot * __restrict__ slots, int which)
   T * __restrict__ t = V.t();
   __shared__ double sh[RED_THREADS];
   double v[1] = {0};
-  for (int k = blockIdx.x * RED_THREADS + threadIdx.x; k < sl.ag.NN; k += nb * RED_THREADS)
+  const int S = nb * RED_THREADS;
+  int k = blockIdx.x * RED_THREADS + threadIdx.x;
+  NodeWalk nw(sl.ag, k, S);
+  for (; k < sl.ag.NN; k += S, nw.step(sl.ag))
   {
-    const double g = aux_gt(sl.ag, r, k);
+    const double g = aux_gt(sl.ag, r, nw.I, nw.J, nw.K);
     const T tk = (T)((double)ginv[k] * g);
     t[k] = tk;
     v[0] += g * (double)tk;
