@@ -59,12 +59,13 @@ wave_lds_sync()
   asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 }
 
-// BATCH = false: one instance, described by the by-value argument d0 (its
-// fields stay in scalar registers), one rhs per work item.  BATCH = true: the
+// BATCH = 0: one instance, described by the by-value argument d0 (its
+// fields stay in scalar registers), one rhs per work item.  BATCH = 2: the
+// instances of the device table descs[], one rhs per work item.  BATCH = 1: the
 // instances of the device table descs[]; a work item loops over a group of rb
 // right-hand sides with the tile's quadrature data held in registers (read
 // from HBM once per group).
-template <int P, typename T, int TX, int TY, int TZ, int MODE, int SP, bool BATCH>
+template <int P, typename T, int TX, int TY, int TZ, int MODE, int SP, int BATCH>
 __global__ void __launch_bounds__((Tile<P, TX, TY, TZ, SP>::NT))
 k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables<T> tb)
 {
@@ -84,16 +85,17 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   // rb consecutive right-hand sides (rb = 1 unless BATCH) ----
   const int w = xcd_remap(blockIdx.x, gridDim.x);
   int di = 0;
-  if constexpr (BATCH)
+  constexpr bool TABLE = BATCH != 0, GROUP = BATCH == 1;
+  if constexpr (TABLE)
     di = __builtin_amdgcn_readfirstlane(find_instance(w, ndesc, [&](int i) { return descs[i].w0; }));
-  const ApplyDesc & ad = BATCH ? descs[di] : d0;
-  const int rb = BATCH ? ad.rb : 1;
-  const int ngrp = BATCH ? (ad.nrhs + rb - 1) / rb : ad.nrhs;
+  const ApplyDesc & ad = TABLE ? descs[di] : d0;
+  const int rb = GROUP ? ad.rb : 1;
+  const int ngrp = GROUP ? (ad.nrhs + rb - 1) / rb : ad.nrhs;
   const int wl = w - ad.w0;
   const int r_begin = (wl % ngrp) * rb;
-  const int r_end = BATCH ? min(ad.nrhs, r_begin + rb) : r_begin + 1;
+  const int r_end = GROUP ? min(ad.nrhs, r_begin + rb) : r_begin + 1;
   const int tcur = wl / ngrp;
-  if (!BATCH && ad.done && ad.done[r_begin * ad.done_stride] != ad.gate)
+  if (!GROUP && ad.done && ad.done[r_begin * ad.done_stride] != ad.gate)
     return;
   const Geom & g = ad.g;
   const T * __restrict__ qd = (const T *)ad.qd;
@@ -223,7 +225,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   bool x_ready = false;
   if (MODE >= 1)
   {
-    if constexpr (!BATCH)
+    if constexpr (!GROUP)
     {
       x = (const T *)ad.x + (int64_t)r_begin * g.off[3];
       load_x(e0, ne);
@@ -234,7 +236,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
 
   for (int rhs = r_begin; rhs < r_end; rhs++)
   {
-  if (BATCH && ad.done && ad.done[rhs * ad.done_stride] != ad.gate)
+  if (GROUP && ad.done && ad.done[rhs * ad.done_stride] != ad.gate)
     continue;
   x = (const T *)ad.x + (int64_t)rhs * g.off[3];
   y = (T *)ad.y + (int64_t)rhs * g.off[3];
@@ -447,13 +449,13 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   if (MODE == 0)
     load_qf(e0, ne);
   // term weights (batched kernel only; the single-instance kernel applies A)
-  const T msc = BATCH ? (T)ad.mscale : T(1), csc = BATCH ? (T)ad.cscale : T(1);
+  const T msc = TABLE ? (T)ad.mscale : T(1), csc = TABLE ? (T)ad.cscale : T(1);
 #pragma unroll
   for (int j = 0; j < QH; j++)
   {
     T u0 = U[0][j], u1 = U[1][j], u2 = U[2][j];
     T c0 = Cu[0][j], c1 = Cu[1][j], c2 = Cu[2][j];
-    if constexpr (BATCH)
+    if constexpr (TABLE)
     {
       u0 *= msc;
       u1 *= msc;
@@ -814,7 +816,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
     else
       wout(std::integral_constant<int, 2>{}, i, i - OB2);
   }
-  if (BATCH && rhs + 1 < r_end)
+  if (GROUP && rhs + 1 < r_end)
     __syncthreads(); // the next rhs overwrites the LDS stages
   }
 }
@@ -824,24 +826,40 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
 // ---------------------------------------------------------------------------
 template <int P, int TX, int TY, int TZ, int MODE, int SP>
 static hipError_t
-launch_one(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, const void * tb, int prec,
-           hipStream_t st)
+launch_one(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, int grouped, const void * tb,
+           int prec, hipStream_t st)
 {
   using C = Tile<P, TX, TY, TZ, SP>;
   if (nwork <= 0)
     return hipSuccess;
-  if (prec == 64 && descs)
-    hipLaunchKernelGGL((k_apply_tile<P, double, TX, TY, TZ, MODE, SP, true>), dim3(nwork), dim3(C::NT), 0, st, d0,
-                       descs, ndesc, *(const Tables<double> *)tb);
-  else if (prec == 64)
-    hipLaunchKernelGGL((k_apply_tile<P, double, TX, TY, TZ, MODE, SP, false>), dim3(nwork), dim3(C::NT), 0, st, d0,
-                       descs, ndesc, *(const Tables<double> *)tb);
-  else if (descs)
-    hipLaunchKernelGGL((k_apply_tile<P, float, TX, TY, TZ, MODE, SP, true>), dim3(nwork), dim3(C::NT), 0, st, d0,
-                       descs, ndesc, *(const Tables<float> *)tb);
+  // BATCH variant: 0 single instance, 1 table with rhs groups, 2 table with one rhs per item
+  const int bv = !descs ? 0 : (grouped ? 1 : 2);
+  if (prec == 64)
+  {
+    const Tables<double> & t = *(const Tables<double> *)tb;
+    if (bv == 0)
+      hipLaunchKernelGGL((k_apply_tile<P, double, TX, TY, TZ, MODE, SP, 0>), dim3(nwork), dim3(C::NT), 0, st, d0,
+                         descs, ndesc, t);
+    else if (bv == 1)
+      hipLaunchKernelGGL((k_apply_tile<P, double, TX, TY, TZ, MODE, SP, 1>), dim3(nwork), dim3(C::NT), 0, st, d0,
+                         descs, ndesc, t);
+    else
+      hipLaunchKernelGGL((k_apply_tile<P, double, TX, TY, TZ, MODE, SP, 2>), dim3(nwork), dim3(C::NT), 0, st, d0,
+                         descs, ndesc, t);
+  }
   else
-    hipLaunchKernelGGL((k_apply_tile<P, float, TX, TY, TZ, MODE, SP, false>), dim3(nwork), dim3(C::NT), 0, st, d0,
-                       descs, ndesc, *(const Tables<float> *)tb);
+  {
+    const Tables<float> & t = *(const Tables<float> *)tb;
+    if (bv == 0)
+      hipLaunchKernelGGL((k_apply_tile<P, float, TX, TY, TZ, MODE, SP, 0>), dim3(nwork), dim3(C::NT), 0, st, d0,
+                         descs, ndesc, t);
+    else if (bv == 1)
+      hipLaunchKernelGGL((k_apply_tile<P, float, TX, TY, TZ, MODE, SP, 1>), dim3(nwork), dim3(C::NT), 0, st, d0,
+                         descs, ndesc, t);
+    else
+      hipLaunchKernelGGL((k_apply_tile<P, float, TX, TY, TZ, MODE, SP, 2>), dim3(nwork), dim3(C::NT), 0, st, d0,
+                         descs, ndesc, t);
+  }
   return hipGetLastError();
 }
 
@@ -926,10 +944,10 @@ apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T *
     d.rb = rb < nrhs ? rb : nrhs;
     if ((e = hipMemcpyAsync(desc_dev, &d, sizeof(d), hipMemcpyHostToDevice, st)) != hipSuccess)
       return e;
-    e = cand->launch(d, desc_dev, 1, d.ntiles * ((nrhs + d.rb - 1) / d.rb), &tb, sizeof(T) == 8 ? 64 : 32, st);
+    e = cand->launch(d, desc_dev, 1, d.ntiles * ((nrhs + d.rb - 1) / d.rb), 1, &tb, sizeof(T) == 8 ? 64 : 32, st);
   }
   else
-    e = cand->launch(d, nullptr, 1, d.ntiles * nrhs, &tb, sizeof(T) == 8 ? 64 : 32, st);
+    e = cand->launch(d, nullptr, 1, d.ntiles * nrhs, 0, &tb, sizeof(T) == 8 ? 64 : 32, st);
   if (e != hipSuccess)
     return e;
   return assemble_planes<T>(d, nullptr, 1, d.prows * nrhs, st);
@@ -958,7 +976,7 @@ apply_scaled(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const
   hipError_t e = hipMemcpyAsync(desc_dev, &d, sizeof(d), hipMemcpyHostToDevice, st);
   if (e != hipSuccess)
     return e;
-  e = cand->launch(d, desc_dev, 1, d.ntiles, &tb, sizeof(T) == 8 ? 64 : 32, st);
+  e = cand->launch(d, desc_dev, 1, d.ntiles, 0, &tb, sizeof(T) == 8 ? 64 : 32, st);
   if (e != hipSuccess)
     return e;
   return assemble_planes<T>(d, nullptr, 1, d.prows, st);
@@ -989,18 +1007,18 @@ template hipError_t apply_op<float>(const Geom &, const PlaneSegs &, const Table
 template <typename T>
 hipError_t
 apply_multi(const TileCandidate * cand, const Tables<T> & tb, const ApplyDesc * descs_dev, int ndesc, int nwork,
-            int npwork, hipStream_t st)
+            int npwork, int grouped, hipStream_t st)
 {
   ApplyDesc dummy{};
-  hipError_t e = cand->launch(dummy, descs_dev, ndesc, nwork, &tb, sizeof(T) == 8 ? 64 : 32, st);
+  hipError_t e = cand->launch(dummy, descs_dev, ndesc, nwork, grouped, &tb, sizeof(T) == 8 ? 64 : 32, st);
   if (e != hipSuccess)
     return e;
   return assemble_planes<T>(dummy, descs_dev, ndesc, npwork, st);
 }
 
-template hipError_t apply_multi<double>(const TileCandidate *, const Tables<double> &, const ApplyDesc *, int, int,
+template hipError_t apply_multi<double>(const TileCandidate *, const Tables<double> &, const ApplyDesc *, int, int, int,
                                         int, hipStream_t);
-template hipError_t apply_multi<float>(const TileCandidate *, const Tables<float> &, const ApplyDesc *, int, int, int,
+template hipError_t apply_multi<float>(const TileCandidate *, const Tables<float> &, const ApplyDesc *, int, int, int, int,
                                        hipStream_t);
 
 } // namespace hc

@@ -529,8 +529,8 @@ k_apply_qp(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables
 
 template <int P, int TX, int TY, int TZ>
 hipError_t
-launch_qp(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, const void * tb, int prec,
-          hipStream_t st)
+launch_qp(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, int grouped, const void * tb,
+          int prec, hipStream_t st)
 {
   using C = QpTile<P, TX, TY, TZ>;
   if (nwork <= 0)
@@ -551,8 +551,8 @@ launch_qp(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, c
 }
 
 #define HC_QP_INST(P, TX, TY, TZ)                                                                                \
-  template hipError_t launch_qp<P, TX, TY, TZ>(const ApplyDesc &, const ApplyDesc *, int, int, const void *, int, \
-                                               hipStream_t);
+  template hipError_t launch_qp<P, TX, TY, TZ>(const ApplyDesc &, const ApplyDesc *, int, int, int, const void *, \
+                                               int, hipStream_t);
 HC_QP_INST(1, 4, 4, 4)
 HC_QP_INST(1, 8, 4, 2)
 HC_QP_INST(2, 4, 2, 2)

@@ -679,7 +679,7 @@ hc_op_profile_apply(hc_op_t op, int prec_bits, const void * x, void * y, int rep
   for (int r = 0; r < reps; r++)
   {
     HC_TRY_HIP(hipEventRecord(ev[3 * r], st));
-    HC_TRY_HIP(pl.cand->launch(d, nullptr, 1, d.ntiles, tb, prec_bits, st));
+    HC_TRY_HIP(pl.cand->launch(d, nullptr, 1, d.ntiles, 0, tb, prec_bits, st));
     HC_TRY_HIP(hipEventRecord(ev[3 * r + 1], st));
     HC_TRY_HIP(f32 ? hc::assemble_planes<float>(d, nullptr, 1, d.prows, st)
                    : hc::assemble_planes<double>(d, nullptr, 1, d.prows, st));

@@ -87,7 +87,8 @@ xcd_remap(int b, int nb)
 
 // A compiled tile variant of the apply kernel (shape + load mode).
 // descs == nullptr: the single instance d0; otherwise ndesc instances.
-using TileLaunch = hipError_t (*)(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork,
+// grouped != 0 (with descs): work items are groups of rb right-hand sides.
+using TileLaunch = hipError_t (*)(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, int grouped,
                                   const void * tables, int prec_bits, hipStream_t st);
 struct TileCandidate
 {
@@ -98,8 +99,8 @@ struct TileCandidate
 };
 // The one-lane-per-quadrature-point tile kernel (hc_apply_qp.hip), mode 8.
 template <int P, int TX, int TY, int TZ>
-hipError_t launch_qp(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, const void * tables,
-                     int prec_bits, hipStream_t st);
+hipError_t launch_qp(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, int grouped,
+                     const void * tables, int prec_bits, hipStream_t st);
 constexpr int MODE_QP = 8;
 // Candidates of order p (contiguous array); returns their count.
 int tile_candidates(int p, const TileCandidate ** out);
@@ -173,7 +174,7 @@ void plane_rows(ApplyDesc & d);
 // nwork = sum of ntiles*nrhs, npwork = sum of prows*nrhs.
 template <typename T>
 hipError_t apply_multi(const TileCandidate * cand, const Tables<T> & tb, const ApplyDesc * descs_dev, int ndesc,
-                       int nwork, int npwork, hipStream_t st);
+                       int nwork, int npwork, int grouped, hipStream_t st);
 
 // Sum of the tile-face partials (fixed order) into y.
 template <typename T>
@@ -276,8 +277,10 @@ struct SlotTable
 {
   const SolveSlot * slots; // device
   int nslots;
-  int nb_max; // max red_blocks(n) over the slots
+  int nb_max;  // max red_blocks(n) over the slots
   int any_aux; // some slot uses the gradient-space correction
+  int nblk;    // blocks of the flattened slot grid: sum of red_blocks(n)
+  const int * blk0; // device, nslots + 1: first block of each slot
 };
 
 // Phase moves at the start of a round (one thread per slot).

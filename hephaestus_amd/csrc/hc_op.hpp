@@ -277,6 +277,7 @@ private:
     ApplyDesc single;  // by-value descriptor (batched == false)
     const ApplyDesc * descs; // device
     int ndesc, nwork, npwork;
+    int grouped; // some instance of the table groups rb > 1 right-hand sides per work item
     const TileCandidate * cand;
     const void * tables; // Tables<double> or Tables<float>
     int prec;

@@ -92,8 +92,32 @@ bad(double v)
 }
 
 // grid-stride loop over the entries of one slot handled by this block
+// (bk: the block's index within its slot)
 #define HC_SLOT_LOOP(i, n, nb)                                                                                    \
-  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < (n); i += (int64_t)(nb)*RED_THREADS)
+  for (int64_t i = (int64_t)bk * RED_THREADS + threadIdx.x; i < (n); i += (int64_t)(nb)*RED_THREADS)
+
+// Block of the flattened slot grid: slot s and block b within the slot (the
+// blocks of slot s are [blk0[s], blk0[s + 1]), blk0[s + 1] - blk0[s] =
+// slot_blocks(n of slot s)).
+struct SlotBlock
+{
+  int s, b;
+};
+__device__ inline SlotBlock
+slot_block(const SlotTable & t)
+{
+  const int g = blockIdx.x;
+  int lo = 0, hi = t.nslots - 1;
+  while (lo < hi)
+  {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t.blk0[mid] <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return {lo, g - t.blk0[lo]};
+}
 
 // Vectors of a slot in the precision of T.
 template <typename T>
@@ -131,11 +155,13 @@ struct SlotVecs<float>
 // FP64 start: r = b - q64 (q64 = A x of this round), p = D^-1 r; partials
 // (b,b), (r, D^-1 r), (r,r).  FP64 slots in PH_INIT.
 __global__ void __launch_bounds__(RED_THREADS)
-k_pcg_init64(const SolveSlot * __restrict__ slots)
+k_pcg_init64(SlotTable t)
 {
-  const SolveSlot & sl = slots[blockIdx.y];
+  const SlotBlock sb = slot_block(t);
+  const int bk = sb.b;
+  const SolveSlot & sl = t.slots[sb.s];
   const int nb = slot_blocks(sl.n);
-  if ((int)blockIdx.x >= nb || sl.out->phase != PH_INIT || !sl.out->prec64)
+  if (sl.out->phase != PH_INIT || !sl.out->prec64)
     return;
   const double * __restrict__ b = sl.b;
   const double * __restrict__ q = sl.q64;
@@ -159,9 +185,9 @@ k_pcg_init64(const SolveSlot * __restrict__ slots)
   block_sum<3>(v, sh);
   if (threadIdx.x == 0)
   {
-    sl.part[blockIdx.x * 4 + 0] = v[0];
-    sl.part[blockIdx.x * 4 + 1] = v[1];
-    sl.part[blockIdx.x * 4 + 2] = v[2];
+    sl.part[bk * 4 + 0] = v[0];
+    sl.part[bk * 4 + 1] = v[1];
+    sl.part[bk * 4 + 2] = v[2];
   }
 }
 
@@ -205,14 +231,16 @@ k_pcg_init64_fin(const SolveSlot * __restrict__ slots, const int * __restrict__ 
 
 template <typename T>
 __global__ void __launch_bounds__(RED_THREADS)
-k_pcg_dot_pq(const SolveSlot * __restrict__ slots, int * git)
+k_pcg_dot_pq(SlotTable t, int * git)
 {
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+  if (blockIdx.x == 0 && threadIdx.x == 0)
     *git = *git + 1; // read by the next kernels only
-  const SolveSlot & sl = slots[blockIdx.y];
+  const SlotBlock sb = slot_block(t);
+  const int bk = sb.b;
+  const SolveSlot & sl = t.slots[sb.s];
   const int nb = slot_blocks(sl.n);
   const SlotVecs<T> V{sl};
-  if ((int)blockIdx.x >= nb || sl.in->run != SlotVecs<T>::RUN)
+  if (sl.in->run != SlotVecs<T>::RUN)
     return;
   const T * __restrict__ p = V.p();
   const T * __restrict__ q = V.q();
@@ -222,18 +250,20 @@ k_pcg_dot_pq(const SolveSlot * __restrict__ slots, int * git)
   v[0] += (double)p[i] * (double)q[i];
   block_sum<1>(v, sh);
   if (threadIdx.x == 0)
-    sl.part[blockIdx.x * 4 + 3] = v[0];
+    sl.part[bk * 4 + 3] = v[0];
 }
 
 template <typename T>
 __global__ void __launch_bounds__(RED_THREADS)
-k_pcg_update_xr(const SolveSlot * __restrict__ slots, const int * __restrict__ git)
+k_pcg_update_xr(SlotTable t, const int * __restrict__ git)
 {
-  const SolveSlot & sl = slots[blockIdx.y];
+  const SlotBlock sb = slot_block(t);
+  const int bk = sb.b;
+  const SolveSlot & sl = t.slots[sb.s];
   const PcgState * s = sl.in;
   const int nb = slot_blocks(sl.n);
   const SlotVecs<T> V{sl};
-  if ((int)blockIdx.x >= nb || s->run != SlotVecs<T>::RUN)
+  if (s->run != SlotVecs<T>::RUN)
     return;
   const int it = *git - 1 - s->it0;
   T * __restrict__ x = V.x();
@@ -259,20 +289,22 @@ k_pcg_update_xr(const SolveSlot * __restrict__ slots, const int * __restrict__ g
   block_sum<2>(v, sh);
   if (threadIdx.x == 0)
   {
-    sl.part[blockIdx.x * 4 + 0] = v[0];
-    sl.part[blockIdx.x * 4 + 1] = v[1];
+    sl.part[bk * 4 + 0] = v[0];
+    sl.part[bk * 4 + 1] = v[1];
   }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(RED_THREADS)
-k_pcg_update_p(const SolveSlot * __restrict__ slots, const int * __restrict__ git)
+k_pcg_update_p(SlotTable t, const int * __restrict__ git)
 {
-  const SolveSlot & sl = slots[blockIdx.y];
+  const SlotBlock sb = slot_block(t);
+  const int bk = sb.b;
+  const SolveSlot & sl = t.slots[sb.s];
   PcgState * s = sl.in;
   const int nb = slot_blocks(sl.n);
   const SlotVecs<T> V{sl};
-  if ((int)blockIdx.x >= nb || s->run != SlotVecs<T>::RUN)
+  if (s->run != SlotVecs<T>::RUN)
     return;
   const int it = *git - 1 - s->it0;
   __shared__ double sh[3 * RED_THREADS];
@@ -283,7 +315,7 @@ k_pcg_update_p(const SolveSlot * __restrict__ slots, const int * __restrict__ gi
   const bool nan = bad(gnew) || bad(rr);
   const bool conv = nan || rr <= s->tol2 * s->bb;
   const bool stop = conv || it + 1 >= s->maxit;
-  if (blockIdx.x == 0 && threadIdx.x == 0)
+  if (bk == 0 && threadIdx.x == 0)
   {
     s->gam[(it + 1) & 1] = gnew;
     s->rr = rr;
@@ -306,9 +338,9 @@ k_pcg_update_p(const SolveSlot * __restrict__ slots, const int * __restrict__ gi
   const T beta = (T)(gnew / gold);
   if (sl.aux)
   {
-    const T * __restrict__ t = V.t();
+    const T * __restrict__ tv = V.t();
     HC_SLOT_LOOP(i, sl.n, nb)
-    p[i] = (dinv[i] * r[i] + (T)aux_g(sl.ag, t, i)) + beta * p[i];
+    p[i] = (dinv[i] * r[i] + (T)aux_g(sl.ag, tv, i)) + beta * p[i];
   }
   else
   {
@@ -324,11 +356,13 @@ k_pcg_update_p(const SolveSlot * __restrict__ slots, const int * __restrict__ gi
 // 2: FP32 slots starting (kp = 3 for both).
 template <typename T>
 __global__ void __launch_bounds__(RED_THREADS)
-k_aux_nodes(const SolveSlot * __restrict__ slots, int which)
+k_aux_nodes(SlotTable t, int which)
 {
-  const SolveSlot & sl = slots[blockIdx.y];
+  const SlotBlock sb = slot_block(t);
+  const int bk = sb.b;
+  const SolveSlot & sl = t.slots[sb.s];
   const int nb = slot_blocks(sl.n);
-  if ((int)blockIdx.x >= nb || !sl.aux)
+  if (!sl.aux)
     return;
   if (which == 0 && sl.in->run != SlotVecs<T>::RUN)
     return;
@@ -339,34 +373,36 @@ k_aux_nodes(const SolveSlot * __restrict__ slots, int which)
   const SlotVecs<T> V{sl};
   const T * __restrict__ r = V.r();
   const T * __restrict__ ginv = V.ginv();
-  T * __restrict__ t = V.t();
+  T * __restrict__ tv = V.t();
   __shared__ double sh[RED_THREADS];
   double v[1] = {0};
   const int S = nb * RED_THREADS;
-  int k = blockIdx.x * RED_THREADS + threadIdx.x;
+  int k = bk * RED_THREADS + threadIdx.x;
   NodeWalk nw(sl.ag, k, S);
   for (; k < sl.ag.NN; k += S, nw.step(sl.ag))
   {
     const double g = aux_gt(sl.ag, r, nw.I, nw.J, nw.K);
     const T tk = (T)((double)ginv[k] * g);
-    t[k] = tk;
+    tv[k] = tk;
     v[0] += g * (double)tk;
   }
   block_sum<1>(v, sh);
   if (threadIdx.x == 0)
-    sl.part[blockIdx.x * 4 + (which == 0 ? 2 : 3)] = v[0];
+    sl.part[bk * 4 + (which == 0 ? 2 : 3)] = v[0];
 }
 
 // First search direction p = D^-1 r + G t of starting aux slots (which as in
 // k_aux_nodes; FP32 slots only on a fresh start, a paused PCG keeps p).
 template <typename T>
 __global__ void __launch_bounds__(RED_THREADS)
-k_aux_pset(const SolveSlot * __restrict__ slots, int which)
+k_aux_pset(SlotTable t, int which)
 {
-  const SolveSlot & sl = slots[blockIdx.y];
+  const SlotBlock sb = slot_block(t);
+  const int bk = sb.b;
+  const SolveSlot & sl = t.slots[sb.s];
   const int nb = slot_blocks(sl.n);
   const PcgState * o = sl.out;
-  if ((int)blockIdx.x >= nb || !sl.aux || o->phase != PH_INIT)
+  if (!sl.aux || o->phase != PH_INIT)
     return;
   if ((which == 1) != (o->prec64 != 0))
     return;
@@ -375,10 +411,10 @@ k_aux_pset(const SolveSlot * __restrict__ slots, int which)
   const SlotVecs<T> V{sl};
   const T * __restrict__ r = V.r();
   const T * __restrict__ dinv = V.dinv();
-  const T * __restrict__ t = V.t();
+  const T * __restrict__ tv = V.t();
   T * __restrict__ p = V.p();
   HC_SLOT_LOOP(i, sl.n, nb)
-  p[i] = dinv[i] * r[i] + (T)aux_g(sl.ag, t, i);
+  p[i] = dinv[i] * r[i] + (T)aux_g(sl.ag, tv, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -420,25 +456,29 @@ k_phase_prep(const SolveSlot * __restrict__ slots, int nslots)
 // x += x32: the FP32 iterate since the last replacement folded into the FP64
 // solution (FP32 slots entering PH_RESID after iterations)
 __global__ void
-k_ir_update(const SolveSlot * __restrict__ slots)
+k_ir_update(SlotTable t)
 {
-  const SolveSlot & sl = slots[blockIdx.y];
+  const SlotBlock sb = slot_block(t);
+  const int bk = sb.b;
+  const SolveSlot & sl = t.slots[sb.s];
   const PcgState * s = sl.out;
   if (s->phase != PH_RESID || s->first || s->prec64)
     return;
   double * __restrict__ x = sl.x;
   const float * __restrict__ d32 = sl.x32;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sl.n; i += (int64_t)gridDim.x * blockDim.x)
+  for (int64_t i = (int64_t)bk * blockDim.x + threadIdx.x; i < sl.n; i += (int64_t)slot_blocks(sl.n) * blockDim.x)
     x[i] += (double)d32[i];
 }
 
 // r64 = b - q64 (q64 = A x), partials (r,r), (b,b); slots in PH_RESID
 __global__ void __launch_bounds__(RED_THREADS)
-k_residual64(const SolveSlot * __restrict__ slots)
+k_residual64(SlotTable t)
 {
-  const SolveSlot & sl = slots[blockIdx.y];
+  const SlotBlock sb = slot_block(t);
+  const int bk = sb.b;
+  const SolveSlot & sl = t.slots[sb.s];
   const int nb = slot_blocks(sl.n);
-  if ((int)blockIdx.x >= nb || sl.out->phase != PH_RESID)
+  if (sl.out->phase != PH_RESID)
     return;
   const double * __restrict__ b = sl.b;
   const double * __restrict__ Ax = sl.q64;
@@ -455,8 +495,8 @@ k_residual64(const SolveSlot * __restrict__ slots)
   block_sum<2>(v, sh);
   if (threadIdx.x == 0)
   {
-    sl.part[blockIdx.x * 4 + 0] = v[0];
-    sl.part[blockIdx.x * 4 + 1] = v[1];
+    sl.part[bk * 4 + 0] = v[0];
+    sl.part[bk * 4 + 1] = v[1];
   }
 }
 
@@ -537,14 +577,16 @@ k_residual_fin(const SolveSlot * __restrict__ slots, int max_outer)
 
 // x -= x32 for slots that reject their last FP32 increment
 __global__ void
-k_ir_revert(const SolveSlot * __restrict__ slots)
+k_ir_revert(SlotTable t)
 {
-  const SolveSlot & sl = slots[blockIdx.y];
+  const SlotBlock sb = slot_block(t);
+  const int bk = sb.b;
+  const SolveSlot & sl = t.slots[sb.s];
   if (!sl.out->revert || sl.out->phase != PH_RESID)
     return;
   double * __restrict__ x = sl.x;
   const float * __restrict__ d32 = sl.x32;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sl.n; i += (int64_t)gridDim.x * blockDim.x)
+  for (int64_t i = (int64_t)bk * blockDim.x + threadIdx.x; i < sl.n; i += (int64_t)slot_blocks(sl.n) * blockDim.x)
     x[i] -= (double)d32[i];
 }
 
@@ -553,12 +595,14 @@ k_ir_revert(const SolveSlot * __restrict__ slots)
 // replacement (then the search direction is kept).  Partials (r32, D^-1 r32),
 // (r32, r32).
 __global__ void __launch_bounds__(RED_THREADS)
-k_pcg_init_rel(const SolveSlot * __restrict__ slots)
+k_pcg_init_rel(SlotTable t)
 {
-  const SolveSlot & sl = slots[blockIdx.y];
+  const SlotBlock sb = slot_block(t);
+  const int bk = sb.b;
+  const SolveSlot & sl = t.slots[sb.s];
   const int nb = slot_blocks(sl.n);
   const PcgState * o = sl.out;
-  if ((int)blockIdx.x >= nb || o->phase != PH_INIT || o->prec64)
+  if (o->phase != PH_INIT || o->prec64)
     return;
   const bool fresh = !(o->iters > 0 && sl.in->first);
   const double * __restrict__ r64 = sl.r64;
@@ -582,8 +626,8 @@ k_pcg_init_rel(const SolveSlot * __restrict__ slots)
   block_sum<2>(v, sh);
   if (threadIdx.x == 0)
   {
-    sl.part[blockIdx.x * 4 + 0] = v[0];
-    sl.part[blockIdx.x * 4 + 1] = v[1];
+    sl.part[bk * 4 + 0] = v[0];
+    sl.part[bk * 4 + 1] = v[1];
   }
 }
 
@@ -696,17 +740,17 @@ k_zero_int(int * v, int n)
 static inline dim3
 slot_grid(const SlotTable & t)
 {
-  return dim3(t.nb_max, t.nslots);
+  return dim3(t.nblk);
 }
 
 hipError_t
 pcg_init64(const SlotTable & t, const int * git_pair, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_pcg_init64, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
+  hipLaunchKernelGGL(k_pcg_init64, slot_grid(t), dim3(RED_THREADS), 0, st, t);
   if (t.any_aux)
   {
-    hipLaunchKernelGGL(k_aux_nodes<double>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, 1);
-    hipLaunchKernelGGL(k_aux_pset<double>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, 1);
+    hipLaunchKernelGGL(k_aux_nodes<double>, slot_grid(t), dim3(RED_THREADS), 0, st, t, 1);
+    hipLaunchKernelGGL(k_aux_pset<double>, slot_grid(t), dim3(RED_THREADS), 0, st, t, 1);
   }
   hipLaunchKernelGGL(k_pcg_init64_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, git_pair + 1);
   return hipGetLastError();
@@ -719,11 +763,11 @@ hipError_t
 pcg_iterate(const SlotTable & t, int * git_pair, hipStream_t st)
 {
   int * git = git_pair + (sizeof(T) == 4 ? 0 : 1);
-  hipLaunchKernelGGL(k_pcg_dot_pq<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, git);
-  hipLaunchKernelGGL(k_pcg_update_xr<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, (const int *)git);
+  hipLaunchKernelGGL(k_pcg_dot_pq<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t, git);
+  hipLaunchKernelGGL(k_pcg_update_xr<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t, (const int *)git);
   if (t.any_aux)
-    hipLaunchKernelGGL(k_aux_nodes<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, 0);
-  hipLaunchKernelGGL(k_pcg_update_p<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, (const int *)git);
+    hipLaunchKernelGGL(k_aux_nodes<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t, 0);
+  hipLaunchKernelGGL(k_pcg_update_p<T>, slot_grid(t), dim3(RED_THREADS), 0, st, t, (const int *)git);
   return hipGetLastError();
 }
 
@@ -740,20 +784,20 @@ phase_prep(const SlotTable & t, hipStream_t st)
 hipError_t
 residual64(const SlotTable & t, int max_outer, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_residual64, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
+  hipLaunchKernelGGL(k_residual64, slot_grid(t), dim3(RED_THREADS), 0, st, t);
   hipLaunchKernelGGL(k_residual_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, max_outer);
-  hipLaunchKernelGGL(k_ir_revert, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
+  hipLaunchKernelGGL(k_ir_revert, slot_grid(t), dim3(RED_THREADS), 0, st, t);
   return hipGetLastError();
 }
 
 hipError_t
 pcg_init_reliable(const SlotTable & t, const int * git_pair, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_pcg_init_rel, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
+  hipLaunchKernelGGL(k_pcg_init_rel, slot_grid(t), dim3(RED_THREADS), 0, st, t);
   if (t.any_aux)
   {
-    hipLaunchKernelGGL(k_aux_nodes<float>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, 2);
-    hipLaunchKernelGGL(k_aux_pset<float>, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots, 2);
+    hipLaunchKernelGGL(k_aux_nodes<float>, slot_grid(t), dim3(RED_THREADS), 0, st, t, 2);
+    hipLaunchKernelGGL(k_aux_pset<float>, slot_grid(t), dim3(RED_THREADS), 0, st, t, 2);
   }
   hipLaunchKernelGGL(k_pcg_init_rel_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, git_pair);
   return hipGetLastError();
@@ -762,7 +806,7 @@ pcg_init_reliable(const SlotTable & t, const int * git_pair, hipStream_t st)
 hipError_t
 ir_update(const SlotTable & t, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_ir_update, slot_grid(t), dim3(RED_THREADS), 0, st, t.slots);
+  hipLaunchKernelGGL(k_ir_update, slot_grid(t), dim3(RED_THREADS), 0, st, t);
   return hipGetLastError();
 }
 

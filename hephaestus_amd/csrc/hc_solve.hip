@@ -104,7 +104,8 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
   const size_t o_slots = 0;
   const size_t o_gath = align256(o_slots + sizeof(SolveSlot) * nslots_);
   const size_t o_git = align256(o_gath + sizeof(PcgState) * nslots_);
-  const size_t o_desc = align256(o_git + 2 * sizeof(int));
+  const size_t o_blk = align256(o_git + 2 * sizeof(int));
+  const size_t o_desc = align256(o_blk + sizeof(int) * (nslots_ + 1));
   const size_t total = o_desc + sizeof(ApplyDesc) * ndesc_total * K_COUNT;
   const size_t own_in = 0;
   const size_t own_out = align256(own_in + sizeof(PcgState) * owned);
@@ -130,6 +131,7 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
   SolveSlot * d_slots = (SolveSlot *)(base + o_slots);
   gathered_ = (PcgState *)(base + o_gath);
   git_ = (int *)(base + o_git);
+  int * d_blk0 = (int *)(base + o_blk);
   ApplyDesc * d_desc = (ApplyDesc *)(base + o_desc);
   int opos = 0;
   for (auto & e : inst_)
@@ -195,10 +197,16 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
       }
     }
   }
+  // flattened slot grid: the blocks of slot i are [blk0[i], blk0[i + 1])
+  std::vector<int> hblk(nslots_ + 1, 0);
+  for (int i = 0; i < nslots_; i++)
+    hblk[i + 1] = hblk[i] + red_blocks(hs[i].n);
   table_.slots = d_slots;
   table_.nslots = nslots_;
   table_.nb_max = nb_max;
   table_.any_aux = aux_ ? 1 : 0;
+  table_.nblk = hblk[nslots_];
+  table_.blk0 = d_blk0;
   // new slots start in the mode's precision
   if (ir)
     any32_ = true;
@@ -228,6 +236,7 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
       G.batched = force_batched || ids.size() > 1 || inst_[ids[0]].nrhs > 1;
       G.descs = d_desc + dpos;
       G.ndesc = (int)ids.size();
+      G.grouped = 0;
       int w0 = 0, pw0 = 0;
       for (int i : ids)
       {
@@ -270,6 +279,8 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
         d.pw0 = pw0;
         w0 += d.ntiles * ((d.nrhs + d.rb - 1) / d.rb);
         pw0 += d.prows * d.nrhs;
+        if (d.rb > 1)
+          G.grouped = 1;
         if (!G.batched)
           G.single = d;
         hd[dpos++] = d;
@@ -280,6 +291,7 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
     }
   }
   HC_TRY_HIP(hipMemcpy(d_slots, hs.data(), sizeof(SolveSlot) * nslots_, hipMemcpyHostToDevice));
+  HC_TRY_HIP(hipMemcpy(d_blk0, hblk.data(), sizeof(int) * (nslots_ + 1), hipMemcpyHostToDevice));
   HC_TRY_HIP(hipMemcpy(d_desc, hd.data(), sizeof(ApplyDesc) * hd.size(), hipMemcpyHostToDevice));
   HC_TRY_HIP(hipMemcpy(git_, git_host, sizeof(git_host), hipMemcpyHostToDevice));
   if (host_cap_ < nslots_)
@@ -303,12 +315,12 @@ Engine::apply(Kind k, hipStream_t st)
     hipError_t e;
     if (G.batched)
       e = G.prec == 64 ? apply_multi<double>(G.cand, *(const Tables<double> *)G.tables, G.descs, G.ndesc, G.nwork,
-                                             G.npwork, st)
+                                             G.npwork, G.grouped, st)
                        : apply_multi<float>(G.cand, *(const Tables<float> *)G.tables, G.descs, G.ndesc, G.nwork,
-                                            G.npwork, st);
+                                            G.npwork, G.grouped, st);
     else
     {
-      e = G.cand->launch(G.single, nullptr, 1, G.nwork, G.tables, G.prec, st);
+      e = G.cand->launch(G.single, nullptr, 1, G.nwork, 0, G.tables, G.prec, st);
       if (e == hipSuccess)
         e = G.prec == 64 ? assemble_planes<double>(G.single, nullptr, 1, G.npwork, st)
                          : assemble_planes<float>(G.single, nullptr, 1, G.npwork, st);
