@@ -133,6 +133,42 @@ aux_diag_inv(const Geom & g, const AuxGeom & a, const Tables<double> & tb, const
   return hipGetLastError();
 }
 
+// Per-block minimum over elements of trace(mass factors) / trace(curl
+// factors) at the first quadrature point: sigma h^2 / nu of the element on an
+// affine hex (the ratio of the two terms on a field of unit reference size).
+__global__ void __launch_bounds__(256)
+k_term_ratio(const double * __restrict__ qd, int64_t E, int Q, double * __restrict__ out)
+{
+  __shared__ double sh[256];
+  double v = 1.0e300;
+  const int QQ = Q * Q;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < E; e += (int64_t)gridDim.x * 256)
+  {
+    const double * f = qd + e * (12 * (int64_t)Q * QQ);
+    const double m = f[0] + f[3 * QQ] + f[5 * QQ];
+    const double c = f[6 * QQ] + f[9 * QQ] + f[11 * QQ];
+    const double r = c > 0.0 ? m / c : 1.0e300;
+    v = r < v ? r : v;
+  }
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s2 = 128; s2 > 0; s2 >>= 1)
+  {
+    if ((int)threadIdx.x < s2 && sh[threadIdx.x + s2] < sh[threadIdx.x])
+      sh[threadIdx.x] = sh[threadIdx.x + s2];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    out[blockIdx.x] = sh[0];
+}
+
+hipError_t
+term_ratio_blocks(const double * qd, int64_t E, int p, double * out, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_term_ratio, dim3(TERM_RATIO_BLOCKS), dim3(256), 0, st, qd, E, p + 1, out);
+  return hipGetLastError();
+}
+
 // Standalone z = B r: t = Dg^-1 G^T r at the nodes, then z = D^-1 r + G t.
 __global__ void
 k_aux_nodes_single(AuxGeom a, const double * __restrict__ ginv, const double * __restrict__ r,

@@ -169,6 +169,16 @@ AuxGeom make_aux_geom(const Geom & g, const double * dg1d);
 // (NN doubles) may be ginv64 itself.
 hipError_t aux_diag_inv(const Geom & g, const AuxGeom & a, const Tables<double> & tb, const double * qd,
                         double * dg_scratch, double * ginv64, float * ginv32, hipStream_t st);
+// Per-block minima (TERM_RATIO_BLOCKS doubles) over the elements of
+// trace(mass factors) / trace(curl factors), i.e. sigma h^2 / nu per element.
+constexpr int TERM_RATIO_BLOCKS = 256;
+hipError_t term_ratio_blocks(const double * qd, int64_t E, int p, double * out, hipStream_t st);
+// FP32 inner solves of FP32 + FP64 refinement are used for an operator only
+// when min_e sigma h_eff^2 / nu (h_eff = h / p) is at least this: FP32
+// rounding of a vector puts curl-curl noise of relative size eps32 = 6e-8 on
+// fields whose true operator action (the gradients, eigenvalues ~ sigma) is
+// sigma h_eff^2 / nu times smaller; the bound keeps that noise below 1 %.
+constexpr double FP32_MIN_TERM_RATIO = 6.0e-6;
 // z = D^-1 r + G Dg^-1 G^T r for one vector (t: node scratch, NN entries).
 hipError_t aux_precond_apply(const AuxGeom & a, const double * dinv, const double * ginv, const double * r, double * t,
                              double * z, hipStream_t st);

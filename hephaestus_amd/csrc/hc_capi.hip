@@ -324,7 +324,8 @@ hc_op_create(const hc_mesh_desc * mesh, int order, const double * nu_e, const do
   const int64_t nq = op->E * 12 * Q * Q * Q;
   int rc;
   if ((rc = op->alloc(op->d_nu, op->E)) || (rc = op->alloc(op->d_sig, op->E)) || (rc = op->alloc(op->qd64, nq)) ||
-      (rc = op->alloc(op->diag, op->N)) || (rc = op->alloc(op->dinv64, op->N)) || (rc = op->alloc(op->desc_dev, 2)))
+      (rc = op->alloc(op->diag, op->N)) || (rc = op->alloc(op->dinv64, op->N)) || (rc = op->alloc(op->desc_dev, 2)) ||
+      (rc = op->alloc(op->d_ratio, hc::TERM_RATIO_BLOCKS)))
   {
     delete op;
     return rc;
@@ -402,7 +403,17 @@ hc_op_setup(hc_op_t op, double dt, void * stream)
   HC_TRY_HIP(hc::invert_diag(op->diag, op->dinv64, op->dinv32, op->N, st));
   // node diagonal of the gradient-space correction (computed in place)
   HC_TRY_HIP(hc::aux_diag_inv(op->g, op->ag, op->tb64, op->qd64, op->ginv64, op->ginv64, op->ginv32, st));
+  // sigma h^2 / nu per element (block minima)
+  HC_TRY_HIP(hc::term_ratio_blocks(op->qd64, op->E, op->p, op->d_ratio, st));
+  double rmin[hc::TERM_RATIO_BLOCKS];
+  HC_TRY_HIP(hipMemcpyAsync(rmin, op->d_ratio, sizeof(rmin), hipMemcpyDeviceToHost, st));
   HC_TRY_HIP(hipStreamSynchronize(st));
+  {
+    double m = 1.0e300;
+    for (int i = 0; i < hc::TERM_RATIO_BLOCKS; i++)
+      m = std::min(m, rmin[i]);
+    op->term_ratio = m / ((double)op->p * op->p);
+  }
   op->setup_done = true;
   op->coef_dirty = false;
   op->last_dt = dt;
@@ -478,6 +489,17 @@ hc_op_set_rhs_block(hc_op_t op, int rhs_block)
   if (!op || rhs_block < 0)
     return fail(HC_EINVAL, "hc_op_set_rhs_block: bad argument");
   op->rhs_block = rhs_block;
+  return HC_OK;
+}
+
+int
+hc_op_term_ratio(hc_op_t op, double * ratio)
+{
+  if (!op || !ratio)
+    return fail(HC_EINVAL, "hc_op_term_ratio: null argument");
+  if (!op->setup_done)
+    return fail(HC_EINVAL, "hc_op_term_ratio: hc_op_setup has not been called");
+  *ratio = op->term_ratio;
   return HC_OK;
 }
 

@@ -265,7 +265,7 @@ struct SolveSlot
   // gradient-space correction of the preconditioner (aux != 0):
   // z = D^-1 r + G Dg^-1 G^T r (hc_aux.hpp); t = Dg^-1 G^T r at the nodes
   int aux;
-  int pad2;
+  int g64; // the FP64 apply group serves this slot: an FP64 inner start waits for it
   const double * ginv64;
   const float * ginv32;
   double * t64;

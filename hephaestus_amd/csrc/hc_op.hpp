@@ -179,6 +179,12 @@ struct hc_op_s
   hc::AuxGeom ag{};
   double * ginv64 = nullptr;
   float * ginv32 = nullptr;
+  // min over elements of sigma h_eff^2 / nu (set at setup); FP32 + FP64
+  // refinement runs its inner solves in FP32 only when this is at least
+  // FP32_MIN_TERM_RATIO, in FP64 otherwise
+  double term_ratio = 0.0;
+  double * d_ratio = nullptr; // TERM_RATIO_BLOCKS block minima
+  bool fp32_inner_ok() const { return term_ratio >= hc::FP32_MIN_TERM_RATIO; }
   hc::ApplyDesc * desc_dev = nullptr; // staging of the batched apply descriptor
   int rhs_block = 0;                    // right-hand sides per work item (0: automatic)
   std::vector<void *> allocs;
@@ -224,6 +230,10 @@ struct EngineInst
   PcgState * in = nullptr;
   PcgState * out = nullptr;
   double * part = nullptr; // nrhs * 4 * RED_BLOCKS
+  // FP32 + refinement: the instance is in the FP64 iteration group (always
+  // when its operator does not admit FP32 inner solves; otherwise once one of
+  // its slots continues in FP64 -- Engine::promote64)
+  bool fp64 = false;
 };
 
 struct SlotStats
@@ -236,8 +246,9 @@ struct SlotStats
   int fp64;       // finished in FP64
 };
 
-// Initial values of the states of a slot that starts a solve in `mode`.
-void initial_states(PcgState & in, PcgState & out, int mode);
+// Initial values of the states of a slot that starts a solve in `mode`
+// (fp32_ok: the operator admits FP32 inner solves, hc_op_s::fp32_inner_ok).
+void initial_states(PcgState & in, PcgState & out, int mode, bool fp32_ok);
 
 class Engine
 {
@@ -263,6 +274,11 @@ public:
   const PcgState * host_states() const { return host_; }
   // Zero the iteration counter (before the first round of a fresh batch).
   int reset_counter(hipStream_t st);
+  // After gather(): instances with a slot that continues in FP64 and that are
+  // not yet in the FP64 iteration group join it; true if the tables must be
+  // rebuilt (set() with instances()) before the next round.
+  bool promote64();
+  const std::vector<EngineInst> & instances() const { return inst_; }
   // Batch solve from fresh states to completion; per-slot statistics.
   int solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double * ms);
   int nslots() const { return nslots_; }

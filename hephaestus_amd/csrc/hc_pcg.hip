@@ -161,7 +161,7 @@ k_pcg_init64(SlotTable t)
   const int bk = sb.b;
   const SolveSlot & sl = t.slots[sb.s];
   const int nb = slot_blocks(sl.n);
-  if (sl.out->phase != PH_INIT || !sl.out->prec64)
+  if (sl.out->phase != PH_INIT || !sl.out->prec64 || !sl.g64)
     return;
   const double * __restrict__ b = sl.b;
   const double * __restrict__ q = sl.q64;
@@ -199,7 +199,7 @@ k_pcg_init64_fin(const SolveSlot * __restrict__ slots, const int * __restrict__ 
   const SolveSlot & sl = slots[blockIdx.x];
   PcgState * s = sl.in;
   PcgState * o = sl.out;
-  if (o->phase != PH_INIT || !o->prec64)
+  if (o->phase != PH_INIT || !o->prec64 || !sl.g64)
     return;
   __shared__ double sh[4 * RED_THREADS];
   double v[4];
@@ -366,7 +366,7 @@ k_aux_nodes(SlotTable t, int which)
     return;
   if (which == 0 && sl.in->run != SlotVecs<T>::RUN)
     return;
-  if (which == 1 && (sl.out->phase != PH_INIT || !sl.out->prec64))
+  if (which == 1 && (sl.out->phase != PH_INIT || !sl.out->prec64 || !sl.g64))
     return;
   if (which == 2 && (sl.out->phase != PH_INIT || sl.out->prec64))
     return;
@@ -404,7 +404,7 @@ k_aux_pset(SlotTable t, int which)
   const PcgState * o = sl.out;
   if (!sl.aux || o->phase != PH_INIT)
     return;
-  if ((which == 1) != (o->prec64 != 0))
+  if ((which == 1) != (o->prec64 != 0) || (which == 1 && !sl.g64))
     return;
   if (which == 2 && o->iters > 0 && sl.in->first)
     return;

@@ -106,6 +106,7 @@ struct Request
   double *f = nullptr, *k = nullptr;
   hc::PcgState *in = nullptr, *out = nullptr;
   double * part = nullptr;
+  bool fp64 = false; // in the engine's FP64 iteration group (hc::EngineInst::fp64)
 };
 
 size_t
@@ -144,7 +145,32 @@ struct hc_server_s
   void loop();
   int admit(std::vector<Request *> & fresh);
   int start_request(Request * r);
+  int set_engine();
 };
+
+// Engine tables over the active requests (one slot each, in order).
+int
+hc_server_s::set_engine()
+{
+  std::vector<hc::EngineInst> inst;
+  for (Request * r : active)
+  {
+    hc::EngineInst e{};
+    e.op = r->op;
+    e.nrhs = 1;
+    e.b = r->f;
+    e.x = r->k;
+    e.rtol = cfg.rtol;
+    e.maxit = cfg.maxit;
+    e.ws = &r->ws;
+    e.in = r->in;
+    e.out = r->out;
+    e.part = r->part;
+    e.fp64 = r->fp64;
+    inst.push_back(e);
+  }
+  return engine.set(inst, cfg.mode, 0, true, cands64, cands32);
+}
 
 int
 hc_server_s::start_request(Request * r)
@@ -173,7 +199,7 @@ hc_server_s::start_request(Request * r)
   HC_TRY_HIP(hc::zero_ess(op->g, r->f, st));
   HC_TRY_HIP(hc::axpby(N, 0.0, r->k, 0.0, r->k, st));
   hc::PcgState hin, hout;
-  hc::initial_states(hin, hout, cfg.mode);
+  hc::initial_states(hin, hout, cfg.mode, op->fp32_inner_ok());
   HC_TRY_HIP(hipMemcpyAsync(r->in, &hin, sizeof(hin), hipMemcpyHostToDevice, st));
   HC_TRY_HIP(hipMemcpyAsync(r->out, &hout, sizeof(hout), hipMemcpyHostToDevice, st));
   HC_TRY_HIP(hipStreamSynchronize(st)); // the host state copies above are from the stack
@@ -256,29 +282,10 @@ hc_server_s::loop()
     }
     if (active.empty())
       continue;
-    if (changed)
+    if (changed && (rc = set_engine()))
     {
-      std::vector<hc::EngineInst> inst;
-      for (Request * r : active)
-      {
-        hc::EngineInst e{};
-        e.op = r->op;
-        e.nrhs = 1;
-        e.b = r->f;
-        e.x = r->k;
-        e.rtol = cfg.rtol;
-        e.maxit = cfg.maxit;
-        e.ws = &r->ws;
-        e.in = r->in;
-        e.out = r->out;
-        e.part = r->part;
-        inst.push_back(e);
-      }
-      if ((rc = engine.set(inst, cfg.mode, 0, true, cands64, cands32)))
-      {
-        set_error(rc);
-        return;
-      }
+      set_error(rc);
+      return;
     }
     if ((rc = engine.round(cfg.round_iters, st)) || (rc = engine.gather(st)))
     {
@@ -289,10 +296,27 @@ hc_server_s::loop()
     // retire finished slots
     const hc::PcgState * hs = engine.host_states();
     std::vector<Request *> keep, gone;
+    bool grow64 = false; // a request whose slot continues in FP64 joins the FP64 group
     for (size_t i = 0; i < active.size(); i++)
-      (hs[i].phase == hc::PH_DONE ? gone : keep).push_back(active[i]);
+    {
+      Request * r = active[i];
+      if (hs[i].prec64 && hs[i].phase != hc::PH_DONE && !r->fp64 && r->op->fp32_inner_ok() &&
+          hc::prec_mode(cfg.mode) == HC_SOLVE_FP32_IR)
+      {
+        r->fp64 = true;
+        grow64 = true;
+      }
+      (hs[i].phase == hc::PH_DONE ? gone : keep).push_back(r);
+    }
     if (gone.empty())
+    {
+      if (grow64 && (rc = set_engine()))
+      {
+        set_error(rc);
+        return;
+      }
       continue;
+    }
     std::vector<hc_request_result> res;
     for (size_t i = 0; i < active.size(); i++)
     {
@@ -339,29 +363,10 @@ hc_server_s::loop()
       active.swap(keep);
       cv_done.notify_all();
     }
-    if (!active.empty())
+    if (!active.empty() && (rc = set_engine()))
     {
-      std::vector<hc::EngineInst> inst;
-      for (Request * r : active)
-      {
-        hc::EngineInst e{};
-        e.op = r->op;
-        e.nrhs = 1;
-        e.b = r->f;
-        e.x = r->k;
-        e.rtol = cfg.rtol;
-        e.maxit = cfg.maxit;
-        e.ws = &r->ws;
-        e.in = r->in;
-        e.out = r->out;
-        e.part = r->part;
-        inst.push_back(e);
-      }
-      if ((rc = engine.set(inst, cfg.mode, 0, true, cands64, cands32)))
-      {
-        set_error(rc);
-        return;
-      }
+      set_error(rc);
+      return;
     }
   }
 }

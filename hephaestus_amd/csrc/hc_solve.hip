@@ -40,13 +40,13 @@ geom_for(const hc_op_s * op, const TileCandidate * c, Geom & g, PlaneSegs & segs
 } // namespace
 
 void
-initial_states(PcgState & in, PcgState & out, int mode)
+initial_states(PcgState & in, PcgState & out, int mode, bool fp32_ok)
 {
   std::memset(&in, 0, sizeof(in));
   std::memset(&out, 0, sizeof(out));
   in.done = 1; // no inner solve running
   out.phase = PH_FIRST;
-  out.prec64 = (prec_mode(mode) == HC_SOLVE_FP32_IR) ? 0 : 1;
+  out.prec64 = (prec_mode(mode) == HC_SOLVE_FP32_IR && fp32_ok) ? 0 : 1;
 }
 
 Engine::~Engine()
@@ -183,6 +183,7 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
       s.in = e.in + r;
       s.out = e.out + r;
       s.part = e.part + (size_t)r * 4 * RED_BLOCKS;
+      s.g64 = (!ir || !op->fp32_inner_ok() || e.fp64) ? 1 : 0;
       if (aux_)
       {
         s.aux = 1;
@@ -207,11 +208,15 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
   table_.any_aux = aux_ ? 1 : 0;
   table_.nblk = hblk[nslots_];
   table_.blk0 = d_blk0;
-  // new slots start in the mode's precision
-  if (ir)
-    any32_ = true;
-  else
-    any64_ = true;
+  // new slots start in the mode's precision (FP64 where the operator does not
+  // admit FP32 inner solves)
+  for (auto & e : inst_)
+  {
+    if (ir && e.op->fp32_inner_ok())
+      any32_ = true;
+    else
+      any64_ = true;
+  }
   // apply groups
   std::vector<ApplyDesc> hd(ndesc_total * K_COUNT);
   size_t dpos = 0;
@@ -223,7 +228,18 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
     const bool prec32 = (k == K_PQ32);
     for (int p = 1; p <= PMAX; p++)
     {
-      const auto & ids = by_order[p];
+      // the FP32 group holds the instances whose slots may run FP32 inner
+      // solves (work items of the others would only be gated off)
+      std::vector<int> ids;
+      for (int i : by_order[p])
+      {
+        const bool may32 = ir && inst_[i].op->fp32_inner_ok();
+        // FP32 group: instances that may run FP32; FP64 iteration group in
+        // FP32 + refinement: the others and those promoted after a slot
+        // continued in FP64 (the residual group K_XQ holds every instance)
+        if (prec32 ? may32 : (k != K_PQ64 || !may32 || inst_[i].fp64))
+          ids.push_back(i);
+      }
       if (ids.empty())
         continue;
       const hc_op_s * op0 = inst_[ids[0]].op;
@@ -386,6 +402,30 @@ Engine::gather(hipStream_t st)
   return HC_OK;
 }
 
+bool
+Engine::promote64()
+{
+  if (mode_ != HC_SOLVE_FP32_IR)
+    return false;
+  bool grow = false;
+  for (auto & e : inst_)
+  {
+    if (e.fp64)
+      continue;
+    for (int r = 0; r < e.nrhs; r++)
+    {
+      const PcgState & h = host_[e.slot0 + r];
+      if (h.prec64 && h.phase != PH_DONE)
+      {
+        e.fp64 = true;
+        grow = grow || e.op->fp32_inner_ok(); // the others are in the group already
+        break;
+      }
+    }
+  }
+  return grow;
+}
+
 SlotStats
 Engine::stats_of(const PcgState & s)
 {
@@ -410,9 +450,14 @@ Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double *
   // fresh states for every slot
   std::vector<PcgState> hin(nslots_), hout(nslots_);
   for (int i = 0; i < nslots_; i++)
-    initial_states(hin[i], hout[i], mode);
-  any32_ = (mode_ == HC_SOLVE_FP32_IR);
-  any64_ = !any32_;
+    initial_states(hin[i], hout[i], mode, true);
+  any32_ = any64_ = false;
+  for (auto & e : inst_)
+    for (int r = 0; r < e.nrhs; r++)
+    {
+      initial_states(hin[e.slot0 + r], hout[e.slot0 + r], mode, e.op->fp32_inner_ok());
+      (hout[e.slot0 + r].prec64 ? any64_ : any32_) = true;
+    }
   for (auto & e : inst_)
   {
     HC_TRY_HIP(hipMemcpyAsync(e.in, hin.data() + e.slot0, sizeof(PcgState) * e.nrhs, hipMemcpyHostToDevice, st));
@@ -442,6 +487,7 @@ Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double *
       return rc;
     if ((rc = gather(st)))
       return rc;
+    const bool grow64 = promote64();
     std::vector<EngineInst> keep;
     std::vector<int> keep_orig;
     for (size_t j = 0; j < inst_.size(); j++)
@@ -475,6 +521,18 @@ Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double *
       if ((rc = set(keep, mode_ | (aux_ ? HC_SOLVE_AUX : 0), rhs_block_, force_batched_, c64, c32)))
         return rc;
       orig.swap(keep_orig);
+    }
+    else if (grow64)
+    {
+      const std::vector<EngineInst> cur = inst_;
+      const TileCandidate * c64[PMAX + 1], *c32[PMAX + 1];
+      for (int p = 0; p <= PMAX; p++)
+      {
+        c64[p] = cands64_[p];
+        c32[p] = cands32_[p];
+      }
+      if ((rc = set(cur, mode_ | (aux_ ? HC_SOLVE_AUX : 0), rhs_block_, force_batched_, c64, c32)))
+        return rc;
     }
   }
   HC_TRY_HIP(hipEventRecord(e1, st));

@@ -34,7 +34,8 @@ SOLVE_AUX = 0x10  # flag: Jacobi + discrete-gradient auxiliary-space correction
 # Entry points declared in include/hc_hcurl.h (checked by the CPU test-suite).
 EXPORTS = [
     "hc_op_create", "hc_op_set_coefficients", "hc_op_setup", "hc_op_ndofs", "hc_op_ess_count",
-    "hc_op_device_bytes", "hc_op_apply", "hc_op_apply_f32", "hc_op_diagonal", "hc_op_precond_apply", "hc_pcg_solve",
+    "hc_op_device_bytes", "hc_op_apply", "hc_op_apply_f32", "hc_op_diagonal", "hc_op_precond_apply",
+    "hc_op_term_ratio", "hc_pcg_solve",
     "hc_op_destroy", "hc_last_error", "hc_fill_uniform", "hc_op_set_tile", "hc_op_get_tile",
     "hc_op_autotune", "hc_tile_list", "hc_op_set_rhs_block", "hc_batch_create", "hc_batch_add", "hc_batch_set_rhs_block",
     "hc_batch_solve", "hc_batch_destroy", "hc_op_rhs_coil", "hc_op_profile_apply", "hc_op_apply_curl", "hc_axpby", "hc_op_zero_ess",
@@ -115,6 +116,7 @@ def load():
     lib.hc_op_apply_f32.argtypes = [vp, vp, vp, C.c_int, vp]
     lib.hc_op_diagonal.argtypes = [vp, vp, vp]
     lib.hc_op_precond_apply.argtypes = [vp, C.c_int, vp, vp, vp]
+    lib.hc_op_term_ratio.argtypes = [vp, C.POINTER(C.c_double)]
     lib.hc_pcg_solve.argtypes = [vp, vp, vp, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(SolveStats), vp]
     lib.hc_op_destroy.argtypes = [vp]
     lib.hc_last_error.restype = C.c_char_p
@@ -264,6 +266,13 @@ class Operator:
         d = torch.empty(self.ndofs, dtype=torch.float64, device="cuda")
         _check(_lib.hc_op_diagonal(self._h, _ptr(d), _stream(stream)), "hc_op_diagonal")
         return d
+
+    @property
+    def term_ratio(self):
+        """min over elements of sigma h^2 / (nu p^2) (from the quadrature factors)."""
+        v = C.c_double()
+        _check(_lib.hc_op_term_ratio(self._h, C.byref(v)), "hc_op_term_ratio")
+        return v.value
 
     def precond_apply(self, r, mode=SOLVE_FP64 | SOLVE_AUX, z=None, stream=None):
         """z = B r with the preconditioner of a solve mode (Jacobi, or with

@@ -80,7 +80,10 @@ extern "C" {
 
 /* solve modes of hc_pcg_solve */
 #define HC_SOLVE_FP64 0    /* plain FP64 Jacobi-PCG */
-#define HC_SOLVE_FP32_IR 1 /* FP32 Jacobi-PCG inner solves + FP64 refinement */
+#define HC_SOLVE_FP32_IR 1 /* FP32 Jacobi-PCG inner solves + FP64 refinement; an
+                              operator whose hc_op_term_ratio is below 6e-6 runs its
+                              inner solves in FP64 (FP32 rounding noise would swamp
+                              the sigma term there) and counts in fp64_fallbacks */
 /* flag, or-ed into either mode: precondition with Jacobi plus the
  * discrete-gradient auxiliary-space correction B = D^-1 + G Dg^-1 G^T
  * (Dg = diag(G^T A G) on the H1 nodes; the one-level form of the HypreAMS
@@ -143,6 +146,13 @@ int64_t hc_op_device_bytes(hc_op_t op);
  * correction; one-level form of HypreAMS, hcurl_formulation.cpp:69-79).
  * r, z: N doubles on the device.  Needs hc_op_setup. */
 int hc_op_precond_apply(hc_op_t op, int mode, const double * r, double * z, void * stream);
+
+/* min over elements of (trace of the sigma-mass factors) / (trace of the
+ * curl-curl factors) at a quadrature point, divided by p^2: sigma h^2 / (nu p^2)
+ * on an affine mesh, the relative size of the operator on gradient fields.
+ * No counterpart in the reference (it selects the precision of FP32 + FP64
+ * refinement).  Needs hc_op_setup. */
+int hc_op_term_ratio(hc_op_t op, double * ratio);
 
 /* y = A x (PEC eliminated).  x, y: device, nrhs vectors (rhs r at r*N).
  * With nrhs > 1 a work item of the kernel applies the operator to a group of

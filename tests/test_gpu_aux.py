@@ -172,3 +172,21 @@ def test_aux_pcg_vs_direct(torch_cuda, hc, O, p, n):
         its[name] = st["iterations"]
     print(f"iterations {its}")
     assert its["aux"] < its["jacobi"]
+
+
+def test_term_ratio_and_fp32_rule(torch_cuda, hc):
+    """hc_op_term_ratio is sigma h^2 / (nu p^2) on the uniform cube (closed form
+    of the quadrature factors); FP32 + FP64 refinement runs the inner solves of
+    an operator below the admission ratio in FP64 (reported as an FP64
+    fallback) and of one above it in FP32."""
+    torch = torch_cuda
+    n, p = 6, 2
+    for sig, nu in ((2.0, 0.5), (1e-6, 1.0)):
+        op = hc.Operator(n, p, nu, sig).setup(1.0)
+        ref = sig * (1.0 / n) ** 2 / nu / p ** 2
+        assert abs(op.term_ratio - ref) <= 1e-12 * ref
+        b = op.rhs_coil(3)
+        x = torch.zeros_like(b)
+        st = op.solve(b, x, rtol=1e-8, maxit=20000, mode=hc.SOLVE_FP32_IR | hc.SOLVE_AUX)
+        assert st["converged"]
+        assert st["fp64_fallbacks"] == (1 if ref < 6e-6 else 0), (sig, nu, st)

@@ -220,3 +220,47 @@ def test_apply_curl_vs_oracle(torch_cuda, hc, O, p, dt):
     # the full operator is unchanged by the curl-only apply
     qd_full = O.pa_setup(m, p, dt * nu, sig)
     assert _rel(op.apply(x).cpu().numpy(), O.pa_apply(m, p, qd_full, ess, x.cpu().numpy())) <= 1e-13
+
+
+def test_batch_fp64_group_promotion(torch_cuda, hc, O):
+    """FP32 + refinement batch mixing an operator below the FP32 admission ratio
+    (FP64 inner solves from the start), one above it whose FP32 inner solve
+    stalls and continues in FP64 (it joins the FP64 iteration group mid-solve)
+    and one that stays FP32: every true residual, recomputed by the oracle,
+    meets rtol; the continuing instance reports its FP64 finish; each result is
+    bitwise the one it gets alone."""
+    torch = torch_cuda
+    specs = [((15, 15, 15), 2, 0.008323536902254423, 49.397593774185665, 1061),
+             ((12, 12, 12), 2, 1e-5, 1.0, 7),
+             ((10, 10, 10), 3, 0.11181569746498954, 0.25449960267461974, 1027)]
+    print("case fp64 group promotion", flush=True)
+    ops, bs, xs, sigs = [], [], [], []
+    for n, p, s_in, s_out, seed in specs:
+        sig = _two_region(n, s_in, s_out)
+        op = hc.Operator(n, p, 1.0, sig).setup(1.0)
+        ops.append(op)
+        bs.append(op.rhs_coil(seed))
+        xs.append(torch.zeros_like(bs[-1]))
+        sigs.append(sig)
+    assert ops[0].term_ratio >= 6e-6 and ops[1].term_ratio < 6e-6 and ops[2].term_ratio >= 6e-6
+    md = hc.SOLVE_FP32_IR | hc.SOLVE_AUX
+    bt = hc.Batch()
+    for op, b, x in zip(ops, bs, xs):
+        bt.add(op, b, x, rtol=1e-8, maxit=20000)
+    rc, stats = bt.solve(md)
+    assert rc == 0 and all(s["converged"] for s in stats)
+    assert stats[1]["fp64_fallbacks"] == 1
+    for i, (op, b, x) in enumerate(zip(ops, bs, xs)):
+        (n, p, _, _, _) = specs[i]
+        m = O.Mesh(*n)
+        qd = O.pa_setup(m, p, 1.0, sigs[i])
+        ess = O.ess_mask(m, p)
+        bh, xh = b.cpu().numpy(), x.cpu().numpy()
+        res = bh - O.pa_apply(m, p, qd, ess, xh)
+        assert np.linalg.norm(res) <= 1e-8 * np.linalg.norm(bh) * (1 + 1e-6), i
+        x1 = torch.zeros_like(b)
+        b1 = hc.Batch()
+        b1.add(op, b, x1, rtol=1e-8, maxit=20000)
+        rc1, st1 = b1.solve(md)
+        assert rc1 == 0 and torch.equal(x1, x), i
+    print("fallbacks", [s["fp64_fallbacks"] for s in stats], "iterations", [s["iterations"] for s in stats])
