@@ -218,7 +218,7 @@ struct PcgState
   int run;       // inner: 0 idle, 1 FP32 iterations running, 2 FP64 running
   int stall;     // outer: FP32 replacements in a row without progress
   int revert;    // outer: 1 while the last FP32 increment is being undone
-  int pad;
+  int outer64;   // outer: finished FP64 inner solves
 };
 constexpr int STATE_STRIDE_INT = sizeof(PcgState) / sizeof(int);
 
@@ -289,8 +289,9 @@ hipError_t phase_prep(const SlotTable & t, hipStream_t st);
 hipError_t ir_update(const SlotTable & t, hipStream_t st);
 // r64 = b - q64 for slots in PH_RESID; outer state update and the move to
 // PH_DONE or PH_INIT (or back to PH_RESID after undoing an FP32 increment
-// that increased the residual).
-hipError_t residual64(const SlotTable & t, int max_outer, hipStream_t st);
+// that increased the residual).  A slot stops after max_outer outer steps, or
+// after max_outer64 finished FP64 inner solves.
+hipError_t residual64(const SlotTable & t, int max_outer, int max_outer64, hipStream_t st);
 // The iteration counters git_pair[0] (FP32 iterations) and git_pair[1] (FP64)
 // give each slot its iteration index: counter - value at the slot's start.
 // FP32 PCG (re)start for FP32 slots in PH_INIT: r32 = (float) r64, x32 = 0,

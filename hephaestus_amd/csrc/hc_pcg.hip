@@ -509,7 +509,7 @@ k_residual64(SlotTable t)
 // rr <= rtol^2 bb, on a NaN, after max_outer steps or when the iteration
 // budget is spent; PH_INIT otherwise.
 __global__ void __launch_bounds__(RED_THREADS)
-k_residual_fin(const SolveSlot * __restrict__ slots, int max_outer)
+k_residual_fin(const SolveSlot * __restrict__ slots, int max_outer, int max_outer64)
 {
   const SolveSlot & sl = slots[blockIdx.x];
   PcgState * s = sl.out;
@@ -538,6 +538,8 @@ k_residual_fin(const SolveSlot * __restrict__ slots, int max_outer)
       s->inner += in->iters - in->counted;
       in->counted = in->iters;
       s->iters += 1;
+      if (s->prec64)
+        s->outer64 += 1;
       if (!s->prec64)
       {
         if (in->status || !(rr <= s->rrmax))
@@ -569,7 +571,8 @@ k_residual_fin(const SolveSlot * __restrict__ slots, int max_outer)
     s->rrmax = rr;
     if (bad(rr))
       s->status = -5;
-    const bool stop = s->status != 0 || rr <= s->tol2 * s->bb || s->iters >= max_outer || s->inner >= sl.maxit;
+    const bool stop = s->status != 0 || rr <= s->tol2 * s->bb || s->iters >= max_outer ||
+                      s->outer64 >= max_outer64 || s->inner >= sl.maxit;
     s->done = stop ? 1 : 0;
     s->phase = stop ? PH_DONE : PH_INIT;
   }
@@ -782,10 +785,10 @@ phase_prep(const SlotTable & t, hipStream_t st)
 }
 
 hipError_t
-residual64(const SlotTable & t, int max_outer, hipStream_t st)
+residual64(const SlotTable & t, int max_outer, int max_outer64, hipStream_t st)
 {
   hipLaunchKernelGGL(k_residual64, slot_grid(t), dim3(RED_THREADS), 0, st, t);
-  hipLaunchKernelGGL(k_residual_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, max_outer);
+  hipLaunchKernelGGL(k_residual_fin, dim3(t.nslots), dim3(RED_THREADS), 0, st, t.slots, max_outer, max_outer64);
   hipLaunchKernelGGL(k_ir_revert, slot_grid(t), dim3(RED_THREADS), 0, st, t);
   return hipGetLastError();
 }

@@ -364,7 +364,7 @@ Engine::round(int iters, hipStream_t st)
     HC_TRY_HIP(ir_update(table_, st));
   if ((rc = apply(K_XQ, st)))
     return rc;
-  HC_TRY_HIP(residual64(table_, max_outer(), st));
+  HC_TRY_HIP(residual64(table_, max_outer(), MAX_OUTER_FP64, st));
   if (ir)
     HC_TRY_HIP(pcg_init_reliable(table_, git_, st));
   HC_TRY_HIP(pcg_init64(table_, git_, st));

@@ -79,7 +79,8 @@ public:
 };
 
 // Jacobi-preconditioned CG on the GPU operator; "Tolerance" / "MaxIter" /
-// "Precision" ("fp64" or "fp32_ir") parameters.  Stopping rule
+// "Precision" ("fp64" or "fp32_ir") / "Preconditioner" ("Jacobi" or "AMS", the
+// latter the one-level gradient-space correction) parameters.  Stopping rule
 // ||b - A x||_2 <= tol ||b||_2 confirmed in FP64.
 class DefaultJacobiPCGSolver
 {

@@ -239,6 +239,14 @@ DefaultJacobiPCGSolver::DefaultJacobiPCGSolver(const InputParameters & params, B
     _mode(params.GetOptionalParam<std::string>("Precision", "fp64") == "fp32_ir" ? HC_SOLVE_FP32_IR
                                                                                  : HC_SOLVE_FP64)
 {
+  // "Preconditioner": "Jacobi" (HypreDiagScale, hephaestus_solvers.hpp:52) or
+  // "AMS" (the gradient-space correction, the one-level form of the HypreAMS
+  // that HCurlFormulation::ConstructJacobianPreconditioner sets)
+  const std::string pc = params.GetOptionalParam<std::string>("Preconditioner", "Jacobi");
+  if (pc == "AMS")
+    _mode |= HC_SOLVE_AUX;
+  else if (pc != "Jacobi")
+    throw Error("DefaultJacobiPCGSolver: Preconditioner must be \"Jacobi\" or \"AMS\", got \"" + pc + "\"");
 }
 
 void

@@ -116,7 +116,8 @@ write_vec(const std::string & path, const std::vector<double> & v)
 }
 
 static void
-test_gpu(const std::string & dir, int n, int order, int steps, double dt, const std::string & precision)
+test_gpu(const std::string & dir, int n, int order, int steps, double dt, const std::string & precision,
+         const std::string & preconditioner)
 {
   StructuredHexMesh mesh = StructuredHexMesh::TwoRegionCube(n);
   FESpaces fespaces;
@@ -142,6 +143,7 @@ test_gpu(const std::string & dir, int n, int order, int steps, double dt, const 
   solver.SetParam("Tolerance", 1e-12);
   solver.SetParam("MaxIter", 20000);
   solver.SetParam("Precision", precision);
+  solver.SetParam("Preconditioner", preconditioner);
   TimeDomainEquationSystemProblemOperator op(es, gfs, bcs, coefs, sources, solver);
   for (int k = 0; k < steps; k++)
   {
@@ -166,7 +168,7 @@ main(int argc, char ** argv)
 {
   if (argc < 2)
   {
-    std::fprintf(stderr, "usage: test_host cpu | gpu <dir> <n> <p> <steps> <dt> <precision>\n");
+    std::fprintf(stderr, "usage: test_host cpu | gpu <dir> <n> <p> <steps> <dt> <precision> [Jacobi|AMS]\n");
     return 2;
   }
   try
@@ -174,7 +176,8 @@ main(int argc, char ** argv)
     if (!std::strcmp(argv[1], "cpu"))
       test_cpu();
     else if (!std::strcmp(argv[1], "gpu") && argc >= 8)
-      test_gpu(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), std::atof(argv[6]), argv[7]);
+      test_gpu(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), std::atof(argv[6]), argv[7],
+               argc >= 9 ? argv[8] : "Jacobi");
     else
     {
       std::fprintf(stderr, "bad arguments\n");

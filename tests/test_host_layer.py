@@ -40,13 +40,13 @@ def _two_region(n, inner, outer):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
-def test_host_layer_time_steps(tmp_path, precision):
+@pytest.mark.parametrize("precision,pc", [("fp64", "Jacobi"), ("fp32_ir", "Jacobi"), ("fp32_ir", "AMS")])
+def test_host_layer_time_steps(tmp_path, precision, pc):
     import scipy.sparse.linalg as spla
     from oracle import oracle as O
     n, p, steps, dt = 6, 2, 3, 0.1
-    print(f"case host layer {precision} n={n} p={p} steps={steps}", flush=True)
-    r = subprocess.run([_built(), "gpu", str(tmp_path), str(n), str(p), str(steps), str(dt), precision],
+    print(f"case host layer {precision} {pc} n={n} p={p} steps={steps}", flush=True)
+    r = subprocess.run([_built(), "gpu", str(tmp_path), str(n), str(p), str(steps), str(dt), precision, pc],
                        capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
