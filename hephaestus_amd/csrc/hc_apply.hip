@@ -45,10 +45,12 @@ struct Tile
   static constexpr int SB = pad_to(3 * LB, 16, 4);
   // wave-synchronous stages: when the (p+1)^2 pencils, stage items and
   // transposed-stage items of an element always fall in one wave (64 divisible
-  // by (p+1)^2, whole waves of pencils, no split of the qz range), every LDS
-  // exchange between the forward and backward stages stays inside a wave and
-  // needs no workgroup barrier -- only the x^T rows and the write-out do.
-  static constexpr bool WS = (SP == 1) && (64 % QQ == 0) && (NP % 64 == 0);
+  // by (p+1)^2, whole waves of pencils, no split of the qz range, and the
+  // stage-x items of a row of TX elements -- D*D*TX, element fastest -- in
+  // whole fractions of a wave), every LDS exchange between the forward and
+  // backward stages stays inside a wave and needs no workgroup barrier --
+  // only the x^T rows and the write-out do.
+  static constexpr bool WS = (SP == 1) && (64 % QQ == 0) && (NP % 64 == 0) && (64 % (D * D * TX) == 0);
 };
 
 // Order the LDS traffic of one wave (all lanes of a wavefront run in lockstep,
@@ -872,12 +874,13 @@ launch_one(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, 
 static const TileCandidate g_cands[] = {
     // first entry of each order = default before tuning
     HC_CAND(1, 4, 4, 4, 1, 2), HC_CAND(1, 4, 4, 4, 1, 1), HC_CAND(1, 8, 4, 2, 1, 1), HC_CAND(1, 8, 4, 2, 0, 2),
-    HC_CAND(1, 8, 8, 1, 0, 1), HC_QP(1, 4, 4, 4), HC_QP(1, 8, 4, 2),
+    HC_CAND(1, 8, 8, 1, 0, 1), HC_CAND(1, 16, 2, 2, 1, 1), HC_QP(1, 4, 4, 4), HC_QP(1, 8, 4, 2),
     HC_CAND(2, 4, 2, 2, 1, 1), HC_CAND(2, 4, 2, 2, 1, 2), HC_CAND(2, 4, 2, 2, 0, 2), HC_CAND(2, 4, 4, 2, 1, 1),
-    HC_CAND(2, 4, 4, 2, 0, 2), HC_CAND(2, 8, 2, 2, 0, 1), HC_QP(2, 4, 2, 2), HC_QP(2, 4, 4, 2),
+    HC_CAND(2, 4, 4, 2, 0, 2), HC_CAND(2, 8, 2, 2, 0, 1), HC_CAND(2, 8, 2, 2, 1, 1), HC_CAND(2, 8, 4, 1, 1, 1),
+    HC_QP(2, 4, 2, 2), HC_QP(2, 4, 4, 2),
     HC_CAND(3, 4, 2, 2, 1, 2), HC_CAND(3, 4, 2, 2, 1, 1), HC_CAND(3, 4, 2, 2, 0, 2), HC_CAND(3, 4, 4, 1, 1, 2),
-    HC_CAND(3, 2, 2, 4, 1, 2), HC_CAND(3, 4, 2, 2, 0, 1), HC_CAND(3, 2, 2, 2, 1, 1), HC_CAND(3, 2, 2, 2, 0, 1),
-    HC_CAND(3, 4, 2, 1, 1, 1), HC_QP(3, 4, 2, 2), HC_QP(3, 2, 2, 2), HC_QP(3, 4, 2, 1),
+    HC_CAND(3, 2, 2, 4, 1, 2), HC_CAND(3, 2, 2, 2, 1, 1), HC_CAND(3, 4, 2, 1, 1, 1), HC_CAND(3, 8, 2, 1, 1, 2),
+    HC_CAND(3, 8, 1, 2, 1, 2), HC_CAND(3, 8, 2, 1, 1, 1),
 };
 #undef HC_CAND
 #undef HC_QP

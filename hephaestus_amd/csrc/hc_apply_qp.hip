@@ -557,9 +557,6 @@ HC_QP_INST(1, 4, 4, 4)
 HC_QP_INST(1, 8, 4, 2)
 HC_QP_INST(2, 4, 2, 2)
 HC_QP_INST(2, 4, 4, 2)
-HC_QP_INST(3, 4, 2, 2)
-HC_QP_INST(3, 2, 2, 2)
-HC_QP_INST(3, 4, 2, 1)
 #undef HC_QP_INST
 
 } // namespace hc

@@ -70,10 +70,12 @@ def _serve(hc, torch, ops, bs, us, mode, budget=1 << 30, max_active=0):
     return res, info
 
 
-@pytest.mark.parametrize("mode", ["fp32_ir", "fp64"])
+@pytest.mark.parametrize("mode", ["fp32_ir", "fp64", "fp32_ir_aux"])
 def test_server_mix_invariance(torch_cuda, hc, mode):
     torch = torch_cuda
-    md = hc.SOLVE_FP32_IR if mode == "fp32_ir" else hc.SOLVE_FP64
+    md = hc.SOLVE_FP32_IR if mode.startswith("fp32_ir") else hc.SOLVE_FP64
+    if mode.endswith("_aux"):
+        md |= hc.SOLVE_AUX
     print(f"case server mix {mode} ({len(MIX)} requests)", flush=True)
     ops, bs, us0 = _setup(torch, hc, MIX)
     mixed = [u.clone() for u in us0]
