@@ -268,7 +268,10 @@ public:
           const TileCandidate * const * cands64 = nullptr, const TileCandidate * const * cands32 = nullptr);
   // One round: phase moves, FP64 residuals and refinement updates, inner
   // starts, then `iters` PCG iterations of every slot in an inner solve.
-  int round(int iters, hipStream_t st);
+  // graph: replay a captured hipGraph of this round's launches (captured on
+  // first use per (iters, running precisions) and table set; st must not be
+  // the null stream).
+  int round(int iters, hipStream_t st, bool graph = false);
   // Copies the outer states of all slots to host_states() (synchronises).
   int gather(hipStream_t st);
   const PcgState * host_states() const { return host_; }
@@ -286,6 +289,8 @@ public:
   static SlotStats stats_of(const PcgState & out);
 
 private:
+  int round_launches(int iters, bool run32, bool run64, hipStream_t st);
+  void drop_graphs();
   struct ApplyGroup
   {
     int order;
@@ -324,6 +329,13 @@ private:
   int host_cap_ = 0;
   bool any32_ = false, any64_ = true; // precisions with running slots
   std::vector<ApplyGroup> groups_[K_COUNT];
+  // captured rounds of the current tables: key = 4 iters + 2 run32 + run64
+  std::vector<std::pair<int, hipGraphExec_t>> graphs_;
+  hipStream_t own_st_ = nullptr; // solves issued on the null stream run here
 };
+
+// Graph capture of the engine's steady-state rounds (process-wide switch,
+// off by default; hc_engine_set_graphs).
+bool graphs_enabled();
 
 } // namespace hc

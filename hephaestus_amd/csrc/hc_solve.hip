@@ -9,6 +9,7 @@
 #include "hc_op.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -22,6 +23,7 @@ namespace
 
 constexpr int MAX_OUTER_FP64 = 5;  // PCG runs (1 + residual-replacement restarts)
 constexpr int MAX_OUTER_IR = 400;  // FP64 residual replacements of the FP32 PCG
+constexpr int GRAPH_MIN_ITERS = 64; // rounds of this many iterations replay a captured graph
 // FP32 PCG: an FP64 residual replacement each time ||r|| fell by 10x
 constexpr double RELIABLE_DELTA2 = 1e-2;
 
@@ -51,6 +53,9 @@ initial_states(PcgState & in, PcgState & out, int mode, bool fp32_ok)
 
 Engine::~Engine()
 {
+  drop_graphs();
+  if (own_st_)
+    (void)hipStreamDestroy(own_st_);
   if (host_)
     (void)hipHostFree(host_);
 }
@@ -65,6 +70,7 @@ int
 Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool force_batched,
             const TileCandidate * const * cands64, const TileCandidate * const * cands32)
 {
+  drop_graphs(); // captured rounds hold the old tables' sizes
   inst_ = inst;
   mode_ = prec_mode(mode);
   aux_ = (mode & HC_SOLVE_AUX) != 0;
@@ -354,8 +360,63 @@ Engine::reset_counter(hipStream_t st)
   return HC_OK;
 }
 
+namespace
+{
+std::atomic<int> g_graphs{0};
+} // namespace
+
+bool
+graphs_enabled()
+{
+  return g_graphs.load() != 0;
+}
+
+void
+Engine::drop_graphs()
+{
+  for (auto & g : graphs_)
+    (void)hipGraphExecDestroy(g.second);
+  graphs_.clear();
+}
+
 int
-Engine::round(int iters, hipStream_t st)
+Engine::round(int iters, hipStream_t st, bool graph)
+{
+  const bool ir = (mode_ == HC_SOLVE_FP32_IR);
+  // the precisions that have slots running, as of the last gathered states
+  // (a slot that switched to FP64 in this round starts iterating next round)
+  const bool run32 = ir && any32_, run64 = !ir || any64_;
+  if (!graph || !st || !graphs_enabled())
+    return round_launches(iters, run32, run64, st);
+  const int key = 4 * iters + 2 * (run32 ? 1 : 0) + (run64 ? 1 : 0);
+  hipGraphExec_t ex = nullptr;
+  for (auto & g : graphs_)
+    if (g.first == key)
+      ex = g.second;
+  if (!ex)
+  {
+    HC_TRY_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = round_launches(iters, run32, run64, st);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc || e != hipSuccess)
+    {
+      if (g)
+        (void)hipGraphDestroy(g);
+      return rc ? rc : fail(HC_EHIP, std::string("round capture: ") + hipGetErrorString(e));
+    }
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess)
+      return fail(HC_EHIP, std::string("round graph instantiation: ") + hipGetErrorString(ei));
+    graphs_.push_back({key, ex});
+  }
+  HC_TRY_HIP(hipGraphLaunch(ex, st));
+  return HC_OK;
+}
+
+int
+Engine::round_launches(int iters, bool run32, bool run64, hipStream_t st)
 {
   const bool ir = (mode_ == HC_SOLVE_FP32_IR);
   int rc;
@@ -368,9 +429,6 @@ Engine::round(int iters, hipStream_t st)
   if (ir)
     HC_TRY_HIP(pcg_init_reliable(table_, git_, st));
   HC_TRY_HIP(pcg_init64(table_, git_, st));
-  // the precisions that have slots running, as of the last gathered states
-  // (a slot that switched to FP64 in this round starts iterating next round)
-  const bool run32 = ir && any32_, run64 = !ir || any64_;
   for (int k = 0; k < iters; k++)
   {
     if (run32)
@@ -440,10 +498,16 @@ Engine::stats_of(const PcgState & s)
 }
 
 int
-Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double * ms)
+Engine::solve(int mode, hipStream_t st_in, std::vector<SlotStats> & stats, double * ms)
 {
   if (prec_mode(mode) != mode_ || ((mode & HC_SOLVE_AUX) != 0) != aux_)
     return fail(HC_EINVAL, "solve: mode differs from the one the batch was set up for");
+  // work of a solve issued on the null stream runs on the engine's own
+  // (blocking, default-priority) stream, which orders with the null stream and
+  // can be captured into graphs
+  if (!st_in && !own_st_)
+    HC_TRY_HIP(hipStreamCreate(&own_st_));
+  hipStream_t st = st_in ? st_in : own_st_;
   int maxit_all = 0;
   for (auto & e : inst_)
     maxit_all = std::max(maxit_all, e.maxit);
@@ -483,7 +547,7 @@ Engine::solve(int mode, hipStream_t st, std::vector<SlotStats> & stats, double *
   const int max_rounds = max_outer() * (maxit_all / 8 + 4) + 8;
   for (;;)
   {
-    if ((rc = round(chunk, st)))
+    if ((rc = round(chunk, st, chunk >= GRAPH_MIN_ITERS)))
       return rc;
     if ((rc = gather(st)))
       return rc;
@@ -651,6 +715,13 @@ struct hc_batch_s
 };
 
 extern "C" {
+
+int
+hc_engine_set_graphs(int enable)
+{
+  hc::g_graphs.store(enable ? 1 : 0);
+  return HC_OK;
+}
 
 int
 hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, int maxit, int mode,

@@ -35,7 +35,7 @@ SOLVE_AUX = 0x10  # flag: Jacobi + discrete-gradient auxiliary-space correction
 EXPORTS = [
     "hc_op_create", "hc_op_set_coefficients", "hc_op_setup", "hc_op_ndofs", "hc_op_ess_count",
     "hc_op_device_bytes", "hc_op_apply", "hc_op_apply_f32", "hc_op_diagonal", "hc_op_precond_apply",
-    "hc_op_term_ratio", "hc_pcg_solve",
+    "hc_op_term_ratio", "hc_engine_set_graphs", "hc_pcg_solve",
     "hc_op_destroy", "hc_last_error", "hc_fill_uniform", "hc_op_set_tile", "hc_op_get_tile",
     "hc_op_autotune", "hc_tile_list", "hc_op_set_rhs_block", "hc_batch_create", "hc_batch_add", "hc_batch_set_rhs_block",
     "hc_batch_solve", "hc_batch_destroy", "hc_op_rhs_coil", "hc_op_profile_apply", "hc_op_apply_curl", "hc_axpby", "hc_op_zero_ess",
@@ -117,6 +117,7 @@ def load():
     lib.hc_op_diagonal.argtypes = [vp, vp, vp]
     lib.hc_op_precond_apply.argtypes = [vp, C.c_int, vp, vp, vp]
     lib.hc_op_term_ratio.argtypes = [vp, C.POINTER(C.c_double)]
+    lib.hc_engine_set_graphs.argtypes = [C.c_int]
     lib.hc_pcg_solve.argtypes = [vp, vp, vp, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(SolveStats), vp]
     lib.hc_op_destroy.argtypes = [vp]
     lib.hc_last_error.restype = C.c_char_p
@@ -175,6 +176,11 @@ def _nrhs(v, n, dtype):
     if v.dim() == 2 and v.shape[1] == n:
         return int(v.shape[0])
     raise ValueError(f"expected shape ({n},) or (nrhs, {n}), got {tuple(v.shape)}")
+
+
+def set_graphs(enable):
+    """Process-wide: replay the solve engine's steady-state rounds from captured hipGraphs."""
+    _check(load().hc_engine_set_graphs(1 if enable else 0), "hc_engine_set_graphs")
 
 
 def tile_list(order):

@@ -154,6 +154,13 @@ int hc_op_precond_apply(hc_op_t op, int mode, const double * r, double * z, void
  * refinement).  Needs hc_op_setup. */
 int hc_op_term_ratio(hc_op_t op, double * ratio);
 
+/* Process-wide switch (default off): batch and single solves replay their
+ * steady-state rounds (64 iterations) from a captured hipGraph, one per table
+ * set, instead of launching every kernel from the host.  No counterpart in the
+ * reference.  Results are identical either way; on one MI355X the replay was
+ * not faster (a round's kernels, not their launches, set its time). */
+int hc_engine_set_graphs(int enable);
+
 /* y = A x (PEC eliminated).  x, y: device, nrhs vectors (rhs r at r*N).
  * With nrhs > 1 a work item of the kernel applies the operator to a group of
  * right-hand sides, reading each tile's quadrature data once per group. */

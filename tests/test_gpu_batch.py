@@ -264,3 +264,52 @@ def test_batch_fp64_group_promotion(torch_cuda, hc, O):
         rc1, st1 = b1.solve(md)
         assert rc1 == 0 and torch.equal(x1, x), i
     print("fallbacks", [s["fp64_fallbacks"] for s in stats], "iterations", [s["iterations"] for s in stats])
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32_ir_aux"])
+def test_graph_rounds_match_eager(torch_cuda, hc, mode):
+    """Solves whose steady-state rounds replay a captured hipGraph (several
+    replays per solve, with the host's state gathers and table rebuilds between
+    them) give bitwise the results and iteration counts of eager launches, on
+    the null stream (engine stream) and on a caller stream, twice in a row."""
+    from hephaestus_amd import hcurl
+    torch = torch_cuda
+    md = hc.SOLVE_FP64 if mode == "fp64" else hc.SOLVE_FP32_IR | hc.SOLVE_AUX
+    print(f"case graph rounds {mode}", flush=True)
+    ops = _instances(hc, SPECS)
+    bs = []
+    for i, (op, _) in enumerate(ops):
+        b = torch.empty(op.ndofs, dtype=torch.float64, device="cuda")
+        op.fill_uniform(300 + i, b)
+        bs.append(b)
+
+    def run(graphs, stream):
+        hcurl.set_graphs(graphs)
+        xs = [torch.zeros_like(b) for b in bs]
+        bt = hc.Batch()
+        for (op, _), b, x in zip(ops, bs, xs):
+            bt.add(op, b, x, rtol=1e-9, maxit=20000)
+        rc, st = bt.solve(md, stream=stream)
+        torch.cuda.synchronize()
+        assert rc == 0
+        return xs, [s["iterations"] for s in st]
+
+    try:
+        x0, it0 = run(False, None)
+        assert max(it0) > 120 + 64  # long enough for several 64-iteration rounds
+        s = torch.cuda.Stream()
+        for stream in (None, s, None):
+            x1, it1 = run(True, stream)
+            assert it1 == it0
+            assert all(torch.equal(a, b) for a, b in zip(x0, x1))
+        # single-operator solve (hc_pcg_solve) on the null stream
+        op = ops[0][0]
+        ya = torch.zeros_like(bs[0])
+        hcurl.set_graphs(False)
+        sa = op.solve(bs[0], ya, rtol=1e-9, maxit=20000, mode=md)
+        yb = torch.zeros_like(bs[0])
+        hcurl.set_graphs(True)
+        sb = op.solve(bs[0], yb, rtol=1e-9, maxit=20000, mode=md)
+        assert sa["iterations"] == sb["iterations"] and torch.equal(ya, yb)
+    finally:
+        hcurl.set_graphs(False)
