@@ -6,7 +6,7 @@
 //   * (the tiled element apply k_apply_tile lives in hc_apply.hip)
 //   * k_assemble_planes  sums the (2 or 4) tile partials of every tile-face DOF
 //                     in a fixed order (bitwise-reproducible results);
-//   * k_diagonal      Jacobi diagonal, DOF-parallel gather (no atomics).
+//   * k_diag_elem     Jacobi diagonal, element-parallel in 8 parity colours (no atomics).
 // The apply is the operator EquationSystem::Mult (equation_system.cpp:200-204)
 // on the PEC-eliminated system of FormLinearSystem (equation_system.cpp:132-190):
 // y_free = (A x_free)_free, y_ess = x_ess.
@@ -209,109 +209,93 @@ k_assemble_planes(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc)
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal (DOF-parallel gather over the <= 4 elements of each DOF)
+// Diagonal: element-parallel in 8 parity colours (elements of one colour share
+// no DOF, so each adds its element diagonal without conflicts; the colours run
+// in a fixed order).  One workgroup per element with its quadrature factors and
+// the 1-D tables in LDS, one thread per local DOF (component-major, x fastest):
+//   d_i += sum_q  v^2 M_cc + c1^2 K_11 + c2^2 K_22 + 2 c1 c2 K_12
+// with v the DOF's basis function (reference component c) and (c1, c2) the
+// two nonzero components of its curl at the point.  PEC DOFs are 1.
 // ---------------------------------------------------------------------------
-__global__ void
-k_diagonal(Geom g, Tables<double> tb, const double * __restrict__ qd, double * __restrict__ diag)
+template <int P>
+__global__ void __launch_bounds__(256)
+k_diag_elem(Geom g, Tables<double> tb, const double * __restrict__ qd, int col, double * __restrict__ diag)
 {
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= g.off[3])
-    return;
-  int c;
-  int64_t gi[3];
-  dof_coords(g, gid, c, gi[0], gi[1], gi[2]);
-  if (is_ess(g, c, gi[0], gi[1], gi[2]))
+  constexpr int O = P, D = P + 1, Q = P + 1, QQ = Q * Q, Q3 = Q * QQ;
+  constexpr int ND = 3 * O * D * D; // local DOFs
+  __shared__ double sq[12 * Q3];
+  __shared__ double sBo[Q * O], sBc[Q * D], sGc[Q * D];
+  const int hx = (g.n[0] - (col & 1) + 1) / 2, hy = (g.n[1] - ((col >> 1) & 1) + 1) / 2;
+  const int b = blockIdx.x;
+  const int ex = 2 * (b % hx) + (col & 1);
+  const int ey = 2 * ((b / hx) % hy) + ((col >> 1) & 1);
+  const int ez = 2 * (b / (hx * hy)) + ((col >> 2) & 1);
+  const int64_t e = ((int64_t)ez * g.n[1] + ey) * g.n[0] + ex;
+  const double * qe = qd + e * (12 * Q3);
+  for (int i = threadIdx.x; i < 12 * Q3; i += blockDim.x)
+    sq[i] = qe[i];
+  for (int i = threadIdx.x; i < Q * D; i += blockDim.x)
   {
-    diag[gid] = 1.0;
+    sBc[i] = tb.Bc[i];
+    sGc[i] = tb.Gc[i];
+    if (i < Q * O)
+      sBo[i] = tb.Bo[i];
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t >= ND)
     return;
-  }
-  const int p = g.p, O = p, D = p + 1, Q = p + 1, QQ = Q * Q;
-  int ne[3];
-  int64_t ec[3][2];
-  int lc[3][2];
-  for (int a = 0; a < 3; a++)
-  {
-    ne[a] = 0;
-    if (a == c)
-    {
-      ec[a][0] = gi[a] / p;
-      lc[a][0] = (int)(gi[a] % p);
-      ne[a] = 1;
-    }
-    else
-    {
-      const int64_t eh = gi[a] / p;
-      const int lh = (int)(gi[a] % p);
-      if (lh == 0 && eh > 0)
-      {
-        ec[a][ne[a]] = eh - 1;
-        lc[a][ne[a]] = p;
-        ne[a]++;
-      }
-      if (eh < g.n[a])
-      {
-        ec[a][ne[a]] = eh;
-        lc[a][ne[a]] = lh;
-        ne[a]++;
-      }
-    }
-  }
+  // local DOF: component c, indices (dx, dy, dz) with O along c, D elsewhere
+  const int c = t / (O * D * D);
+  const int r = t - c * (O * D * D);
+  const int n0 = (c == 0) ? O : D, n1 = (c == 1) ? O : D;
+  const int dx = r % n0, dy = (r / n0) % n1, dz = r / (n0 * n1);
+  // 1-D factors per axis: value (open or closed basis) and derivative (closed)
+  const double * fx = (c == 0) ? sBo + dx : sBc + dx;
+  const double * fy = (c == 1) ? sBo + dy : sBc + dy;
+  const double * fz = (c == 2) ? sBo + dz : sBc + dz;
+  const int sx = (c == 0) ? O : D, sy = (c == 1) ? O : D, sz = (c == 2) ? O : D;
+  const int km = (c == 0) ? 0 : ((c == 1) ? 3 : 5);
+  const int k1 = (c == 0) ? 9 : 6, k2 = (c == 2) ? 9 : 11, k12 = (c == 0) ? 10 : ((c == 1) ? 8 : 7);
   double s = 0.0;
-  for (int kz = 0; kz < ne[2]; kz++)
-    for (int ky = 0; ky < ne[1]; ky++)
-      for (int kx = 0; kx < ne[0]; kx++)
+  for (int qz = 0; qz < Q; qz++)
+    for (int qy = 0; qy < Q; qy++)
+      for (int qx = 0; qx < Q; qx++)
       {
-        const int64_t e = (ec[2][kz] * g.n[1] + ec[1][ky]) * g.n[0] + ec[0][kx];
-        const int dx = lc[0][kx], dy = lc[1][ky], dz = lc[2][kz];
-        const double * qe = qd + e * 12 * Q * QQ;
-        for (int qz = 0; qz < Q; qz++)
-          for (int qy = 0; qy < Q; qy++)
-            for (int qx = 0; qx < Q; qx++)
-            {
-              const double * qq = qe + qz * 12 * QQ + qy * Q + qx;
-              double v, c1, c2;
-              int k1, k2, k12, km;
-              if (c == 0)
-              {
-                const double a = tb.Bo[qx * O + dx], b = tb.Bc[qy * D + dy], bp = tb.Gc[qy * D + dy];
-                const double cc = tb.Bc[qz * D + dz], cp = tb.Gc[qz * D + dz];
-                v = a * b * cc;
-                c1 = a * b * cp;   // curl_y
-                c2 = -a * bp * cc; // curl_z
-                km = 0;
-                k1 = 9;
-                k2 = 11;
-                k12 = 10;
-              }
-              else if (c == 1)
-              {
-                const double a = tb.Bc[qx * D + dx], ap = tb.Gc[qx * D + dx], b = tb.Bo[qy * O + dy];
-                const double cc = tb.Bc[qz * D + dz], cp = tb.Gc[qz * D + dz];
-                v = a * b * cc;
-                c1 = -a * b * cp; // curl_x
-                c2 = ap * b * cc; // curl_z
-                km = 3;
-                k1 = 6;
-                k2 = 11;
-                k12 = 8;
-              }
-              else
-              {
-                const double a = tb.Bc[qx * D + dx], ap = tb.Gc[qx * D + dx];
-                const double b = tb.Bc[qy * D + dy], bp = tb.Gc[qy * D + dy], cc = tb.Bo[qz * O + dz];
-                v = a * b * cc;
-                c1 = a * bp * cc;  // curl_x
-                c2 = -ap * b * cc; // curl_y
-                km = 5;
-                k1 = 6;
-                k2 = 9;
-                k12 = 7;
-              }
-              s += v * v * qq[km * QQ] + c1 * c1 * qq[k1 * QQ] + c2 * c2 * qq[k2 * QQ] +
-                   2.0 * c1 * c2 * qq[k12 * QQ];
-            }
+        const double a = fx[qx * sx], bb = fy[qy * sy], cc = fz[qz * sz];
+        const double v = a * bb * cc;
+        double c1, c2;
+        if (c == 0)
+        {
+          c1 = a * bb * sGc[qz * D + dz];  // curl_y
+          c2 = -a * sGc[qy * D + dy] * cc; // curl_z
+        }
+        else if (c == 1)
+        {
+          c1 = -a * bb * sGc[qz * D + dz]; // curl_x
+          c2 = sGc[qx * D + dx] * bb * cc; // curl_z
+        }
+        else
+        {
+          c1 = a * sGc[qy * D + dy] * cc;  // curl_x
+          c2 = -sGc[qx * D + dx] * bb * cc; // curl_y
+        }
+        const double * f = sq + qz * 12 * QQ + qy * Q + qx;
+        s += v * v * f[km * QQ] + c1 * c1 * f[k1 * QQ] + c2 * c2 * f[k2 * QQ] + 2.0 * c1 * c2 * f[k12 * QQ];
       }
-  diag[gid] = s;
+  const int64_t gi[3] = {(int64_t)ex * P + dx, (int64_t)ey * P + dy, (int64_t)ez * P + dz};
+  const int64_t gid = g.off[c] + (gi[2] * g.cd[c][1] + gi[1]) * g.cd[c][0] + gi[0];
+  if (is_ess(g, c, gi[0], gi[1], gi[2]))
+    diag[gid] = 1.0;
+  else
+    diag[gid] += s;
+}
+
+__global__ void
+k_fill0(double * __restrict__ v, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -388,7 +372,31 @@ cast_f32(const double * a, float * b, int64_t n, hipStream_t st)
 hipError_t
 diagonal(const Geom & g, const Tables<double> & tb, const double * qd, double * d, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_diagonal, dim3(div_up(g.off[3], 256)), dim3(256), 0, st, g, tb, qd, d);
+  hipLaunchKernelGGL(k_fill0, dim3(4096), dim3(256), 0, st, d, g.off[3]);
+  const int nd = 3 * g.p * (g.p + 1) * (g.p + 1);
+  const int nt = (nd + 63) / 64 * 64;
+  for (int col = 0; col < 8; col++)
+  {
+    const int hx = (g.n[0] - (col & 1) + 1) / 2, hy = (g.n[1] - ((col >> 1) & 1) + 1) / 2;
+    const int hz = (g.n[2] - ((col >> 2) & 1) + 1) / 2;
+    const int nb = hx * hy * hz;
+    if (nb <= 0)
+      continue;
+    switch (g.p)
+    {
+    case 1:
+      hipLaunchKernelGGL(k_diag_elem<1>, dim3(nb), dim3(nt), 0, st, g, tb, qd, col, d);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_diag_elem<2>, dim3(nb), dim3(nt), 0, st, g, tb, qd, col, d);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_diag_elem<3>, dim3(nb), dim3(nt), 0, st, g, tb, qd, col, d);
+      break;
+    default:
+      return hipErrorInvalidValue;
+    }
+  }
   return hipGetLastError();
 }
 
