@@ -10,8 +10,12 @@ the whole mesh, with x, y and the quadrature data resident in HBM.
 Printed: one JSON line with value = FP64 apply GDOF/s, a roofline object for the
 apply (algorithmic bytes B = 8*2N + 8*12*E*Q^3 per launch over the measured
 launch time), the CPU baseline (the oracle's OpenMP PA apply on the host cores,
-bounded sample) and the solve figures (FP64 Jacobi-PCG vs FP32 apply + FP64
-refinement, both to a true relative residual of 1e-8).
+bounded sample) and the solve figures, all to a true relative residual of 1e-8:
+the 48^3 order-3 solve, 32 right-hand sides on 32^3 order 2, 64 uneven
+operators (edge 8..40, order 1..3, random two-region sigma) batched vs one at a
+time, and a Poisson stream of backward-Euler step requests over those 64
+operators (p50/p99); each with plain Jacobi and with the gradient-space
+correction of the preconditioner, FP64 vs FP32 apply + FP64 refinement.
 """
 from __future__ import annotations
 
@@ -102,7 +106,7 @@ def solve_entry(st, nsolves=1):
             "solves_per_s": round(1000.0 * nsolves / st["solve_ms"], 4) if st["solve_ms"] > 0 else None}
 
 
-def solves_config2(torch, H, op, x, args):
+def solves_single_48(torch, H, op, x, args):
     """48^3 p=3 two-region: FP64 Jacobi-PCG vs FP32 PCG + FP64 refinement to a
     true relative residual of 1e-8, with a coil current right-hand side
     (divergence-free, seed 2) and, second, with a uniform random right-hand
@@ -125,12 +129,12 @@ def solves_config2(torch, H, op, x, args):
                           raise_on_fail=False)
             entry[name] = solve_entry(st)
             entry[name]["fp64_continuations"] = st["fp64_fallbacks"] if (mode & 0xF) == H.SOLVE_FP32_IR else 0
-            print(f"[bench] config2 {rhs} solve {name}: {entry[name]}", file=sys.stderr, flush=True)
+            print(f"[bench] 48^3 p3 {rhs} solve {name}: {entry[name]}", file=sys.stderr, flush=True)
         sol[rhs] = entry
     return sol
 
 
-def solves_config3(torch, H, args):
+def solves_multi_rhs_32(torch, H, args):
     """32^3 p=2 two-region, 32 coil right-hand sides (seeds 3..34): plain FP64
     Jacobi-PCG one right-hand side at a time vs one batched FP32 PCG + FP64
     refinement solve of all 32, both to a true relative residual of 1e-8; each
@@ -165,7 +169,7 @@ def solves_config3(torch, H, args):
     res["aux_speedup_vs_fp64_single"] = round(res["fp32_ir_aux_batched"]["solves_per_s"] / base, 3)
     res["aux_speedup_vs_fp64_aux_single"] = round(res["fp32_ir_aux_batched"]["solves_per_s"] /
                                                   res["fp64_aux_single"]["solves_per_s"], 3)
-    print(f"[bench] config3: {res}", file=sys.stderr, flush=True)
+    print(f"[bench] 32^3 p2 32-rhs: {res}", file=sys.stderr, flush=True)
     del op
     return res
 
@@ -177,7 +181,7 @@ def _splitmix_u01(seed, i):
     return ((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0)
 
 
-def config4_specs(count=64):
+def uneven_specs(count=64):
     """Uneven instances: cube edge n ~ U{8..40}, order p ~ U{1,2,3}, sigma/dt of
     the inner region [0.25,0.75]^3 and of the outer region each 10^U[-6,4];
     instance i draws from splitmix64 with seed 4 + i."""
@@ -192,9 +196,9 @@ def config4_specs(count=64):
     return out
 
 
-def build_config4(torch, H, count=64):
+def build_uneven(torch, H, count=64):
     ops, bs = [], []
-    for i, (n, p, s_in, s_out) in enumerate(config4_specs(count)):
+    for i, (n, p, s_in, s_out) in enumerate(uneven_specs(count)):
         op = H.Operator(n, p, 1.0, two_region(n, s_in, s_out)).setup(1.0)
         ops.append(op)
         bs.append(op.rhs_coil(1000 + i))
@@ -202,7 +206,7 @@ def build_config4(torch, H, count=64):
     return ops, bs
 
 
-def solves_config4(torch, H, ops, bs, args):
+def solves_uneven(torch, H, ops, bs, args):
     """64 uneven instances, coil right-hand sides, rtol 1e-8: one batched
     FP32 PCG + FP64 refinement solve (FP64 continuation per instance where FP32
     stalls) against the plain FP64 path solving the instances one at a time;
@@ -239,13 +243,13 @@ def solves_config4(torch, H, ops, bs, args):
     res["aux_speedup_vs_fp64_single"] = round(res["fp32_ir_aux_batched"]["solves_per_s"] / base, 3)
     res["aux_speedup_vs_fp64_aux_single"] = round(res["fp32_ir_aux_batched"]["solves_per_s"] /
                                                   res["fp64_aux_single"]["solves_per_s"], 3)
-    print(f"[bench] config4: {res}", file=sys.stderr, flush=True)
+    print(f"[bench] 64 uneven: {res}", file=sys.stderr, flush=True)
     return res
 
 
-def serve_config5(torch, H, ops, bs, args, fp64_single_sps, aux=True):
+def serve_poisson(torch, H, ops, bs, args, fp64_single_sps, aux=True):
     """Poisson arrivals (mean 200 req/s, seed 5) of backward-Euler step solves
-    drawn uniformly from the config-4 operators (resident), FP32 PCG + FP64
+    drawn uniformly from the 64 uneven operators (resident), FP32 PCG + FP64
     refinement to 1e-8 (preconditioner: Jacobi with the gradient-space
     correction when aux, else Jacobi), continuous batching under an HBM budget."""
     rng = np.random.default_rng(5)
@@ -269,7 +273,7 @@ def serve_config5(torch, H, ops, bs, args, fp64_single_sps, aux=True):
     span_ms = max(r["arrival_ms"] + r["latency_ms"] for r in out)
     sps = nreq / (span_ms / 1000.0)
     res = {"workload": f"{nreq} requests, Poisson arrivals mean 200 req/s (seed 5), operators drawn from the "
-                       f"config-4 mix, u_n = 0, rtol 1e-8, FP32 PCG + FP64 refinement, "
+                       f"64-operator uneven mix, u_n = 0, rtol 1e-8, FP32 PCG + FP64 refinement, "
                        f"{'Jacobi + gradient-space correction' if aux else 'Jacobi'} preconditioner",
            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
            "mean_queue_ms": round(float(np.mean([r["queue_ms"] for r in out])), 2),
@@ -279,7 +283,7 @@ def serve_config5(torch, H, ops, bs, args, fp64_single_sps, aux=True):
            "budget_gb": args.serve_budget_gb, "peak_budget_use_gb": round(info["peak_bytes"] / 2 ** 30, 2),
            "rounds": info["rounds"], "wall_s": round(wall, 2),
            "speedup_vs_fp64_single": round(sps / fp64_single_sps, 3) if fp64_single_sps else None}
-    print(f"[bench] config5{' aux' if aux else ''}: {res}", file=sys.stderr, flush=True)
+    print(f"[bench] serving{' aux' if aux else ' jacobi'}: {res}", file=sys.stderr, flush=True)
     return res
 
 
@@ -379,16 +383,16 @@ def main():
                        "GB_s": round(B32 / ms32 / 1e6, 1)},
     }
     if not args.no_solve:
-        out["solves"] = {"config2": solves_config2(torch, H, op, x, args),
-                         "config3": solves_config3(torch, H, args)}
+        out["solves"] = {"single_48_p3": solves_single_48(torch, H, op, x, args),
+                         "multi_rhs_32_p2": solves_multi_rhs_32(torch, H, args)}
         if not args.no_batch:
             del op
             torch.cuda.empty_cache()
-            ops4, bs4 = build_config4(torch, H)
-            c4 = solves_config4(torch, H, ops4, bs4, args)
-            out["solves"]["config4"] = c4
-            out["serving"] = serve_config5(torch, H, ops4, bs4, args, c4["fp64_single"]["solves_per_s"], aux=True)
-            out["serving_jacobi"] = serve_config5(torch, H, ops4, bs4, args, c4["fp64_single"]["solves_per_s"],
+            ops4, bs4 = build_uneven(torch, H)
+            c4 = solves_uneven(torch, H, ops4, bs4, args)
+            out["solves"]["uneven_64"] = c4
+            out["serving"] = serve_poisson(torch, H, ops4, bs4, args, c4["fp64_single"]["solves_per_s"], aux=True)
+            out["serving_jacobi"] = serve_poisson(torch, H, ops4, bs4, args, c4["fp64_single"]["solves_per_s"],
                                                   aux=False)
             del ops4, bs4
             torch.cuda.empty_cache()
