@@ -100,6 +100,64 @@ def cpu_baseline(n, p, sig, budget_s):
             "ms_per_apply": t * 1e3}
 
 
+def cpu_csr_path(torch, H, budget_s):
+    """The reference's own per-iteration kernel on the host, restated: an
+    assembled CSR matrix (full assembly with PEC elimination, oracle/hc_oracle.c,
+    OpenMP) -- its SpMV at 32^3 order 2 (median of a bounded sample) beside the
+    GPU FP64 apply of the same operator, and a CSR Jacobi-PCG solve at 16^3
+    order 1 (sigma/dt = 1, nu = 1, uniform RHS seed 1, rtol 1e-10) beside the
+    GPU FP64 solve of the same problem."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O
+    O.build()
+    out = {"cores": O.num_threads(), "kind": "port"}
+    n, p = 32, 2
+    sig = two_region(n)
+    m = O.Mesh(n)
+    qd = O.pa_setup(m, p, 1.0, sig)
+    ess = O.ess_mask(m, p)
+    t0 = time.perf_counter()
+    A = O.Csr(m, p, qd, ess)
+    t_asm = time.perf_counter() - t0
+    x = O.rand_uniform(2, A.n)
+    A.apply(x)
+    times = []
+    t_start = time.perf_counter()
+    while time.perf_counter() - t_start < budget_s and len(times) < 20:
+        t0 = time.perf_counter()
+        A.apply(x)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    op = H.Operator(n, p, 1.0, sig).setup(1.0)
+    op.autotune(reps=3)
+    xd = torch.empty(op.ndofs, dtype=torch.float64, device="cuda")
+    op.fill_uniform(2, xd)
+    yd = torch.empty_like(xd)
+    ms = time_applies(torch, lambda: op.apply(xd, yd), 30, 3)
+    out["spmv_32_p2"] = {"nnz": A.nnz, "assembly_s": round(t_asm, 2), "cpu_ms": round(t * 1e3, 2),
+                         "cpu_GDOF_s": round(A.n / t / 1e9, 4), "gpu_apply_ms": round(ms, 4),
+                         "gpu_GDOF_s": round(op.ndofs / ms / 1e6, 3), "sample": f"{len(times)} SpMVs"}
+    del A, op
+    n, p = 16, 1
+    m = O.Mesh(n)
+    qd = O.pa_setup(m, p, 1.0, 1.0)
+    ess = O.ess_mask(m, p)
+    A = O.Csr(m, p, qd, ess)
+    b = O.rhs_uniform(m, p, 1, ess)
+    d = O.pa_diagonal(m, p, qd, ess)
+    t0 = time.perf_counter()
+    _, st = A.pcg(d, b, 1e-10, 10000)
+    t_cpu = time.perf_counter() - t0
+    op = H.Operator(n, p, 1.0, 1.0).setup(1.0)
+    bd = torch.from_numpy(b).cuda()
+    xd = torch.zeros_like(bd)
+    sg = op.solve(bd, xd, rtol=1e-10, maxit=10000, mode=H.SOLVE_FP64)
+    out["pcg_16_p1"] = {"cpu_iterations": int(st["iterations"]), "cpu_ms": round(t_cpu * 1e3, 2),
+                        "gpu_iterations": sg["iterations"], "gpu_ms": round(sg["solve_ms"], 2)}
+    print(f"[bench] CPU CSR path {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def solve_entry(st, nsolves=1):
     return {"iterations": st["iterations"], "outer": st["outer_iterations"], "converged": bool(st["converged"]),
             "true_rel_res": st["rel_res"], "solve_ms": round(st["solve_ms"], 3),
@@ -402,6 +460,8 @@ def main():
         cb = cpu_baseline(n, p, sig, args.cpu_budget)
         out["cpu_baseline"] = cb
         print(f"[bench] CPU baseline {cb}", file=sys.stderr, flush=True)
+        if not args.no_solve:
+            out["cpu_csr_path"] = cpu_csr_path(torch, H, min(args.cpu_budget, 5.0))
     print(json.dumps(out))
 
 

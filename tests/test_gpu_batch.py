@@ -83,12 +83,14 @@ def test_multi_rhs_apply(torch_cuda, hc, O, p, n, rb):
         assert _rel(Y32[r], Yh[r]) <= 2e-6
 
 
-@pytest.mark.parametrize("mode", ["fp64", "fp32_ir"])
+@pytest.mark.parametrize("mode", ["fp64", "fp32_ir", "fp64_aux", "fp32_ir_aux"])
 def test_multi_rhs_solve(torch_cuda, hc, O, mode):
     torch = torch_cuda
     n, p = (8, 7, 6), 2
     print(f"case multi-rhs solve {mode} n={n} p={p}", flush=True)
-    md = hc.SOLVE_FP64 if mode == "fp64" else hc.SOLVE_FP32_IR
+    md = hc.SOLVE_FP64 if mode.startswith("fp64") else hc.SOLVE_FP32_IR
+    if mode.endswith("_aux"):
+        md |= hc.SOLVE_AUX
     m = O.Mesh(*n)
     sig = _two_region(n, 100.0, 0.1)
     op = hc.Operator(n, p, 1.0, sig).setup(1.0)
