@@ -173,8 +173,9 @@ int hc_op_apply_f32(hc_op_t op, const float * x, float * y, int nrhs, void * str
  * quadrature data of hc_op_setup (no second assembly). */
 int hc_op_apply_curl(hc_op_t op, const double * x, double * y, void * stream);
 /* Right-hand sides per work item of the multi-RHS apply and solve (0 =
- * automatic: the fewest groups that still give the GPU enough work items).
- * Results do not depend on it. */
+ * automatic: one per work item; a larger group holds the tile's quadrature
+ * data in registers across its right-hand sides).  Results do not depend on
+ * it. */
 int hc_op_set_rhs_block(hc_op_t op, int rhs_block);
 /* Element-tile shape of the apply kernel.  Each order has a set of compiled
  * tile variants: tile extent tx x ty x tz elements and a load mode (0:
