@@ -174,20 +174,28 @@ __global__ void
 k_aux_nodes_single(AuxGeom a, const double * __restrict__ ginv, const double * __restrict__ r,
                    double * __restrict__ t)
 {
-  const int S = gridDim.x * blockDim.x;
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  NodeWalk nw(a, k, S);
-  for (; k < a.NN; k += S, nw.step(a))
-    t[k] = ginv[k] * aux_gt(a, r, nw.I, nw.J, nw.K);
+  with_order(a, [&](auto pc) {
+    constexpr int P = decltype(pc)::value;
+    const DgTab<P> tab = load_dg<P>(a);
+    const int S = gridDim.x * blockDim.x;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    NodeWalk nw(a, k, S);
+    for (; k < a.NN; k += S, nw.step(a))
+      t[k] = ginv[k] * aux_gt<P>(a, tab, r, nw.I, nw.J, nw.K);
+  });
 }
 
 __global__ void
 k_aux_z_single(AuxGeom a, const double * __restrict__ dinv, const double * __restrict__ r,
                const double * __restrict__ t, double * __restrict__ z)
 {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.off[3];
-       i += (int64_t)gridDim.x * blockDim.x)
-    z[i] = dinv[i] * r[i] + aux_g(a, t, i);
+  with_order(a, [&](auto pc) {
+    constexpr int P = decltype(pc)::value;
+    const DgTab<P> tab = load_dg<P>(a);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.off[3];
+         i += (int64_t)gridDim.x * blockDim.x)
+      z[i] = dinv[i] * r[i] + aux_g<P>(a, tab, t, i);
+  });
 }
 
 hipError_t

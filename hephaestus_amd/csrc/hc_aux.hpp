@@ -23,6 +23,8 @@
 
 #include "hc_common.hpp"
 
+#include <type_traits>
+
 namespace hc
 {
 
@@ -57,12 +59,33 @@ fast_divmod(int n, int d, double rd, int & q, int & r)
   }
 }
 
-// (G^T r) at node (I, J, K).
-template <typename T>
-__device__ inline double
-aux_gt(const AuxGeom & a, const T * __restrict__ r, int I, int J, int K)
+// The 1D derivative table Dg of order P in registers: its entries are
+// wave-uniform, and a row or column chosen by a per-lane index is a short
+// select chain instead of an indexed memory load.
+template <int P>
+struct DgTab
 {
-  const int p = a.p, D = p + 1;
+  double d[P][P + 1];
+};
+
+template <int P>
+__device__ inline DgTab<P>
+load_dg(const AuxGeom & a)
+{
+  DgTab<P> t;
+#pragma unroll
+  for (int i = 0; i < P; i++)
+#pragma unroll
+    for (int m = 0; m <= P; m++)
+      t.d[i][m] = a.dg[i * (P + 1) + m];
+  return t;
+}
+
+// (G^T r) at node (I, J, K), order P.
+template <int P, typename T>
+__device__ inline double
+aux_gt(const AuxGeom & a, const DgTab<P> & tab, const T * __restrict__ r, int I, int J, int K)
+{
   const int idx[3] = {I, J, K};
   double g = 0.0;
 #pragma unroll
@@ -87,18 +110,31 @@ aux_gt(const AuxGeom & a, const T * __restrict__ r, int I, int J, int K)
       base = a.off[2] + (int64_t)J * a.cd[2][0] + I;
       stride = a.cd[2][0] * a.cd[2][1];
     }
-    // the (at most two) elements along c holding the node: local index m
-#pragma unroll
-    for (int side = 0; side < 2; side++)
+    // the (at most two) elements along c holding the node: local index m of
+    // the node in the lower-numbered one's upper end (m = P) and in the other
+    const int e = ic / P, m = ic % P;
+    if (e < a.n[c])
     {
-      const int e = side == 0 ? ic / p : ic / p - 1;
-      const int m = side == 0 ? ic % p : p;
-      const bool ok = side == 0 ? (e < a.n[c]) : (ic % p == 0 && ic > 0);
-      if (!ok)
-        continue;
-      const T * rl = r + base + (int64_t)(e * p) * stride;
-      for (int i = 0; i < p; i++)
-        g += a.dg[i * D + m] * (double)rl[(int64_t)i * stride];
+      double col[P];
+#pragma unroll
+      for (int i = 0; i < P; i++)
+      {
+        col[i] = tab.d[i][0];
+#pragma unroll
+        for (int k = 1; k < P; k++)
+          col[i] = (m == k) ? tab.d[i][k] : col[i];
+      }
+      const T * rl = r + base + (int64_t)(e * P) * stride;
+#pragma unroll
+      for (int i = 0; i < P; i++)
+        g += col[i] * (double)rl[(int64_t)i * stride];
+    }
+    if (m == 0 && ic > 0)
+    {
+      const T * rl = r + base + (int64_t)((e - 1) * P) * stride;
+#pragma unroll
+      for (int i = 0; i < P; i++)
+        g += tab.d[i][P] * (double)rl[(int64_t)i * stride];
     }
   }
   return g;
@@ -139,27 +175,56 @@ struct NodeWalk
   }
 };
 
-// (G t) at ND DOF i.
-template <typename T>
+// (G t) at ND DOF i, order P.
+template <int P, typename T>
 __device__ inline double
-aux_g(const AuxGeom & a, const T * __restrict__ t, int64_t i)
+aux_g(const AuxGeom & a, const DgTab<P> & tab, const T * __restrict__ t, int64_t i)
 {
-  const int p = a.p, D = p + 1;
   const int c = (i < a.off[1]) ? 0 : ((i < a.off[2]) ? 1 : 2);
   const int li = (int)(i - a.off[c]);
   int i0, rest, i1, i2;
   fast_divmod(li, a.cd[c][0], a.rcd[c][0], rest, i0);
   fast_divmod(rest, a.cd[c][1], a.rcd[c][1], i2, i1);
   const int ic = (c == 0) ? i0 : ((c == 1) ? i1 : i2);
-  const int e = ic / p, l = ic % p;
+  const int e = ic / P, l = ic % P;
   int node[3] = {i0, i1, i2};
-  node[c] = e * p;
+  node[c] = e * P;
   const int stride = (c == 0) ? 1 : ((c == 1) ? a.M[0] : a.M[0] * a.M[1]);
   const T * tl = t + ((int64_t)node[2] * a.M[1] + node[1]) * a.M[0] + node[0];
+  double row[P + 1];
+#pragma unroll
+  for (int m = 0; m <= P; m++)
+  {
+    row[m] = tab.d[0][m];
+#pragma unroll
+    for (int k = 1; k < P; k++)
+      row[m] = (l == k) ? tab.d[k][m] : row[m];
+  }
   double v = 0.0;
-  for (int m = 0; m <= p; m++)
-    v += a.dg[l * D + m] * (double)tl[(int64_t)m * stride];
+#pragma unroll
+  for (int m = 0; m <= P; m++)
+    v += row[m] * (double)tl[(int64_t)m * stride];
   return v;
+}
+
+// Runs f with the order of a as a compile-time constant (orders 1..PMAX).
+static_assert(PMAX == 3, "with_order dispatches orders 1..3");
+template <typename F>
+__device__ inline void
+with_order(const AuxGeom & a, F && f)
+{
+  switch (a.p)
+  {
+  case 1:
+    f(std::integral_constant<int, 1>{});
+    break;
+  case 2:
+    f(std::integral_constant<int, 2>{});
+    break;
+  default:
+    f(std::integral_constant<int, 3>{});
+    break;
+  }
 }
 
 // Host: geometry of the auxiliary space of g, with the 1D derivative matrix dg1d.

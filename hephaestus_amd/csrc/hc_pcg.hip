@@ -339,8 +339,12 @@ k_pcg_update_p(SlotTable t, const int * __restrict__ git)
   if (sl.aux)
   {
     const T * __restrict__ tv = V.t();
-    HC_SLOT_LOOP(i, sl.n, nb)
-    p[i] = (dinv[i] * r[i] + (T)aux_g(sl.ag, tv, i)) + beta * p[i];
+    with_order(sl.ag, [&](auto pc) {
+      constexpr int P = decltype(pc)::value;
+      const DgTab<P> tab = load_dg<P>(sl.ag);
+      HC_SLOT_LOOP(i, sl.n, nb)
+      p[i] = (dinv[i] * r[i] + (T)aux_g<P>(sl.ag, tab, tv, i)) + beta * p[i];
+    });
   }
   else
   {
@@ -376,16 +380,20 @@ k_aux_nodes(SlotTable t, int which)
   T * __restrict__ tv = V.t();
   __shared__ double sh[RED_THREADS];
   double v[1] = {0};
-  const int S = nb * RED_THREADS;
-  int k = bk * RED_THREADS + threadIdx.x;
-  NodeWalk nw(sl.ag, k, S);
-  for (; k < sl.ag.NN; k += S, nw.step(sl.ag))
-  {
-    const double g = aux_gt(sl.ag, r, nw.I, nw.J, nw.K);
-    const T tk = (T)((double)ginv[k] * g);
-    tv[k] = tk;
-    v[0] += g * (double)tk;
-  }
+  with_order(sl.ag, [&](auto pc) {
+    constexpr int P = decltype(pc)::value;
+    const DgTab<P> tab = load_dg<P>(sl.ag);
+    const int S = nb * RED_THREADS;
+    int k = bk * RED_THREADS + threadIdx.x;
+    NodeWalk nw(sl.ag, k, S);
+    for (; k < sl.ag.NN; k += S, nw.step(sl.ag))
+    {
+      const double g = aux_gt<P>(sl.ag, tab, r, nw.I, nw.J, nw.K);
+      const T tk = (T)((double)ginv[k] * g);
+      tv[k] = tk;
+      v[0] += g * (double)tk;
+    }
+  });
   block_sum<1>(v, sh);
   if (threadIdx.x == 0)
     sl.part[bk * 4 + (which == 0 ? 2 : 3)] = v[0];
@@ -413,8 +421,12 @@ k_aux_pset(SlotTable t, int which)
   const T * __restrict__ dinv = V.dinv();
   const T * __restrict__ tv = V.t();
   T * __restrict__ p = V.p();
-  HC_SLOT_LOOP(i, sl.n, nb)
-  p[i] = dinv[i] * r[i] + (T)aux_g(sl.ag, tv, i);
+  with_order(sl.ag, [&](auto pc) {
+    constexpr int P = decltype(pc)::value;
+    const DgTab<P> tab = load_dg<P>(sl.ag);
+    HC_SLOT_LOOP(i, sl.n, nb)
+    p[i] = dinv[i] * r[i] + (T)aux_g<P>(sl.ag, tab, tv, i);
+  });
 }
 
 // ---------------------------------------------------------------------------
