@@ -97,14 +97,14 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   const int r_begin = (wl % ngrp) * rb;
   const int r_end = GROUP ? min(ad.nrhs, r_begin + rb) : r_begin + 1;
   const int tcur = wl / ngrp;
-  if (!GROUP && ad.done && ad.done[r_begin * ad.done_stride] != ad.gate)
+  if (!GROUP && ad.done && gmem(ad.done)[r_begin * ad.done_stride] != ad.gate)
     return;
   const Geom & g = ad.g;
-  const T * __restrict__ qd = (const T *)ad.qd;
+  const gptr<const T> __restrict__ qd = gmem((const T *)ad.qd);
   // vectors of the current right-hand side (reset per rhs below)
-  const T * x = (const T *)ad.x;
-  T * y = (T *)ad.y;
-  T * part = (T *)ad.part;
+  gptr<const T> x = gmem((const T *)ad.x);
+  gptr<T> y = gmem((T *)ad.y);
+  gptr<T> part = gmem((T *)ad.part);
   __shared__ T sA[TE * SA];
   __shared__ T sB[TE * SB];
 
@@ -158,13 +158,13 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
     const bool pin = pact && pl0 < ne_[0] && pl1 < ne_[1] && pl2 < ne_[2];
     // inactive lanes read element 0 (in bounds) and discard the values
     const int64_t e = pin ? ((int64_t)(e0_[2] + pl2) * g.n[1] + (e0_[1] + pl1)) * g.n[0] + (e0_[0] + pl0) : 0;
-    const T * qe = qd + e * (12 * Q * QQ) + (pin ? pqy * Q + pqx : 0);
+    const gptr<const T> qe = qd + e * (12 * Q * QQ) + (pin ? pqy * Q + pqx : 0);
 #pragma unroll
     for (int j = 0; j < QH; j++)
     {
       const int qz = ph * QH + j;
       const bool ld = pin && qz < Q;
-      const T * qq = qe + (ld ? qz : 0) * 12 * QQ;
+      const gptr<const T> qq = qe + (ld ? qz : 0) * 12 * QQ;
 #pragma unroll
       for (int k = 0; k < 12; k++)
       {
@@ -187,7 +187,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
                        (iiz == 0 && (pec & 16)) || (iiz == cd[0][2] - 1 && (pec & 32));
     if (iin && !ress0)
     {
-      const T * row = x + g.off[0] + ((int64_t)iiz * cd[0][1] + iiy) * cd[0][0] + iix;
+      const gptr<const T> row = x + g.off[0] + ((int64_t)iiz * cd[0][1] + iiy) * cd[0][0] + iix;
 #pragma unroll
       for (int d = 0; d < O; d++)
         xr0[d] = row[d];
@@ -196,7 +196,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
     const bool ress1 = (iiz == 0 && (pec & 16)) || (iiz == cd[1][2] - 1 && (pec & 32));
     if (iin && idy < O && !ress1)
     {
-      const T * row = x + g.off[1] + ((int64_t)iiz * cd[1][1] + iiy) * cd[1][0] + iix;
+      const gptr<const T> row = x + g.off[1] + ((int64_t)iiz * cd[1][1] + iiy) * cd[1][0] + iix;
 #pragma unroll
       for (int d = 0; d < D; d++)
         xr1[d] = row[d];
@@ -209,7 +209,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
     const bool ress2 = (iiy == 0 && (pec & 4)) || (iiy == cd[2][1] - 1 && (pec & 8));
     if (iin && idz < O && !ress2)
     {
-      const T * row = x + g.off[2] + ((int64_t)iiz * cd[2][1] + iiy) * cd[2][0] + iix;
+      const gptr<const T> row = x + g.off[2] + ((int64_t)iiz * cd[2][1] + iiy) * cd[2][0] + iix;
 #pragma unroll
       for (int d = 0; d < D; d++)
         xr2[d] = row[d];
@@ -229,7 +229,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   {
     if constexpr (!GROUP)
     {
-      x = (const T *)ad.x + (int64_t)r_begin * g.off[3];
+      x = gmem((const T *)ad.x) + (int64_t)r_begin * g.off[3];
       load_x(e0, ne);
       x_ready = true;
     }
@@ -238,11 +238,11 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
 
   for (int rhs = r_begin; rhs < r_end; rhs++)
   {
-  if (GROUP && ad.done && ad.done[rhs * ad.done_stride] != ad.gate)
+  if (GROUP && ad.done && gmem(ad.done)[rhs * ad.done_stride] != ad.gate)
     continue;
-  x = (const T *)ad.x + (int64_t)rhs * g.off[3];
-  y = (T *)ad.y + (int64_t)rhs * g.off[3];
-  part = (T *)ad.part + (int64_t)rhs * 4 * g.off[3];
+  x = gmem((const T *)ad.x) + (int64_t)rhs * g.off[3];
+  y = gmem((T *)ad.y) + (int64_t)rhs * g.off[3];
+  part = gmem((T *)ad.part) + (int64_t)rhs * 4 * g.off[3];
   if (!x_ready)
     load_x(e0, ne);
   x_ready = false;
@@ -750,6 +750,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   // tile to the partials (slot bit0 = lower-numbered closed axis, bit1 = the
   // other; set when this tile holds the face as its upper face), PEC DOFs copy
   // x (written by the tile that holds them as a lower face) ----
+  const SegOffsets eos = seg_offsets(ad.segs);
   auto wout = [&](auto cc, int i, int r) {
     constexpr int c = decltype(cc)::value;
     constexpr int B0 = (c == 0) ? TX * P : TX * P + 1;
@@ -802,7 +803,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
       const int j1 = (alo == 2) ? gi[1] : gi[2];
       const int n0 = (alo == 0) ? dims[1] : dims[0];
       const int n1 = (alo == 2) ? dims[1] : dims[2];
-      const int64_t eo = (alo == 0) ? ad.segs.eoff[c][0] : ((alo == 1) ? ad.segs.eoff[c][1] : ad.segs.eoff[c][2]);
+      const int64_t eo = seg_offset<c>(eos, alo);
       const int64_t pid = eo + ((int64_t)k * n1 + j1) * n0 + j0;
       part[slot * N + pid] = sOut[i];
     }

@@ -57,13 +57,13 @@ k_apply_qp(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables
   const int r_begin = (wl % ngrp) * rb;
   const int r_end = BATCH ? min(ad.nrhs, r_begin + rb) : r_begin + 1;
   const int tcur = wl / ngrp;
-  if (!BATCH && ad.done && ad.done[r_begin * ad.done_stride] != ad.gate)
+  if (!BATCH && ad.done && gmem(ad.done)[r_begin * ad.done_stride] != ad.gate)
     return;
   const Geom & g = ad.g;
-  const T * __restrict__ qd = (const T *)ad.qd;
-  const T * x = (const T *)ad.x;
-  T * y = (T *)ad.y;
-  T * part = (T *)ad.part;
+  const gptr<const T> __restrict__ qd = gmem((const T *)ad.qd);
+  gptr<const T> x = gmem((const T *)ad.x);
+  gptr<T> y = gmem((T *)ad.y);
+  gptr<T> part = gmem((T *)ad.part);
   __shared__ T sE[TE * EL];
   // 1D tables in LDS: the lanes index them with their own point / DOF index
   __shared__ T sBo[QMAXC * PMAX], sBc[QMAXC * QMAXC], sGc[QMAXC * QMAXC];
@@ -113,7 +113,7 @@ k_apply_qp(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables
   T qf[12];
   {
     const int64_t e = eact ? ((int64_t)gez * g.n[1] + gey) * g.n[0] + gex : 0;
-    const T * qe = qd + e * (12 * Q3) + (eact ? qz * 12 * Q2 + qy * Q + qx : 0);
+    const gptr<const T> qe = qd + e * (12 * Q3) + (eact ? qz * 12 * Q2 + qy * Q + qx : 0);
 #pragma unroll
     for (int k = 0; k < 12; k++)
     {
@@ -127,11 +127,11 @@ k_apply_qp(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables
 
   for (int rhs = r_begin; rhs < r_end; rhs++)
   {
-    if (BATCH && ad.done && ad.done[rhs * ad.done_stride] != ad.gate)
+    if (BATCH && ad.done && gmem(ad.done)[rhs * ad.done_stride] != ad.gate)
       continue;
-    x = (const T *)ad.x + (int64_t)rhs * N;
-    y = (T *)ad.y + (int64_t)rhs * N;
-    part = (T *)ad.part + (int64_t)rhs * 4 * N;
+    x = gmem((const T *)ad.x) + (int64_t)rhs * N;
+    y = gmem((T *)ad.y) + (int64_t)rhs * N;
+    part = gmem((T *)ad.part) + (int64_t)rhs * 4 * N;
 
     // ---- element DOFs of x into R0 (PEC DOFs read as zero) ----
     if (eact)
@@ -413,6 +413,7 @@ k_apply_qp(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables
     constexpr int BY0 = TX * P + 1, BY1 = TY * P, BY2 = TZ * P + 1;
     constexpr int BZ0 = TX * P + 1, BZ1 = TY * P + 1, BZ2 = TZ * P;
     constexpr int OB1 = BX0 * BX1 * BX2, OB2 = OB1 + BY0 * BY1 * BY2, OBN = OB2 + BZ0 * BZ1 * BZ2;
+    const SegOffsets eos = seg_offsets(ad.segs);
     for (int i = tid; i < OBN; i += NT)
     {
       int c, r, b0, b1;
@@ -515,7 +516,7 @@ k_apply_qp(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables
         const int j1 = (alo == 2) ? gi[1] : gi[2];
         const int n0 = (alo == 0) ? dims[1] : dims[0];
         const int n1 = (alo == 2) ? dims[1] : dims[2];
-        const int64_t eo = ad.segs.eoff[c][alo];
+        const int64_t eo = seg_offset(eos, c, alo);
         const int64_t pid = eo + ((int64_t)(ga / ta - 1) * n1 + j1) * n0 + j0;
         part[slot * N + pid] = v;
       }

@@ -76,6 +76,68 @@ find_instance(int w, int ndesc, F key)
 // range of work items, so neighbouring tiles (which read the same x halo rows
 // and write parts of the same cache lines of y) run on one XCD at about the
 // same time and meet in its L2.  A bijection of [0, nb).
+// Pointers typed as global memory.  A pointer read from a descriptor table in
+// device memory is otherwise generic, and generic (flat) accesses also count
+// on the LDS counter: every LDS wait of a kernel would then wait for its
+// outstanding loads and stores as well.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ inline gptr<T>
+gmem(T * p)
+{
+  return (gptr<T>)p;
+}
+
+// A wave-uniform 64-bit value held in scalar registers.
+__device__ inline int64_t
+uniform64(int64_t v)
+{
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Offsets of the compact-partials segments (component c, plane axis a) of an
+// instance, read once before a write-out loop into scalar registers.  Named
+// fields, no array: an array selected by a per-lane axis would be placed in
+// scratch memory, and inside the loop any per-lane load makes each face DOF
+// wait for every store the thread issued before it (loads and stores share
+// one memory counter).
+struct SegOffsets
+{
+  int64_t x0, x1, x2, y0, y1, y2, z0, z1, z2;
+};
+__device__ inline SegOffsets
+seg_offsets(const PlaneSegs & segs)
+{
+  SegOffsets o;
+  o.x0 = uniform64(segs.eoff[0][0]);
+  o.x1 = uniform64(segs.eoff[0][1]);
+  o.x2 = uniform64(segs.eoff[0][2]);
+  o.y0 = uniform64(segs.eoff[1][0]);
+  o.y1 = uniform64(segs.eoff[1][1]);
+  o.y2 = uniform64(segs.eoff[1][2]);
+  o.z0 = uniform64(segs.eoff[2][0]);
+  o.z1 = uniform64(segs.eoff[2][1]);
+  o.z2 = uniform64(segs.eoff[2][2]);
+  return o;
+}
+template <int c>
+__device__ inline int64_t
+seg_offset(const SegOffsets & o, int a)
+{
+  const int64_t e0 = c == 0 ? o.x0 : (c == 1 ? o.y0 : o.z0);
+  const int64_t e1 = c == 0 ? o.x1 : (c == 1 ? o.y1 : o.z1);
+  const int64_t e2 = c == 0 ? o.x2 : (c == 1 ? o.y2 : o.z2);
+  return (a == 0) ? e0 : ((a == 1) ? e1 : e2);
+}
+__device__ inline int64_t
+seg_offset(const SegOffsets & o, int c, int a)
+{
+  return (c == 0) ? seg_offset<0>(o, a) : ((c == 1) ? seg_offset<1>(o, a) : seg_offset<2>(o, a));
+}
+
 constexpr int NXCD = 8;
 __device__ inline int
 xcd_remap(int b, int nb)
