@@ -750,7 +750,12 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   // tile to the partials (slot bit0 = lower-numbered closed axis, bit1 = the
   // other; set when this tile holds the face as its upper face), PEC DOFs copy
   // x (written by the tile that holds them as a lower face) ----
-  const SegOffsets eos = seg_offsets(ad.segs);
+  // descriptor-table instances: segment offsets held in scalar registers; the
+  // by-value instance reads them where used (fewer live registers, measured
+  // faster)
+  SegOffsets eos{};
+  if constexpr (TABLE)
+    eos = seg_offsets(ad.segs);
   auto wout = [&](auto cc, int i, int r) {
     constexpr int c = decltype(cc)::value;
     constexpr int B0 = (c == 0) ? TX * P : TX * P + 1;
@@ -803,7 +808,11 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
       const int j1 = (alo == 2) ? gi[1] : gi[2];
       const int n0 = (alo == 0) ? dims[1] : dims[0];
       const int n1 = (alo == 2) ? dims[1] : dims[2];
-      const int64_t eo = seg_offset<c>(eos, alo);
+      int64_t eo;
+      if constexpr (TABLE)
+        eo = seg_offset<c>(eos, alo);
+      else
+        eo = (alo == 0) ? ad.segs.eoff[c][0] : ((alo == 1) ? ad.segs.eoff[c][1] : ad.segs.eoff[c][2]);
       const int64_t pid = eo + ((int64_t)k * n1 + j1) * n0 + j0;
       part[slot * N + pid] = sOut[i];
     }

@@ -413,7 +413,11 @@ k_apply_qp(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables
     constexpr int BY0 = TX * P + 1, BY1 = TY * P, BY2 = TZ * P + 1;
     constexpr int BZ0 = TX * P + 1, BZ1 = TY * P + 1, BZ2 = TZ * P;
     constexpr int OB1 = BX0 * BX1 * BX2, OB2 = OB1 + BY0 * BY1 * BY2, OBN = OB2 + BZ0 * BZ1 * BZ2;
-    const SegOffsets eos = seg_offsets(ad.segs);
+    // descriptor-table instances: segment offsets held in scalar registers;
+    // the by-value instance reads them where used (fewer live registers)
+    SegOffsets eos{};
+    if constexpr (BATCH)
+      eos = seg_offsets(ad.segs);
     for (int i = tid; i < OBN; i += NT)
     {
       int c, r, b0, b1;
@@ -516,7 +520,7 @@ k_apply_qp(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables
         const int j1 = (alo == 2) ? gi[1] : gi[2];
         const int n0 = (alo == 0) ? dims[1] : dims[0];
         const int n1 = (alo == 2) ? dims[1] : dims[2];
-        const int64_t eo = seg_offset(eos, c, alo);
+        const int64_t eo = BATCH ? seg_offset(eos, c, alo) : ad.segs.eoff[c][alo];
         const int64_t pid = eo + ((int64_t)(ga / ta - 1) * n1 + j1) * n0 + j0;
         part[slot * N + pid] = v;
       }
