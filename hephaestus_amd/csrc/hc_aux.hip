@@ -207,4 +207,83 @@ aux_precond_apply(const AuxGeom & a, const double * dinv, const double * ginv, c
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// AV block operator pieces.  Unknowns: a = dA/dt on the ND DOFs, v = V on the
+// H1 nodes.  With s = a0 + G v (a0: a with its PEC DOFs zeroed),
+//   ya = dt K a0 + M s   on the free DOFs, ya = a on the PEC DOFs,
+//   yv = G^T M s         at every node,
+// i.e. CurlCurl + VectorFEMass + MixedVectorGradient (sigma grad V, u') in the
+// A rows and VectorFEWeakDivergence (sigma u, grad V') + Diffusion
+// (sigma grad V, grad V') in the V rows, the forms stated at
+// src/kernels/{mixed_vector_gradient,vector_fe_weak_divergence,diffusion}_kernel.hpp
+// with the kernels registered at src/formulations/AV/av_formulation.cpp:150-196.
+// M G is exact (G maps into the ND space) and K G = 0, so the mixed and
+// diffusion blocks are M G and G^T M G.
+// ---------------------------------------------------------------------------
+__global__ void
+k_av_gather(Geom g, AuxGeom a, const double * __restrict__ av, const double * __restrict__ v,
+            double * __restrict__ s)
+{
+  with_order(a, [&](auto pc) {
+    constexpr int P = decltype(pc)::value;
+    const DgTab<P> tab = load_dg<P>(a);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.off[3];
+         i += (int64_t)gridDim.x * blockDim.x)
+    {
+      int c;
+      int64_t i0, i1, i2;
+      dof_coords(g, i, c, i0, i1, i2);
+      const double a0 = is_ess(g, c, i0, i1, i2) ? 0.0 : av[i];
+      s[i] = a0 + aux_g<P>(a, tab, v, i);
+    }
+  });
+}
+
+__global__ void
+k_av_finish(Geom g, const double * __restrict__ av, const double * __restrict__ k, double * __restrict__ ya)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.off[3]; i += (int64_t)gridDim.x * blockDim.x)
+  {
+    int c;
+    int64_t i0, i1, i2;
+    dof_coords(g, i, c, i0, i1, i2);
+    ya[i] = is_ess(g, c, i0, i1, i2) ? av[i] : ya[i] + k[i];
+  }
+}
+
+__global__ void
+k_grad_transpose(AuxGeom a, const double * __restrict__ w, double * __restrict__ yv)
+{
+  with_order(a, [&](auto pc) {
+    constexpr int P = decltype(pc)::value;
+    const DgTab<P> tab = load_dg<P>(a);
+    const int S = gridDim.x * blockDim.x;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    NodeWalk nw(a, k, S);
+    for (; k < a.NN; k += S, nw.step(a))
+      yv[k] = aux_gt<P>(a, tab, w, nw.I, nw.J, nw.K);
+  });
+}
+
+hipError_t
+av_gather(const Geom & g, const AuxGeom & a, const double * av, const double * v, double * s, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_av_gather, dim3(4096), dim3(256), 0, st, g, a, av, v, s);
+  return hipGetLastError();
+}
+
+hipError_t
+av_finish(const Geom & g, const double * av, const double * k, double * ya, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_av_finish, dim3(4096), dim3(256), 0, st, g, av, k, ya);
+  return hipGetLastError();
+}
+
+hipError_t
+grad_transpose(const AuxGeom & a, const double * w, double * yv, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_grad_transpose, dim3(2048), dim3(256), 0, st, a, w, yv);
+  return hipGetLastError();
+}
+
 } // namespace hc

@@ -244,6 +244,15 @@ hipError_t term_ratio_blocks(const double * qd, int64_t E, int p, double * out, 
 // fields whose true operator action (the gradients, eigenvalues ~ sigma) is
 // sigma h_eff^2 / nu times smaller; the bound keeps that noise below 1 %.
 constexpr double FP32_MIN_TERM_RATIO = 6.0e-6;
+// AV block pieces (the blocks of src/formulations/AV/av_formulation.cpp:150-196
+// built from G and the ND operator):
+// s = a0 + G v with a0 = a zeroed on the PEC DOFs of g (all ND DOFs).
+hipError_t av_gather(const Geom & g, const AuxGeom & a, const double * av, const double * v, double * s,
+                     hipStream_t st);
+// ya = ya + k on the free DOFs, ya = a on the PEC DOFs of g.
+hipError_t av_finish(const Geom & g, const double * av, const double * k, double * ya, hipStream_t st);
+// yv = G^T w at every node (no essential nodes).
+hipError_t grad_transpose(const AuxGeom & a, const double * w, double * yv, hipStream_t st);
 // z = D^-1 r + G Dg^-1 G^T r for one vector (t: node scratch, NN entries).
 hipError_t aux_precond_apply(const AuxGeom & a, const double * dinv, const double * ginv, const double * r, double * t,
                              double * z, hipStream_t st);

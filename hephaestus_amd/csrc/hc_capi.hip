@@ -1,6 +1,7 @@
 // C ABI of libhcurl.so (declared in include/hc_hcurl.h): operator handles.
 // See that header for the reference interface each entry point replaces.
 #include "hc_op.hpp"
+#include "hc_aux.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -480,6 +481,49 @@ hc_op_apply_curl(hc_op_t op, const double * x, double * y, void * stream)
   // the quadrature data hold dt * nu: scale the curl term back by 1/dt
   HC_TRY_HIP(hc::apply_scaled<double>(op->plan64.g, op->plan64.segs, op->tb64, op->qd64, x, y, op->ws.part64, 0.0,
                                       1.0 / op->last_dt, op->plan64.cand, op->desc_dev, (hipStream_t)stream));
+  return HC_OK;
+}
+
+int64_t
+hc_op_nnodes(hc_op_t op)
+{
+  if (!op)
+  {
+    (void)fail(HC_EINVAL, "hc_op_nnodes: null handle");
+    return -1;
+  }
+  return op->ag.NN;
+}
+
+int
+hc_op_av_apply(hc_op_t op, const double * a, const double * v, double * ya, double * yv, void * stream)
+{
+  if (!op || !a || !v || !ya || !yv)
+    return fail(HC_EINVAL, "hc_op_av_apply: null argument");
+  if (!op->setup_done)
+    return fail(HC_EINVAL, "hc_op_av_apply: hc_op_setup has not been called");
+  if ((const double *)ya == a || (const double *)ya == v || (const double *)yv == a || (const double *)yv == v ||
+      ya == yv)
+    return fail(HC_EINVAL, "hc_op_av_apply: outputs must not alias the inputs or each other");
+  int rc = op->ensure_ws(1, false);
+  if (rc)
+    return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const hc::ApplyPlan & pl = op->plan64;
+  hc::Geom g0 = pl.g;
+  g0.pec = 0; // the mass term of s acts on every DOF (v has no essential nodes)
+  double * s = op->ws.r64;
+  double * k = op->ws.p64;
+  // s = a0 + G v
+  HC_TRY_HIP(hc::av_gather(pl.g, op->ag, a, v, s, st));
+  // ya = M s (mass term alone, no elimination), yv = G^T M s
+  HC_TRY_HIP(hc::apply_scaled<double>(g0, pl.segs, op->tb64, op->qd64, s, ya, op->ws.part64, 1.0, 0.0, pl.cand,
+                                      op->desc_dev, st));
+  HC_TRY_HIP(hc::grad_transpose(op->ag, ya, yv, st));
+  // k = dt K a0 on the free DOFs (the quadrature data hold dt nu); ya += k there, ya = a on PEC DOFs
+  HC_TRY_HIP(hc::apply_scaled<double>(pl.g, pl.segs, op->tb64, op->qd64, a, k, op->ws.part64, 0.0, 1.0, pl.cand,
+                                      op->desc_dev, st));
+  HC_TRY_HIP(hc::av_finish(pl.g, a, k, ya, st));
   return HC_OK;
 }
 

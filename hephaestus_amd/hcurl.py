@@ -39,6 +39,7 @@ EXPORTS = [
     "hc_op_destroy", "hc_last_error", "hc_fill_uniform", "hc_op_set_tile", "hc_op_get_tile",
     "hc_op_autotune", "hc_tile_list", "hc_op_set_rhs_block", "hc_batch_create", "hc_batch_add", "hc_batch_set_rhs_block",
     "hc_batch_solve", "hc_batch_destroy", "hc_op_rhs_coil", "hc_op_profile_apply", "hc_op_apply_curl", "hc_axpby", "hc_op_zero_ess",
+    "hc_op_nnodes", "hc_op_av_apply",
     "hc_server_request_bytes", "hc_server_create", "hc_server_set_tile", "hc_server_submit", "hc_server_poll",
     "hc_server_info", "hc_server_destroy",
 ]
@@ -135,6 +136,9 @@ def load():
     lib.hc_batch_destroy.argtypes = [vp]
     lib.hc_op_rhs_coil.argtypes = [vp, C.c_uint64, vp, vp]
     lib.hc_op_apply_curl.argtypes = [vp, vp, vp, vp]
+    lib.hc_op_nnodes.argtypes = [vp]
+    lib.hc_op_nnodes.restype = C.c_int64
+    lib.hc_op_av_apply.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.hc_axpby.argtypes = [C.c_int64, C.c_double, vp, C.c_double, vp, vp]
     lib.hc_op_zero_ess.argtypes = [vp, vp, vp]
     lib.hc_server_request_bytes.argtypes = [C.c_int64, C.c_int]
@@ -258,6 +262,28 @@ class Operator:
             y = torch.empty_like(x)
         _check(_lib.hc_op_apply_curl(self._h, _ptr(x), _ptr(y), _stream(stream)), "hc_op_apply_curl")
         return y
+
+    @property
+    def nnodes(self):
+        """H1 nodes of the mesh and order (the AV scalar-potential space)."""
+        return int(_lib.hc_op_nnodes(self._h))
+
+    def av_apply(self, a, v, ya=None, yv=None, stream=None):
+        """AV block operator: (ya, yv) = [[dt K + M, M G], [G^T M, G^T M G]] (a, v),
+        PEC DOFs of a eliminated; a of shape (N,), v of shape (nnodes,)."""
+        import torch
+        if _nrhs(a, self.ndofs, torch.float64) != 1:
+            raise ValueError("av_apply takes one vector a")
+        nn = self.nnodes
+        if v.dtype != torch.float64 or v.numel() != nn or not v.is_contiguous():
+            raise ValueError(f"v must be a contiguous float64 tensor of {nn} entries")
+        if ya is None:
+            ya = torch.empty_like(a)
+        if yv is None:
+            yv = torch.empty_like(v)
+        _check(_lib.hc_op_av_apply(self._h, _ptr(a), _ptr(v), _ptr(ya), _ptr(yv), _stream(stream)),
+               "hc_op_av_apply")
+        return ya, yv
 
     def apply_f32(self, x, y=None, stream=None):
         import torch

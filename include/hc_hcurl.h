@@ -172,6 +172,22 @@ int hc_op_apply_f32(hc_op_t op, const float * x, float * y, int nrhs, void * str
  * right-hand side term -K u_n of a backward-Euler step), applied from the
  * quadrature data of hc_op_setup (no second assembly). */
 int hc_op_apply_curl(hc_op_t op, const double * x, double * y, void * stream);
+/* H1 nodes of the operator's mesh and order: (n0 p + 1)(n1 p + 1)(n2 p + 1),
+ * x fastest (the scalar-potential space of the AV block operator). */
+int64_t hc_op_nnodes(hc_op_t op);
+/* Block operator of the AV formulation (AVEquationSystem::AddKernels,
+ * src/formulations/AV/av_formulation.cpp:150-196) on the operator's mesh,
+ * coefficients and dt.  Unknowns: a = dA/dt on the ND DOFs, v = V on the
+ * H1 nodes (hc_op_nnodes).  With G the discrete gradient (H1 -> ND) and a0 = a
+ * zeroed on the PEC DOFs:
+ *   ya = (dt K_nu + M_sigma) a0 + M_sigma G v   CurlCurl + VectorFEMass +
+ *        MixedVectorGradient (sigma grad V, u') (mixed_vector_gradient_kernel.hpp:8)
+ *   yv = G^T M_sigma a0 + G^T M_sigma G v       VectorFEWeakDivergence (sigma u,
+ *        grad V') (vector_fe_weak_divergence_kernel.hpp:8) + Diffusion
+ *        (sigma grad V, grad V') (diffusion_kernel.hpp:8)
+ * and ya = a on the PEC DOFs (hc_op_apply's elimination); v has no essential
+ * nodes.  Outputs must not alias the inputs.  Device pointers, FP64. */
+int hc_op_av_apply(hc_op_t op, const double * a, const double * v, double * ya, double * yv, void * stream);
 /* Right-hand sides per work item of the multi-RHS apply and solve (0 =
  * automatic: one per work item; a larger group holds the tile's quadrature
  * data in registers across its right-hand sides).  Results do not depend on

@@ -281,3 +281,82 @@ def two_region_coef(n, inner, outer, lo=0.25, hi=0.75):
     Z, Y, X = np.meshgrid(ins, ins, ins, indexing="ij")
     m = (X & Y & Z).ravel()
     return np.where(m, inner, outer).astype(np.float64)
+
+
+# ---------------------------------------------------------------------------
+# AV formulation blocks (src/formulations/AV/av_formulation.cpp:150-196): the
+# A rows hold CurlCurl (dt alpha), VectorFEMass (beta) and MixedVectorGradient
+# (sigma grad V, u') (src/kernels/mixed_vector_gradient_kernel.hpp:8); the V
+# rows VectorFEWeakDivergence (sigma u, grad V')
+# (src/kernels/vector_fe_weak_divergence_kernel.hpp:8) and Diffusion
+# (sigma grad V, grad V') (src/kernels/diffusion_kernel.hpp:8).  V lives on the
+# continuous Q_p space of the closed (Gauss-Lobatto) points, one node per point
+# of the (n0 p + 1)(n1 p + 1)(n2 p + 1) grid, x fastest.  Its gradients lie in
+# the ND_p space, so every block is a product of the ND mass matrix M_sigma, the
+# curl-curl matrix and the discrete gradient G.
+# ---------------------------------------------------------------------------
+def _lagrange_derivatives(nodes, x):
+    """D[i, m] = l_m'(x_i) for the Lagrange basis l_m on `nodes`:
+    l_m'(x) = l_m(x) sum_{k != m} 1 / (x - x_k), evaluated by the product rule
+    term by term (no division by x - x_k, so x may coincide with a node)."""
+    n = len(nodes)
+    D = np.zeros((len(x), n))
+    for m in range(n):
+        den = np.prod([nodes[m] - nodes[k] for k in range(n) if k != m])
+        for i, xi in enumerate(x):
+            s = 0.0
+            for k in range(n):
+                if k == m:
+                    continue
+                s += np.prod([xi - nodes[j] for j in range(n) if j not in (m, k)])
+            D[i, m] = s / den
+    return D
+
+
+def node_dims(m: Mesh, p):
+    return (m.nx * p + 1, m.ny * p + 1, m.nz * p + 1)
+
+
+def grad_matrix(m: Mesh, p):
+    """Discrete gradient G (H1 nodes -> ND DOFs) as scipy CSR.  The ND DOF of
+    component c at open point i of element e along c (closed indices j, k on
+    the other axes) is the tangential derivative in reference coordinates,
+    sum_m l_m'(xi_i) phi(node e p + m along c, j, k): exact on Q_p fields."""
+    import scipy.sparse as sp
+    D = _lagrange_derivatives(gauss_lobatto(p + 1), gauss_legendre(p)[0])  # [open i][closed m]
+    n = (m.nx, m.ny, m.nz)
+    M = node_dims(m, p)
+    rows, cols, vals = [], [], []
+    off = 0
+    for c in range(3):
+        dims = [n[a] * p if a == c else n[a] * p + 1 for a in range(3)]
+        k = np.arange(dims[0] * dims[1] * dims[2])
+        idx = [k % dims[0], (k // dims[0]) % dims[1], k // (dims[0] * dims[1])]
+        e, i = idx[c] // p, idx[c] % p
+        for mm in range(p + 1):
+            node = list(idx)
+            node[c] = e * p + mm
+            rows.append(off + k)
+            cols.append((node[2] * M[1] + node[1]) * M[0] + node[0])
+            vals.append(D[i, mm])
+        off += k.size
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(off, M[0] * M[1] * M[2]))
+
+
+def av_apply(m: Mesh, p, nu, sigma, dt, a, v):
+    """AV block operator on (a = dA/dt on the ND DOFs, v = V on the nodes):
+        ya = (dt K_nu + M_sigma) a0 + M_sigma G v,   ya = a on the PEC DOFs,
+        yv = G^T M_sigma a0 + G^T M_sigma G v,
+    a0 = a zeroed on the PEC DOFs (the elimination of
+    src/equation_systems/equation_system.cpp:132-190); v has no essential nodes."""
+    ess = ess_mask(m, p).astype(bool)
+    G = grad_matrix(m, p)
+    a = np.asarray(a, dtype=np.float64)
+    a0 = np.where(ess, 0.0, a)
+    s = a0 + G @ v
+    Ms = pa_apply_raw(m, p, pa_setup(m, p, 0.0, sigma), s)
+    Ka = pa_apply_raw(m, p, pa_setup(m, p, dt * elem_coef(m, nu), 0.0), a0)
+    ya = np.where(ess, a, Ka + Ms)
+    yv = G.T @ Ms
+    return ya, yv

@@ -1,0 +1,102 @@
+"""AV formulation block operator on the GPU (needs an MI355X), through the C ABI
+(hc_op_av_apply): against the oracle's av_apply on small box and trilinear
+meshes, orders 1..3, PEC on all faces and on a subset (<= 1e-13 relative), and
+at 48^3 order 3 (9,082,800 ND DOFs, 912,673 nodes) by its gauge null space
+(a, v) = (-G phi, phi) without PEC and its symmetry with PEC.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import hephaestus_amd as H
+    H.load()
+    from oracle import oracle as O
+    O.build()
+    return torch, H, O
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+CASES = [(1, (5, 4, 3), 0x3F, False), (2, (4, 3, 5), 0x3F, True), (3, (3, 4, 3), 0x3F, False),
+         (2, (4, 4, 3), 0x19, False), (3, (3, 3, 4), 0x26, True), (1, (6, 3, 4), 0, True)]
+
+
+@pytest.mark.parametrize("p,n,pec,tri", CASES)
+def test_av_apply_vs_oracle(env, p, n, pec, tri):
+    torch, H, O = env
+    print(f"case av p={p} n={n} pec={pec:#x} trilinear={tri}", flush=True)
+    rng = np.random.default_rng(900 + p + pec)
+    m0 = O.Mesh(*n)
+    vert = None
+    if tri:
+        vert = m0.box_vertices() + 0.04 * rng.standard_normal(((n[0] + 1) * (n[1] + 1) * (n[2] + 1), 3)) / max(n)
+    m = O.Mesh(*n, vert=vert, pec_faces=pec)
+    E = m.nelem
+    nu, sig = rng.uniform(0.5, 2.0, E), 10.0 ** rng.uniform(-3, 3, E)
+    dt = 0.37
+    a = rng.standard_normal(O.ndofs(m, p))
+    v = rng.standard_normal(int(np.prod(O.node_dims(m, p))))
+    ya_ref, yv_ref = O.av_apply(m, p, nu, sig, dt, a, v)
+    op = H.Operator(n, p, nu, sig, pec_faces=pec, vert=vert).setup(dt)
+    assert op.nnodes == v.size
+    ya, yv = op.av_apply(torch.from_numpy(a).cuda(), torch.from_numpy(v).cuda())
+    assert _rel(ya.cpu().numpy(), ya_ref) <= 1e-13
+    assert _rel(yv.cpu().numpy(), yv_ref) <= 1e-13
+
+
+def _gll(p):
+    inner = np.polynomial.legendre.Legendre.basis(p).deriv().roots()
+    return np.concatenate([[0.0], np.sort((inner + 1.0) / 2.0), [1.0]])
+
+
+def test_av_apply_full_size_properties(env):
+    """48^3 order 3.  Gauge null space without PEC: phi = x + 2y - z at the
+    nodes has G phi = (h, 2h, -h) on the x, y, z DOFs (DOF = h times the
+    tangential component), so (a, v) = (-G phi, phi) maps to zero.  Symmetry
+    with PEC on all faces on PEC-consistent random vectors."""
+    torch, H, O = env
+    n, p = 48, 3
+    print(f"case av full size n={n} p={p}", flush=True)
+    sig = O.two_region_coef(n, 1e4, 1e-6)
+    h = 1.0 / n
+    op = H.Operator(n, p, 1.0, sig, pec_faces=0).setup(1.0)
+    N, NN = op.ndofs, op.nnodes
+    assert NN == (n * p + 1) ** 3
+    e = np.minimum(np.arange(n * p + 1) // p, n - 1)
+    xs = (e + _gll(p)[np.arange(n * p + 1) - e * p]) * h
+    Z, Y, X = np.meshgrid(xs, xs, xs, indexing="ij")
+    phi = torch.from_numpy((X + 2.0 * Y - Z).ravel()).cuda()
+    nc = N // 3   # the three components have equal counts on a cube
+    a = torch.empty(N, dtype=torch.float64, device="cuda")
+    a[:nc] = -h
+    a[nc:2 * nc] = -2.0 * h
+    a[2 * nc:] = h
+    ya, yv = op.av_apply(a, phi)
+    r = torch.empty_like(a)
+    op.fill_uniform(9, r)
+    w = torch.empty_like(phi)
+    w.uniform_(-1.0, 1.0, generator=torch.Generator(device="cuda").manual_seed(4))
+    za, zv = op.av_apply(r, w)
+    assert float(torch.linalg.norm(ya)) <= 1e-12 * float(torch.linalg.norm(za))
+    assert float(torch.linalg.norm(yv)) <= 1e-12 * float(torch.linalg.norm(zv))
+    del op
+    op = H.Operator(n, p, 1.0, sig).setup(1.0)
+    a1, a2 = torch.empty(N, dtype=torch.float64, device="cuda"), torch.empty(N, dtype=torch.float64, device="cuda")
+    op.fill_uniform(11, a1)
+    op.fill_uniform(12, a2)
+    v2 = torch.empty_like(phi)
+    v2.uniform_(-1.0, 1.0, generator=torch.Generator(device="cuda").manual_seed(5))
+    ya1, yv1 = op.av_apply(a1, w)
+    ya2, yv2 = op.av_apply(a2, v2)
+    lhs = float(torch.dot(a1, ya2) + torch.dot(w, yv2))
+    rhs = float(torch.dot(ya1, a2) + torch.dot(yv1, v2))
+    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + abs(rhs))
