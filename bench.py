@@ -440,6 +440,19 @@ def main():
         "fp32_apply": {"ms": round(ms32, 5), "GDOF_s": round(N / ms32 / 1e6, 3),
                        "GB_s": round(B32 / ms32 / 1e6, 1)},
     }
+    # AV formulation block operator on the same mesh: (a, V) with V on the H1 nodes
+    NN = op.nnodes
+    v = torch.empty(NN, dtype=torch.float64, device="cuda")
+    v.uniform_(-1.0, 1.0, generator=torch.Generator(device="cuda").manual_seed(6))
+    ya, yv = torch.empty_like(x), torch.empty_like(v)
+    ms_av = time_applies(torch, lambda: op.av_apply(x, v, ya, yv), args.steps, args.warmup)
+    out["av_apply"] = {"workload": f"AV block operator (dt K + M, M G; G^T M, G^T M G), {n}^3 order {p}, "
+                                   f"{N} ND DOFs + {NN} H1 nodes",
+                       "ms": round(ms_av, 5), "GDOF_s": round((N + NN) / ms_av / 1e6, 3),
+                       "kernels": "gather a0 + G v, mass-only tile apply, G^T, curl-only tile apply, finish"}
+    print(f"[bench] AV block apply {ms_av * 1e3:.1f} us  {(N + NN) / ms_av / 1e6:.1f} GDOF/s",
+          file=sys.stderr, flush=True)
+    del v, ya, yv
     if not args.no_solve:
         out["solves"] = {"single_48_p3": solves_single_48(torch, H, op, x, args),
                          "multi_rhs_32_p2": solves_multi_rhs_32(torch, H, args)}
