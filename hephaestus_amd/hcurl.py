@@ -39,7 +39,7 @@ EXPORTS = [
     "hc_op_destroy", "hc_last_error", "hc_fill_uniform", "hc_op_set_tile", "hc_op_get_tile",
     "hc_op_autotune", "hc_tile_list", "hc_op_set_rhs_block", "hc_batch_create", "hc_batch_add", "hc_batch_set_rhs_block",
     "hc_batch_solve", "hc_batch_destroy", "hc_op_rhs_coil", "hc_op_profile_apply", "hc_op_apply_curl", "hc_axpby", "hc_op_zero_ess",
-    "hc_op_nnodes", "hc_op_av_apply",
+    "hc_op_nnodes", "hc_op_av_apply", "hc_op_av_solve",
     "hc_server_request_bytes", "hc_server_create", "hc_server_set_tile", "hc_server_submit", "hc_server_poll",
     "hc_server_info", "hc_server_destroy",
 ]
@@ -139,6 +139,7 @@ def load():
     lib.hc_op_nnodes.argtypes = [vp]
     lib.hc_op_nnodes.restype = C.c_int64
     lib.hc_op_av_apply.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.hc_op_av_solve.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_int, C.POINTER(SolveStats), vp]
     lib.hc_axpby.argtypes = [C.c_int64, C.c_double, vp, C.c_double, vp, vp]
     lib.hc_op_zero_ess.argtypes = [vp, vp, vp]
     lib.hc_server_request_bytes.argtypes = [C.c_int64, C.c_int]
@@ -284,6 +285,22 @@ class Operator:
         _check(_lib.hc_op_av_apply(self._h, _ptr(a), _ptr(v), _ptr(ya), _ptr(yv), _stream(stream)),
                "hc_op_av_apply")
         return ya, yv
+
+    def av_solve(self, ba, bv, a, v, rtol=1e-8, maxit=10000, stream=None, raise_on_fail=True):
+        """PCG on the AV block system; (a, v) hold the initial guess and receive
+        the solution."""
+        import torch
+        for t, n in ((ba, self.ndofs), (a, self.ndofs), (bv, self.nnodes), (v, self.nnodes)):
+            if t.dtype != torch.float64 or t.numel() != n or not t.is_contiguous():
+                raise ValueError("av_solve: contiguous float64 vectors of ndofs / nnodes entries")
+        st = SolveStats()
+        rc = _lib.hc_op_av_solve(self._h, _ptr(ba), _ptr(bv), _ptr(a), _ptr(v), float(rtol), int(maxit),
+                                 C.byref(st), _stream(stream))
+        if rc != HC_OK and (raise_on_fail or rc not in (HC_ENOTCONV, HC_ENAN)):
+            _check(rc, "hc_op_av_solve")
+        d = st.as_dict()
+        d["rc"] = rc
+        return d
 
     def apply_f32(self, x, y=None, stream=None):
         import torch

@@ -188,6 +188,17 @@ int64_t hc_op_nnodes(hc_op_t op);
  * and ya = a on the PEC DOFs (hc_op_apply's elimination); v has no essential
  * nodes.  Outputs must not alias the inputs.  Device pointers, FP64. */
 int hc_op_av_apply(hc_op_t op, const double * a, const double * v, double * ya, double * yv, void * stream);
+/* Solve of the AV block system (the reference: HypreGMRES + BoomerAMG on the
+ * monolithic matrix, src/problem_builders/problem_builder_base.cpp:206-218,
+ * for AVFormulation): the block operator of hc_op_av_apply is symmetric
+ * positive semi-definite (zero on the gauge fields (-G phi, phi)), so this is
+ * PCG with block-Jacobi preconditioning (diag(A)^-1 on the ND DOFs, 1 on PEC
+ * DOFs; diag(G^T M G)^-1 on the nodes).  (a, v) hold the initial guess and
+ * receive the solution; its gauge component stays that of the guess.  Stops
+ * when ||b - AV x|| <= rtol ||b|| (FP64, both blocks); stats->rel_res is the
+ * true residual of the result.  Allocates its work vectors per call. */
+int hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a, double * v, double rtol,
+                   int maxit, hc_solve_stats * stats, void * stream);
 /* Right-hand sides per work item of the multi-RHS apply and solve (0 =
  * automatic: one per work item; a larger group holds the tile's quadrature
  * data in registers across its right-hand sides).  Results do not depend on

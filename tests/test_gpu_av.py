@@ -100,3 +100,42 @@ def test_av_apply_full_size_properties(env):
     lhs = float(torch.dot(a1, ya2) + torch.dot(w, yv2))
     rhs = float(torch.dot(ya1, a2) + torch.dot(yv1, v2))
     assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + abs(rhs))
+
+
+@pytest.mark.parametrize("p,n,pec", [(1, (6, 5, 4), 0x3F), (2, (4, 4, 3), 0x3F), (3, (3, 3, 3), 0x3F),
+                                     (2, (4, 3, 4), 0x19)])
+def test_av_solve_consistent_rhs(env, p, n, pec):
+    """b = AV x_true; PCG from zero converges to rtol in the true residual
+    (recomputed by the oracle's av_apply), and the gauge-invariant parts of
+    the solution equal those of x_true: s = a0 + G v (the field sigma acts on)
+    and a on the PEC DOFs."""
+    torch, H, O = env
+    print(f"case av solve p={p} n={n} pec={pec:#x}", flush=True)
+    rng = np.random.default_rng(1200 + p + pec)
+    m = O.Mesh(*n, pec_faces=pec)
+    E = m.nelem
+    nu, sig = rng.uniform(0.5, 2.0, E), 10.0 ** rng.uniform(-1, 1, E)
+    dt = 0.5
+    a_t = rng.standard_normal(O.ndofs(m, p))
+    v_t = rng.standard_normal(int(np.prod(O.node_dims(m, p))))
+    ba, bv = O.av_apply(m, p, nu, sig, dt, a_t, v_t)
+    op = H.Operator(n, p, nu, sig, pec_faces=pec).setup(dt)
+    a = torch.zeros(a_t.size, dtype=torch.float64, device="cuda")
+    v = torch.zeros(v_t.size, dtype=torch.float64, device="cuda")
+    rtol = 1e-10
+    st = op.av_solve(torch.from_numpy(ba).cuda(), torch.from_numpy(bv).cuda(), a, v, rtol=rtol, maxit=20000)
+    assert st["converged"] and st["rel_res"] <= rtol * 1.01
+    ah, vh = a.cpu().numpy(), v.cpu().numpy()
+    ra, rv = O.av_apply(m, p, nu, sig, dt, ah, vh)
+    res = np.sqrt(np.sum((ba - ra) ** 2) + np.sum((bv - rv) ** 2)) / np.sqrt(np.sum(ba ** 2) + np.sum(bv ** 2))
+    assert res <= rtol * 1.05
+    ess = O.ess_mask(m, p).astype(bool)
+    G = O.grad_matrix(m, p)
+    s_t = np.where(ess, 0.0, a_t) + G @ v_t
+    s = np.where(ess, 0.0, ah) + G @ vh
+    assert _rel(s, s_t) <= 1e-6
+    assert np.abs(ah[ess] - a_t[ess]).max() <= 1e-9 * np.abs(a_t).max()
+    # a second run from the same start is bitwise identical
+    a2, v2 = torch.zeros_like(a), torch.zeros_like(v)
+    st2 = op.av_solve(torch.from_numpy(ba).cuda(), torch.from_numpy(bv).cuda(), a2, v2, rtol=rtol, maxit=20000)
+    assert st2["iterations"] == st["iterations"] and torch.equal(a2, a) and torch.equal(v2, v)
