@@ -452,6 +452,18 @@ def main():
                        "kernels": "gather a0 + G v, mass-only tile apply, G^T, curl-only tile apply, finish"}
     print(f"[bench] AV block apply {ms_av * 1e3:.1f} us  {(N + NN) / ms_av / 1e6:.1f} GDOF/s",
           file=sys.stderr, flush=True)
+    # AV block solve on a consistent right-hand side b = AV (a_t, v_t), random state
+    if not args.no_solve:
+        op.av_apply(x, v, ya, yv)
+        a0, v0 = torch.zeros_like(x), torch.zeros_like(v)
+        st = op.av_solve(ya, yv, a0, v0, rtol=1e-8, maxit=5000, raise_on_fail=False)
+        out["av_solve"] = {"workload": f"AV block system, {n}^3 order {p}, b = AV (a_t, v_t) with a_t the "
+                                       f"uniform-[-1,1) ND vector (seed 2), v_t uniform nodes (seed 6); "
+                                       f"PCG with block-Jacobi, rtol 1e-8, from zero",
+                           "iterations": st["iterations"], "converged": bool(st["converged"]),
+                           "true_rel_res": st["rel_res"], "solve_ms": round(st["solve_ms"], 3)}
+        print(f"[bench] AV block solve: {out['av_solve']}", file=sys.stderr, flush=True)
+        del a0, v0
     del v, ya, yv
     if not args.no_solve:
         out["solves"] = {"single_48_p3": solves_single_48(torch, H, op, x, args),
