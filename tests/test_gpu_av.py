@@ -1,7 +1,7 @@
 """AV formulation block operator on the GPU (needs an MI355X), through the C ABI
 (hc_op_av_apply): against the oracle's av_apply on small box and trilinear
 meshes, orders 1..3, PEC on all faces and on a subset (<= 1e-13 relative), and
-at 48^3 order 3 (9,082,800 ND DOFs, 912,673 nodes) by its gauge null space
+at 48^3 order 3 (9,082,800 ND DOFs, 3,048,625 nodes) by its gauge null space
 (a, v) = (-G phi, phi) without PEC and its symmetry with PEC.
 """
 import numpy as np
