@@ -230,23 +230,22 @@ k_av_gather(Geom g, AuxGeom a, const double * __restrict__ av, const double * __
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.off[3];
          i += (int64_t)gridDim.x * blockDim.x)
     {
-      int c;
-      int64_t i0, i1, i2;
-      dof_coords(g, i, c, i0, i1, i2);
+      int c, i0, i1, i2;
+      aux_dof_coords(a, i, c, i0, i1, i2);
       const double a0 = is_ess(g, c, i0, i1, i2) ? 0.0 : av[i];
-      s[i] = a0 + aux_g<P>(a, tab, v, i);
+      s[i] = a0 + aux_g_at<P>(a, tab, v, c, i0, i1, i2);
     }
   });
 }
 
 __global__ void
-k_av_finish(Geom g, const double * __restrict__ av, const double * __restrict__ k, double * __restrict__ ya)
+k_av_finish(Geom g, AuxGeom a, const double * __restrict__ av, const double * __restrict__ k,
+            double * __restrict__ ya)
 {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.off[3]; i += (int64_t)gridDim.x * blockDim.x)
   {
-    int c;
-    int64_t i0, i1, i2;
-    dof_coords(g, i, c, i0, i1, i2);
+    int c, i0, i1, i2;
+    aux_dof_coords(a, i, c, i0, i1, i2);
     ya[i] = is_ess(g, c, i0, i1, i2) ? av[i] : ya[i] + k[i];
   }
 }
@@ -273,9 +272,9 @@ av_gather(const Geom & g, const AuxGeom & a, const double * av, const double * v
 }
 
 hipError_t
-av_finish(const Geom & g, const double * av, const double * k, double * ya, hipStream_t st)
+av_finish(const Geom & g, const AuxGeom & a, const double * av, const double * k, double * ya, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_av_finish, dim3(4096), dim3(256), 0, st, g, av, k, ya);
+  hipLaunchKernelGGL(k_av_finish, dim3(4096), dim3(256), 0, st, g, a, av, k, ya);
   return hipGetLastError();
 }
 

@@ -175,16 +175,23 @@ struct NodeWalk
   }
 };
 
-// (G t) at ND DOF i, order P.
-template <int P, typename T>
-__device__ inline double
-aux_g(const AuxGeom & a, const DgTab<P> & tab, const T * __restrict__ t, int64_t i)
+// Component and grid indices of ND DOF i (divisions by the component grid
+// sizes through fast_divmod).
+__device__ inline void
+aux_dof_coords(const AuxGeom & a, int64_t i, int & c, int & i0, int & i1, int & i2)
 {
-  const int c = (i < a.off[1]) ? 0 : ((i < a.off[2]) ? 1 : 2);
+  c = (i < a.off[1]) ? 0 : ((i < a.off[2]) ? 1 : 2);
   const int li = (int)(i - a.off[c]);
-  int i0, rest, i1, i2;
+  int rest;
   fast_divmod(li, a.cd[c][0], a.rcd[c][0], rest, i0);
   fast_divmod(rest, a.cd[c][1], a.rcd[c][1], i2, i1);
+}
+
+// (G t) at the ND DOF of component c at (i0, i1, i2), order P.
+template <int P, typename T>
+__device__ inline double
+aux_g_at(const AuxGeom & a, const DgTab<P> & tab, const T * __restrict__ t, int c, int i0, int i1, int i2)
+{
   const int ic = (c == 0) ? i0 : ((c == 1) ? i1 : i2);
   const int e = ic / P, l = ic % P;
   int node[3] = {i0, i1, i2};
@@ -205,6 +212,16 @@ aux_g(const AuxGeom & a, const DgTab<P> & tab, const T * __restrict__ t, int64_t
   for (int m = 0; m <= P; m++)
     v += row[m] * (double)tl[(int64_t)m * stride];
   return v;
+}
+
+// (G t) at ND DOF i, order P.
+template <int P, typename T>
+__device__ inline double
+aux_g(const AuxGeom & a, const DgTab<P> & tab, const T * __restrict__ t, int64_t i)
+{
+  int c, i0, i1, i2;
+  aux_dof_coords(a, i, c, i0, i1, i2);
+  return aux_g_at<P>(a, tab, t, c, i0, i1, i2);
 }
 
 // Runs f with the order of a as a compile-time constant (orders 1..PMAX).
@@ -250,7 +267,8 @@ constexpr double FP32_MIN_TERM_RATIO = 6.0e-6;
 hipError_t av_gather(const Geom & g, const AuxGeom & a, const double * av, const double * v, double * s,
                      hipStream_t st);
 // ya = ya + k on the free DOFs, ya = a on the PEC DOFs of g.
-hipError_t av_finish(const Geom & g, const double * av, const double * k, double * ya, hipStream_t st);
+hipError_t av_finish(const Geom & g, const AuxGeom & a, const double * av, const double * k, double * ya,
+                     hipStream_t st);
 // yv = G^T w at every node (no essential nodes).
 hipError_t grad_transpose(const AuxGeom & a, const double * w, double * yv, hipStream_t st);
 // z = D^-1 r + G Dg^-1 G^T r for one vector (t: node scratch, NN entries).

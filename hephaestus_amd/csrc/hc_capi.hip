@@ -523,7 +523,7 @@ hc_op_av_apply(hc_op_t op, const double * a, const double * v, double * ya, doub
   // k = dt K a0 on the free DOFs (the quadrature data hold dt nu); ya += k there, ya = a on PEC DOFs
   HC_TRY_HIP(hc::apply_scaled<double>(pl.g, pl.segs, op->tb64, op->qd64, a, k, op->ws.part64, 0.0, 1.0, pl.cand,
                                       op->desc_dev, st));
-  HC_TRY_HIP(hc::av_finish(pl.g, a, k, ya, st));
+  HC_TRY_HIP(hc::av_finish(pl.g, op->ag, a, k, ya, st));
   return HC_OK;
 }
 
