@@ -365,7 +365,7 @@ def main():
                     help="also run a 1 GiB device copy (reference kernel for the HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1",
                           "metric": METRIC, "n_gpus": args.gpus}))
         sys.exit(2)
 

@@ -131,7 +131,7 @@ void
 EquationSystem::FormLinearSystem(DeviceVector & X, DeviceVector & B)
 {
   if (_test_var_names.size() != 1)
-    throw Error("FormLinearSystem: this build handles one test variable (the H(curl) system)");
+    throw Error("FormLinearSystem: one test variable (the H(curl) system) is supported");
   const auto & name = _test_var_names[0];
   BilinearForm * blf = _blfs.Get(name);
   LinearForm * lf = _lfs.Get(name);
