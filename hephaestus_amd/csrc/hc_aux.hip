@@ -264,6 +264,25 @@ k_grad_transpose(AuxGeom a, const double * __restrict__ w, double * __restrict__
   });
 }
 
+__global__ void
+k_grad_apply(AuxGeom a, const double * __restrict__ v, double * __restrict__ y)
+{
+  with_order(a, [&](auto pc) {
+    constexpr int P = decltype(pc)::value;
+    const DgTab<P> tab = load_dg<P>(a);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.off[3];
+         i += (int64_t)gridDim.x * blockDim.x)
+      y[i] = aux_g<P>(a, tab, v, i);
+  });
+}
+
+hipError_t
+grad_apply(const AuxGeom & a, const double * v, double * y, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_grad_apply, dim3(4096), dim3(256), 0, st, a, v, y);
+  return hipGetLastError();
+}
+
 hipError_t
 av_gather(const Geom & g, const AuxGeom & a, const double * av, const double * v, double * s, hipStream_t st)
 {

@@ -269,6 +269,8 @@ hipError_t av_gather(const Geom & g, const AuxGeom & a, const double * av, const
 // ya = ya + k on the free DOFs, ya = a on the PEC DOFs of g.
 hipError_t av_finish(const Geom & g, const AuxGeom & a, const double * av, const double * k, double * ya,
                      hipStream_t st);
+// y = G v on every ND DOF.
+hipError_t grad_apply(const AuxGeom & a, const double * v, double * y, hipStream_t st);
 // yv = G^T w at every node (no essential nodes).
 hipError_t grad_transpose(const AuxGeom & a, const double * w, double * yv, hipStream_t st);
 // z = D^-1 r + G Dg^-1 G^T r for one vector (t: node scratch, NN entries).

@@ -177,10 +177,199 @@ read_partials(double * dev, double * host, int nsums, hipStream_t st)
   return HC_OK;
 }
 
+__global__ void
+k_fill_value(int64_t n, double v, double * __restrict__ y)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = v;
+}
+
+// Preconditioned CG on one vector of n entries: apply(p, q) computes q = S p,
+// dinv the diagonal preconditioner; x holds the initial guess.  Stops when
+// ||r|| <= rtol ||b|| (recursive residual).  r, z, p, q: n-entry work vectors;
+// part: 2 * AV_BLOCKS device doubles; h: as many host doubles.
+template <typename Apply>
+static int
+pcg_one(int64_t n, Apply && apply, const double * dinv, const double * b, double * x, double rtol, int maxit,
+        double * r, double * z, double * p, double * q, double * part, double * h, hipStream_t st, int & iters,
+        bool & conv, int & status)
+{
+  const dim3 G(AV_BLOCKS), T(AV_THREADS);
+  iters = 0;
+  status = HC_OK;
+  int rc = apply(x, q);
+  if (rc)
+    return rc;
+  hipLaunchKernelGGL(k_av_dot, G, T, 0, st, n, b, b, part);
+  if ((rc = read_partials(part, h, 1, st)))
+    return fail(rc, "PCG: reduction failed");
+  const double bb = host_sum(h, AV_BLOCKS);
+  hipLaunchKernelGGL(k_av_start, G, T, 0, st, n, b, (const double *)q, dinv, r, z, p, part);
+  if ((rc = read_partials(part, h, 2, st)))
+    return fail(rc, "PCG: reduction failed");
+  double gam = host_sum(h, AV_BLOCKS), rr = host_sum(h + AV_BLOCKS, AV_BLOCKS);
+  const double tol2 = rtol * rtol * bb;
+  conv = bb == 0.0 || rr <= tol2;
+  while (!conv && iters < maxit)
+  {
+    if ((rc = apply(p, q)))
+      return rc;
+    hipLaunchKernelGGL(k_av_dot, G, T, 0, st, n, (const double *)p, (const double *)q, part);
+    if ((rc = read_partials(part, h, 1, st)))
+      return fail(rc, "PCG: reduction failed");
+    const double pq = host_sum(h, AV_BLOCKS);
+    if (!std::isfinite(pq) || !(pq > 0.0))
+    {
+      status = std::isfinite(pq) ? HC_ENOTCONV : HC_ENAN;
+      break;
+    }
+    hipLaunchKernelGGL(k_av_update, G, T, 0, st, n, gam / pq, (const double *)p, (const double *)q, dinv, x, r, z,
+                       part);
+    if ((rc = read_partials(part, h, 2, st)))
+      return fail(rc, "PCG: reduction failed");
+    const double gnew = host_sum(h, AV_BLOCKS);
+    rr = host_sum(h + AV_BLOCKS, AV_BLOCKS);
+    iters++;
+    if (!std::isfinite(gnew) || !std::isfinite(rr))
+    {
+      status = HC_ENAN;
+      break;
+    }
+    conv = rr <= tol2;
+    if (conv)
+      break;
+    hipLaunchKernelGGL(k_av_xpby, dim3(2048), dim3(256), 0, st, n, (const double *)z, gnew / gam, p);
+    gam = gnew;
+  }
+  if (status == HC_OK && !conv)
+    status = HC_ENOTCONV;
+  return HC_OK;
+}
+
 } // namespace hc
 
 using hc::AV_BLOCKS;
 using hc::AV_THREADS;
+
+// DivFreeSource::Apply (src/sources/div_free_source.cpp:71-114) for the
+// source linear form j = (J, phi_i):
+//   g = M^-1 j                 (H(curl) mass with unit coefficient, no
+//                               essential DOFs: div_free_source.cpp:76-89)
+//   g <- g - G Q,  (G^T M G) Q = G^T M g   (the Helmholtz projection
+//                               P(g) = g - grad Q, (grad.P(g), q) = 0 for all q
+//                               in H1, div_free_source.cpp:7-17,93-104, no
+//                               boundary conditions)
+//   lf += M g                  (div_free_source.cpp:107-112)
+// Both solves are Jacobi-preconditioned CG to rtol (the reference uses its
+// default GMRES, hephaestus_solvers.hpp); the projection's singular system
+// (constants) has a consistent right-hand side (G 1 = 0).  src receives the
+// projected source field g, Q (optional) the potential.
+extern "C" int
+hc_op_div_free_source(hc_op_t op, const double * j, double * src, double * lf, double * Q, int project,
+                      double rtol, int maxit, hc_solve_stats * stats, void * stream)
+{
+  if (!op || !j || !src || !lf)
+    return hc::fail(HC_EINVAL, "hc_op_div_free_source: null argument");
+  if (!(rtol > 0.0) || maxit < 1)
+    return hc::fail(HC_EINVAL, "hc_op_div_free_source: rtol must be > 0 and maxit >= 1");
+  int rc = op->ensure_ws(1, false);
+  if (rc)
+    return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t N = op->N, NN = op->ag.NN, E = op->E;
+  const int Qp = op->p + 1;
+  hc::AvBuffers buf;
+  double * qdu = buf.get((size_t)12 * E * Qp * Qp * Qp);
+  double *c0 = buf.get(E), *c1 = buf.get(E), *dM = buf.get(N);
+  double *r = buf.get(N), *z = buf.get(N), *p = buf.get(N), *q = buf.get(N), *t = buf.get(N);
+  double *rn = buf.get(NN), *zn = buf.get(NN), *pn = buf.get(NN), *qn = buf.get(NN), *dS = buf.get(NN),
+         *Qv = buf.get(NN), *bn = buf.get(NN);
+  double * part = buf.get(2 * AV_BLOCKS);
+  if (!qdu || !c0 || !c1 || !dM || !r || !z || !p || !q || !t || !rn || !zn || !pn || !qn || !dS || !Qv || !bn ||
+      !part)
+    return hc::fail(HC_ENOMEM, "hc_op_div_free_source: device allocation failed");
+  std::vector<double> h(2 * AV_BLOCKS);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return hc::fail(HC_EHIP, "hc_op_div_free_source: event creation failed");
+  struct Ev
+  {
+    hipEvent_t a, b;
+    ~Ev()
+    {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } ev{e0, e1};
+  (void)hipEventRecord(e0, st);
+  const hc::ApplyPlan & pl = op->plan64;
+  hc::Geom g0 = pl.g;
+  g0.pec = 0;
+  // unit-coefficient mass factors on the operator's geometry
+  hipLaunchKernelGGL(hc::k_fill_value, dim3(256), dim3(256), 0, st, E, 0.0, c0);
+  hipLaunchKernelGGL(hc::k_fill_value, dim3(256), dim3(256), 0, st, E, 1.0, c1);
+  HC_TRY_HIP(hc::pa_setup(op->g, op->tb64, op->d_vert, op->h, c0, c1, 1.0, qdu, st));
+  HC_TRY_HIP(hc::diagonal(g0, op->tb64, qdu, dM, st));
+  HC_TRY_HIP(hc::invert_diag(dM, dM, nullptr, N, st));
+  auto mass = [&](const double * x, double * y) -> int {
+    HC_TRY_HIP(hc::apply_scaled<double>(g0, pl.segs, op->tb64, qdu, x, y, op->ws.part64, 1.0, 0.0, pl.cand,
+                                        op->desc_dev, st));
+    return HC_OK;
+  };
+  int it1 = 0, it2 = 0, s1 = HC_OK, s2 = HC_OK;
+  bool c1ok = false, c2ok = true;
+  // g = M^-1 j
+  hipLaunchKernelGGL(hc::k_fill_value, dim3(2048), dim3(256), 0, st, N, 0.0, src);
+  if ((rc = hc::pcg_one(N, mass, dM, j, src, rtol, maxit, r, z, p, q, part, h.data(), st, it1, c1ok, s1)))
+    return rc;
+  hipLaunchKernelGGL(hc::k_fill_value, dim3(2048), dim3(256), 0, st, NN, 0.0, Qv);
+  if (project && s1 == HC_OK)
+  {
+    // b_n = G^T M g; S = G^T M G on the nodes, diag(S)^-1 from the unit mass factors
+    if ((rc = mass(src, t)))
+      return rc;
+    HC_TRY_HIP(hc::grad_transpose(op->ag, t, bn, st));
+    HC_TRY_HIP(hc::aux_diag_inv(g0, op->ag, op->tb64, qdu, dS, dS, nullptr, st));
+    auto stiff = [&](const double * x, double * y) -> int {
+      HC_TRY_HIP(hc::grad_apply(op->ag, x, t, st));
+      int e = mass(t, q);
+      if (e)
+        return e;
+      HC_TRY_HIP(hc::grad_transpose(op->ag, q, y, st));
+      return HC_OK;
+    };
+    // the stiffness apply uses t and q: the node PCG's own vectors are rn..qn
+    if ((rc = hc::pcg_one(NN, stiff, dS, bn, Qv, rtol, maxit, rn, zn, pn, qn, part, h.data(), st, it2, c2ok, s2)))
+      return rc;
+    // g <- g - G Q
+    HC_TRY_HIP(hc::grad_apply(op->ag, Qv, t, st));
+    HC_TRY_HIP(hc::axpby(N, -1.0, t, 1.0, src, st));
+  }
+  // lf += M g
+  if ((rc = mass(src, t)))
+    return rc;
+  HC_TRY_HIP(hc::axpby(N, 1.0, t, 1.0, lf, st));
+  if (Q)
+    HC_TRY_HIP(hipMemcpyAsync(Q, Qv, sizeof(double) * NN, hipMemcpyDeviceToDevice, st));
+  (void)hipEventRecord(e1, st);
+  HC_TRY_HIP(hipStreamSynchronize(st));
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  const int status = s1 != HC_OK ? s1 : s2;
+  if (stats)
+  {
+    *stats = hc_solve_stats{};
+    stats->iterations = it1 + it2;
+    stats->outer_iterations = project ? 2 : 1;
+    stats->converged = (c1ok && c2ok && status == HC_OK) ? 1 : 0;
+    stats->status = status;
+    stats->solve_ms = ms;
+  }
+  if (status != HC_OK)
+    return hc::fail(status, status == HC_ENAN ? "hc_op_div_free_source: NaN/Inf in a reduction"
+                                              : "hc_op_div_free_source: tolerance not reached in maxit");
+  return HC_OK;
+}
 
 extern "C" int
 hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a, double * v, double rtol, int maxit,

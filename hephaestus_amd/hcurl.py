@@ -39,7 +39,7 @@ EXPORTS = [
     "hc_op_destroy", "hc_last_error", "hc_fill_uniform", "hc_op_set_tile", "hc_op_get_tile",
     "hc_op_autotune", "hc_tile_list", "hc_op_set_rhs_block", "hc_batch_create", "hc_batch_add", "hc_batch_set_rhs_block",
     "hc_batch_solve", "hc_batch_destroy", "hc_op_rhs_coil", "hc_op_profile_apply", "hc_op_apply_curl", "hc_axpby", "hc_op_zero_ess",
-    "hc_op_nnodes", "hc_op_av_apply", "hc_op_av_solve",
+    "hc_op_nnodes", "hc_op_av_apply", "hc_op_av_solve", "hc_op_div_free_source",
     "hc_server_request_bytes", "hc_server_create", "hc_server_set_tile", "hc_server_submit", "hc_server_poll",
     "hc_server_info", "hc_server_destroy",
 ]
@@ -140,6 +140,8 @@ def load():
     lib.hc_op_nnodes.restype = C.c_int64
     lib.hc_op_av_apply.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.hc_op_av_solve.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_int, C.POINTER(SolveStats), vp]
+    lib.hc_op_div_free_source.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_double, C.c_int,
+                                          C.POINTER(SolveStats), vp]
     lib.hc_axpby.argtypes = [C.c_int64, C.c_double, vp, C.c_double, vp, vp]
     lib.hc_op_zero_ess.argtypes = [vp, vp, vp]
     lib.hc_server_request_bytes.argtypes = [C.c_int64, C.c_int]
@@ -301,6 +303,20 @@ class Operator:
         d = st.as_dict()
         d["rc"] = rc
         return d
+
+    def div_free_source(self, j, lf, project=True, rtol=1e-10, maxit=20000, stream=None):
+        """DivFreeSource::Apply: g = M^-1 j (unit mass), optional Helmholtz
+        projection g - grad Q, lf += M g.  Returns (g, Q, stats)."""
+        import torch
+        _nrhs(j, self.ndofs, torch.float64)
+        _nrhs(lf, self.ndofs, torch.float64)
+        g = torch.empty_like(j)
+        Q = torch.empty(self.nnodes, dtype=torch.float64, device=j.device)
+        st = SolveStats()
+        _check(_lib.hc_op_div_free_source(self._h, _ptr(j), _ptr(g), _ptr(lf), _ptr(Q), int(bool(project)),
+                                          float(rtol), int(maxit), C.byref(st), _stream(stream)),
+               "hc_op_div_free_source")
+        return g, Q, st.as_dict()
 
     def apply_f32(self, x, y=None, stream=None):
         import torch

@@ -199,6 +199,17 @@ int hc_op_av_apply(hc_op_t op, const double * a, const double * v, double * ya, 
  * true residual of the result.  Allocates its work vectors per call. */
 int hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a, double * v, double rtol,
                    int maxit, hc_solve_stats * stats, void * stream);
+/* DivFreeSource::Apply (src/sources/div_free_source.cpp:71-114) for a source
+ * linear form j = (J, phi_i) (e.g. from hc_op_rhs_coil): g = M^-1 j with the
+ * unit-coefficient H(curl) mass matrix (no essential DOFs); when project != 0
+ * the Helmholtz projection g <- g - G Q, (G^T M G) Q = G^T M g (Q up to a
+ * constant, no boundary conditions, as the reference's HelmholtzProjector with
+ * an empty BCMap); then lf += M g.  src receives g, Q (optional, H1 nodes) the
+ * potential.  Both solves: Jacobi-PCG to rtol (the reference: its default
+ * GMRES).  stats: iterations of both solves summed.  Allocates its work
+ * vectors and the unit-mass quadrature factors per call. */
+int hc_op_div_free_source(hc_op_t op, const double * j, double * src, double * lf, double * Q, int project,
+                          double rtol, int maxit, hc_solve_stats * stats, void * stream);
 /* Right-hand sides per work item of the multi-RHS apply and solve (0 =
  * automatic: one per work item; a larger group holds the tile's quadrature
  * data in registers across its right-hand sides).  Results do not depend on

@@ -139,3 +139,39 @@ def test_av_solve_consistent_rhs(env, p, n, pec):
     a2, v2 = torch.zeros_like(a), torch.zeros_like(v)
     st2 = op.av_solve(torch.from_numpy(ba).cuda(), torch.from_numpy(bv).cuda(), a2, v2, rtol=rtol, maxit=20000)
     assert st2["iterations"] == st["iterations"] and torch.equal(a2, a) and torch.equal(v2, v)
+
+
+@pytest.mark.parametrize("p,n,tri", [(1, (6, 5, 4), False), (2, (4, 4, 3), True), (3, (3, 3, 4), False)])
+def test_div_free_source(env, p, n, tri):
+    """DivFreeSource::Apply on j = M x for a random (not divergence-free) field
+    x: the mass solve recovers g + G Q = x, the projected g is discretely
+    divergence-free (G^T M g = 0) while x is not, and lf gains M g; M (unit
+    coefficient) and G from the oracle."""
+    torch, H, O = env
+    print(f"case div-free source p={p} n={n} trilinear={tri}", flush=True)
+    rng = np.random.default_rng(1500 + p)
+    m0 = O.Mesh(*n)
+    vert = None
+    if tri:
+        vert = m0.box_vertices() + 0.04 * rng.standard_normal(((n[0] + 1) * (n[1] + 1) * (n[2] + 1), 3)) / max(n)
+    m = O.Mesh(*n, vert=vert)
+    E = m.nelem
+    N = O.ndofs(m, p)
+    M = O.Csr(m, p, O.pa_setup(m, p, 0.0, 1.0), np.zeros(N, dtype=np.uint8)).to_scipy().tocsr()
+    G = O.grad_matrix(m, p)
+    x = rng.standard_normal(N)
+    j = M @ x
+    lf0 = rng.standard_normal(N)
+    op = H.Operator(n, p, rng.uniform(0.5, 2.0, E), 10.0 ** rng.uniform(-2, 2, E), vert=vert).setup(0.3)
+    lf = torch.from_numpy(lf0.copy()).cuda()
+    g, Q, st = op.div_free_source(torch.from_numpy(j).cuda(), lf, project=True, rtol=1e-12, maxit=20000)
+    assert st["converged"]
+    gh, Qh, lfh = g.cpu().numpy(), Q.cpu().numpy(), lf.cpu().numpy()
+    assert _rel(gh + G @ Qh, x) <= 1e-8
+    div_x = G.T @ (M @ x)
+    assert np.linalg.norm(G.T @ (M @ gh)) <= 1e-9 * np.linalg.norm(div_x)
+    assert _rel(lfh - lf0, M @ gh) <= 1e-10
+    # without the projection g is the mass solve alone
+    lf2 = torch.from_numpy(lf0.copy()).cuda()
+    g2, _, st2 = op.div_free_source(torch.from_numpy(j).cuda(), lf2, project=False, rtol=1e-12, maxit=20000)
+    assert st2["converged"] and _rel(g2.cpu().numpy(), x) <= 1e-9
