@@ -257,9 +257,9 @@ using hc::AV_THREADS;
 //                               essential DOFs: div_free_source.cpp:76-89)
 //   g <- g - G Q,  (G^T M G) Q = G^T M g   (the Helmholtz projection
 //                               P(g) = g - grad Q, (grad.P(g), q) = 0 for all q
-//                               in H1, div_free_source.cpp:7-17,93-104, no
+//                               in H1, div_free_source.cpp:7-17,93-106, no
 //                               boundary conditions)
-//   lf += M g                  (div_free_source.cpp:107-112)
+//   lf += M g                  (div_free_source.cpp:108-113)
 // Both solves are Jacobi-preconditioned CG to rtol (the reference uses its
 // default GMRES, hephaestus_solvers.hpp); the projection's singular system
 // (constants) has a consistent right-hand side (G 1 = 0).  src receives the
