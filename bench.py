@@ -399,6 +399,15 @@ def main():
     tile_ach = B / (prof["tile_us"] * 1e-6) / 1e9
     print(f"[bench] FP64 tile kernel {prof['tile_us']:.1f} us ({tile_ach:.0f} GB/s algorithmic), "
           f"tile-face assembly {prof['planes_us']:.1f} us", file=sys.stderr, flush=True)
+    # measured device copy of 1 GiB (read + write), the achievable-HBM reference
+    ca = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    ca.fill_(1.0)
+    cb = torch.empty_like(ca)
+    ms_copy = time_applies(torch, lambda: cb.copy_(ca), 10, 2)
+    copy_gbs = 2.0 * ca.numel() * 8 / (ms_copy * 1e-3) / 1e9
+    print(f"[bench] device copy 1 GiB: {ms_copy * 1e3:.1f} us = {copy_gbs:.0f} GB/s (read + write)",
+          file=sys.stderr, flush=True)
+    del ca, cb
     if args.copy_calib:
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
         c = torch.empty_like(a)
@@ -434,6 +443,8 @@ def main():
                      "frac": round(tile_ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, p),
                      "kernel": "k_apply_tile (FP64)", "kernel_us": round(prof["tile_us"], 2),
                      "algorithmic_bytes": B,
+                     "measured_copy_GB_s": round(copy_gbs, 1),
+                     "frac_of_measured_copy": round(tile_ach / copy_gbs, 4),
                      "whole_apply": {"us": round(ms * 1e3, 2), "GB_s": round(achieved, 1),
                                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                                      "assembly_kernel_us": round(prof["planes_us"], 2)}},
