@@ -319,6 +319,31 @@ private:
   double _scale;
 };
 
+// DivFreeSource (src/sources/div_free_source.cpp:71-120) for the coil current
+// J of hc_op_rhs_coil: g = M^-1 (J, phi_i) with the unit-coefficient H(curl)
+// mass matrix, optionally the Helmholtz projection g - grad Q (no boundary
+// conditions), then lf += M g; SubtractSource removes M g again.
+class DeviceVector;
+class DivFreeSource : public Source
+{
+public:
+  explicit DivFreeSource(uint64_t seed, bool perform_helmholtz_projection = true, double tol = 1e-12,
+                         int maxit = 20000);
+  ~DivFreeSource() override;
+  void Apply(LinearForm * lf) override;
+  void SubtractSource(DeviceVector * gf) const;
+  // g of the last Apply (the divergence-free source field)
+  const DeviceVector & SourceField() const;
+
+private:
+  uint64_t _seed;
+  bool _project;
+  double _tol;
+  int _maxit;
+  DeviceVector * _g = nullptr;  // g
+  DeviceVector * _mg = nullptr; // M g
+};
+
 using Sources = NamedFieldsMap<Source>;
 
 } // namespace hephaestus

@@ -211,4 +211,53 @@ CoilSource::Apply(LinearForm * lf)
   lf->Axpby(_scale, b, 1.0);
 }
 
+DivFreeSource::DivFreeSource(uint64_t seed, bool perform_helmholtz_projection, double tol, int maxit)
+  : _seed(seed), _project(perform_helmholtz_projection), _tol(tol), _maxit(maxit)
+{
+}
+
+DivFreeSource::~DivFreeSource()
+{
+  delete _g;
+  delete _mg;
+}
+
+void
+DivFreeSource::Apply(LinearForm * lf)
+{
+  if (!lf->Operator())
+    throw Error("DivFreeSource: the linear form has no operator (build the bilinear forms first)");
+  const int64_t n = lf->Size();
+  DeviceVector j(n);
+  CheckHc(hc_op_rhs_coil(lf->Operator(), _seed, j.Data(), nullptr), "DivFreeSource");
+  if (!_g)
+    _g = new DeviceVector(n);
+  if (!_mg)
+    _mg = new DeviceVector(n);
+  _g->SetSize(n);
+  _mg->SetSize(n);
+  _mg->Fill(0.0);
+  hc_solve_stats st{};
+  CheckHc(hc_op_div_free_source(lf->Operator(), j.Data(), _g->Data(), _mg->Data(), nullptr, _project ? 1 : 0, _tol,
+                                _maxit, &st, nullptr),
+          "DivFreeSource");
+  lf->Axpby(1.0, *_mg, 1.0);
+}
+
+void
+DivFreeSource::SubtractSource(DeviceVector * gf) const
+{
+  if (!_mg)
+    throw Error("DivFreeSource::SubtractSource before Apply");
+  gf->Axpby(-1.0, *_mg, 1.0);
+}
+
+const DeviceVector &
+DivFreeSource::SourceField() const
+{
+  if (!_g)
+    throw Error("DivFreeSource::SourceField before Apply");
+  return *_g;
+}
+
 } // namespace hephaestus

@@ -1,6 +1,6 @@
 // Tests of the host C++ layer.
 //   test_host cpu                         framework logic, no device needed
-//   test_host gpu <dir> <n> <p> <steps> <dt> <precision>
+//   test_host gpu <dir> <n> <p> <steps> <dt> <precision> [Jacobi|AMS] [Coil|DivFree|DivFreeProj]
 //                                         backward-Euler steps of the H(curl)
 //                                         curl-curl system on the device; writes
 //                                         u after each step to <dir>/u_<k>.bin
@@ -117,7 +117,7 @@ write_vec(const std::string & path, const std::vector<double> & v)
 
 static void
 test_gpu(const std::string & dir, int n, int order, int steps, double dt, const std::string & precision,
-         const std::string & preconditioner)
+         const std::string & preconditioner, const std::string & source)
 {
   StructuredHexMesh mesh = StructuredHexMesh::TwoRegionCube(n);
   FESpaces fespaces;
@@ -131,7 +131,14 @@ test_gpu(const std::string & dir, int n, int order, int steps, double dt, const 
   BCMap bcs;
   bcs.Register("pec", std::make_shared<VectorDirichletBC>("pec", HC_FACE_ALL));
   Sources sources;
-  sources.Register("coil", std::make_shared<CoilSource>(11));
+  std::shared_ptr<DivFreeSource> dfs;
+  if (source == "Coil")
+    sources.Register("coil", std::make_shared<CoilSource>(11));
+  else
+  {
+    dfs = std::make_shared<DivFreeSource>(11, source == "DivFreeProj");
+    sources.Register("coil", dfs);
+  }
   InputParameters esp;
   esp.SetParam("HCurlVarName", std::string("h"));
   esp.SetParam("AlphaCoefName", std::string("nu"));
@@ -153,6 +160,18 @@ test_gpu(const std::string & dir, int n, int order, int steps, double dt, const 
     EXPECT(st.converged == 1);
     write_vec(dir + "/u_" + std::to_string(k) + ".bin", gfs.Get("h")->Download());
   }
+  if (dfs)
+  {
+    // SubtractSource takes back what Apply added
+    DeviceVector t(dfs->SourceField().Size());
+    t.Fill(0.0);
+    dfs->SubtractSource(&t);
+    std::vector<double> th = t.Download();
+    double s2 = 0.0;
+    for (double v : th)
+      s2 += v * v;
+    EXPECT(s2 > 0.0);
+  }
   // dt change re-assembles (dt rule), an unchanged dt does not
   BilinearForm & blf = es.GetGradient();
   es.SetTimeStep(dt);
@@ -168,7 +187,8 @@ main(int argc, char ** argv)
 {
   if (argc < 2)
   {
-    std::fprintf(stderr, "usage: test_host cpu | gpu <dir> <n> <p> <steps> <dt> <precision> [Jacobi|AMS]\n");
+    std::fprintf(stderr, "usage: test_host cpu | gpu <dir> <n> <p> <steps> <dt> <precision> [Jacobi|AMS] "
+                         "[Coil|DivFree|DivFreeProj]\n");
     return 2;
   }
   try
@@ -177,7 +197,7 @@ main(int argc, char ** argv)
       test_cpu();
     else if (!std::strcmp(argv[1], "gpu") && argc >= 8)
       test_gpu(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), std::atof(argv[6]), argv[7],
-               argc >= 9 ? argv[8] : "Jacobi");
+               argc >= 9 ? argv[8] : "Jacobi", argc >= 10 ? argv[9] : "Coil");
     else
     {
       std::fprintf(stderr, "bad arguments\n");

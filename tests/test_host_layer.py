@@ -7,7 +7,9 @@ kernel registration, error paths).  GPU: backward-Euler steps of the two-region
 curl-curl system driven entirely from C++, against the same steps computed by
 the oracle with a sparse direct solve: u_{k+1} = u_k + dt k,
 (dt K_nu + M_sigma) k = b_coil - K_nu u_k; relative L2 <= 1e-8 (PCG stops at
-1e-12 relative residual).
+1e-12 relative residual).  With DivFreeSource in place of the coil source the
+right-hand side is M g, g = M^-1 b_coil (unit mass), optionally Helmholtz
+projected, restated for the oracle with scipy sparse solves.
 """
 import os
 import subprocess
@@ -39,14 +41,34 @@ def _two_region(n, inner, outer):
     return np.where((X & Y & Z).ravel(), inner, outer).astype(np.float64)
 
 
+def _div_free_rhs(O, m, p, j, project):
+    """DivFreeSource::Apply restated with scipy: g = M^-1 j (unit mass, no
+    elimination), optionally g - G Q with (G^T M G) Q = G^T M g (Q pinned at
+    node 0: the system is singular in constants), then M g."""
+    import scipy.sparse.linalg as spla
+    N = O.ndofs(m, p)
+    M = O.Csr(m, p, O.pa_setup(m, p, 0.0, 1.0), np.zeros(N, dtype=np.uint8)).to_scipy().tocsc()
+    g = spla.spsolve(M, j)
+    if project:
+        G = O.grad_matrix(m, p)
+        S = (G.T @ M @ G).tocsc()
+        rhs = G.T @ (M @ g)
+        Q = np.zeros(S.shape[0])
+        Q[1:] = spla.spsolve(S[1:, 1:], rhs[1:])
+        g = g - G @ Q
+    return M @ g
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision,pc", [("fp64", "Jacobi"), ("fp32_ir", "Jacobi"), ("fp32_ir", "AMS")])
-def test_host_layer_time_steps(tmp_path, precision, pc):
+@pytest.mark.parametrize("precision,pc,source", [("fp64", "Jacobi", "Coil"), ("fp32_ir", "Jacobi", "Coil"),
+                                                 ("fp32_ir", "AMS", "Coil"), ("fp64", "Jacobi", "DivFree"),
+                                                 ("fp64", "Jacobi", "DivFreeProj")])
+def test_host_layer_time_steps(tmp_path, precision, pc, source):
     import scipy.sparse.linalg as spla
     from oracle import oracle as O
     n, p, steps, dt = 6, 2, 3, 0.1
-    print(f"case host layer {precision} {pc} n={n} p={p} steps={steps}", flush=True)
-    r = subprocess.run([_built(), "gpu", str(tmp_path), str(n), str(p), str(steps), str(dt), precision, pc],
+    print(f"case host layer {precision} {pc} {source} n={n} p={p} steps={steps}", flush=True)
+    r = subprocess.run([_built(), "gpu", str(tmp_path), str(n), str(p), str(steps), str(dt), precision, pc, source],
                        capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -61,6 +83,8 @@ def test_host_layer_time_steps(tmp_path, precision, pc):
     A = O.Csr(m, p, qA, ess).to_scipy().tocsc()
     solve = spla.factorized(A)
     b = O.rhs_coil(m, p, 11, ess)
+    if source != "Coil":
+        b = _div_free_rhs(O, m, p, b, source == "DivFreeProj")
     u = np.zeros(O.ndofs(m, p))
     for k in range(steps):
         f = b - O.pa_apply(m, p, qK, ess, u)
