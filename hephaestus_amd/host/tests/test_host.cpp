@@ -5,9 +5,11 @@
 //                                         curl-curl system on the device; writes
 //                                         u after each step to <dir>/u_<k>.bin
 #include "hephaestus/equation_system.hpp"
+#include "hephaestus/outputs.hpp"
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -101,6 +103,47 @@ test_cpu()
     InputParameters bad;
     CurlCurlKernel k(bad); // CoefficientName missing
   }));
+  // cell-centre evaluation: a constant field u on an affine (sheared) mesh
+  // has DOFs (J e_c) . u, and every cell returns u
+  {
+    StructuredHexMesh am;
+    am.nx = 3;
+    am.ny = 2;
+    am.nz = 2;
+    const double A[3][3] = {{1.0, 0.2, 0.0}, {0.0, 0.5, 0.1}, {0.3, 0.0, 0.8}};
+    for (int k = 0; k <= am.nz; k++)
+      for (int j = 0; j <= am.ny; j++)
+        for (int i = 0; i <= am.nx; i++)
+        {
+          const double r[3] = {(double)i / am.nx, (double)j / am.ny, (double)k / am.nz};
+          for (int a = 0; a < 3; a++)
+            am.vert.push_back(A[a][0] * r[0] + A[a][1] * r[1] + A[a][2] * r[2] + 0.1 * a);
+        }
+    const double u[3] = {1.0, -2.0, 0.5};
+    const int n[3] = {am.nx, am.ny, am.nz};
+    for (int p = 1; p <= 3; p++)
+    {
+      std::vector<double> dofs;
+      for (int c = 0; c < 3; c++)
+      {
+        // J e_c = A e_c / n_c (reference edge of unit length)
+        double t = 0.0;
+        for (int a = 0; a < 3; a++)
+          t += A[a][c] / n[c] * u[a];
+        int64_t cnt = 1;
+        for (int a = 0; a < 3; a++)
+          cnt *= (a == c) ? (int64_t)n[a] * p : (int64_t)n[a] * p + 1;
+        dofs.insert(dofs.end(), cnt, t);
+      }
+      const std::vector<double> cc = EvaluateCellCenters(am, p, dofs);
+      double err = 0.0;
+      for (int64_t e = 0; e < am.NumElements(); e++)
+        for (int a = 0; a < 3; a++)
+          err = std::max(err, std::fabs(cc[3 * e + a] - u[a]));
+      EXPECT(err <= 1e-13);
+    }
+    EXPECT(throws([&] { EvaluateCellCenters(am, 2, std::vector<double>(5)); }));
+  }
   // without a device, allocation fails with an Error (not an abort)
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -171,6 +214,15 @@ test_gpu(const std::string & dir, int n, int order, int steps, double dt, const 
     for (double v : th)
       s2 += v * v;
     EXPECT(s2 > 0.0);
+  }
+  // field output: cycle 0 at Reset, cycle 1 after the steps
+  {
+    Outputs outs(gfs);
+    outs.Register("vtk", std::make_shared<VTKDataCollection>(dir + "/fields"));
+    outs.SetOutputFieldNames({"h"});
+    outs.Reset();
+    outs.Write(dt * steps);
+    EXPECT(outs.Cycle() == 1);
   }
   // dt change re-assembles (dt rule), an unchanged dt does not
   BilinearForm & blf = es.GetGradient();

@@ -9,7 +9,8 @@ the oracle with a sparse direct solve: u_{k+1} = u_k + dt k,
 (dt K_nu + M_sigma) k = b_coil - K_nu u_k; relative L2 <= 1e-8 (PCG stops at
 1e-12 relative residual).  With DivFreeSource in place of the coil source the
 right-hand side is M g, g = M^-1 b_coil (unit mass), optionally Helmholtz
-projected, restated for the oracle with scipy sparse solves.
+projected, restated for the oracle with scipy sparse solves.  The VTK field
+output written after the steps holds the field at the element centres.
 """
 import os
 import subprocess
@@ -94,3 +95,31 @@ def test_host_layer_time_steps(tmp_path, precision, pc, source):
         err = np.linalg.norm(ug - u) / np.linalg.norm(u)
         print(f"step {k}: relative L2 difference {err:.3e}")
         assert err <= 1e-8
+    _check_vtk(tmp_path / "fields_000001.vtk", n, p, np.fromfile(tmp_path / f"u_{steps - 1}.bin", dtype=np.float64))
+
+
+def _check_vtk(path, n, p, u):
+    """The VTK output's cell vectors equal the field at the element centres,
+    evaluated here from the DOFs with numpy Lagrange polynomials (box mesh:
+    u_c = u_hat_c / h)."""
+    import brute
+    b = brute.Basis1D(p)
+    bo, bc = b.open(np.array([0.5]))[:, 0], b.closed(np.array([0.5]))[:, 0]
+    h = 1.0 / n
+    ref = np.zeros((n, n, n, 3))
+    off = 0
+    for c in range(3):
+        dims = brute.comp_dims((n, n, n), p, c)
+        U = u[off:off + int(np.prod(dims))].reshape(dims[2], dims[1], dims[0])
+        off += int(np.prod(dims))
+        w = [bo if a == c else bc for a in range(3)]
+        for ez in range(n):
+            for ey in range(n):
+                for ex in range(n):
+                    blk = U[ez * p:ez * p + len(w[2]), ey * p:ey * p + len(w[1]), ex * p:ex * p + len(w[0])]
+                    ref[ez, ey, ex, c] = np.einsum("k,j,i,kji->", w[2], w[1], w[0], blk) / h
+    lines = path.read_text().splitlines()
+    assert lines[0].startswith("# vtk DataFile") and "STRUCTURED_GRID" in lines[3]
+    i = lines.index("VECTORS h double")
+    vals = np.array([[float(t) for t in l.split()] for l in lines[i + 1:i + 1 + n ** 3]])
+    assert np.abs(vals - ref.reshape(-1, 3)).max() <= 1e-12 * np.abs(ref).max()
