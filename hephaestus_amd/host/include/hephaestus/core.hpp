@@ -214,15 +214,18 @@ class FESpace
 {
 public:
   FESpace(const StructuredHexMesh * mesh, const std::string & fec_name);
-  FESpace(const StructuredHexMesh * mesh, int order) : _mesh(mesh), _order(order) {}
+  FESpace(const StructuredHexMesh * mesh, int order, bool h1 = false) : _mesh(mesh), _order(order), _h1(h1) {}
   const StructuredHexMesh * Mesh() const { return _mesh; }
   int Order() const { return _order; }
+  // H1_3D_P<p> (continuous Q_p on the Gauss-Lobatto points) rather than ND_3D_P<p>
+  bool IsH1() const { return _h1; }
   int64_t GetTrueVSize() const;
   int64_t GetVSize() const { return GetTrueVSize(); }
 
 private:
   const StructuredHexMesh * _mesh;
   int _order;
+  bool _h1 = false;
 };
 
 // Device vector (hipMalloc'd) of doubles.

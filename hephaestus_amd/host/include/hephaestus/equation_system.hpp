@@ -78,6 +78,47 @@ public:
   std::string _h_curl_var_name, _alpha_coef_name, _beta_coef_name, _dtalpha_coef_name;
 };
 
+// AV formulation (src/formulations/AV/av_formulation.cpp:119-196): unknowns
+// dA/dt (ND) and V (H1) of
+//   (dt alpha curl dA/dt, curl A') + (beta dA/dt, A') + (sigma grad V, A')
+//       = -(alpha curl A_n, curl A') + sources
+//   (sigma dA/dt, grad V') + (sigma grad V, grad V') = 0
+// with sigma = beta.  The bilinear forms are those of the reference's kernels;
+// BuildBilinearForms assembles the A-A block as one device operator and the
+// other blocks are applied through it (hc_op_av_apply).  Solve uses the block
+// solver hc_op_av_solve; Step advances A by dt.
+class AVEquationSystem : public TimeDependentEquationSystem
+{
+public:
+  using MixedBilinearFormKernel = Kernel<MixedBilinearForm>;
+  explicit AVEquationSystem(const InputParameters & params);
+  void Init(GridFunctions & gridfunctions, const FESpaces & fespaces, BCMap & bc_map,
+            Coefficients & coefficients) override;
+  void AddKernels() override;
+  using EquationSystem::AddKernel;
+  // mixed kernel with trial variable trial_var_name in the rows of test_var_name
+  void AddKernel(const std::string & trial_var_name, const std::string & test_var_name,
+                 std::shared_ptr<MixedBilinearFormKernel> kernel);
+  void BuildBilinearForms() override;
+  // the block system with the current forms; (dA_dt, V) hold the initial guess
+  hc_solve_stats Solve(DeviceVector & dA_dt, DeviceVector & V, double tol, int maxit);
+  // one backward-Euler step: build, solve, A <- A + dt dA/dt (V and dA/dt kept
+  // in their grid functions)
+  hc_solve_stats Step(double dt, GridFunctions & gridfunctions, BCMap & bc_map, Sources & sources, double tol,
+                      int maxit);
+  std::string _a_name, _v_name, _alpha_coef_name, _beta_coef_name, _dtalpha_coef_name, _neg_beta_coef_name;
+  ConstantCoefficient _neg_coef{-1.0};
+
+private:
+  std::map<std::pair<std::string, std::string>, std::vector<std::shared_ptr<MixedBilinearFormKernel>>>
+      _mblf_kernels;
+  std::map<std::pair<std::string, std::string>, std::unique_ptr<MixedBilinearForm>> _mblfs;
+  GridFunctions * _gfs = nullptr;
+  const FESpaces * _fes = nullptr;
+  BCMap * _bcs = nullptr;
+  Coefficients * _coefs = nullptr;
+};
+
 // Jacobi-preconditioned CG on the GPU operator; "Tolerance" / "MaxIter" /
 // "Precision" ("fp64" or "fp32_ir") / "Preconditioner" ("Jacobi" or "AMS", the
 // latter the one-level gradient-space correction) parameters.  Stopping rule

@@ -20,6 +20,19 @@ struct VectorFEMassIntegrator
 {
   const Coefficient * coef;
 };
+// integrators of the AV formulation's other blocks
+struct DiffusionIntegrator
+{
+  const Coefficient * coef;
+};
+struct MixedVectorGradientIntegrator
+{
+  const Coefficient * coef;
+};
+struct VectorFEWeakDivergenceIntegrator
+{
+  const Coefficient * coef;
+};
 
 // ParBilinearForm counterpart: a(u,w) = sum (a_i curl u, curl w) + sum (b_j u, w)
 // assembled as one libhcurl operator (partial assembly: quadrature-point
@@ -37,11 +50,17 @@ public:
 
   void AddDomainIntegrator(CurlCurlIntegrator i) { _curl.push_back(i.coef); }
   void AddDomainIntegrator(VectorFEMassIntegrator i) { _mass.push_back(i.coef); }
+  // H1 forms (the AV system's (sigma grad V, grad V')) are recorded, not
+  // assembled on their own: AVEquationSystem builds them into its block operator
+  void AddDomainIntegrator(DiffusionIntegrator i) { _diff.push_back(i.coef); }
   void ClearIntegrators()
   {
     _curl.clear();
     _mass.clear();
+    _diff.clear();
   }
+  const std::vector<const Coefficient *> & MassCoefficients() const { return _mass; }
+  const std::vector<const Coefficient *> & DiffusionCoefficients() const { return _diff; }
   // HC_FACE_* bits of the PEC faces (fixed once the form is assembled)
   void SetEssentialFaces(int faces) { _ess_faces = faces; }
   void Assemble();
@@ -58,13 +77,36 @@ public:
 
 private:
   FESpace * _fes;
-  std::vector<const Coefficient *> _curl, _mass;
+  std::vector<const Coefficient *> _curl, _mass, _diff;
   hc_op_t _op = nullptr;
   std::vector<double> _nu, _sig;
   int _prec = HC_PREC_FP64 | HC_PREC_FP32;
   int _ess_faces = HC_FACE_ALL;
   int _last_work = 0;
   mutable DeviceVector _tmp;
+};
+
+// ParMixedBilinearForm counterpart (trial space -> test space): records the
+// coefficients of its integrators; AVEquationSystem builds its blocks.
+class MixedBilinearForm
+{
+public:
+  MixedBilinearForm(FESpace * trial, FESpace * test) : _trial(trial), _test(test) {}
+  void AddDomainIntegrator(MixedVectorGradientIntegrator i) { _grad.push_back(i.coef); }
+  void AddDomainIntegrator(VectorFEWeakDivergenceIntegrator i) { _wdiv.push_back(i.coef); }
+  void ClearIntegrators()
+  {
+    _grad.clear();
+    _wdiv.clear();
+  }
+  FESpace * TrialFESpace() const { return _trial; }
+  FESpace * TestFESpace() const { return _test; }
+  const std::vector<const Coefficient *> & GradientCoefficients() const { return _grad; }
+  const std::vector<const Coefficient *> & WeakDivergenceCoefficients() const { return _wdiv; }
+
+private:
+  FESpace *_trial, *_test;
+  std::vector<const Coefficient *> _grad, _wdiv;
 };
 
 // ParLinearForm counterpart: a device vector on the space.
@@ -116,6 +158,42 @@ public:
   void Init(GridFunctions & gridfunctions, const FESpaces & fespaces, BCMap & bc_map,
             Coefficients & coefficients) override;
   void Apply(BilinearForm * blf) override;
+  std::string _coef_name;
+  Coefficient * _coef{nullptr};
+};
+
+// (sigma grad V, grad V') (src/kernels/diffusion_kernel.{hpp,cpp})
+class DiffusionKernel : public Kernel<BilinearForm>
+{
+public:
+  explicit DiffusionKernel(const InputParameters & params);
+  void Init(GridFunctions & gridfunctions, const FESpaces & fespaces, BCMap & bc_map,
+            Coefficients & coefficients) override;
+  void Apply(BilinearForm * blf) override;
+  std::string _coef_name;
+  Coefficient * _coef{nullptr};
+};
+
+// (sigma grad V, u') (src/kernels/mixed_vector_gradient_kernel.{hpp,cpp})
+class MixedVectorGradientKernel : public Kernel<MixedBilinearForm>
+{
+public:
+  explicit MixedVectorGradientKernel(const InputParameters & params);
+  void Init(GridFunctions & gridfunctions, const FESpaces & fespaces, BCMap & bc_map,
+            Coefficients & coefficients) override;
+  void Apply(MixedBilinearForm * mblf) override;
+  std::string _coef_name;
+  Coefficient * _coef{nullptr};
+};
+
+// (sigma u, grad V') (src/kernels/vector_fe_weak_divergence_kernel.{hpp,cpp})
+class VectorFEWeakDivergenceKernel : public Kernel<MixedBilinearForm>
+{
+public:
+  explicit VectorFEWeakDivergenceKernel(const InputParameters & params);
+  void Init(GridFunctions & gridfunctions, const FESpaces & fespaces, BCMap & bc_map,
+            Coefficients & coefficients) override;
+  void Apply(MixedBilinearForm * mblf) override;
   std::string _coef_name;
   Coefficient * _coef{nullptr};
 };

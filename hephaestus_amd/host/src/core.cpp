@@ -106,11 +106,15 @@ Coefficients::Coefficients()
 // ---------------------------------------------------------------------------
 FESpace::FESpace(const StructuredHexMesh * mesh, const std::string & fec_name) : _mesh(mesh)
 {
-  // "ND_3D_P<p>"
-  const std::string pre = "ND_3D_P";
-  if (fec_name.compare(0, pre.size(), pre) != 0)
-    throw Error("FESpace: only Nedelec spaces ND_3D_P<p> are supported, got '" + fec_name + "'");
-  _order = std::atoi(fec_name.c_str() + pre.size());
+  // "ND_3D_P<p>" or "H1_3D_P<p>"
+  const std::string nd = "ND_3D_P", h1 = "H1_3D_P";
+  if (fec_name.compare(0, nd.size(), nd) == 0)
+    _h1 = false;
+  else if (fec_name.compare(0, h1.size(), h1) == 0)
+    _h1 = true;
+  else
+    throw Error("FESpace: only ND_3D_P<p> and H1_3D_P<p> spaces are supported, got '" + fec_name + "'");
+  _order = std::atoi(fec_name.c_str() + nd.size());
   if (_order < 1 || _order > 3)
     throw Error("FESpace: order must be 1..3, got '" + fec_name + "'");
 }
@@ -119,6 +123,8 @@ int64_t
 FESpace::GetTrueVSize() const
 {
   const int64_t p = _order, nx = _mesh->nx, ny = _mesh->ny, nz = _mesh->nz;
+  if (_h1)
+    return (nx * p + 1) * (ny * p + 1) * (nz * p + 1);
   return nx * p * (ny * p + 1) * (nz * p + 1) + (nx * p + 1) * ny * p * (nz * p + 1) +
          (nx * p + 1) * (ny * p + 1) * nz * p;
 }

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 
 namespace hephaestus
 {
@@ -229,6 +230,151 @@ CurlCurlEquationSystem::AddKernels()
   InputParameters mass;
   mass.SetParam("CoefficientName", _beta_coef_name);
   AddKernel(du_dt, std::shared_ptr<ParBilinearFormKernel>(std::make_shared<VectorFEMassKernel>(mass)));
+}
+
+// ---------------------------------------------------------------------------
+AVEquationSystem::AVEquationSystem(const InputParameters & params)
+  : _a_name(params.GetParam<std::string>("VectorPotentialName")),
+    _v_name(params.GetParam<std::string>("ScalarPotentialName")),
+    _alpha_coef_name(params.GetParam<std::string>("AlphaCoefName")),
+    _beta_coef_name(params.GetParam<std::string>("BetaCoefName")),
+    _dtalpha_coef_name("dt_" + _alpha_coef_name),
+    _neg_beta_coef_name("negative_" + _beta_coef_name)
+{
+}
+
+void
+AVEquationSystem::Init(GridFunctions & gridfunctions, const FESpaces & fespaces, BCMap & bc_map,
+                       Coefficients & coefficients)
+{
+  coefficients._scalars.Register(
+      _dtalpha_coef_name,
+      std::make_shared<TransformedCoefficient>(&_dt_coef, coefficients._scalars.Get(_alpha_coef_name), prodFunc));
+  coefficients._scalars.Register(
+      _neg_beta_coef_name,
+      std::make_shared<TransformedCoefficient>(&_neg_coef, coefficients._scalars.Get(_beta_coef_name), prodFunc));
+  TimeDependentEquationSystem::Init(gridfunctions, fespaces, bc_map, coefficients);
+  for (auto & kv : _mblf_kernels)
+    for (auto & k : kv.second)
+      k->Init(gridfunctions, fespaces, bc_map, coefficients);
+  _gfs = &gridfunctions;
+  _fes = &fespaces;
+  _bcs = &bc_map;
+  _coefs = &coefficients;
+}
+
+void
+AVEquationSystem::AddKernel(const std::string & trial_var_name, const std::string & test_var_name,
+                            std::shared_ptr<MixedBilinearFormKernel> kernel)
+{
+  AddTestVariableNameIfMissing(test_var_name);
+  _mblf_kernels[{trial_var_name, test_var_name}].push_back(std::move(kernel));
+}
+
+void
+AVEquationSystem::AddKernels()
+{
+  AddTrialVariableNameIfMissing(_a_name);
+  const std::string da_dt = GetTimeDerivativeName(_a_name);
+  AddTrialVariableNameIfMissing(_v_name);
+  // (alpha curl A_n, curl dA'/dt)
+  InputParameters weak;
+  weak.SetParam("CoupledVariableName", _a_name);
+  weak.SetParam("CoefficientName", _alpha_coef_name);
+  EquationSystem::AddKernel(da_dt, std::shared_ptr<ParLinearFormKernel>(std::make_shared<WeakCurlCurlKernel>(weak)));
+  // (alpha dt curl dA/dt, curl dA'/dt)
+  InputParameters cc;
+  cc.SetParam("CoefficientName", _dtalpha_coef_name);
+  EquationSystem::AddKernel(da_dt, std::shared_ptr<ParBilinearFormKernel>(std::make_shared<CurlCurlKernel>(cc)));
+  // (beta dA/dt, dA'/dt)
+  InputParameters mass;
+  mass.SetParam("CoefficientName", _beta_coef_name);
+  EquationSystem::AddKernel(da_dt, std::shared_ptr<ParBilinearFormKernel>(std::make_shared<VectorFEMassKernel>(mass)));
+  // (sigma grad V, dA'/dt)
+  InputParameters grad;
+  grad.SetParam("CoefficientName", _beta_coef_name);
+  AddKernel(_v_name, da_dt, std::make_shared<MixedVectorGradientKernel>(grad));
+  // (sigma grad V, grad V')
+  InputParameters diff;
+  diff.SetParam("CoefficientName", _beta_coef_name);
+  EquationSystem::AddKernel(_v_name, std::shared_ptr<ParBilinearFormKernel>(std::make_shared<DiffusionKernel>(diff)));
+  // (sigma dA/dt, grad V')
+  InputParameters wdiv;
+  wdiv.SetParam("CoefficientName", _beta_coef_name);
+  AddKernel(da_dt, _v_name, std::make_shared<VectorFEWeakDivergenceKernel>(wdiv));
+}
+
+void
+AVEquationSystem::BuildBilinearForms()
+{
+  EquationSystem::BuildBilinearForms();
+  for (auto & kv : _mblf_kernels)
+  {
+    auto & f = _mblfs[kv.first];
+    if (!f)
+      f = std::make_unique<MixedBilinearForm>(_gfs->Get(kv.first.first)->ParFESpace(),
+                                              _gfs->Get(kv.first.second)->ParFESpace());
+    f->ClearIntegrators();
+    for (auto & k : kv.second)
+      k->Apply(f.get());
+  }
+  // the block operator applies every block through the A-A operator's mass
+  // factors: the mixed and diffusion coefficients must equal its sigma
+  const std::string da_dt = GetTimeDerivativeName(_a_name);
+  const StructuredHexMesh & mesh = *_gfs->Get(da_dt)->ParFESpace()->Mesh();
+  auto sum = [&](const std::vector<const Coefficient *> & cs) {
+    std::vector<double> v(mesh.NumElements(), 0.0);
+    for (const Coefficient * c : cs)
+    {
+      const auto e = c->ElementValues(mesh);
+      for (size_t i = 0; i < v.size(); i++)
+        v[i] += e[i];
+    }
+    return v;
+  };
+  const std::vector<double> sig = sum(_blfs.Get(da_dt)->MassCoefficients());
+  const std::vector<double> sdiff = sum(_blfs.Get(_v_name)->DiffusionCoefficients());
+  const std::vector<double> sgrad = sum(_mblfs.at({_v_name, da_dt})->GradientCoefficients());
+  const std::vector<double> swdiv = sum(_mblfs.at({da_dt, _v_name})->WeakDivergenceCoefficients());
+  if (sdiff != sig || sgrad != sig || swdiv != sig)
+    throw Error("AVEquationSystem: the mass, mixed-gradient, weak-divergence and diffusion coefficients must be "
+                "the same sigma");
+}
+
+hc_solve_stats
+AVEquationSystem::Solve(DeviceVector & dA_dt, DeviceVector & V, double tol, int maxit)
+{
+  const std::string da_dt = GetTimeDerivativeName(_a_name);
+  hc_op_t op = _blfs.Get(da_dt)->Handle();
+  LinearForm * la = _lfs.Get(da_dt);
+  LinearForm * lv = _lfs.Get(_v_name);
+  // B: the linear forms with the PEC rows of the A part set to the (zero)
+  // tangential values, as FormLinearSystem does
+  DeviceVector Ba(la->Size()), Bv(lv->Size());
+  Ba.CopyFrom(*la);
+  Bv.CopyFrom(*lv);
+  CheckHc(hc_op_zero_ess(op, Ba.Data(), nullptr), "AVEquationSystem::Solve");
+  hc_solve_stats st{};
+  CheckHc(hc_op_av_solve(op, Ba.Data(), Bv.Data(), dA_dt.Data(), V.Data(), tol, maxit, &st, nullptr),
+          "AVEquationSystem::Solve");
+  return st;
+}
+
+hc_solve_stats
+AVEquationSystem::Step(double dt, GridFunctions & gridfunctions, BCMap & bc_map, Sources & sources, double tol,
+                       int maxit)
+{
+  const std::string da_dt = GetTimeDerivativeName(_a_name);
+  SetTimeStep(dt);
+  UpdateEquationSystem(bc_map, sources);
+  GridFunction * a = gridfunctions.Get(_a_name);
+  GridFunction * k = gridfunctions.Get(da_dt);
+  GridFunction * v = gridfunctions.Get(_v_name);
+  k->Fill(0.0);
+  v->Fill(0.0);
+  const hc_solve_stats st = Solve(*k, *v, tol, maxit);
+  a->Axpby(dt, *k, 1.0);
+  return st;
 }
 
 // ---------------------------------------------------------------------------

@@ -13,6 +13,8 @@ BilinearForm::~BilinearForm()
 void
 BilinearForm::Assemble()
 {
+  if (_fes->IsH1())
+    return; // recorded only (see AddDomainIntegrator(DiffusionIntegrator))
   const StructuredHexMesh & mesh = *_fes->Mesh();
   const int64_t E = mesh.NumElements();
   std::vector<double> nu(E, 0.0), sig(E, 0.0);
@@ -90,6 +92,57 @@ void
 CurlCurlKernel::Apply(BilinearForm * blf)
 {
   blf->AddDomainIntegrator(CurlCurlIntegrator{_coef});
+}
+
+DiffusionKernel::DiffusionKernel(const InputParameters & params)
+  : Kernel(params), _coef_name(params.GetParam<std::string>("CoefficientName"))
+{
+}
+
+void
+DiffusionKernel::Init(GridFunctions &, const FESpaces &, BCMap &, Coefficients & coefficients)
+{
+  _coef = coefficients._scalars.Get(_coef_name);
+}
+
+void
+DiffusionKernel::Apply(BilinearForm * blf)
+{
+  blf->AddDomainIntegrator(DiffusionIntegrator{_coef});
+}
+
+MixedVectorGradientKernel::MixedVectorGradientKernel(const InputParameters & params)
+  : Kernel(params), _coef_name(params.GetParam<std::string>("CoefficientName"))
+{
+}
+
+void
+MixedVectorGradientKernel::Init(GridFunctions &, const FESpaces &, BCMap &, Coefficients & coefficients)
+{
+  _coef = coefficients._scalars.Get(_coef_name);
+}
+
+void
+MixedVectorGradientKernel::Apply(MixedBilinearForm * mblf)
+{
+  mblf->AddDomainIntegrator(MixedVectorGradientIntegrator{_coef});
+}
+
+VectorFEWeakDivergenceKernel::VectorFEWeakDivergenceKernel(const InputParameters & params)
+  : Kernel(params), _coef_name(params.GetParam<std::string>("CoefficientName"))
+{
+}
+
+void
+VectorFEWeakDivergenceKernel::Init(GridFunctions &, const FESpaces &, BCMap &, Coefficients & coefficients)
+{
+  _coef = coefficients._scalars.Get(_coef_name);
+}
+
+void
+VectorFEWeakDivergenceKernel::Apply(MixedBilinearForm * mblf)
+{
+  mblf->AddDomainIntegrator(VectorFEWeakDivergenceIntegrator{_coef});
 }
 
 VectorFEMassKernel::VectorFEMassKernel(const InputParameters & params)

@@ -1,6 +1,7 @@
 // Tests of the host C++ layer.
 //   test_host cpu                         framework logic, no device needed
 //   test_host gpu <dir> <n> <p> <steps> <dt> <precision> [Jacobi|AMS] [Coil|DivFree|DivFreeProj]
+//   test_host av <dir> <n> <p> <steps> <dt>  AV formulation steps (a, da_dt, v per step)
 //                                         backward-Euler steps of the H(curl)
 //                                         curl-curl system on the device; writes
 //                                         u after each step to <dir>/u_<k>.bin
@@ -82,7 +83,9 @@ test_cpu()
   // FE space sizes: N = 3 (np)(np+1)^2 on an n^3 cube
   FESpace fes(&mesh, "ND_3D_P2");
   EXPECT(fes.Order() == 2 && fes.GetTrueVSize() == 3 * 16 * 17 * 17);
-  EXPECT(throws([&] { FESpace bad(&mesh, "H1_3D_P1"); }));
+  FESpace h1(&mesh, "H1_3D_P2");
+  EXPECT(h1.IsH1() && h1.GetTrueVSize() == 17 * 17 * 17);
+  EXPECT(throws([&] { FESpace bad(&mesh, "L2_3D_P1"); }));
   EXPECT(throws([&] { FESpace bad(&mesh, "ND_3D_P7"); }));
   // boundary conditions
   BCMap bcs;
@@ -143,6 +146,19 @@ test_cpu()
       EXPECT(err <= 1e-13);
     }
     EXPECT(throws([&] { EvaluateCellCenters(am, 2, std::vector<double>(5)); }));
+  }
+  // kernel registration of the AV formulation (av_formulation.cpp:150-196)
+  {
+    InputParameters avp;
+    avp.SetParam("VectorPotentialName", std::string("a"));
+    avp.SetParam("ScalarPotentialName", std::string("v"));
+    avp.SetParam("AlphaCoefName", std::string("nu"));
+    avp.SetParam("BetaCoefName", std::string("sigma"));
+    AVEquationSystem av(avp);
+    av.AddKernels();
+    EXPECT(av._test_var_names.size() == 2 && av._test_var_names[0] == "da_dt" && av._test_var_names[1] == "v");
+    EXPECT(av._trial_var_names.size() == 2);
+    EXPECT(av._dtalpha_coef_name == "dt_nu" && av._neg_beta_coef_name == "negative_sigma");
   }
   // without a device, allocation fails with an Error (not an abort)
   int ndev = 0;
@@ -234,6 +250,46 @@ test_gpu(const std::string & dir, int n, int order, int steps, double dt, const 
   std::printf("gpu tests: %s\n", g_fail ? "FAILED" : "ok");
 }
 
+// Backward-Euler steps of the AV system with the projected coil source; writes
+// A, dA/dt and V after each step to <dir>/{a,k,v}_<k>.bin.
+static void
+test_av(const std::string & dir, int n, int order, int steps, double dt)
+{
+  StructuredHexMesh mesh = StructuredHexMesh::TwoRegionCube(n);
+  FESpaces fespaces;
+  fespaces.Register("HCurl", std::make_shared<FESpace>(&mesh, "ND_3D_P" + std::to_string(order)));
+  fespaces.Register("H1", std::make_shared<FESpace>(&mesh, "H1_3D_P" + std::to_string(order)));
+  GridFunctions gfs;
+  gfs.Register("a", std::make_shared<GridFunction>(fespaces.Get("HCurl")));
+  gfs.Register("da_dt", std::make_shared<GridFunction>(fespaces.Get("HCurl")));
+  gfs.Register("v", std::make_shared<GridFunction>(fespaces.Get("H1")));
+  Coefficients coefs;
+  coefs._scalars.Register("nu", std::make_shared<PWConstCoefficient>(std::vector<double>{1.0, 0.5}));
+  coefs._scalars.Register("sigma", std::make_shared<PWConstCoefficient>(std::vector<double>{1e-2, 1e2}));
+  BCMap bcs;
+  bcs.Register("pec", std::make_shared<VectorDirichletBC>("pec", HC_FACE_ALL));
+  Sources sources;
+  sources.Register("coil", std::make_shared<DivFreeSource>(11, true));
+  InputParameters avp;
+  avp.SetParam("VectorPotentialName", std::string("a"));
+  avp.SetParam("ScalarPotentialName", std::string("v"));
+  avp.SetParam("AlphaCoefName", std::string("nu"));
+  avp.SetParam("BetaCoefName", std::string("sigma"));
+  AVEquationSystem es(avp);
+  es.AddKernels();
+  es.Init(gfs, fespaces, bcs, coefs);
+  for (int k = 0; k < steps; k++)
+  {
+    const hc_solve_stats st = es.Step(dt, gfs, bcs, sources, 1e-11, 20000);
+    std::printf("av step %d: iterations %d rel_res %.3e converged %d\n", k, st.iterations, st.rel_res, st.converged);
+    EXPECT(st.converged == 1);
+    write_vec(dir + "/a_" + std::to_string(k) + ".bin", gfs.Get("a")->Download());
+    write_vec(dir + "/k_" + std::to_string(k) + ".bin", gfs.Get("da_dt")->Download());
+    write_vec(dir + "/v_" + std::to_string(k) + ".bin", gfs.Get("v")->Download());
+  }
+  std::printf("av tests: %s\n", g_fail ? "FAILED" : "ok");
+}
+
 int
 main(int argc, char ** argv)
 {
@@ -247,6 +303,8 @@ main(int argc, char ** argv)
   {
     if (!std::strcmp(argv[1], "cpu"))
       test_cpu();
+    else if (!std::strcmp(argv[1], "av") && argc >= 7)
+      test_av(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), std::atof(argv[6]));
     else if (!std::strcmp(argv[1], "gpu") && argc >= 8)
       test_gpu(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), std::atof(argv[6]), argv[7],
                argc >= 9 ? argv[8] : "Jacobi", argc >= 10 ? argv[9] : "Coil");
