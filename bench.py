@@ -105,8 +105,8 @@ def cpu_csr_path(torch, H, budget_s):
     assembled CSR matrix (full assembly with PEC elimination, oracle/hc_oracle.c,
     OpenMP) -- its SpMV at 32^3 order 2 (median of a bounded sample) beside the
     GPU FP64 apply of the same operator, and a CSR Jacobi-PCG solve at 16^3
-    order 1 (sigma/dt = 1, nu = 1, uniform RHS seed 1, rtol 1e-10) beside the
-    GPU FP64 solve of the same problem."""
+    order 1 (sigma/dt = 1, nu = 1, uniform RHS seed 1, rtol 1e-10) and the same
+    solve with the oracle's PA apply beside the GPU FP64 solve of the problem."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import oracle as O
     O.build()
@@ -152,7 +152,12 @@ def cpu_csr_path(torch, H, budget_s):
     bd = torch.from_numpy(b).cuda()
     xd = torch.zeros_like(bd)
     sg = op.solve(bd, xd, rtol=1e-10, maxit=10000, mode=H.SOLVE_FP64)
+    # the same solve with the oracle's PA apply in the CG loop (MFEM-PA-equivalent path)
+    t0 = time.perf_counter()
+    _, st_pa = O.pcg_pa(m, p, qd, ess, d, b, 1e-10, 10000)
+    t_pa = time.perf_counter() - t0
     out["pcg_16_p1"] = {"cpu_iterations": int(st["iterations"]), "cpu_ms": round(t_cpu * 1e3, 2),
+                        "cpu_pa_iterations": int(st_pa["iterations"]), "cpu_pa_ms": round(t_pa * 1e3, 2),
                         "gpu_iterations": sg["iterations"], "gpu_ms": round(sg["solve_ms"], 2)}
     print(f"[bench] CPU CSR path {out}", file=sys.stderr, flush=True)
     return out
