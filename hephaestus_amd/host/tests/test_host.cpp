@@ -6,6 +6,7 @@
 //                                         curl-curl system on the device; writes
 //                                         u after each step to <dir>/u_<k>.bin
 #include "hephaestus/equation_system.hpp"
+#include "hephaestus/mesh_io.hpp"
 #include "hephaestus/outputs.hpp"
 
 #include <hip/hip_runtime_api.h>
@@ -17,6 +18,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <sstream>
 
 using namespace hephaestus;
 
@@ -159,6 +161,91 @@ test_cpu()
     EXPECT(av._test_var_names.size() == 2 && av._test_var_names[0] == "da_dt" && av._test_var_names[1] == "v");
     EXPECT(av._trial_var_names.size() == 2);
     EXPECT(av._dtalpha_coef_name == "dt_nu" && av._neg_beta_coef_name == "negative_sigma");
+  }
+  // MFEM mesh v1.0 round trips (box with attributes; sheared anisotropic grid),
+  // a file with permuted vertex ids, and refusal of unstructured meshes
+  {
+    const std::string tmp = "/tmp/hephaestus_mesh_io_test.mesh";
+    StructuredHexMesh box = StructuredHexMesh::TwoRegionCube(4);
+    WriteMFEMMesh(box, tmp);
+    StructuredHexMesh rb = ReadMFEMMesh(tmp);
+    EXPECT(rb.nx == 4 && rb.ny == 4 && rb.nz == 4 && rb.attributes == box.attributes);
+    bool same = rb.vert.size() == 3 * 125;
+    for (int k = 0; k <= 4 && same; k++)
+      for (int j = 0; j <= 4; j++)
+        for (int i = 0; i <= 4; i++)
+        {
+          const size_t v = ((size_t)k * 5 + j) * 5 + i;
+          same = same && rb.vert[3 * v] == i / 4.0 && rb.vert[3 * v + 1] == j / 4.0 && rb.vert[3 * v + 2] == k / 4.0;
+        }
+    EXPECT(same);
+    StructuredHexMesh sh;
+    sh.nx = 3;
+    sh.ny = 2;
+    sh.nz = 5;
+    for (int k = 0; k <= sh.nz; k++)
+      for (int j = 0; j <= sh.ny; j++)
+        for (int i = 0; i <= sh.nx; i++)
+        {
+          sh.vert.push_back(0.3 * i + 0.05 * j * k);
+          sh.vert.push_back(0.5 * j + 0.01 * i);
+          sh.vert.push_back(0.2 * k - 0.02 * i * j);
+        }
+    sh.attributes.resize(sh.NumElements());
+    for (int64_t e = 0; e < sh.NumElements(); e++)
+      sh.attributes[e] = 1 + (int)(e % 3);
+    WriteMFEMMesh(sh, tmp);
+    StructuredHexMesh rs = ReadMFEMMesh(tmp);
+    EXPECT(rs.nx == 3 && rs.ny == 2 && rs.nz == 5 && rs.vert == sh.vert && rs.attributes == sh.attributes);
+    // the same mesh with its vertex ids reversed in the file
+    {
+      std::vector<std::string> lines;
+      {
+        std::ifstream in(tmp);
+        std::string l;
+        while (std::getline(in, l))
+          lines.push_back(l);
+      }
+      const int64_t nv = 4 * 3 * 6;
+      std::ofstream w(tmp);
+      for (size_t i = 0; i < lines.size(); i++)
+      {
+        w << lines[i] << "\n";
+        if (lines[i] == "elements")
+        {
+          const int64_t ne = std::stoll(lines[++i]);
+          w << lines[i] << "\n";
+          for (int64_t e = 0; e < ne; e++)
+          {
+            std::istringstream es(lines[++i]);
+            int a, g;
+            es >> a >> g;
+            w << a << " " << g;
+            int64_t v;
+            while (es >> v)
+              w << " " << (nv - 1 - v);
+            w << "\n";
+          }
+        }
+        else if (lines[i] == "vertices")
+        {
+          w << lines[i + 1] << "\n" << lines[i + 2] << "\n"; // count, dimension
+          for (int64_t v = nv - 1; v >= 0; v--)
+            w << lines[i + 3 + v] << "\n";
+          i += 2 + nv;
+        }
+      }
+    }
+    StructuredHexMesh rp = ReadMFEMMesh(tmp);
+    EXPECT(rp.nx == 3 && rp.ny == 2 && rp.nz == 5 && rp.vert == sh.vert);
+    {
+      std::ofstream w(tmp);
+      w << "MFEM mesh v1.0\ndimension\n3\nelements\n1\n1 4 0 1 2 3\nboundary\n0\nvertices\n4\n3\n"
+           "0 0 0\n1 0 0\n0 1 0\n0 0 1\n";
+    }
+    EXPECT(throws([&] { ReadMFEMMesh(tmp); }));
+    EXPECT(throws([&] { ReadMFEMMesh("/nonexistent/x.mesh"); }));
+    std::remove(tmp.c_str());
   }
   // without a device, allocation fails with an Error (not an abort)
   int ndev = 0;
