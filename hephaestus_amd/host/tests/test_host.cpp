@@ -2,6 +2,7 @@
 //   test_host cpu                         framework logic, no device needed
 //   test_host gpu <dir> <n> <p> <steps> <dt> <precision> [Jacobi|AMS] [Coil|DivFree|DivFreeProj]
 //   test_host av <dir> <n> <p> <steps> <dt>  AV formulation steps (a, da_dt, v per step)
+//   test_host meshio <dir> <n> <p>            apply on a mesh read back from an MFEM mesh file
 //                                         backward-Euler steps of the H(curl)
 //                                         curl-curl system on the device; writes
 //                                         u after each step to <dir>/u_<k>.bin
@@ -377,6 +378,43 @@ test_av(const std::string & dir, int n, int order, int steps, double dt)
   std::printf("av tests: %s\n", g_fail ? "FAILED" : "ok");
 }
 
+// An operator on a mesh read back from an MFEM mesh file (trilinear-vertex
+// path) applies as the one on the box mesh it was written from.
+static void
+test_meshio_gpu(const std::string & dir, int n, int order)
+{
+  StructuredHexMesh box = StructuredHexMesh::TwoRegionCube(n);
+  WriteMFEMMesh(box, dir + "/cube.mesh");
+  StructuredHexMesh rd = ReadMFEMMesh(dir + "/cube.mesh");
+  Coefficients coefs;
+  coefs._scalars.Register("sigma", std::make_shared<PWConstCoefficient>(std::vector<double>{1e-2, 1e2}));
+  coefs._scalars.Register("nu", std::make_shared<ConstantCoefficient>(1.0));
+  std::vector<double> y[2];
+  const StructuredHexMesh * meshes[2] = {&box, &rd};
+  for (int k = 0; k < 2; k++)
+  {
+    FESpace fes(meshes[k], "ND_3D_P" + std::to_string(order));
+    BilinearForm blf(&fes);
+    blf.AddDomainIntegrator(CurlCurlIntegrator{coefs._scalars.Get("nu")});
+    blf.AddDomainIntegrator(VectorFEMassIntegrator{coefs._scalars.Get("sigma")});
+    blf.SetEssentialFaces(HC_FACE_ALL);
+    blf.Assemble();
+    DeviceVector x(fes.GetTrueVSize()), yy(fes.GetTrueVSize());
+    CheckHc(hc_fill_uniform(blf.Handle(), 5, x.Data(), x.Size(), nullptr), "meshio");
+    blf.Mult(x, yy);
+    y[k] = yy.Download();
+  }
+  double d2 = 0.0, n2 = 0.0;
+  for (size_t i = 0; i < y[0].size(); i++)
+  {
+    d2 += (y[1][i] - y[0][i]) * (y[1][i] - y[0][i]);
+    n2 += y[0][i] * y[0][i];
+  }
+  std::printf("meshio: relative difference %.3e\n", std::sqrt(d2 / n2));
+  EXPECT(y[0].size() == y[1].size() && d2 <= 1e-24 * n2);
+  std::printf("meshio tests: %s\n", g_fail ? "FAILED" : "ok");
+}
+
 int
 main(int argc, char ** argv)
 {
@@ -390,6 +428,8 @@ main(int argc, char ** argv)
   {
     if (!std::strcmp(argv[1], "cpu"))
       test_cpu();
+    else if (!std::strcmp(argv[1], "meshio") && argc >= 5)
+      test_meshio_gpu(argv[2], std::atoi(argv[3]), std::atoi(argv[4]));
     else if (!std::strcmp(argv[1], "av") && argc >= 7)
       test_av(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), std::atof(argv[6]));
     else if (!std::strcmp(argv[1], "gpu") && argc >= 8)

@@ -179,3 +179,15 @@ def test_host_layer_av_steps(tmp_path):
         e2 = np.linalg.norm(K @ ag - K @ A) / np.linalg.norm(K @ A)
         print(f"av step {k}: a0 + G v rel diff {e1:.3e}, K A rel diff {e2:.3e}")
         assert e1 <= 1e-5 and e2 <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("p", [1, 3])
+def test_host_layer_mesh_file(tmp_path, p):
+    """The operator on a mesh written to and read back from the MFEM v1.0 format
+    applies as the operator on the original box mesh (relative L2 <= 1e-12)."""
+    print(f"case host mesh file p={p}", flush=True)
+    r = subprocess.run([_built(), "meshio", str(tmp_path), "5", str(p)], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "meshio tests: ok" in r.stdout
