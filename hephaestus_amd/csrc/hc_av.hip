@@ -345,10 +345,13 @@ hc_op_div_free_source(hc_op_t op, const double * j, double * src, double * lf, d
     HC_TRY_HIP(hc::grad_apply(op->ag, Qv, t, st));
     HC_TRY_HIP(hc::axpby(N, -1.0, t, 1.0, src, st));
   }
-  // lf += M g
-  if ((rc = mass(src, t)))
-    return rc;
-  HC_TRY_HIP(hc::axpby(N, 1.0, t, 1.0, lf, st));
+  // lf += M g (left untouched when a solve failed)
+  if (s1 == HC_OK && s2 == HC_OK)
+  {
+    if ((rc = mass(src, t)))
+      return rc;
+    HC_TRY_HIP(hc::axpby(N, 1.0, t, 1.0, lf, st));
+  }
   if (Q)
     HC_TRY_HIP(hipMemcpyAsync(Q, Qv, sizeof(double) * NN, hipMemcpyDeviceToDevice, st));
   (void)hipEventRecord(e1, st);

@@ -206,8 +206,9 @@ int hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a,
  * constant, no boundary conditions, as the reference's HelmholtzProjector with
  * an empty BCMap); then lf += M g.  src receives g, Q (optional, H1 nodes) the
  * potential.  Both solves: Jacobi-PCG to rtol (the reference: its default
- * GMRES).  stats: iterations of both solves summed.  Allocates its work
- * vectors and the unit-mass quadrature factors per call. */
+ * GMRES).  stats: iterations of both solves summed; lf is left unchanged when
+ * a solve fails.  Allocates its work vectors and the unit-mass quadrature
+ * factors per call. */
 int hc_op_div_free_source(hc_op_t op, const double * j, double * src, double * lf, double * Q, int project,
                           double rtol, int maxit, hc_solve_stats * stats, void * stream);
 /* Right-hand sides per work item of the multi-RHS apply and solve (0 =

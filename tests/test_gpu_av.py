@@ -175,3 +175,8 @@ def test_div_free_source(env, p, n, tri):
     lf2 = torch.from_numpy(lf0.copy()).cuda()
     g2, _, st2 = op.div_free_source(torch.from_numpy(j).cuda(), lf2, project=False, rtol=1e-12, maxit=20000)
     assert st2["converged"] and _rel(g2.cpu().numpy(), x) <= 1e-9
+    # a solve that cannot converge reports an error and leaves lf unchanged
+    lf3 = torch.from_numpy(lf0.copy()).cuda()
+    with pytest.raises(H.HcError):
+        op.div_free_source(torch.from_numpy(j).cuda(), lf3, project=True, rtol=1e-14, maxit=1)
+    assert torch.equal(lf3.cpu(), torch.from_numpy(lf0))
