@@ -29,6 +29,24 @@ next_line(std::istream & in, std::string & line)
   return false;
 }
 
+// integer of a header line, or an Error naming the file
+int64_t
+to_int(const std::string & s, const std::string & path)
+{
+  try
+  {
+    size_t pos = 0;
+    const long long v = std::stoll(s, &pos);
+    if (pos != s.size())
+      throw std::invalid_argument(s);
+    return v;
+  }
+  catch (const std::exception &)
+  {
+    throw Error("ReadMFEMMesh: expected an integer in " + path + ", got '" + s + "'");
+  }
+}
+
 constexpr int CUBE = 5;
 
 // corner offsets (dx, dy, dz) of the local vertices of an MFEM hexahedron
@@ -56,14 +74,14 @@ ReadMFEMMesh(const std::string & path)
     if (line == "dimension")
     {
       next_line(in, line);
-      dim = std::stoi(line);
+      dim = (int)to_int(line, path);
       if (dim != 3)
         throw Error("ReadMFEMMesh: only 3D meshes are supported");
     }
     else if (line == "elements")
     {
       next_line(in, line);
-      const int64_t ne = std::stoll(line);
+      const int64_t ne = to_int(line, path);
       for (int64_t e = 0; e < ne; e++)
       {
         if (!next_line(in, line))
@@ -86,7 +104,7 @@ ReadMFEMMesh(const std::string & path)
     else if (line == "boundary")
     {
       next_line(in, line);
-      const int64_t nb = std::stoll(line);
+      const int64_t nb = to_int(line, path);
       for (int64_t b = 0; b < nb; b++)
         if (!next_line(in, line))
           throw Error("ReadMFEMMesh: truncated boundary section");
@@ -94,9 +112,9 @@ ReadMFEMMesh(const std::string & path)
     else if (line == "vertices")
     {
       next_line(in, line);
-      nverts = std::stoll(line);
+      nverts = to_int(line, path);
       next_line(in, line);
-      if (std::stoi(line) != 3)
+      if (to_int(line, path) != 3)
         throw Error("ReadMFEMMesh: vertices must have 3 coordinates");
       coords.resize(3 * nverts);
       for (int64_t v = 0; v < nverts; v++)

@@ -141,6 +141,8 @@ EvaluateCellCenters(const StructuredHexMesh & mesh, int p, const std::vector<dou
                            A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
                            A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
         const size_t e = ((size_t)ez * n[1] + ey) * n[0] + ex;
+        if (!(std::fabs(det) > 0.0) || !std::isfinite(det))
+          throw Error("EvaluateCellCenters: degenerate element " + std::to_string(e));
         for (int k = 0; k < 3; k++)
         {
           double M[3][3];

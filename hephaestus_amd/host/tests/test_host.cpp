@@ -245,6 +245,12 @@ test_cpu()
            "0 0 0\n1 0 0\n0 1 0\n0 0 1\n";
     }
     EXPECT(throws([&] { ReadMFEMMesh(tmp); }));
+    {
+      // a non-numeric count is an Error, not an uncaught std::invalid_argument
+      std::ofstream w(tmp);
+      w << "MFEM mesh v1.0\ndimension\nthree\n";
+    }
+    EXPECT(throws([&] { ReadMFEMMesh(tmp); }));
     EXPECT(throws([&] { ReadMFEMMesh("/nonexistent/x.mesh"); }));
     std::remove(tmp.c_str());
   }
