@@ -149,6 +149,28 @@ test_cpu()
       EXPECT(err <= 1e-13);
     }
     EXPECT(throws([&] { EvaluateCellCenters(am, 2, std::vector<double>(5)); }));
+    {
+      // all vertices of the plane z = 0 moved onto z = 1/nz: the first layer of
+      // elements has zero volume, and the evaluation refuses it
+      StructuredHexMesh flat = am;
+      for (int j = 0; j <= am.ny; j++)
+        for (int i = 0; i <= am.nx; i++)
+        {
+          const size_t v0 = (size_t)j * (am.nx + 1) + i;
+          const size_t v1 = v0 + (size_t)(am.nx + 1) * (am.ny + 1);
+          for (int a = 0; a < 3; a++)
+            flat.vert[3 * v0 + a] = am.vert[3 * v1 + a];
+        }
+      std::vector<double> dofs;
+      for (int c = 0; c < 3; c++)
+      {
+        int64_t cnt = 1;
+        for (int a = 0; a < 3; a++)
+          cnt *= (a == c) ? (int64_t)n[a] : (int64_t)n[a] + 1;
+        dofs.insert(dofs.end(), cnt, 1.0);
+      }
+      EXPECT(throws([&] { EvaluateCellCenters(flat, 1, dofs); }));
+    }
   }
   // kernel registration of the AV formulation (av_formulation.cpp:150-196)
   {
