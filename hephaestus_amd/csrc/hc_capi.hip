@@ -233,6 +233,19 @@ apply_t(hc_op_t op, const T * x, T * y, int nrhs, const int * done, hipStream_t 
   return HC_OK;
 }
 
+// Per-element coefficients must be finite and non-negative (a negative value
+// makes the operator indefinite; zero switches a term off, as in the forms of
+// WeakCurlCurlKernel or of a unit mass matrix).
+int
+check_coefficients(int64_t E, const double * nu_e, const double * sigma_e, const char * who)
+{
+  for (int64_t e = 0; e < E; e++)
+    if (!(nu_e[e] >= 0) || !std::isfinite(nu_e[e]) || !(sigma_e[e] >= 0) || !std::isfinite(sigma_e[e]))
+      return fail(HC_EINVAL, std::string(who) + ": coefficients must be finite and non-negative (element " +
+                                 std::to_string(e) + ")");
+  return HC_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -259,6 +272,16 @@ hc_op_create(const hc_mesh_desc * mesh, int order, const double * nu_e, const do
     const int64_t m[3] = {(int64_t)mesh->nx * order + 1, (int64_t)mesh->ny * order + 1, (int64_t)mesh->nz * order + 1};
     if (m[0] * m[1] * m[2] >= (int64_t(1) << 31))
       return fail(HC_EINVAL, "hc_op_create: mesh too large (each vector component must hold fewer than 2^31 DOFs)");
+  }
+  if (!mesh->vert)
+    for (int a = 0; a < 3; a++)
+      if (!(mesh->len[a] > 0) || !std::isfinite(mesh->len[a]) || !std::isfinite(mesh->x0[a]))
+        return fail(HC_EINVAL, "hc_op_create: box lengths must be positive and finite, the origin finite");
+  {
+    const int64_t E = (int64_t)mesh->nx * mesh->ny * mesh->nz;
+    int rc = check_coefficients(E, nu_e, sigma_e, "hc_op_create");
+    if (rc)
+      return rc;
   }
   if (!(precision & (HC_PREC_FP64 | HC_PREC_FP32)))
     precision = HC_PREC_FP64;
@@ -373,6 +396,8 @@ hc_op_set_coefficients(hc_op_t op, const double * nu_e, const double * sigma_e)
 {
   if (!op || !nu_e || !sigma_e)
     return fail(HC_EINVAL, "hc_op_set_coefficients: null argument");
+  if (int rc = check_coefficients(op->E, nu_e, sigma_e, "hc_op_set_coefficients"))
+    return rc;
   op->nu.assign(nu_e, nu_e + op->E);
   op->sig.assign(sigma_e, sigma_e + op->E);
   op->coef_dirty = true;

@@ -334,6 +334,10 @@ private:
   hipStream_t own_st_ = nullptr; // solves issued on the null stream run here
 };
 
+// Geometry of op with the tile decomposition of candidate c, and the
+// tile-face assembly segments of that decomposition.
+void geom_for(const hc_op_s * op, const TileCandidate * c, Geom & g, PlaneSegs & segs);
+
 // Graph capture of the engine's steady-state rounds (process-wide switch,
 // off by default; hc_engine_set_graphs).
 bool graphs_enabled();

@@ -193,8 +193,14 @@ hc_server_s::start_request(Request * r)
   r->part = (double *)p;
   // f = b_src - K u_n  (K from the operator's quadrature data; its apply
   // writes u_ess on PEC rows, zeroed next), k = 0
-  HC_TRY_HIP(hc::apply_scaled<double>(op->plan64.g, op->plan64.segs, op->tb64, op->qd64, r->u, r->f,
-                                      r->ws.part64, 0.0, 1.0 / op->last_dt, cands64[op->p], op->desc_dev, st));
+  // the geometry follows the server's tile variant of this order (the
+  // operator's own plan may hold another decomposition)
+  const hc::TileCandidate * cand = cands64[op->p] ? cands64[op->p] : op->plan64.cand;
+  hc::Geom g;
+  hc::PlaneSegs segs;
+  hc::geom_for(op, cand, g, segs);
+  HC_TRY_HIP(hc::apply_scaled<double>(g, segs, op->tb64, op->qd64, r->u, r->f, r->ws.part64, 0.0,
+                                      1.0 / op->last_dt, cand, op->desc_dev, st));
   HC_TRY_HIP(hc::axpby(N, 1.0, r->bsrc, -1.0, r->f, st));
   HC_TRY_HIP(hc::zero_ess(op->g, r->f, st));
   HC_TRY_HIP(hc::axpby(N, 0.0, r->k, 0.0, r->k, st));
@@ -271,6 +277,11 @@ hc_server_s::loop()
         break;
     if (rc)
     {
+      // the admitted requests stay owned by the server (released with it)
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        active.insert(active.end(), fresh.begin(), fresh.end());
+      }
       set_error(rc);
       return;
     }

@@ -27,6 +27,8 @@ constexpr int GRAPH_MIN_ITERS = 64; // rounds of this many iterations replay a c
 // FP32 PCG: an FP64 residual replacement each time ||r|| fell by 10x
 constexpr double RELIABLE_DELTA2 = 1e-2;
 
+} // namespace
+
 void
 geom_for(const hc_op_s * op, const TileCandidate * c, Geom & g, PlaneSegs & segs)
 {
@@ -38,8 +40,6 @@ geom_for(const hc_op_s * op, const TileCandidate * c, Geom & g, PlaneSegs & segs
     g.nt[a] = (g.n[a] + g.T[a] - 1) / g.T[a];
   segs = make_plane_segs(g);
 }
-
-} // namespace
 
 void
 initial_states(PcgState & in, PcgState & out, int mode, bool fp32_ok)
@@ -714,6 +714,15 @@ struct hc_batch_s
   std::vector<hc::SlotStats> stats;
 };
 
+static int
+total_rhs(const hc_batch_s * bt)
+{
+  int n = 0;
+  for (const auto & e : bt->entries)
+    n += e.nrhs;
+  return n;
+}
+
 extern "C" {
 
 int
@@ -836,7 +845,17 @@ hc_batch_solve(hc_batch_t bt, int mode, hc_solve_stats * stats, void * stream)
   }
   double ms = 0;
   if ((rc = bt->engine.solve(mode, (hipStream_t)stream, bt->stats, &ms)))
+  {
+    // a failed solve may leave the engine on a compacted instance list: the
+    // next call rebuilds the tables from every entry
+    bt->dirty = true;
     return rc;
+  }
+  if (bt->stats.size() < (size_t)total_rhs(bt))
+  {
+    bt->dirty = true;
+    return hc::fail(HC_EHIP, "hc_batch_solve: engine returned fewer slot statistics than right-hand sides");
+  }
   int pos = 0;
   hc_solve_stats all{};
   all.converged = 1;

@@ -47,6 +47,16 @@ to_int(const std::string & s, const std::string & path)
   }
 }
 
+// A section count of the header: an integer in [0, 2^31).
+int64_t
+to_count(const std::string & s, const std::string & path, const char * what)
+{
+  const int64_t v = to_int(s, path);
+  if (v < 0 || v >= (int64_t(1) << 31))
+    throw Error(std::string("ReadMFEMMesh: ") + what + " count out of range in " + path + ": " + s);
+  return v;
+}
+
 constexpr int CUBE = 5;
 
 // corner offsets (dx, dy, dz) of the local vertices of an MFEM hexahedron
@@ -81,7 +91,7 @@ ReadMFEMMesh(const std::string & path)
     else if (line == "elements")
     {
       next_line(in, line);
-      const int64_t ne = to_int(line, path);
+      const int64_t ne = to_count(line, path, "element");
       for (int64_t e = 0; e < ne; e++)
       {
         if (!next_line(in, line))
@@ -104,7 +114,7 @@ ReadMFEMMesh(const std::string & path)
     else if (line == "boundary")
     {
       next_line(in, line);
-      const int64_t nb = to_int(line, path);
+      const int64_t nb = to_count(line, path, "boundary");
       for (int64_t b = 0; b < nb; b++)
         if (!next_line(in, line))
           throw Error("ReadMFEMMesh: truncated boundary section");
@@ -112,7 +122,7 @@ ReadMFEMMesh(const std::string & path)
     else if (line == "vertices")
     {
       next_line(in, line);
-      nverts = to_int(line, path);
+      nverts = to_count(line, path, "vertex");
       next_line(in, line);
       if (to_int(line, path) != 3)
         throw Error("ReadMFEMMesh: vertices must have 3 coordinates");

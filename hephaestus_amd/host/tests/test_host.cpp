@@ -273,6 +273,22 @@ test_cpu()
       w << "MFEM mesh v1.0\ndimension\nthree\n";
     }
     EXPECT(throws([&] { ReadMFEMMesh(tmp); }));
+    {
+      // negative and huge section counts are Errors, not length_error / bad_alloc
+      std::ofstream w(tmp);
+      w << "MFEM mesh v1.0\ndimension\n3\nelements\n0\nboundary\n0\nvertices\n-8\n3\n";
+    }
+    EXPECT(throws([&] { ReadMFEMMesh(tmp); }));
+    {
+      std::ofstream w(tmp);
+      w << "MFEM mesh v1.0\ndimension\n3\nelements\n-1\n";
+    }
+    EXPECT(throws([&] { ReadMFEMMesh(tmp); }));
+    {
+      std::ofstream w(tmp);
+      w << "MFEM mesh v1.0\ndimension\n3\nvertices\n99999999999\n3\n";
+    }
+    EXPECT(throws([&] { ReadMFEMMesh(tmp); }));
     EXPECT(throws([&] { ReadMFEMMesh("/nonexistent/x.mesh"); }));
     std::remove(tmp.c_str());
   }
