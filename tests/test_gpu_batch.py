@@ -166,6 +166,47 @@ def test_batch_uneven_mix_invariance(torch_cuda, hc, O, mode):
         assert st1[0]["iterations"] == stats[i]["iterations"]
 
 
+# sigma/dt at both ends of the uneven-batch range [1e-6, 1e4] (inner / outer
+# region), orders 1-3, uneven edges
+SPAN = [((8, 9, 8), 1, 1e4, 1e-6), ((10, 8, 9), 2, 1e-6, 1e4), ((8, 8, 7), 3, 1e4, 1e-6),
+        ((12, 10, 8), 1, 1e-6, 1e-6), ((9, 7, 10), 2, 1e4, 1e4), ((7, 8, 8), 3, 1e-6, 1e4)]
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32_ir_aux"])
+def test_batch_sigma_span(torch_cuda, hc, O, mode):
+    """Uneven batch whose instances span sigma/dt 1e-6 and 1e4, coil right-hand
+    sides: every instance converges, its true residual recomputed by the oracle
+    meets rtol 1e-8, and it is bitwise the same solved alone."""
+    torch = torch_cuda
+    md = hc.SOLVE_FP64 if mode == "fp64" else hc.SOLVE_FP32_IR | hc.SOLVE_AUX
+    print(f"case sigma span {mode} ({len(SPAN)} instances)", flush=True)
+    ops = _instances(hc, SPAN)
+    rtol = 1e-8
+    bs = [op.rhs_coil(200 + i) for i, (op, _) in enumerate(ops)]
+    xs = [torch.zeros_like(b) for b in bs]
+    bt = hc.Batch()
+    for (op, _), b, x in zip(ops, bs, xs):
+        bt.add(op, b, x, rtol=rtol, maxit=50000)
+    rc, stats = bt.solve(md)
+    assert rc == 0 and all(s["converged"] for s in stats)
+    for i, ((op, sig), b, x) in enumerate(zip(ops, bs, xs)):
+        n, p = SPAN[i][0], SPAN[i][1]
+        m = O.Mesh(*n)
+        qd = O.pa_setup(m, p, 1.0, sig)
+        ess = O.ess_mask(m, p)
+        bh, xh = b.cpu().numpy(), x.cpu().numpy()
+        res = np.linalg.norm(bh - O.pa_apply(m, p, qd, ess, xh)) / np.linalg.norm(bh)
+        print(f"instance {i} n={n} p={p} sigma {SPAN[i][2]:.0e}/{SPAN[i][3]:.0e}: "
+              f"{stats[i]['iterations']} it, oracle residual {res:.2e}", flush=True)
+        assert res <= rtol * (1 + 1e-6), i
+        x1 = torch.zeros_like(b)
+        b1 = hc.Batch()
+        b1.add(op, b, x1, rtol=rtol, maxit=50000)
+        rc1, st1 = b1.solve(md)
+        assert rc1 == 0
+        assert torch.equal(x1, x), f"instance {i} differs alone vs in the batch"
+
+
 def test_batch_errors(torch_cuda, hc):
     torch = torch_cuda
     op = hc.Operator(4, 1, 1.0, 1.0, precision=hc.PREC_FP64).setup(1.0)

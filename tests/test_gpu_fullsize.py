@@ -171,23 +171,24 @@ def test_full_size_solves(env):
 
 
 def test_multi_rhs_32_order2_batched_vs_single(env):
-    """32^3 order 2 (811,200 DOFs), four coil right-hand sides solved as one
-    batch and one at a time: each converges to 1e-8 in the true residual and
-    the batched solution equals the single solve's to 1e-12 relative."""
+    """32^3 order 2 (811,200 DOFs), the 32 coil right-hand sides of the
+    multi-RHS workload (seeds 3..34) solved as one batch and one at a time:
+    each converges to 1e-8 in the true residual and the batched solution
+    equals the single solve's to 1e-12 relative."""
     torch, H = env
-    n, p = 32, 2
-    print(f"case multi-rhs n={n} p={p}", flush=True)
+    n, p, nr = 32, 2, 32
+    print(f"case multi-rhs n={n} p={p} nrhs={nr}", flush=True)
     op = H.Operator(n, p, 1.0, _sigma(n)).setup(1.0)
     N = op.ndofs
-    B = torch.empty((4, N), dtype=torch.float64, device="cuda")
-    for r in range(4):
+    B = torch.empty((nr, N), dtype=torch.float64, device="cuda")
+    for r in range(nr):
         op.rhs_coil(3 + r, B[r])
     rtol = 1e-8
     for mode in (H.SOLVE_FP64, H.SOLVE_FP32_IR | H.SOLVE_AUX):
         X = torch.zeros_like(B)
         st = op.solve(B, X, rtol=rtol, maxit=20000, mode=mode)
         assert st["converged"], mode
-        for r in range(4):
+        for r in range(nr):
             assert _true_rel_res(torch, op, B[r], X[r]) <= rtol * 1.01, (mode, r)
             x1 = torch.zeros_like(B[r])
             assert op.solve(B[r].contiguous(), x1, rtol=rtol, maxit=20000, mode=mode)["converged"]

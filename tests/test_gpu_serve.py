@@ -3,11 +3,15 @@
 Each request computes u <- u + dt k with (dt K + M) k = b_coil - K u.  Checks:
 one request against the oracle's step: the served k meets rtol 1e-10 on the
 oracle's assembled operator and u agrees with a sparse direct solve to relative
-L2 <= 1e-6 (rtol times the conditioning of this system); every request of
-a mix of uneven instances (orders 1-3, sizes 5-9) is bitwise identical to the
-same request served alone (mix invariance); an HBM budget that holds only two
-requests at a time bounds the admitted set and still completes every request;
-requests that can never fit are refused.
+L2 <= 1e-6 (rtol times the conditioning of this system), also with a server
+tile variant that differs from the operator's own (autotuned) one; every
+request of a mix of uneven instances (orders 1-3, sizes 5-9) is bitwise
+identical to the same request served alone (mix invariance), both when all
+arrive at once and when they arrive one after another (scheduled and Poisson
+arrival times, at most three admitted at once) so that later requests join a
+running batch; an HBM budget that holds only two requests at a time bounds the
+admitted set and still completes every request; requests that can never fit
+are refused.
 """
 import numpy as np
 import pytest
@@ -56,10 +60,10 @@ def _setup(torch, hc, specs):
     return ops, bs, us0
 
 
-def _serve(hc, torch, ops, bs, us, mode, budget=1 << 30, max_active=0):
+def _serve(hc, torch, ops, bs, us, mode, budget=1 << 30, max_active=0, arrivals=None):
     srv = hc.Server(budget, mode=mode, rtol=1e-9, maxit=20000, max_active=max_active, round_iters=8)
     for i, (op, b, u) in enumerate(zip(ops, bs, us)):
-        srv.submit(op, b, u, DT, tag=i)
+        srv.submit(op, b, u, DT, tag=i, arrival_ms=-1.0 if arrivals is None else float(arrivals[i]))
     res = {}
     while len(res) < len(ops):
         for r in srv.poll(wait_ms=2000):
@@ -91,6 +95,37 @@ def test_server_mix_invariance(torch_cuda, hc, mode):
         assert not torch.equal(alone, us0[i])
 
 
+@pytest.mark.parametrize("arrivals", ["staggered", "poisson"])
+def test_server_late_arrivals_join_bitwise(torch_cuda, hc, arrivals):
+    """Requests arrive while others are on the GPU (arrival times 4 ms apart, or
+    exponential inter-arrival times of mean 4 ms from a seed; at most three
+    admitted at once): later requests are admitted into the running batch as
+    slots free up, and each result is bitwise the one of the same request
+    served alone, with the same iteration count."""
+    torch = torch_cuda
+    md = hc.SOLVE_FP32_IR | hc.SOLVE_AUX
+    print(f"case server late arrivals {arrivals}", flush=True)
+    ops, bs, us0 = _setup(torch, hc, MIX)
+    if arrivals == "staggered":
+        at = [4.0 * i for i in range(len(MIX))]
+    else:
+        at = np.cumsum(np.random.default_rng(5).exponential(4.0, len(MIX))).tolist()
+    mixed = [u.clone() for u in us0]
+    res, info = _serve(hc, torch, ops, bs, mixed, md, max_active=3, arrivals=at)
+    assert info["completed"] == len(MIX)
+    assert all(r["converged"] for r in res.values())
+    # some request was admitted while another one was running
+    span = {t: (r["arrival_ms"] + r["queue_ms"], r["arrival_ms"] + r["latency_ms"]) for t, r in res.items()}
+    joined = [(i, j) for i in span for j in span if i != j and span[i][0] < span[j][0] < span[i][1]]
+    assert joined, span
+    for i in range(len(MIX)):
+        assert res[i]["arrival_ms"] >= at[i] - 1e-6
+        alone = us0[i].clone()
+        r1, _ = _serve(hc, torch, [ops[i]], [bs[i]], [alone], md)
+        assert r1[0]["iterations"] == res[i]["iterations"], i
+        assert torch.equal(alone, mixed[i]), f"request {i} differs alone vs joining a running batch"
+
+
 def test_server_budget_admission(torch_cuda, hc):
     torch = torch_cuda
     print("case server budget", flush=True)
@@ -112,12 +147,14 @@ def test_server_budget_admission(torch_cuda, hc):
     srv.close()
 
 
-def test_server_step_vs_oracle(torch_cuda, hc):
+@pytest.mark.parametrize("tiles", ["operator", "server_tile"])
+def test_server_step_vs_oracle(torch_cuda, hc, tiles):
     import scipy.sparse.linalg as spla
+    from hephaestus_amd import hcurl
     from oracle import oracle as O
     torch = torch_cuda
     n, p = 6, 2
-    print("case server step vs oracle", flush=True)
+    print(f"case server step vs oracle ({tiles})", flush=True)
     nu = _two_region(n, 0.5, 1.0)
     sig = _two_region(n, 1e2, 1e-2)
     op = hc.Operator(n, p, nu, sig).setup(DT)
@@ -127,6 +164,14 @@ def test_server_step_vs_oracle(torch_cuda, hc):
     u.mul_(0.1)
     u0 = u.cpu().numpy()
     srv = hc.Server(1 << 28, mode=hc.SOLVE_FP32_IR, rtol=1e-10, maxit=20000)
+    if tiles == "server_tile":
+        # the operator keeps its autotuned variant; the server runs another one
+        op.autotune(reps=2)
+        own = op.get_tile(64)
+        other = [t for t in hcurl.tile_list(p) if t != own and t[3] < 8][-1]
+        srv.set_tile(p, 64, *other)
+        srv.set_tile(p, 32, *other)
+        print(f"operator tile {own}, server tile {other}", flush=True)
     srv.submit(op, b, u, DT, tag=7)
     r = srv.poll(wait_ms=30000)
     srv.close()
