@@ -1,5 +1,6 @@
 """Discretisation check of the HIP solve (needs an MI355X): the method of
-manufactured solutions on hex meshes up to 32^3, the GPU-path counterpart of
+manufactured solutions on hex meshes up to 32^3 (24^3 at order 3; solves to
+rtol 1e-11, far below the discretisation error), the GPU-path counterpart of
 the reference's MMS rate test (test/integration/test_aform_source.cpp:197-208,
 window 1.85 < r < 3.0 for order 2).
 
@@ -187,7 +188,7 @@ def test_mms_load_vector_vs_bruteforce():
         assert np.array_equal(_ess_mask(n, p), B.ess_mask((n, n, n), p).astype(bool)), p
 
 
-@pytest.mark.parametrize("p,ns", [(1, (8, 16, 32)), (2, (8, 16, 32)), (3, (8, 16, 32))])
+@pytest.mark.parametrize("p,ns", [(1, (8, 16, 32)), (2, (8, 16, 32)), (3, (6, 12, 24))])
 def test_mms_rate_gpu(env, p, ns):
     torch, H = env
     errs = []
@@ -200,7 +201,7 @@ def test_mms_rate_gpu(env, p, ns):
         op = H.Operator(n, p, NU, SIG, precision=H.PREC_FP64).setup(1.0)
         bd = torch.from_numpy(b).cuda()
         x = torch.zeros_like(bd)
-        st = op.solve(bd, x, rtol=1e-12, maxit=50000, mode=H.SOLVE_FP64 | H.SOLVE_AUX)
+        st = op.solve(bd, x, rtol=1e-11, maxit=50000, mode=H.SOLVE_FP64 | H.SOLVE_AUX)
         assert st["converged"], st
         mass = H.Operator(n, p, 0.0, 1.0, precision=H.PREC_FP64).setup(1.0)
         Mx = mass.apply(x)
