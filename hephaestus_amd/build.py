@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libhcurl.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["hc_kernels.hip", "hc_apply.hip", "hc_apply_qp.hip", "hc_pcg.hip", "hc_capi.hip", "hc_solve.hip", "hc_serve.hip", "hc_aux.hip", "hc_av.hip"]
+SOURCES = ["hc_kernels.hip", "hc_apply.hip", "hc_apply_qp.hip", "hc_pcg.hip", "hc_capi.hip", "hc_solve.hip", "hc_serve.hip", "hc_aux.hip", "hc_av.hip", "hc_noalloc.hip"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-munsafe-fp-atomics", "-pthread", "-Wall", "-Wno-unused-result", "-Wno-unused-function"]

@@ -242,7 +242,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
     continue;
   x = gmem((const T *)ad.x) + (int64_t)rhs * g.off[3];
   y = gmem((T *)ad.y) + (int64_t)rhs * g.off[3];
-  part = gmem((T *)ad.part) + (int64_t)rhs * 4 * g.off[3];
+  part = gmem((T *)ad.part) + (int64_t)rhs * 4 * part_stride(ad);
   if (!x_ready)
     load_x(e0, ne);
   x_ready = false;
@@ -814,7 +814,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
       else
         eo = (alo == 0) ? ad.segs.eoff[c][0] : ((alo == 1) ? ad.segs.eoff[c][1] : ad.segs.eoff[c][2]);
       const int64_t pid = eo + ((int64_t)k * n1 + j1) * n0 + j0;
-      part[slot * N + pid] = sOut[i];
+      part[slot * part_stride(ad) + pid] = sOut[i];
     }
     else
       y[gid] = sOut[i];

@@ -131,7 +131,7 @@ k_apply_qp(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables
       continue;
     x = gmem((const T *)ad.x) + (int64_t)rhs * N;
     y = gmem((T *)ad.y) + (int64_t)rhs * N;
-    part = gmem((T *)ad.part) + (int64_t)rhs * 4 * N;
+    part = gmem((T *)ad.part) + (int64_t)rhs * 4 * part_stride(ad);
 
     // ---- element DOFs of x into R0 (PEC DOFs read as zero) ----
     if (eact)
@@ -522,7 +522,7 @@ k_apply_qp(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables
         const int n1 = (alo == 2) ? dims[1] : dims[2];
         const int64_t eo = BATCH ? seg_offset(eos, c, alo) : ad.segs.eoff[c][alo];
         const int64_t pid = eo + ((int64_t)(ga / ta - 1) * n1 + j1) * n0 + j0;
-        part[slot * N + pid] = v;
+        part[slot * part_stride(ad) + pid] = v;
       }
       else
         y[gid] = v;

@@ -387,6 +387,12 @@ hc_op_create(const hc_mesh_desc * mesh, int order, const double * nu_e, const do
       return fail(HC_EHIP, "hc_op_create: vertex upload failed");
     }
   }
+  // the scratch address of hc_op_apply_noalloc on this device
+  if ((rc = hc::noalloc_setup()))
+  {
+    delete op;
+    return rc;
+  }
   *out = op;
   return HC_OK;
 }

@@ -166,7 +166,8 @@ k_assemble_planes(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc)
   const int k = row / n1 + 1; // plane index 1 .. nt[a]-1
   const int j1 = row % n1;
   const int64_t N = g.off[3];
-  const T * part = (const T *)d.part + (int64_t)rhs * 4 * N;
+  const int64_t PS = part_stride(d);
+  const T * part = (const T *)d.part + (int64_t)rhs * 4 * PS;
   T * y = (T *)d.y + (int64_t)rhs * N;
   const int a2 = 3 - c - a; // the other closed axis of component c
   const int step2 = g.T[a2] * g.p;
@@ -199,11 +200,11 @@ k_assemble_planes(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc)
     const int64_t pid = ebase + j0; // compact entry (contiguous in j0)
     T v = part[pid];
     if (on_lo)
-      v += part[N + pid];
+      v += part[PS + pid];
     if (on_hi)
-      v += part[2 * N + pid];
+      v += part[2 * PS + pid];
     if (on_lo && on_hi)
-      v += part[3 * N + pid];
+      v += part[3 * PS + pid];
     y[gid] = v;
   }
 }

@@ -40,7 +40,7 @@ struct ApplyDesc
   const void * qd; // quadrature data (element type of the launch)
   const void * x;
   void * y;
-  void * part;     // 4 N per rhs: slot s of compact plane entry j at s*N + j
+  void * part;     // 4 slots per rhs: slot s of compact plane entry j at s*pstride + j
   const int * done;// per-rhs flags at done[r * done_stride], or null: rhs r
   int done_stride; // is applied only while its flag equals `gate`
   int gate = 0;
@@ -52,7 +52,15 @@ struct ApplyDesc
   int prows;       // plane rows per rhs
   double mscale = 1.0; // weight of the mass term (0: curl-curl part only)
   double cscale = 1.0; // weight of the curl-curl term
+  int64_t pstride = 0; // distance between the partial slots (0: N); a rhs takes 4 slots
 };
+
+// Slot stride of the tile-face partials of an instance.
+__host__ __device__ inline int64_t
+part_stride(const ApplyDesc & d)
+{
+  return d.pstride ? d.pstride : d.g.off[3];
+}
 
 // Locate the instance of a flattened work item: largest i with key(i) <= w.
 template <typename F>
@@ -156,7 +164,8 @@ struct TileCandidate
 {
   int p;
   TileShape shape;
-  int mode; // 0: qdata loaded at the quadrature stage, 1: at entry; +4: split qz range; 8: qp kernel
+  int mode; // 0: qdata loaded at the quadrature stage, 1: at entry; +2: per-component
+            // transposed-x stage and write-out (lean LDS); +4: split qz range; 8: qp kernel
   TileLaunch launch;
 };
 // The one-lane-per-quadrature-point tile kernel (hc_apply_qp.hip), mode 8.

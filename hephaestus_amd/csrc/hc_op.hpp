@@ -334,6 +334,10 @@ private:
   hipStream_t own_st_ = nullptr; // solves issued on the null stream run here
 };
 
+// Resolves the current device's address of the static scratch of
+// hc_op_apply_noalloc (hc_noalloc.hip); hc_op_create calls it once per operator.
+int noalloc_setup();
+
 // Geometry of op with the tile decomposition of candidate c, and the
 // tile-face assembly segments of that decomposition.
 void geom_for(const hc_op_s * op, const TileCandidate * c, Geom & g, PlaneSegs & segs);

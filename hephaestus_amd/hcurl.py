@@ -41,7 +41,7 @@ EXPORTS = [
     "hc_batch_solve", "hc_batch_destroy", "hc_op_rhs_coil", "hc_op_profile_apply", "hc_op_apply_curl", "hc_axpby", "hc_op_zero_ess",
     "hc_op_nnodes", "hc_op_av_apply", "hc_op_av_solve", "hc_op_div_free_source",
     "hc_server_request_bytes", "hc_server_create", "hc_server_set_tile", "hc_server_submit", "hc_server_poll",
-    "hc_server_info", "hc_server_destroy",
+    "hc_server_info", "hc_server_destroy", "hc_op_apply_noalloc", "hc_noalloc_scratch_doubles",
 ]
 
 
@@ -115,6 +115,8 @@ def load():
     lib.hc_op_device_bytes.restype = C.c_int64
     lib.hc_op_apply.argtypes = [vp, vp, vp, C.c_int, vp]
     lib.hc_op_apply_f32.argtypes = [vp, vp, vp, C.c_int, vp]
+    lib.hc_op_apply_noalloc.argtypes = [vp, vp, vp, vp]
+    lib.hc_noalloc_scratch_doubles.restype = C.c_int64
     lib.hc_op_diagonal.argtypes = [vp, vp, vp]
     lib.hc_op_precond_apply.argtypes = [vp, C.c_int, vp, vp, vp]
     lib.hc_op_term_ratio.argtypes = [vp, C.POINTER(C.c_double)]
@@ -317,6 +319,16 @@ class Operator:
                                           float(rtol), int(maxit), C.byref(st), _stream(stream)),
                "hc_op_div_free_source")
         return g, Q, st.as_dict()
+
+    def apply_noalloc(self, x, y=None, stream=None):
+        """y = A x (FP64, one right-hand side) through hc_op_apply_noalloc (no
+        device allocation; partials in the library's static scratch)."""
+        import torch
+        _nrhs(x, self.ndofs, torch.float64)
+        if y is None:
+            y = torch.empty_like(x)
+        _check(_lib.hc_op_apply_noalloc(self._h, _ptr(x), _ptr(y), _stream(stream)), "hc_op_apply_noalloc")
+        return y
 
     def apply_f32(self, x, y=None, stream=None):
         import torch

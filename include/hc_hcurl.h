@@ -166,6 +166,17 @@ int hc_engine_set_graphs(int enable);
  * right-hand sides, reading each tile's quadrature data once per group. */
 int hc_op_apply(hc_op_t op, const double * x, double * y, int nrhs, void * stream);
 int hc_op_apply_f32(hc_op_t op, const float * x, float * y, int nrhs, void * stream);
+/* y = A x (FP64, one right-hand side) without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops).  Same kernels and result bits
+ * as hc_op_apply with nrhs = 1; the tile-face partials live in a static device
+ * array of hc_noalloc_scratch_doubles() doubles per GPU (32 MiB), shared by
+ * every call on that GPU: calls that overlap on one GPU race.  HC_EINVAL when
+ * the operator's partials (4 slots of its tile-plane DOFs) exceed it.  The
+ * scratch address is resolved at hc_op_create (or at the first call on a
+ * device), so a call inside a capture only launches kernels. */
+int hc_op_apply_noalloc(hc_op_t op, const double * x, double * y, void * stream);
+int64_t hc_noalloc_scratch_doubles(void);
 /* y = K x with K the curl-curl part alone, (nu curl u, curl w), same PEC
  * convention as hc_op_apply (y_ess = x_ess).  The operator of
  * WeakCurlCurlKernel (src/kernels/weak_curl_curl_kernel.cpp:22-31: the
