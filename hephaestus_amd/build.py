@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libhcurl.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["hc_kernels.hip", "hc_apply.hip", "hc_apply_qp.hip", "hc_pcg.hip", "hc_capi.hip", "hc_solve.hip", "hc_serve.hip", "hc_aux.hip", "hc_av.hip", "hc_noalloc.hip"]
+SOURCES = ["hc_kernels.hip", "hc_apply.hip", "hc_apply_qp.hip", "hc_pcg.hip", "hc_capi.hip", "hc_solve.hip", "hc_serve.hip", "hc_aux.hip", "hc_av.hip", "hc_noalloc.hip", "hc_mg.hip"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-munsafe-fp-atomics", "-pthread", "-Wall", "-Wno-unused-result", "-Wno-unused-function"]
@@ -40,7 +40,7 @@ def build(verbose: bool = True) -> str:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", *objs, "-o", tmp,
-           "-Wl,-rpath,/opt/rocm/lib"]
+           "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")

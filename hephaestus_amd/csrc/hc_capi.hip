@@ -449,6 +449,7 @@ hc_op_setup(hc_op_t op, double dt, void * stream)
   op->setup_done = true;
   op->coef_dirty = false;
   op->last_dt = dt;
+  op->setup_version++; // a multigrid hierarchy of the old setup is rebuilt on its next use
   return HC_OK;
 }
 

@@ -146,6 +146,8 @@ struct ApplyPlan
 };
 
 class Engine;
+struct MgHier;
+void mg_free(MgHier * h);
 
 } // namespace hc
 
@@ -192,6 +194,10 @@ struct hc_op_s
   hc::DevBlock ws_mem;
   hc::Workspace ws;
   hc::Engine * engine = nullptr;
+  // geometric multigrid hierarchy of HC_SOLVE_MG (hc_mg.hip), built on first
+  // use and rebuilt when setup_version (counts the re-assemblies) moved on
+  hc::MgHier * mg = nullptr;
+  int setup_version = 0;
 
   template <typename T>
   int alloc(T *& ptr, int64_t count)
@@ -341,6 +347,12 @@ int noalloc_setup();
 // Geometry of op with the tile decomposition of candidate c, and the
 // tile-face assembly segments of that decomposition.
 void geom_for(const hc_op_s * op, const TileCandidate * c, Geom & g, PlaneSegs & segs);
+
+// PCG with the geometric multigrid preconditioner (hc_mg.hip), FP64, one
+// right-hand side; mg_levels reports the hierarchy (built if needed).
+int mg_pcg_solve(hc_op_t op, const double * b, double * x, double rtol, int maxit, hc_solve_stats * stats,
+                 hipStream_t st);
+int mg_levels(hc_op_t op, hipStream_t st, int * nlev, int64_t * ncoarse);
 
 // Graph capture of the engine's steady-state rounds (process-wide switch,
 // off by default; hc_engine_set_graphs).

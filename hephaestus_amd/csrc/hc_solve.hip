@@ -643,6 +643,7 @@ hc_op_s::ensure_ws(int nrhs, bool need32)
 hc_op_s::~hc_op_s()
 {
   delete engine;
+  hc::mg_free(mg);
   for (void * v : allocs)
     (void)hipFree(v);
 }
@@ -744,6 +745,38 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
     return hc::fail(HC_EINVAL, "hc_pcg_solve: nrhs must be >= 1");
   if (!(rtol > 0) || maxit < 1)
     return hc::fail(HC_EINVAL, "hc_pcg_solve: rtol must be > 0 and maxit >= 1");
+  if (mode & HC_SOLVE_MG)
+  {
+    // geometric multigrid V-cycle preconditioner, FP64 (HC_SOLVE_AUX implied:
+    // the hybrid smoother carries the gradient-space correction)
+    if (hc::prec_mode(mode & ~HC_SOLVE_MG) != HC_SOLVE_FP64 || (mode & ~(HC_SOLVE_MG | HC_SOLVE_AUX)))
+      return hc::fail(HC_EINVAL, "hc_pcg_solve: HC_SOLVE_MG runs in FP64 (mode HC_SOLVE_FP64 | HC_SOLVE_MG)");
+    hc_solve_stats all{};
+    all.converged = 1;
+    int rc_all = HC_OK;
+    for (int r = 0; r < nrhs; r++)
+    {
+      hc_solve_stats s{};
+      const int rc = hc::mg_pcg_solve(op, b + (int64_t)r * op->N, x + (int64_t)r * op->N, rtol, maxit, &s,
+                                      (hipStream_t)stream);
+      if (rc && rc != HC_ENOTCONV && rc != HC_ENAN)
+        return rc;
+      all.iterations = std::max(all.iterations, s.iterations);
+      all.outer_iterations = std::max(all.outer_iterations, s.outer_iterations);
+      all.rel_res = std::max(all.rel_res, s.rel_res);
+      all.solve_ms += s.solve_ms;
+      if (!s.converged)
+        all.converged = 0;
+      if (rc && !rc_all)
+      {
+        rc_all = rc;
+        all.status = rc;
+      }
+    }
+    if (stats)
+      *stats = all;
+    return rc_all ? hc::fail(rc_all, std::string("hc_pcg_solve: ") + hc_last_error()) : HC_OK;
+  }
   const int pm = hc::prec_mode(mode);
   if ((pm != HC_SOLVE_FP64 && pm != HC_SOLVE_FP32_IR) || (mode & ~(HC_SOLVE_AUX | 0xF)))
     return hc::fail(HC_EINVAL, "hc_pcg_solve: unknown mode");

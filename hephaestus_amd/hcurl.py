@@ -30,6 +30,7 @@ PREC_FP32 = 0x2
 SOLVE_FP64 = 0
 SOLVE_FP32_IR = 1
 SOLVE_AUX = 0x10  # flag: Jacobi + discrete-gradient auxiliary-space correction
+SOLVE_MG = 0x20  # flag: geometric multigrid V-cycle preconditioner (FP64)
 
 # Entry points declared in include/hc_hcurl.h (checked by the CPU test-suite).
 EXPORTS = [
@@ -42,6 +43,7 @@ EXPORTS = [
     "hc_op_nnodes", "hc_op_av_apply", "hc_op_av_solve", "hc_op_div_free_source",
     "hc_server_request_bytes", "hc_server_create", "hc_server_set_tile", "hc_server_submit", "hc_server_poll",
     "hc_server_info", "hc_server_destroy", "hc_op_apply_noalloc", "hc_noalloc_scratch_doubles",
+    "hc_mg_levels", "hc_mg_level_op", "hc_mg_transfer",
 ]
 
 
@@ -122,6 +124,9 @@ def load():
     lib.hc_op_term_ratio.argtypes = [vp, C.POINTER(C.c_double)]
     lib.hc_engine_set_graphs.argtypes = [C.c_int]
     lib.hc_pcg_solve.argtypes = [vp, vp, vp, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(SolveStats), vp]
+    lib.hc_mg_levels.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int64), vp]
+    lib.hc_mg_level_op.argtypes = [vp, C.c_int, C.POINTER(vp)]
+    lib.hc_mg_transfer.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
     lib.hc_op_destroy.argtypes = [vp]
     lib.hc_last_error.restype = C.c_char_p
     lib.hc_fill_uniform.argtypes = [vp, C.c_uint64, vp, C.c_int64, vp]
@@ -364,7 +369,8 @@ class Operator:
     def solve(self, b, x, rtol=1e-8, maxit=10000, mode=SOLVE_FP64, stream=None, raise_on_fail=True):
         """Jacobi-PCG (mode SOLVE_FP64) or FP32 PCG + FP64 refinement on A x = b;
         mode | SOLVE_AUX adds the gradient-space correction to the Jacobi
-        preconditioner.  b, x of shape (N,) or (nrhs, N), x holds the initial guess."""
+        preconditioner; SOLVE_FP64 | SOLVE_MG preconditions with the geometric
+        multigrid V-cycle.  b, x of shape (N,) or (nrhs, N), x holds the initial guess."""
         import torch
         nrhs = _nrhs(b, self.ndofs, torch.float64)
         if _nrhs(x, self.ndofs, torch.float64) != nrhs:
@@ -420,16 +426,46 @@ class Operator:
         _check(_lib.hc_op_rhs_coil(self._h, C.c_uint64(seed), _ptr(b), _stream(stream)), "hc_op_rhs_coil")
         return b
 
+    def mg_levels(self, stream=None):
+        """(levels, DOFs of the coarsest) of the SOLVE_MG hierarchy (built if needed)."""
+        nl, nc = C.c_int(), C.c_int64()
+        _check(_lib.hc_mg_levels(self._h, C.byref(nl), C.byref(nc), _stream(stream)), "hc_mg_levels")
+        return int(nl.value), int(nc.value)
+
+    def mg_level(self, level):
+        """The operator of a hierarchy level (borrowed: valid until this
+        operator is set up again or closed)."""
+        h = C.c_void_p()
+        _check(_lib.hc_mg_level_op(self._h, int(level), C.byref(h)), "hc_mg_level_op")
+        return _BorrowedOperator(h, self)
+
+    def mg_transfer(self, level, prolong, x, y, stream=None):
+        """y = P x (prolong: level+1 -> level) or y = P^T x, PEC entries zeroed
+        (restrict: level -> level+1)."""
+        _check(_lib.hc_mg_transfer(self._h, int(level), 1 if prolong else 0, _ptr(x), _ptr(y), _stream(stream)),
+               "hc_mg_transfer")
+        return y
+
     def close(self):
-        if getattr(self, "_h", None) and self._h.value:
+        if getattr(self, "_h", None) and self._h.value and not getattr(self, "_borrowed", False):
             _lib.hc_op_destroy(self._h)
-            self._h = C.c_void_p()
+        self._h = C.c_void_p()
 
     def __del__(self):
         try:
             self.close()
         except Exception:
             pass
+
+
+class _BorrowedOperator(Operator):
+    """An operator handle owned by another (a multigrid level): never destroyed here."""
+
+    def __init__(self, h, owner):
+        self._borrowed = True
+        self._h = h
+        self._owner = owner
+        self.ndofs = int(_lib.hc_op_ndofs(h))
 
 
 class Batch:

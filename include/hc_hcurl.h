@@ -90,6 +90,19 @@ extern "C" {
  * preconditioner that HCurlFormulation::ConstructJacobianPreconditioner sets,
  * src/formulations/HCurl/hcurl_formulation.cpp:69-79) instead of Jacobi alone */
 #define HC_SOLVE_AUX 0x10
+/* flag for hc_pcg_solve (with HC_SOLVE_FP64; box meshes whose element counts
+ * halve down to a mesh of at most 12000 DOFs): precondition with a geometric
+ * multigrid V-cycle over the nested meshes n, n/2, n/4, ... (same order, PEC
+ * faces, dt; coefficients averaged over the eight children): on every level
+ * two pre- and two post-smoothing steps of damped Richardson with the hybrid
+ * smoother D^-1 + G Dg^-1 G^T (edge Jacobi plus node Jacobi through the
+ * discrete gradient), the transfers of the nested ND spaces, and a dense
+ * Cholesky solve on the coarsest level.  The geometric counterpart, on
+ * structured meshes, of the multigrid auxiliary-space preconditioner HypreAMS
+ * that HCurlFormulation::ConstructJacobianPreconditioner sets
+ * (src/formulations/HCurl/hcurl_formulation.cpp:69-79).  Built on first use
+ * and kept with the operator until its next re-assembly. */
+#define HC_SOLVE_MG 0x20
 
 typedef struct hc_op_s * hc_op_t;
 typedef struct hc_batch_s * hc_batch_t;
@@ -259,6 +272,18 @@ int hc_pcg_solve(hc_op_t op,
                  int mode,
                  hc_solve_stats * stats,
                  void * stream);
+
+/* The multigrid hierarchy of HC_SOLVE_MG (built here if needed), for callers
+ * that inspect or test it; no reference counterpart (HypreAMS keeps its
+ * hierarchy internal, src/formulations/HCurl/hcurl_formulation.cpp:69-79).
+ * hc_mg_levels: number of levels (level 0 = op itself) and the DOFs of the
+ * coarsest.  hc_mg_level_op: the operator of a level (borrowed: valid until
+ * op is set up again or destroyed; level 0 returns op).  hc_mg_transfer:
+ * prolong (dir 1: y on level l = P x, x on level l+1) or restrict (dir 0:
+ * y on level l+1 = P^T x with PEC entries zeroed, x on level l). */
+int hc_mg_levels(hc_op_t op, int * nlevels, int64_t * coarse_ndofs, void * stream);
+int hc_mg_level_op(hc_op_t op, int level, hc_op_t * level_op);
+int hc_mg_transfer(hc_op_t op, int level, int dir, const double * x, double * y, void * stream);
 
 /* Batches of solves over several operator instances (differing sizes and
  * orders), advanced together: every iteration applies all instances of one
