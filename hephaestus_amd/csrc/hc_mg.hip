@@ -5,24 +5,26 @@
 // (src/formulations/HCurl/hcurl_formulation.cpp:69-79, AMS in singular mode):
 // an auxiliary-space method whose node-space (gradient) and vector-node-space
 // corrections are multigrid cycles.  This is the geometric counterpart on the
-// nested structured meshes: a V-cycle over the meshes n, n/2, n/4, ... with
-// the same ND_p space on each, the hybrid smoother of Hiptmair on every level
-// (damped Richardson with B = D^-1 + G Dg^-1 G^T: Jacobi on the edges plus
-// Jacobi on the node space through the discrete gradient, the correction that
-// AMS's gradient part performs), and a dense Cholesky solve on the coarsest
-// level.
+// nested structured meshes: a V-cycle whose levels halve the mesh (while the
+// element counts are even and the coefficients constant on 2x2x2 blocks, so
+// that each coarse operator is the Galerkin product of the finer one), then
+// drop the order to 1 on the same mesh, then (order 1, only while too large
+// for the coarsest solve) halve the mesh with averaged coefficients; the
+// hybrid smoother of Hiptmair on every level (B = D^-1 + G Dg^-1 G^T: Jacobi
+// on the edges plus Jacobi on the node space through the discrete gradient,
+// the correction that AMS's gradient part performs) inside a Chebyshev
+// iteration; and a dense Cholesky solve on the coarsest level.
 //
-// Levels: level 0 is the caller's operator; level l+1 halves every element
-// count of level l (all must be even), with the coefficients averaged over the
-// eight children, the same order, PEC faces and dt.  Transfers: the ND spaces
-// are nested, so prolongation evaluates the coarse field at the fine DOF
-// functionals -- per component a tensor product of 1-D interpolations (open
-// Gauss-Legendre basis along the component's axis, closed Gauss-Lobatto basis
-// along the others) times h_f / h_c = 1/2 (covariant scaling of the
-// tangential DOF); restriction is its transpose.  Both run as three 1-D passes.
-// The V-cycle (nu_1 = nu_2 smoothing steps with the same damped symmetric
-// smoother, exact coarse solve) is a fixed symmetric positive definite
-// operator, so plain PCG applies.
+// Transfers: the ND spaces are nested (ND_p on a mesh inside ND_p on its
+// refinement, ND_1 inside ND_p on the same mesh), so prolongation evaluates
+// the coarse field at the fine DOF functionals -- per component a tensor
+// product of 1-D interpolations (open Gauss-Legendre basis along the
+// component's axis, closed Gauss-Lobatto basis along the others) times
+// h_f / h_c along the component's axis (covariant scaling of the tangential
+// DOF); restriction is its transpose.  Both run as three 1-D passes.  The
+// V-cycle (the same polynomial smoother before and after an exact coarse
+// solve) is a fixed symmetric positive definite operator, so plain PCG
+// applies.
 #include "hc_op.hpp"
 
 #include <rocblas/rocblas.h>
@@ -42,6 +44,9 @@ namespace
 {
 
 constexpr int MG_SMOOTH = 2;            // pre- and post-smoothing steps
+// Chebyshev smoothing range lam_hi / lam_lo per order (measured on the
+// two-region coil problems: wider ranges pay off at higher order)
+constexpr double MG_CHEB_RANGE[4] = {0.0, 6.0, 6.0, 20.0};
 constexpr int64_t MG_COARSE_TARGET = 4000; // stop coarsening at this many DOFs
 constexpr int64_t MG_COARSE_MAX = 12000;   // largest coarsest level (dense inverse)
 constexpr int MG_POWER_ITERS = 15;      // power iterations for the smoother's lambda_max
@@ -54,30 +59,30 @@ constexpr int DOT_THREADS = 256;
 
 // One 1-D pass of a tensor-product transfer: out = scale * M (along axis) in,
 // + out when accumulate.  in: dims d[0..2] (x fastest); out: the same dims but
-// d[axis] -> m (the rows of M, CSR rp/ci/w).
-__global__ void
+// d[axis] -> m (the rows of M, CSR rp/ci/w).  Grid: x over the (i0, i1)
+// plane of the output, y over i2.
+__global__ void __launch_bounds__(256)
 k_pass1d(const double * __restrict__ in, double * __restrict__ out, int d0, int d1, int d2, int axis, int m,
          const int * __restrict__ rp, const int * __restrict__ ci, const double * __restrict__ w, double scale,
          int accumulate)
 {
-  const int o0 = axis == 0 ? m : d0, o1 = axis == 1 ? m : d1, o2 = axis == 2 ? m : d2;
-  const int64_t total = (int64_t)o0 * o1 * o2;
-  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x)
-  {
-    const int i0 = (int)(o % o0);
-    const int i1 = (int)((o / o0) % o1);
-    const int i2 = (int)(o / ((int64_t)o0 * o1));
-    const int row = axis == 0 ? i0 : (axis == 1 ? i1 : i2);
-    double s = 0.0;
-    for (int k = rp[row]; k < rp[row + 1]; k++)
-    {
-      const int j = ci[k];
-      const int j0 = axis == 0 ? j : i0, j1 = axis == 1 ? j : i1, j2 = axis == 2 ? j : i2;
-      s += w[k] * in[((int64_t)j2 * d1 + j1) * d0 + j0];
-    }
-    s *= scale;
-    out[o] = accumulate ? out[o] + s : s;
-  }
+  const unsigned o0 = axis == 0 ? m : d0, o1 = axis == 1 ? m : d1;
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= o0 * o1)
+    return;
+  const int i1 = (int)(t / o0);
+  const int i0 = (int)(t - (unsigned)i1 * o0);
+  const int i2 = (int)blockIdx.y;
+  const int row = axis == 0 ? i0 : (axis == 1 ? i1 : i2);
+  // input offset of (j, i) along the axis: base + j * stride
+  const int64_t stride = axis == 0 ? 1 : (axis == 1 ? (int64_t)d0 : (int64_t)d0 * d1);
+  const int64_t base = ((int64_t)(axis == 2 ? 0 : i2) * d1 + (axis == 1 ? 0 : i1)) * d0 + (axis == 0 ? 0 : i0);
+  double s = 0.0;
+  for (int k = rp[row]; k < rp[row + 1]; k++)
+    s += w[k] * in[base + ci[k] * stride];
+  s *= scale;
+  const int64_t o = ((int64_t)i2 * o1 + i1) * o0 + i0;
+  out[o] = accumulate ? out[o] + s : s;
 }
 
 // Block partials of (a, b) and (a, a) (fixed grid and order).
@@ -123,6 +128,33 @@ k_xr_update(int64_t n, double alpha, const double * __restrict__ p, const double
   }
 }
 
+// Chebyshev step: d = a z + b d (d = a z with flag 1: d not read); x = x + d
+// (x = d with flag 2: x not read)
+__global__ void
+k_cheb_update(int64_t n, double a, double b, const double * __restrict__ z, double * __restrict__ d,
+              double * __restrict__ x, int flags)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+  {
+    const double di = (flags & 1) ? a * z[i] : a * z[i] + b * d[i];
+    d[i] = di;
+    x[i] = (flags & 2) ? di : x[i] + di;
+  }
+}
+
+// Upper triangle of a column-major n x n matrix from its lower triangle.
+__global__ void
+k_symmetrize_lower(double * __restrict__ A, int64_t n)
+{
+  const int64_t total = n * n;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x)
+  {
+    const int64_t i = k % n, j = k / n; // element (i, j)
+    if (i < j)
+      A[k] = A[j + i * n];
+  }
+}
+
 // Identity columns j0 .. j0 + nb - 1 of an n x n matrix (column-major, nb columns).
 __global__ void
 k_identity_cols(double * __restrict__ e, int64_t n, int j0, int nb)
@@ -154,66 +186,6 @@ lagrange1(int n, const double * nodes, int j, double t)
     if (m != j)
       v *= (t - nodes[m]) / (nodes[j] - nodes[m]);
   return v;
-}
-
-struct Csr1
-{
-  std::vector<int> rp, ci;
-  std::vector<double> w;
-  int rows = 0, cols = 0;
-};
-
-// Prolongation of 1-D DOFs from nc coarse to 2 nc fine elements: open (p
-// Gauss-Legendre points per element) or closed (p+1 Gauss-Lobatto, shared at
-// element ends).  Row = fine DOF, entries = coarse basis values at its point.
-Csr1
-prolong_1d(int nc, int p, bool closed, const double * gl, const double * gll)
-{
-  Csr1 m;
-  const int nf = 2 * nc;
-  const int k = closed ? p + 1 : p; // local nodes per element
-  const double * nodes = closed ? gll : gl;
-  m.rows = closed ? nf * p + 1 : nf * p;
-  m.cols = closed ? nc * p + 1 : nc * p;
-  m.rp.push_back(0);
-  for (int gf = 0; gf < m.rows; gf++)
-  {
-    const int ef = std::min(gf / p, nf - 1);
-    const int jf = gf - ef * p;
-    const int ec = ef / 2, s = ef % 2;
-    const double t = (s + nodes[jf]) / 2.0; // position in the coarse element
-    for (int jc = 0; jc < k; jc++)
-    {
-      const double v = lagrange1(k, nodes, jc, t);
-      m.ci.push_back(ec * p + jc);
-      m.w.push_back(v);
-    }
-    m.rp.push_back((int)m.ci.size());
-  }
-  return m;
-}
-
-Csr1
-transpose(const Csr1 & a)
-{
-  Csr1 t;
-  t.rows = a.cols;
-  t.cols = a.rows;
-  std::vector<std::vector<std::pair<int, double>>> rows(t.rows);
-  for (int i = 0; i < a.rows; i++)
-    for (int k = a.rp[i]; k < a.rp[i + 1]; k++)
-      rows[a.ci[k]].push_back({i, a.w[k]}); // ascending fine index: a fixed summation order
-  t.rp.push_back(0);
-  for (auto & r : rows)
-  {
-    for (auto & e : r)
-    {
-      t.ci.push_back(e.first);
-      t.w.push_back(e.second);
-    }
-    t.rp.push_back((int)t.ci.size());
-  }
-  return t;
 }
 
 // Gauss-Legendre / Gauss-Lobatto points on [0,1] (Newton on P_n / P'_{n-1})
@@ -273,6 +245,69 @@ points01(int n, bool lobatto, double * x)
   }
 }
 
+struct Csr1
+{
+  std::vector<int> rp, ci;
+  std::vector<double> w;
+  int rows = 0, cols = 0;
+};
+
+// Prolongation of 1-D DOFs from nc coarse elements of order pc to nf = r nc
+// fine elements (r = 1 or 2) of order pf >= pc: open (p Gauss-Legendre points
+// per element) or closed (p+1 Gauss-Lobatto, shared at element ends).  Row =
+// fine DOF, entries = the coarse basis at its point (the coarse field is in
+// the fine space: ND_pc on a mesh is contained in ND_pf on its refinement).
+Csr1
+prolong_1d(int nc, int pc, int nf, int pf, bool closed)
+{
+  double gf[PMAX + 2], gc[PMAX + 2];
+  points01(closed ? pf + 1 : pf, closed, gf);
+  points01(closed ? pc + 1 : pc, closed, gc);
+  Csr1 m;
+  const int r = nf / nc;
+  const int kc = closed ? pc + 1 : pc; // coarse local nodes per element
+  m.rows = closed ? nf * pf + 1 : nf * pf;
+  m.cols = closed ? nc * pc + 1 : nc * pc;
+  m.rp.push_back(0);
+  for (int g = 0; g < m.rows; g++)
+  {
+    const int ef = std::min(g / pf, nf - 1);
+    const int jf = g - ef * pf;
+    const int ec = ef / r, sub = ef % r;
+    const double t = (sub + gf[jf]) / r; // position in the coarse element
+    for (int jc = 0; jc < kc; jc++)
+    {
+      m.ci.push_back(ec * pc + jc);
+      m.w.push_back(lagrange1(kc, gc, jc, t));
+    }
+    m.rp.push_back((int)m.ci.size());
+  }
+  return m;
+}
+
+Csr1
+transpose(const Csr1 & a)
+{
+  Csr1 t;
+  t.rows = a.cols;
+  t.cols = a.rows;
+  std::vector<std::vector<std::pair<int, double>>> rows(t.rows);
+  for (int i = 0; i < a.rows; i++)
+    for (int k = a.rp[i]; k < a.rp[i + 1]; k++)
+      rows[a.ci[k]].push_back({i, a.w[k]}); // ascending fine index: a fixed summation order
+  t.rp.push_back(0);
+  for (auto & r : rows)
+  {
+    for (auto & e : r)
+    {
+      t.ci.push_back(e.first);
+      t.w.push_back(e.second);
+    }
+    t.rp.push_back((int)t.ci.size());
+  }
+  return t;
+}
+
 // device copy of a CSR 1-D matrix (carved out of a DevBlock)
 struct Csr1Dev
 {
@@ -290,8 +325,10 @@ struct MgLevel
   bool owned = false;
   int n[3] = {0, 0, 0};
   int64_t N = 0;
-  double *x = nullptr, *b = nullptr, *r = nullptr, *z = nullptr;
-  double omega = 1.0;
+  double *x = nullptr, *b = nullptr, *r = nullptr, *z = nullptr, *d = nullptr;
+  double omega = 1.0; // Richardson damping 1 / (1.05 lambda_max)
+  double lam = 1.0;   // lambda_max(B A) estimate
+  double cheb = 0.0;  // Chebyshev range lam_hi / lam_lo of the smoother (<= 1: Richardson)
   // transfers to this level from the next coarser one: [axis][open=0/closed=1]
   Csr1Dev P[3][2], R[3][2];
 };
@@ -310,6 +347,7 @@ struct MgHier
   DevBlock ainv_mem;
   int64_t NL = 0;
   rocblas_handle blas = nullptr;
+  int nsmooth = MG_SMOOTH;   // smoothing steps before and after the coarse correction
   ~MgHier()
   {
     for (auto & l : lev)
@@ -366,13 +404,14 @@ transfer(MgHier & h, int l, bool prolong, const double * x, double * y, hipStrea
 {
   const MgLevel & f = h.lev[l];
   const MgLevel & c = h.lev[l + 1];
-  const int p = f.op->p;
   int64_t offf = 0, offc = 0;
   for (int comp = 0; comp < 3; comp++)
   {
     int df[3], dc[3];
-    comp_dims(f.n, p, comp, df);
-    comp_dims(c.n, p, comp, dc);
+    comp_dims(f.n, f.op->p, comp, df);
+    comp_dims(c.n, c.op->p, comp, dc);
+    // covariant DOF (h times the tangential component): h_f / h_c along the component's axis
+    const double scale = (double)c.n[comp] / f.n[comp];
     const double * src = x + (prolong ? offc : offf);
     double * dst = y + (prolong ? offf : offc);
     int d[3];
@@ -383,12 +422,11 @@ transfer(MgHier & h, int l, bool prolong, const double * x, double * y, hipStrea
       const Csr1Dev & m = prolong ? f.P[a][kind] : f.R[a][kind];
       double * out = (a == 2) ? dst : (a == 0 ? h.tmp1 : h.tmp2);
       const double * in = (a == 0) ? src : (a == 1 ? h.tmp1 : h.tmp2);
-      int64_t total = (int64_t)m.rows;
-      for (int q = 0; q < 3; q++)
-        if (q != a)
-          total *= d[q];
-      hipLaunchKernelGGL(k_pass1d, dim3(grid_for(total)), dim3(256), 0, st, in, out, d[0], d[1], d[2], a, m.rows,
-                         m.rp, m.ci, m.w, a == 2 ? 0.5 : 1.0, (a == 2 && prolong) ? 1 : 0);
+      const int64_t plane = (int64_t)(a == 0 ? m.rows : d[0]) * (a == 1 ? m.rows : d[1]);
+      const int o2 = a == 2 ? m.rows : d[2];
+      hipLaunchKernelGGL(k_pass1d, dim3((unsigned)((plane + 255) / 256), (unsigned)o2), dim3(256), 0, st, in, out,
+                         d[0], d[1], d[2], a, m.rows, m.rp, m.ci, m.w, a == 2 ? scale : 1.0,
+                         (a == 2 && prolong) ? 1 : 0);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess)
         return e;
@@ -417,6 +455,58 @@ residual(MgLevel & L, const double * b, const double * x, double * r, hipStream_
   return HC_OK;
 }
 
+// h.nsmooth steps of the smoother on a level, from x = 0 when zero_start.
+// Richardson (cheb_range 0): x += omega B (b - A x).  Chebyshev (cheb_range
+// alpha > 1): the Chebyshev iteration for B A on [lam_hi / alpha, lam_hi],
+// lam_hi = 1.1 lambda_max (Saad, Iterative Methods for Sparse Linear
+// Systems, 2nd ed., Algorithm 12.1, with the residual recomputed each step).
+// Either error propagator is a polynomial in B A, self-adjoint in the A
+// inner product, so the same smoother before and after the coarse correction
+// keeps the V-cycle symmetric.
+int
+smooth(MgHier & h, MgLevel & L, bool zero_start, hipStream_t st)
+{
+  double theta = 0, delta = 0, sigma1 = 0, rho0 = 0;
+  if (L.cheb > 1.0)
+  {
+    const double hi = 1.1 * L.lam, lo = hi / L.cheb;
+    theta = 0.5 * (hi + lo);
+    delta = 0.5 * (hi - lo);
+    sigma1 = theta / delta;
+    rho0 = 1.0 / sigma1;
+  }
+  for (int s = 0; s < h.nsmooth; s++)
+  {
+    const bool first = zero_start && s == 0;
+    if (int rc = residual(L, L.b, first ? nullptr : L.x, L.r, st))
+      return rc;
+    if (int rc = hc_op_precond_apply(L.op, HC_SOLVE_FP64 | HC_SOLVE_AUX, L.r, L.z, st))
+      return rc;
+    if (L.cheb > 1.0)
+    {
+      double a, b;
+      if (s == 0)
+      {
+        a = 1.0 / theta;
+        b = 0.0;
+      }
+      else
+      {
+        const double rho1 = 1.0 / (2.0 * sigma1 - rho0);
+        a = 2.0 * rho1 / delta;
+        b = rho1 * rho0;
+        rho0 = rho1;
+      }
+      hipLaunchKernelGGL(k_cheb_update, dim3(grid_for(L.N)), dim3(256), 0, st, L.N, a, b, L.z, L.d, L.x,
+                         (s == 0 ? 1 : 0) | (first ? 2 : 0));
+      HC_TRY_HIP(hipGetLastError());
+    }
+    else
+      HC_TRY_HIP(axpby(L.N, L.omega, L.z, first ? 0.0 : 1.0, L.x, st));
+  }
+  return HC_OK;
+}
+
 // x_l <- V-cycle(b_l) with x_l starting from zero
 int
 vcycle(MgHier & h, int l, hipStream_t st)
@@ -424,37 +514,23 @@ vcycle(MgHier & h, int l, hipStream_t st)
   MgLevel & L = h.lev[l];
   if (l == (int)h.lev.size() - 1)
   {
+    // the inverse is stored full (symmetric): x = A^-T b reads it by columns
     const double one = 1.0, zero = 0.0;
     if (rocblas_set_stream(h.blas, st) != rocblas_status_success ||
-        rocblas_dsymv(h.blas, rocblas_fill_lower, (rocblas_int)L.N, &one, h.Ainv, (rocblas_int)L.N, L.b, 1, &zero,
-                      L.x, 1) != rocblas_status_success)
-      return fail(HC_EHIP, "multigrid: coarse solve (rocblas_dsymv) failed");
+        rocblas_dgemv(h.blas, rocblas_operation_transpose, (rocblas_int)L.N, (rocblas_int)L.N, &one, h.Ainv,
+                      (rocblas_int)L.N, L.b, 1, &zero, L.x, 1) != rocblas_status_success)
+      return fail(HC_EHIP, "multigrid: coarse solve (rocblas_dgemv) failed");
     return HC_OK;
   }
-  // pre-smoothing from x = 0
-  for (int s = 0; s < MG_SMOOTH; s++)
-  {
-    if (int rc = residual(L, L.b, s == 0 ? nullptr : L.x, L.r, st))
-      return rc;
-    if (int rc = hc_op_precond_apply(L.op, HC_SOLVE_FP64 | HC_SOLVE_AUX, L.r, L.z, st))
-      return rc;
-    HC_TRY_HIP(axpby(L.N, L.omega, L.z, s == 0 ? 0.0 : 1.0, L.x, st));
-  }
+  if (int rc = smooth(h, L, true, st)) // pre-smoothing from x = 0
+    return rc;
   if (int rc = residual(L, L.b, L.x, L.r, st))
     return rc;
   HC_TRY_HIP(transfer(h, l, false, L.r, h.lev[l + 1].b, st));
   if (int rc = vcycle(h, l + 1, st))
     return rc;
   HC_TRY_HIP(transfer(h, l, true, h.lev[l + 1].x, L.x, st));
-  for (int s = 0; s < MG_SMOOTH; s++)
-  {
-    if (int rc = residual(L, L.b, L.x, L.r, st))
-      return rc;
-    if (int rc = hc_op_precond_apply(L.op, HC_SOLVE_FP64 | HC_SOLVE_AUX, L.r, L.z, st))
-      return rc;
-    HC_TRY_HIP(axpby(L.N, L.omega, L.z, 1.0, L.x, st));
-  }
-  return HC_OK;
+  return smooth(h, L, false, st); // post-smoothing
 }
 
 Csr1Dev
@@ -521,7 +597,6 @@ mg_build(hc_op_t op, hipStream_t st)
     return fail(HC_EINVAL, "multigrid: needs a box mesh (nested coarse meshes)");
   MgHier * h = new MgHier();
   std::unique_ptr<MgHier> guard(h);
-  const int p = op->p;
   MgLevel L0;
   L0.op = op;
   for (int a = 0; a < 3; a++)
@@ -529,30 +604,65 @@ mg_build(hc_op_t op, hipStream_t st)
   L0.N = op->N;
   h->lev.push_back(L0);
   std::vector<double> nu = op->nu, sig = op->sig;
+  // Coarsening, one step per level: halve the mesh while every element count
+  // is even and the coefficients are constant on each 2x2x2 block of children
+  // (the coarse operator is then the Galerkin product); else drop the order
+  // to 1 on the same mesh (ND_1 is contained in ND_p); else, at order 1 and
+  // only while the level is too large for the dense coarsest solve, halve the
+  // mesh with the coefficients averaged over the children.
   while (h->lev.back().N > MG_COARSE_TARGET)
   {
     const MgLevel & f = h->lev.back();
-    if ((f.n[0] % 2) || (f.n[1] % 2) || (f.n[2] % 2) || f.n[0] < 2 || f.n[1] < 2 || f.n[2] < 2)
+    const int fp = f.op->p;
+    const bool even = f.n[0] % 2 == 0 && f.n[1] % 2 == 0 && f.n[2] % 2 == 0;
+    bool aligned = even;
+    for (int k = 0; aligned && k < f.n[2]; k++)
+      for (int j = 0; aligned && j < f.n[1]; j++)
+        for (int i = 0; aligned && i < f.n[0]; i++)
+        {
+          const int64_t e = ((int64_t)k * f.n[1] + j) * f.n[0] + i;
+          const int64_t e0 = ((int64_t)(k & ~1) * f.n[1] + (j & ~1)) * f.n[0] + (i & ~1);
+          aligned = nu[e] == nu[e0] && sig[e] == sig[e0];
+        }
+    int step; // 0: halve (exact), 1: order -> 1, 2: halve (averaged)
+    if (aligned)
+      step = 0;
+    else if (fp > 1)
+      step = 1;
+    else if (even && f.N > MG_COARSE_MAX)
+      step = 2;
+    else
       break;
     MgLevel c;
     for (int a = 0; a < 3; a++)
-      c.n[a] = f.n[a] / 2;
-    const int64_t Ec = (int64_t)c.n[0] * c.n[1] * c.n[2];
-    std::vector<double> nuc(Ec, 0.0), sigc(Ec, 0.0);
-    for (int k = 0; k < c.n[2]; k++)
-      for (int j = 0; j < c.n[1]; j++)
-        for (int i = 0; i < c.n[0]; i++)
-        {
-          const int64_t ec = ((int64_t)k * c.n[1] + j) * c.n[0] + i;
-          for (int dz = 0; dz < 2; dz++)
-            for (int dy = 0; dy < 2; dy++)
-              for (int dx = 0; dx < 2; dx++)
-              {
-                const int64_t ef = ((int64_t)(2 * k + dz) * f.n[1] + (2 * j + dy)) * f.n[0] + (2 * i + dx);
-                nuc[ec] += 0.125 * nu[ef];
-                sigc[ec] += 0.125 * sig[ef];
-              }
-        }
+      c.n[a] = step == 1 ? f.n[a] : f.n[a] / 2;
+    const int cp = step == 1 ? 1 : fp;
+    std::vector<double> nuc, sigc;
+    if (step == 1)
+    {
+      nuc = nu;
+      sigc = sig;
+    }
+    else
+    {
+      const int64_t Ec = (int64_t)c.n[0] * c.n[1] * c.n[2];
+      nuc.assign(Ec, 0.0);
+      sigc.assign(Ec, 0.0);
+      for (int k = 0; k < c.n[2]; k++)
+        for (int j = 0; j < c.n[1]; j++)
+          for (int i = 0; i < c.n[0]; i++)
+          {
+            const int64_t ec = ((int64_t)k * c.n[1] + j) * c.n[0] + i;
+            for (int dz = 0; dz < 2; dz++)
+              for (int dy = 0; dy < 2; dy++)
+                for (int dx = 0; dx < 2; dx++)
+                {
+                  const int64_t ef = ((int64_t)(2 * k + dz) * f.n[1] + (2 * j + dy)) * f.n[0] + (2 * i + dx);
+                  nuc[ec] += 0.125 * nu[ef];
+                  sigc[ec] += 0.125 * sig[ef];
+                }
+          }
+    }
     hc_mesh_desc d{};
     d.nx = c.n[0];
     d.ny = c.n[1];
@@ -564,17 +674,22 @@ mg_build(hc_op_t op, hipStream_t st)
       d.len[a] = op->h[a] * op->g.n[a];
     }
     d.pec_faces = op->g.pec;
-    int rc = hc_op_create(&d, p, nuc.data(), sigc.data(), HC_PREC_FP64, &c.op);
+    int rc = hc_op_create(&d, cp, nuc.data(), sigc.data(), HC_PREC_FP64, &c.op);
     if (rc)
       return rc;
     c.owned = true;
     h->lev.push_back(c);
     if ((rc = hc_op_setup(h->lev.back().op, op->last_dt, st)))
       return rc;
+    // the level's own fastest tile variant (its mesh differs from the finest)
+    if ((rc = hc_op_autotune(h->lev.back().op, 3, nullptr, st)))
+      return rc;
     h->lev.back().N = h->lev.back().op->N;
     nu.swap(nuc);
     sig.swap(sigc);
   }
+  for (auto & L : h->lev)
+    L.cheb = MG_CHEB_RANGE[std::min(std::max(L.op->p, 1), 3)];
   const int nl = (int)h->lev.size();
   // a single level (a mesh small enough already) is solved by the dense inverse alone
   h->NL = h->lev.back().N;
@@ -583,16 +698,13 @@ mg_build(hc_op_t op, hipStream_t st)
                                " DOFs (element counts must halve, while even, down to a mesh of at most " +
                                std::to_string(MG_COARSE_MAX) + " DOFs)");
   // transfer matrices
-  double gl[PMAX + 1], gll[PMAX + 2];
-  points01(p, false, gl);
-  points01(p + 1, true, gll);
   std::vector<Csr1> mats; // per level pair: [axis][kind] P then R
   size_t bytes = 0;
   for (int l = 0; l + 1 < nl; l++)
     for (int a = 0; a < 3; a++)
       for (int kind = 0; kind < 2; kind++)
       {
-        Csr1 P = prolong_1d(h->lev[l + 1].n[a], p, kind == 1, gl, gll);
+        Csr1 P = prolong_1d(h->lev[l + 1].n[a], h->lev[l + 1].op->p, h->lev[l].n[a], h->lev[l].op->p, kind == 1);
         Csr1 R = transpose(P);
         bytes += csr_bytes(P) + csr_bytes(R);
         mats.push_back(std::move(P));
@@ -600,7 +712,7 @@ mg_build(hc_op_t op, hipStream_t st)
       }
   // vectors: x, b, r, z per level; two temporaries of the finest size; dot partials
   for (auto & L : h->lev)
-    bytes += 4 * align256(sizeof(double) * L.N);
+    bytes += 5 * align256(sizeof(double) * L.N);
   bytes += 3 * align256(sizeof(double) * op->N) + align256(sizeof(double) * 2 * DOT_BLOCKS);
   if (int rc = h->mem.alloc(bytes))
     return rc;
@@ -615,7 +727,7 @@ mg_build(hc_op_t op, hipStream_t st)
       }
   for (auto & L : h->lev)
   {
-    double ** v[4] = {&L.x, &L.b, &L.r, &L.z};
+    double ** v[5] = {&L.x, &L.b, &L.r, &L.z, &L.d};
     for (double ** pp : v)
     {
       *pp = (double *)q;
@@ -636,6 +748,7 @@ mg_build(hc_op_t op, hipStream_t st)
     double lam = 1.0;
     if (int rc = estimate_lambda(*h, h->lev[l], st, lam))
       return rc;
+    h->lev[l].lam = lam;
     h->lev[l].omega = 1.0 / (1.05 * lam);
   }
   // coarsest level: dense matrix by probing, Cholesky inverse (lower triangle)
@@ -673,6 +786,8 @@ mg_build(hc_op_t op, hipStream_t st)
     if (hinfo[0] || hinfo[1])
       return fail(HC_EHIP, "multigrid: coarsest-level matrix is not positive definite (info " +
                                std::to_string(hinfo[0]) + ", " + std::to_string(hinfo[1]) + ")");
+    hipLaunchKernelGGL(k_symmetrize_lower, dim3(grid_for(n * n)), dim3(256), 0, st, h->Ainv, n);
+    HC_TRY_HIP(hipGetLastError());
   }
   h->version = op->setup_version;
   op->mg = guard.release();
@@ -712,12 +827,17 @@ mg_pcg_solve(hc_op_t op, const double * b, double * x, double rtol, int maxit, h
   int iters = 0, outer = 0, status = HC_OK;
   bool conv = false;
   double rr = 0.0;
-  auto precond = [&](const double * rin, double * zout) -> int {
-    HC_TRY_HIP(hipMemcpyAsync(L.b, rin, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
-    if (int rc = vcycle(h, 0, st))
-      return rc;
-    HC_TRY_HIP(hipMemcpyAsync(zout, L.x, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
-    return HC_OK;
+  // the V-cycle reads the finest level's b and writes its x: point them at
+  // the PCG's residual and preconditioned residual for the call
+  auto precond = [&](double * rin, double * zout) -> int {
+    double * const b0 = L.b;
+    double * const x0 = L.x;
+    L.b = rin;
+    L.x = zout;
+    const int rc = vcycle(h, 0, st);
+    L.b = b0;
+    L.x = x0;
+    return rc;
   };
   for (outer = 1; outer <= 1 + MAX_RESTARTS && !conv && status == HC_OK; outer++)
   {

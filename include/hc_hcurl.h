@@ -90,16 +90,19 @@ extern "C" {
  * preconditioner that HCurlFormulation::ConstructJacobianPreconditioner sets,
  * src/formulations/HCurl/hcurl_formulation.cpp:69-79) instead of Jacobi alone */
 #define HC_SOLVE_AUX 0x10
-/* flag for hc_pcg_solve (with HC_SOLVE_FP64; box meshes whose element counts
- * halve down to a mesh of at most 12000 DOFs): precondition with a geometric
- * multigrid V-cycle over the nested meshes n, n/2, n/4, ... (same order, PEC
- * faces, dt; coefficients averaged over the eight children): on every level
- * two pre- and two post-smoothing steps of damped Richardson with the hybrid
- * smoother D^-1 + G Dg^-1 G^T (edge Jacobi plus node Jacobi through the
- * discrete gradient), the transfers of the nested ND spaces, and a dense
- * Cholesky solve on the coarsest level.  The geometric counterpart, on
- * structured meshes, of the multigrid auxiliary-space preconditioner HypreAMS
- * that HCurlFormulation::ConstructJacobianPreconditioner sets
+/* flag for hc_pcg_solve (with HC_SOLVE_FP64; box meshes): precondition with a
+ * multigrid V-cycle.  Levels: the mesh halves while its element counts are
+ * even and the coefficients are constant on each 2x2x2 block of children;
+ * then the order drops to 1 on the same mesh; then, at order 1 and only
+ * while a level exceeds 12000 DOFs, the mesh halves with the coefficients
+ * averaged (same PEC faces and dt throughout) -- a mesh that stays above
+ * 12000 DOFs is refused.  On every level two pre- and two post-smoothing
+ * steps of Chebyshev iteration with the hybrid smoother D^-1 + G Dg^-1 G^T
+ * (edge Jacobi plus node Jacobi through the discrete gradient), the
+ * transfers of the nested ND spaces, and a dense Cholesky solve on the
+ * coarsest level.  The geometric counterpart, on structured meshes, of the
+ * multigrid auxiliary-space preconditioner HypreAMS that
+ * HCurlFormulation::ConstructJacobianPreconditioner sets
  * (src/formulations/HCurl/hcurl_formulation.cpp:69-79).  Built on first use
  * and kept with the operator until its next re-assembly. */
 #define HC_SOLVE_MG 0x20

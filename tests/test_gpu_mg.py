@@ -1,13 +1,13 @@
 """Geometric multigrid preconditioner HC_SOLVE_MG (needs an MI355X).
 
 Transfers, pinned against the oracle: the ND spaces of the nested meshes are
-nested and both meshes integrate exactly, so with coefficients constant on
-every 2x2x2 block of fine elements the coarse operator is the Galerkin product
-P^T A_f P = A_c.  The check applies the GPU prolongation P and restriction
+nested (as is ND_1 in ND_p on one mesh) and both levels integrate exactly, so
+with coefficients constant on every 2x2x2 block of fine elements (or on the
+same mesh) the coarse operator is the Galerkin product P^T A_f P = A_c.  The check applies the GPU prolongation P and restriction
 P^T between the oracle's assembled fine and coarse operators (scipy CSR on the
 CPU) on random coarse vectors: R (A_f (P u)) = A_c u to 1e-12, for p = 1..3,
-PEC on all faces and on a partial set, on every level pair of a three-level
-hierarchy.  A wrong 1-D node set, a missing h_f / h_c scaling, a transposed
+PEC on all faces and on a partial set, mesh-halving and order-dropping steps,
+on every level pair of a three-level hierarchy.  A wrong 1-D node set, a missing h_f / h_c scaling, a transposed
 axis or a wrong PEC treatment breaks it.  The restriction is the transpose of
 the prolongation: (R x, y) = (x, P y) for y with zero PEC entries.
 
@@ -17,7 +17,8 @@ with the coefficients averaged over the eight children and the same dt
 dt (the hierarchy is rebuilt).
 
 Solves: MG-PCG reaches rtol on the oracle's assembled operator and agrees
-with a sparse direct solve on two-region coefficients (two and three levels);
+with a sparse direct solve on two-region coefficients (two and three levels,
+an odd mesh through an order-1 level);
 a mesh small enough for the coarsest level is solved directly (one
 iteration); iteration counts stay flat from 16^3 to 32^3 (order 2); meshes
 that cannot be coarsened down to a small level, and FP32 modes, are refused.
@@ -72,20 +73,30 @@ def _oracle_A(O, n, p, nu, sig, dt, pec):
     return O.Csr(m, p, qd, ess).to_scipy().tocsr(), ess.astype(bool)
 
 
-@pytest.mark.parametrize("p,nc,pec", [(1, 8, 0x3F), (2, 4, 0x3F), (3, 3, 0x3F), (2, 4, 0x15), (1, 8, 0x26)])
-def test_galerkin_transfer_vs_oracle(env, p, nc, pec):
+@pytest.mark.parametrize("p,nc,pec,step", [(1, 8, 0x3F, "h"), (2, 4, 0x3F, "h"), (3, 3, 0x3F, "h"), (2, 4, 0x15, "h"),
+                                          (1, 8, 0x26, "h"), (3, 5, 0x3F, "p"), (2, 6, 0x19, "p")])
+def test_galerkin_transfer_vs_oracle(env, p, nc, pec, step):
+    """step h: the fine mesh halves (coefficients constant on 2x2x2 blocks);
+    step p: odd or non-aligned coefficients, the order drops to 1 on the same
+    mesh (ND_1 inside ND_p, random per-element coefficients)."""
     torch, H, O = env
     rng = np.random.default_rng(10 * p + nc)
-    nf = 2 * nc
-    nu_c = rng.uniform(0.5, 2.0, nc ** 3)
-    sig_c = 10.0 ** rng.uniform(-2, 2, nc ** 3)
     dt = 0.3
-    print(f"case mg galerkin p={p} coarse {nc}^3 pec {pec:#x}", flush=True)
-    op = H.Operator(nf, p, _expand(nu_c, nc), _expand(sig_c, nc), precision=H.PREC_FP64, pec_faces=pec).setup(dt)
+    print(f"case mg galerkin p={p} coarse {nc}^3 pec {pec:#x} step {step}", flush=True)
+    if step == "h":
+        nf, pc = 2 * nc, p
+        nu_c = rng.uniform(0.5, 2.0, nc ** 3)
+        sig_c = 10.0 ** rng.uniform(-2, 2, nc ** 3)
+        nu_f, sig_f = _expand(nu_c, nc), _expand(sig_c, nc)
+    else:
+        nf, pc = nc, 1
+        nu_f = nu_c = rng.uniform(0.5, 2.0, nc ** 3)
+        sig_f = sig_c = 10.0 ** rng.uniform(-2, 2, nc ** 3)
+    op = H.Operator(nf, p, nu_f, sig_f, precision=H.PREC_FP64, pec_faces=pec).setup(dt)
     nl, ncoarse = op.mg_levels()
-    assert nl == 2 and ncoarse == O.ndofs(O.Mesh(nc), p)
-    Af, ess_f = _oracle_A(O, nf, p, _expand(nu_c, nc), _expand(sig_c, nc), dt, pec)
-    Ac, ess_c = _oracle_A(O, nc, p, nu_c, sig_c, dt, pec)
+    assert nl == 2 and ncoarse == O.ndofs(O.Mesh(nc), pc)
+    Af, ess_f = _oracle_A(O, nf, p, nu_f, sig_f, dt, pec)
+    Ac, ess_c = _oracle_A(O, nc, pc, nu_c, sig_c, dt, pec)
     for k in range(3):
         u = rng.standard_normal(Ac.shape[0])
         u[ess_c] = 0.0
@@ -153,12 +164,14 @@ def test_three_levels_galerkin_and_level_ops(env):
             direct.fill_uniform(3 + lvl, x)
             lo = op.mg_level(lvl)
             assert lo.ndofs == direct.ndofs
+            direct.set_tile(64, *lo.get_tile(64))  # (levels run their own autotuned tile variant)
             assert torch.equal(lo.apply(x), direct.apply(x)), (dt, lvl)
             direct.close()
     op.close()
 
 
-@pytest.mark.parametrize("n,p,levels,direct", [(8, 2, 2, True), (16, 1, 2, True), (12, 2, 3, False)])
+@pytest.mark.parametrize("n,p,levels,direct", [(8, 2, 2, True), (16, 1, 2, True), (12, 2, 2, False),
+                                               (16, 2, 3, False), (9, 3, 2, False)])
 def test_mg_solve_vs_direct(env, n, p, levels, direct):
     import scipy.sparse.linalg as spla
     torch, H, O = env
@@ -181,7 +194,7 @@ def test_mg_solve_vs_direct(env, n, p, levels, direct):
     print(f"  {st['iterations']} iterations, oracle residual {res:.2e}, vs direct {err:.2e}")
     assert st["converged"] and res <= 1e-10 * 1.05
     assert err <= 1e-7
-    assert st["iterations"] <= 80  # (interfaces off the coarse grids: 12 -> 6 -> 3 splits them)
+    assert st["iterations"] <= 40
     op.close()
 
 
@@ -225,7 +238,7 @@ def test_mg_iterations_flat_in_h(env):
 def test_mg_refusals(env):
     torch, H, O = env
     print("case mg refusals", flush=True)
-    op = H.Operator(9, 3, 1.0, 1.0, precision=H.PREC_FP64).setup(1.0)
+    op = H.Operator(17, 1, 1.0, 1.0, precision=H.PREC_FP64).setup(1.0)  # odd, order 1, 16524 DOFs
     b = op.rhs_coil(1)
     x = torch.zeros_like(b)
     with pytest.raises(H.HcError) as e:
