@@ -402,7 +402,7 @@ def solves_uneven(torch, H, ops, bs, args):
     return res, x_refs
 
 
-def serve_poisson(torch, H, ops, bs, args, c4, x_refs, aux=True, rate=200.0, nreq=None):
+def serve_poisson(torch, H, ops, bs, args, c4, x_refs, aux=True, rate=200.0, nreq=None, mg=False):
     """Poisson arrivals (mean `rate` req/s, seed 5) of backward-Euler step solves
     drawn uniformly from the 64 uneven operators (resident), FP32 PCG + FP64
     refinement to 1e-8 (preconditioner: Jacobi with the gradient-space
@@ -415,8 +415,8 @@ def serve_poisson(torch, H, ops, bs, args, c4, x_refs, aux=True, rate=200.0, nre
     arrivals_ms = np.cumsum(gaps) * 1000.0
     pick = rng.integers(0, len(ops), nreq)
     us = [torch.zeros_like(bs[k]) for k in pick]
-    srv = H.Server(args.serve_budget_gb * (1 << 30), mode=H.SOLVE_FP32_IR | (H.SOLVE_AUX if aux else 0), rtol=1e-8,
-                   maxit=args.solve_maxit, round_iters=16)
+    mode = H.SOLVE_FP64 | H.SOLVE_MG if mg else H.SOLVE_FP32_IR | (H.SOLVE_AUX if aux else 0)
+    srv = H.Server(args.serve_budget_gb * (1 << 30), mode=mode, rtol=1e-8, maxit=args.solve_maxit, round_iters=16)
     t0 = time.perf_counter()
     for i in range(nreq):
         srv.submit(ops[pick[i]], bs[pick[i]], us[i], 1.0, tag=i, arrival_ms=float(arrivals_ms[i]))
@@ -430,12 +430,14 @@ def serve_poisson(torch, H, ops, bs, args, c4, x_refs, aux=True, rate=200.0, nre
     span_ms = max(r["arrival_ms"] + r["latency_ms"] for r in out)
     sps = nreq / (span_ms / 1000.0)
     err = max_err(rel_err(torch, us[i], x_refs[pick[i]]) for i in range(nreq))
-    base = c4["fp64_aux_single" if aux else "fp64_single"]["solves_per_s"]
+    same = "fp64_mg_single" if mg else ("fp64_aux_single" if aux else "fp64_single")
+    base = c4[same]["solves_per_s"]
+    how = ("FP64 multigrid PCG, 4 worker streams (gradient-space PCG where a mesh has no hierarchy)" if mg else
+           f"FP32 PCG + FP64 refinement, {'Jacobi + gradient-space correction' if aux else 'Jacobi'} preconditioner")
     res = {"workload": f"{nreq} requests, Poisson arrivals mean {rate:g} req/s (seed 5), operators drawn from the "
-                       f"64-operator uneven mix, u_n = 0, rtol 1e-8, FP32 PCG + FP64 refinement, "
-                       f"{'Jacobi + gradient-space correction' if aux else 'Jacobi'} preconditioner",
+                       f"64-operator uneven mix, u_n = 0, rtol 1e-8, {how}",
            "arrival_rate_per_s": rate, "max_err_vs_fp64_ref": err,
-           f"speedup_vs_{'fp64_aux_single' if aux else 'fp64_single'} (same preconditioner)": round(sps / base, 3),
+           f"speedup_vs_{same} (same preconditioner, one at a time)": round(sps / base, 3),
            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
            "mean_queue_ms": round(float(np.mean([r["queue_ms"] for r in out])), 2),
            "solves_per_s": round(sps, 3), "converged": int(sum(r["converged"] for r in out)),
@@ -446,7 +448,7 @@ def serve_poisson(torch, H, ops, bs, args, c4, x_refs, aux=True, rate=200.0, nre
            "speedup_vs_fp64_single (plain FP64 Jacobi path)": round(sps / c4["fp64_single"]["solves_per_s"], 3),
            "speedup_vs_fp64_mg_single (fastest one-at-a-time FP64 path)":
                round(sps / c4["fp64_mg_single"]["solves_per_s"], 3)}
-    print(f"[bench] serving{' aux' if aux else ' jacobi'}: {res}", file=sys.stderr, flush=True)
+    print(f"[bench] serving{' mg' if mg else (' aux' if aux else ' jacobi')}: {res}", file=sys.stderr, flush=True)
     return res
 
 
@@ -596,6 +598,10 @@ def main():
             sust = 0.5 * out["serving"]["solves_per_s"]
             out["serving_sustained"] = serve_poisson(torch, H, ops4, bs4, args, c4, x_refs, aux=True,
                                                      rate=round(sust, 1), nreq=min(64, args.serve_requests))
+            out["serving_mg"] = serve_poisson(torch, H, ops4, bs4, args, c4, x_refs, mg=True)
+            sust = 0.5 * out["serving_mg"]["solves_per_s"]
+            out["serving_mg_sustained"] = serve_poisson(torch, H, ops4, bs4, args, c4, x_refs, mg=True,
+                                                        rate=round(sust, 1), nreq=min(64, args.serve_requests))
             del ops4, bs4, x_refs
             torch.cuda.empty_cache()
     if not args.no_cpu:

@@ -57,7 +57,7 @@ struct DevBlock
 inline int
 prec_mode(int mode)
 {
-  return mode & ~HC_SOLVE_AUX;
+  return mode & ~(HC_SOLVE_AUX | HC_SOLVE_MG);
 }
 
 inline size_t

@@ -15,6 +15,14 @@
 // that arrive while others run join the running batch.  Each slot's
 // arithmetic does not depend on the other slots: a request's result is the
 // same alone and in any mix (same tile variant per order).
+//
+// With HC_SOLVE_MG the solve of a request is the multigrid PCG of hc_mg.hip,
+// whose V-cycle does not batch across operators: up to max_active worker
+// threads (default 4) each take the next arrived request whose operator is
+// not in use by another worker, solve it on their own stream and retire it, so
+// the requests of different operators overlap on the device and a request
+// waits only for admission.  The result of a request is again independent of
+// the others (its operator's hierarchy, built on first use, is the same).
 #include "hc_op.hpp"
 
 #include <algorithm>
@@ -25,6 +33,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <set>
 #include <thread>
 
 namespace
@@ -117,6 +126,15 @@ request_bytes(int64_t N, bool ir)
          2 * align256(sizeof(hc::PcgState)) + align256(sizeof(double) * 4 * hc::RED_BLOCKS);
 }
 
+// multigrid mode: f and k (the solver's vectors belong to the operator)
+size_t
+request_bytes_mg(int64_t N)
+{
+  return 2 * hc::align256(sizeof(double) * N);
+}
+
+constexpr int MG_WORKERS_DEFAULT = 4;
+
 } // namespace
 
 struct hc_server_s
@@ -131,6 +149,11 @@ struct hc_server_s
   std::vector<Request *> active;
   std::deque<hc_request_result> results;
   std::thread th;
+  // multigrid mode: worker threads, operators in use
+  bool mg = false;
+  std::vector<std::thread> workers;
+  std::set<hc_op_t> busy;
+  int in_flight = 0;
   bool stop = false;
   int error = HC_OK;
   std::string error_msg;
@@ -143,6 +166,8 @@ struct hc_server_s
   int peak_active = 0;
 
   void loop();
+  void mg_worker();
+  int mg_request(Request * r, hipStream_t ws);
   int admit(std::vector<Request *> & fresh);
   int start_request(Request * r);
   int set_engine();
@@ -382,12 +407,142 @@ hc_server_s::loop()
   }
 }
 
+// One multigrid request on the worker's stream: f = b_src - K u, A k = f by
+// multigrid PCG (PCG with the gradient-space correction when the operator's
+// mesh has no hierarchy), u += dt k.
+int
+hc_server_s::mg_request(Request * r, hipStream_t ws)
+{
+  hc_op_t op = r->op;
+  const int64_t N = op->N;
+  char * p = (char *)mem.base + r->off;
+  r->f = (double *)p;
+  r->k = (double *)(p + hc::align256(sizeof(double) * N));
+  if (int rc = hc_op_apply_curl(op, r->u, r->f, ws))
+    return rc;
+  HC_TRY_HIP(hc::axpby(N, 1.0, r->bsrc, -1.0, r->f, ws));
+  HC_TRY_HIP(hc::zero_ess(op->g, r->f, ws));
+  HC_TRY_HIP(hc::axpby(N, 0.0, r->k, 0.0, r->k, ws));
+  int nlev = 0;
+  int64_t nc = 0;
+  const bool has_mg = hc::mg_levels(op, ws, &nlev, &nc) == HC_OK;
+  hc_solve_stats s{};
+  int rc = has_mg ? hc::mg_pcg_solve(op, r->f, r->k, cfg.rtol, cfg.maxit, &s, ws)
+                  : hc_pcg_solve(op, r->f, r->k, 1, cfg.rtol, cfg.maxit, HC_SOLVE_FP64 | HC_SOLVE_AUX, &s, ws);
+  if (rc && rc != HC_ENOTCONV && rc != HC_ENAN)
+    return rc;
+  HC_TRY_HIP(hc::axpby(N, r->dt, r->k, 1.0, r->u, ws));
+  HC_TRY_HIP(hipStreamSynchronize(ws));
+  hc_request_result o{};
+  o.tag = r->tag;
+  o.status = s.status ? s.status : (s.converged ? HC_OK : HC_ENOTCONV);
+  o.converged = s.converged;
+  o.iterations = s.iterations;
+  o.outer_iterations = s.outer_iterations;
+  o.rel_res = s.rel_res;
+  o.fp64_fallback = 0;
+  const auto now = Clock::now();
+  o.latency_ms = ms_between(r->arrival, now);
+  o.queue_ms = ms_between(r->arrival, r->admitted);
+  o.arrival_ms = ms_between(t0, r->arrival);
+  std::lock_guard<std::mutex> lk(mu);
+  results.push_back(o);
+  n_completed++;
+  cv_done.notify_all();
+  return HC_OK;
+}
+
+void
+hc_server_s::mg_worker()
+{
+  (void)hipSetDevice(0);
+  hipStream_t ws = nullptr;
+  if (hipStreamCreate(&ws) != hipSuccess)
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    error = hc::fail(HC_EHIP, "server: hipStreamCreate failed");
+    error_msg = hc_last_error();
+    cv_done.notify_all();
+    return;
+  }
+  for (;;)
+  {
+    Request * r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      for (;;)
+      {
+        if (error || (stop && pending.empty()))
+          break;
+        // the first arrived request whose operator is free and whose vectors fit
+        const auto now = Clock::now();
+        Clock::time_point wake = Clock::time_point::max();
+        for (auto it = pending.begin(); it != pending.end(); ++it)
+        {
+          if ((*it)->arrival > now)
+          {
+            wake = (*it)->arrival;
+            break;
+          }
+          if (busy.count((*it)->op))
+            continue;
+          size_t off;
+          const size_t need = request_bytes_mg((*it)->op->N);
+          if (!arena.alloc(need, off))
+            break; // first in, first admitted: wait for memory to free up
+          r = *it;
+          r->off = off;
+          r->bytes = need;
+          r->admitted = now;
+          pending.erase(it);
+          busy.insert(r->op);
+          in_flight++;
+          peak_active = std::max(peak_active, in_flight);
+          break;
+        }
+        if (r)
+          break;
+        if (wake != Clock::time_point::max())
+          cv_work.wait_until(lk, wake);
+        else
+          cv_work.wait(lk);
+      }
+    }
+    if (!r)
+      break;
+    const int rc = mg_request(r, ws);
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      arena.release(r->off, r->bytes);
+      busy.erase(r->op);
+      in_flight--;
+      if (rc && !error)
+      {
+        error = rc;
+        error_msg = hc_last_error();
+      }
+      delete r;
+      cv_done.notify_all();
+    }
+    cv_work.notify_all(); // an operator or memory became free
+    if (rc)
+      break;
+  }
+  (void)hipStreamSynchronize(ws);
+  (void)hipStreamDestroy(ws);
+  cv_work.notify_all();
+}
+
 extern "C" {
 
 int64_t
 hc_server_request_bytes(int64_t ndofs, int mode)
 {
-  return ndofs > 0 ? (int64_t)request_bytes(ndofs, hc::prec_mode(mode) == HC_SOLVE_FP32_IR) : -1;
+  if (ndofs <= 0)
+    return -1;
+  if (mode & HC_SOLVE_MG)
+    return (int64_t)request_bytes_mg(ndofs);
+  return (int64_t)request_bytes(ndofs, hc::prec_mode(mode) == HC_SOLVE_FP32_IR);
 }
 
 int
@@ -397,8 +552,10 @@ hc_server_create(const hc_server_config * cfg, hc_server_t * out)
     return hc::fail(HC_EINVAL, "hc_server_create: null argument");
   if (cfg->hbm_budget_bytes <= 0 || !(cfg->rtol > 0) || cfg->maxit < 1 ||
       (hc::prec_mode(cfg->mode) != HC_SOLVE_FP64 && hc::prec_mode(cfg->mode) != HC_SOLVE_FP32_IR) ||
-      (cfg->mode & ~(HC_SOLVE_AUX | 0xF)))
+      (cfg->mode & ~(HC_SOLVE_AUX | HC_SOLVE_MG | 0xF)))
     return hc::fail(HC_EINVAL, "hc_server_create: budget > 0, rtol > 0, maxit >= 1 and a solve mode are required");
+  if ((cfg->mode & HC_SOLVE_MG) && hc::prec_mode(cfg->mode) != HC_SOLVE_FP64)
+    return hc::fail(HC_EINVAL, "hc_server_create: HC_SOLVE_MG runs in FP64 (mode HC_SOLVE_FP64 | HC_SOLVE_MG)");
   hc_server_t s = new hc_server_s();
   s->cfg = *cfg;
   if (s->cfg.round_iters <= 0)
@@ -423,7 +580,15 @@ hc_server_create(const hc_server_config * cfg, hc_server_t * out)
       s->cands64[p] = s->cands32[p] = &c[0];
   }
   s->t0 = Clock::now();
-  s->th = std::thread([s] { s->loop(); });
+  s->mg = (cfg->mode & HC_SOLVE_MG) != 0;
+  if (s->mg)
+  {
+    const int nw = cfg->max_active > 0 ? cfg->max_active : MG_WORKERS_DEFAULT;
+    for (int i = 0; i < nw; i++)
+      s->workers.emplace_back([s] { s->mg_worker(); });
+  }
+  else
+    s->th = std::thread([s] { s->loop(); });
   *out = s;
   return HC_OK;
 }
@@ -458,7 +623,9 @@ hc_server_submit(hc_server_t s, hc_op_t op, const double * b_src, double * u, do
   const bool ir = hc::prec_mode(s->cfg.mode) == HC_SOLVE_FP32_IR;
   if (ir && !op->qd32)
     return hc::fail(HC_EINVAL, "hc_server_submit: FP32 refinement needs an operator created with HC_PREC_FP32");
-  if (request_bytes(op->N, ir) > (size_t)s->cfg.hbm_budget_bytes)
+  if (s->mg && !op->qd64)
+    return hc::fail(HC_EINVAL, "hc_server_submit: multigrid needs an operator created with HC_PREC_FP64");
+  if ((s->mg ? request_bytes_mg(op->N) : request_bytes(op->N, ir)) > (size_t)s->cfg.hbm_budget_bytes)
     return hc::fail(HC_EBUSY, "hc_server_submit: the request's work vectors exceed the whole memory budget");
   Request * r = new Request();
   r->op = op;
@@ -518,7 +685,7 @@ hc_server_info(hc_server_t s, hc_server_stats * st)
   st->completed = s->n_completed;
   st->rounds = s->n_rounds;
   st->pending = (int64_t)s->pending.size();
-  st->active = (int64_t)s->active.size();
+  st->active = s->mg ? (int64_t)s->in_flight : (int64_t)s->active.size();
   st->peak_active = s->peak_active;
   st->budget_bytes = s->cfg.hbm_budget_bytes;
   st->peak_bytes = (int64_t)s->arena.peak();
@@ -548,6 +715,9 @@ hc_server_destroy(hc_server_t s)
   s->cv_work.notify_all();
   if (s->th.joinable())
     s->th.join();
+  for (auto & w : s->workers)
+    if (w.joinable())
+      w.join();
   (void)hipStreamSynchronize(s->st);
   (void)hipStreamDestroy(s->st);
   delete s;

@@ -320,13 +320,20 @@ int hc_batch_destroy(hc_batch_t batch);
  * fewer than max_active are running, advances all admitted requests together
  * (one launch per ND order per operator apply), and retires each as soon as it
  * converges; a request's result does not depend on what else is running.
+ * With mode HC_SOLVE_FP64 | HC_SOLVE_MG each request is solved by the
+ * multigrid PCG of hc_pcg_solve (the gradient-space PCG for an operator whose
+ * mesh has no hierarchy): max_active worker threads (0 = 4), each with its own
+ * stream, take the next arrived request whose operator no other worker is
+ * using, so requests of different operators run concurrently; round_iters and
+ * hc_server_set_tile do not apply (operators keep their own tile variants).
  * The reference serves one problem per process and has no counterpart. */
 typedef struct hc_server_s * hc_server_t;
 typedef struct
 {
   int64_t hbm_budget_bytes; /* device memory for the work vectors of admitted requests */
   int max_active;           /* most requests running at once (0 = no limit) */
-  int mode;                 /* HC_SOLVE_FP64 or HC_SOLVE_FP32_IR */
+  int mode;                 /* HC_SOLVE_FP64 or HC_SOLVE_FP32_IR (| HC_SOLVE_AUX), or
+                               HC_SOLVE_FP64 | HC_SOLVE_MG */
   double rtol;              /* true relative residual of each solve */
   int maxit;                /* inner PCG iterations per request */
   int round_iters;          /* PCG iterations between scheduling decisions (0 = 16) */
@@ -353,7 +360,8 @@ typedef struct
   double elapsed_ms;
 } hc_server_stats;
 /* Work-vector bytes one request of an operator with N DOFs takes from the
- * budget (mode HC_SOLVE_FP64 or HC_SOLVE_FP32_IR). */
+ * budget (for the server's mode; with HC_SOLVE_MG the solver's vectors belong
+ * to the operator and only f, k count). */
 int64_t hc_server_request_bytes(int64_t ndofs, int mode);
 int hc_server_create(const hc_server_config * cfg, hc_server_t * out);
 /* Tile variant used for all operators of an order (before any submit). */

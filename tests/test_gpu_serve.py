@@ -4,7 +4,8 @@ Each request computes u <- u + dt k with (dt K + M) k = b_coil - K u.  Checks:
 one request against the oracle's step: the served k meets rtol 1e-10 on the
 oracle's assembled operator and u agrees with a sparse direct solve to relative
 L2 <= 1e-6 (rtol times the conditioning of this system), also with a server
-tile variant that differs from the operator's own (autotuned) one; every
+tile variant that differs from the operator's own (autotuned) one and with the
+multigrid workers (HC_SOLVE_MG); every
 request of a mix of uneven instances (orders 1-3, sizes 5-9) is bitwise
 identical to the same request served alone (mix invariance), both when all
 arrive at once and when they arrive one after another (scheduled and Poisson
@@ -74,12 +75,14 @@ def _serve(hc, torch, ops, bs, us, mode, budget=1 << 30, max_active=0, arrivals=
     return res, info
 
 
-@pytest.mark.parametrize("mode", ["fp32_ir", "fp64", "fp32_ir_aux"])
+@pytest.mark.parametrize("mode", ["fp32_ir", "fp64", "fp32_ir_aux", "fp64_mg"])
 def test_server_mix_invariance(torch_cuda, hc, mode):
     torch = torch_cuda
     md = hc.SOLVE_FP32_IR if mode.startswith("fp32_ir") else hc.SOLVE_FP64
     if mode.endswith("_aux"):
         md |= hc.SOLVE_AUX
+    if mode.endswith("_mg"):
+        md |= hc.SOLVE_MG
     print(f"case server mix {mode} ({len(MIX)} requests)", flush=True)
     ops, bs, us0 = _setup(torch, hc, MIX)
     mixed = [u.clone() for u in us0]
@@ -95,15 +98,16 @@ def test_server_mix_invariance(torch_cuda, hc, mode):
         assert not torch.equal(alone, us0[i])
 
 
-@pytest.mark.parametrize("arrivals", ["staggered", "poisson"])
-def test_server_late_arrivals_join_bitwise(torch_cuda, hc, arrivals):
+@pytest.mark.parametrize("arrivals,mode", [("staggered", "fp32_ir_aux"), ("poisson", "fp32_ir_aux"),
+                                           ("poisson", "fp64_mg")])
+def test_server_late_arrivals_join_bitwise(torch_cuda, hc, arrivals, mode):
     """Requests arrive while others are on the GPU (arrival times 4 ms apart, or
     exponential inter-arrival times of mean 4 ms from a seed; at most three
-    admitted at once): later requests are admitted into the running batch as
-    slots free up, and each result is bitwise the one of the same request
+    admitted at once, three workers with multigrid): later requests are admitted
+    as slots free up, and each result is bitwise the one of the same request
     served alone, with the same iteration count."""
     torch = torch_cuda
-    md = hc.SOLVE_FP32_IR | hc.SOLVE_AUX
+    md = hc.SOLVE_FP32_IR | hc.SOLVE_AUX if mode == "fp32_ir_aux" else hc.SOLVE_FP64 | hc.SOLVE_MG
     print(f"case server late arrivals {arrivals}", flush=True)
     ops, bs, us0 = _setup(torch, hc, MIX)
     if arrivals == "staggered":
@@ -147,7 +151,7 @@ def test_server_budget_admission(torch_cuda, hc):
     srv.close()
 
 
-@pytest.mark.parametrize("tiles", ["operator", "server_tile"])
+@pytest.mark.parametrize("tiles", ["operator", "server_tile", "multigrid"])
 def test_server_step_vs_oracle(torch_cuda, hc, tiles):
     import scipy.sparse.linalg as spla
     from hephaestus_amd import hcurl
@@ -163,7 +167,8 @@ def test_server_step_vs_oracle(torch_cuda, hc, tiles):
     op.fill_uniform(6, u)
     u.mul_(0.1)
     u0 = u.cpu().numpy()
-    srv = hc.Server(1 << 28, mode=hc.SOLVE_FP32_IR, rtol=1e-10, maxit=20000)
+    srv = hc.Server(1 << 28, mode=hc.SOLVE_FP64 | hc.SOLVE_MG if tiles == "multigrid" else hc.SOLVE_FP32_IR,
+                    rtol=1e-10, maxit=20000)
     if tiles == "server_tile":
         # the operator keeps its autotuned variant; the server runs another one
         op.autotune(reps=2)
@@ -195,3 +200,29 @@ def test_server_step_vs_oracle(torch_cuda, hc, tiles):
     print(f"server step vs oracle: relative L2 {err:.3e}, oracle residual of k {res:.3e}")
     assert res <= 1e-10 * 1.05
     assert err <= 1e-6
+
+
+MIX_MG = [(16, 2, 1e2, 1e-2), (12, 3, 10.0, 0.1), (17, 1, 1.0, 1.0), (9, 3, 50.0, 0.5), (20, 1, 2.0, 2.0),
+          (16, 2, 1e-3, 1e3)]
+
+
+def test_server_multigrid_multilevel_mix(torch_cuda, hc):
+    """Multigrid workers on operators with real hierarchies (halved meshes, an
+    order-1 level, an inexact coarsest solve at 12^3 p3 and 9^3 p3, a 17^3
+    order-1 mesh without a hierarchy served by the gradient-space PCG): every
+    request converges, several run at once, and each is bitwise the same
+    request served alone."""
+    torch = torch_cuda
+    md = hc.SOLVE_FP64 | hc.SOLVE_MG
+    print("case server multigrid multilevel mix", flush=True)
+    ops, bs, us0 = _setup(torch, hc, MIX_MG)
+    mixed = [u.clone() for u in us0]
+    res, info = _serve(hc, torch, ops, bs, mixed, md, max_active=3)
+    assert info["completed"] == len(MIX_MG)
+    assert all(r["converged"] and r["rel_res"] <= 1e-9 * (1 + 1e-6) for r in res.values()), res
+    assert info["peak_active"] >= 2
+    for i in range(len(MIX_MG)):
+        alone = us0[i].clone()
+        r1, _ = _serve(hc, torch, [ops[i]], [bs[i]], [alone], md)
+        assert r1[0]["iterations"] == res[i]["iterations"]
+        assert torch.equal(alone, mixed[i]), f"request {i} differs alone vs in the mix"
