@@ -18,6 +18,7 @@
 #include "hc_op.hpp"
 
 #include <cmath>
+#include <initializer_list>
 #include <vector>
 
 namespace hc
@@ -59,34 +60,7 @@ k_av_dot(int64_t n, const double * __restrict__ x, const double * __restrict__ y
     out[blockIdx.x] = s;
 }
 
-// x += alpha p, r -= alpha q, z = d r; out[b] = block parts of (r, z), out[AV_BLOCKS + b] of (r, r)
-__global__ void __launch_bounds__(AV_THREADS)
-k_av_update(int64_t n, double alpha, const double * __restrict__ p, const double * __restrict__ q,
-            const double * __restrict__ d, double * __restrict__ x, double * __restrict__ r, double * __restrict__ z,
-            double * __restrict__ out)
-{
-  __shared__ double sh[AV_THREADS];
-  double rz = 0.0, rr = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * AV_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * AV_THREADS)
-  {
-    x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
-    const double zi = d[i] * ri;
-    r[i] = ri;
-    z[i] = zi;
-    rz += ri * zi;
-    rr += ri * ri;
-  }
-  rz = av_block_sum(rz, sh);
-  rr = av_block_sum(rr, sh);
-  if (threadIdx.x == 0)
-  {
-    out[blockIdx.x] = rz;
-    out[AV_BLOCKS + blockIdx.x] = rr;
-  }
-}
-
-// r = b - q, z = d r, p = z; out as k_av_update
+// r = b - q, z = d r, p = z; out[b] = block parts of (r, z), out[AV_BLOCKS + b] of (r, r)
 __global__ void __launch_bounds__(AV_THREADS)
 k_av_start(int64_t n, const double * __restrict__ b, const double * __restrict__ q, const double * __restrict__ d,
            double * __restrict__ r, double * __restrict__ z, double * __restrict__ p, double * __restrict__ out)
@@ -112,10 +86,153 @@ k_av_start(int64_t n, const double * __restrict__ b, const double * __restrict__
   }
 }
 
-// p = z + beta p
-__global__ void
-k_av_xpby(int64_t n, const double * __restrict__ z, double beta, double * __restrict__ p)
+// Sum of n partials by one workgroup of AV_THREADS in a fixed order.
+__device__ inline double
+fixed_sum(const double * __restrict__ a, int n, double * sh)
 {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += AV_THREADS)
+    s += a[i];
+  return av_block_sum(s, sh);
+}
+
+} // namespace
+
+// State of a device-driven PCG (in device memory; the host reads it every
+// PCG_CHUNK iterations).  Partials: [0, 2 AV_BLOCKS) one dot per vector block,
+// [2 AV_BLOCKS, 6 AV_BLOCKS) the (r, z), (r, r) pairs of up to two blocks.
+struct PcgDev
+{
+  double bb, tol2, gam, alpha, beta, rr;
+  int it, maxit, done, status;
+};
+constexpr int PCG_CHUNK = 8;
+constexpr int PCG_PARTS = 6 * AV_BLOCKS;
+
+namespace
+{
+
+__global__ void __launch_bounds__(AV_THREADS)
+k_pcgd_start(const double * __restrict__ part, int nb, double rtol, int maxit, PcgDev * s)
+{
+  __shared__ double sh[AV_THREADS];
+  double bb = 0.0, gam = 0.0, rr = 0.0;
+  for (int k = 0; k < nb; k++)
+  {
+    bb += fixed_sum(part + k * AV_BLOCKS, AV_BLOCKS, sh);
+    gam += fixed_sum(part + (2 + 2 * k) * AV_BLOCKS, AV_BLOCKS, sh);
+    rr += fixed_sum(part + (3 + 2 * k) * AV_BLOCKS, AV_BLOCKS, sh);
+  }
+  if (threadIdx.x == 0)
+  {
+    s->bb = bb;
+    s->tol2 = rtol * rtol * bb;
+    s->gam = gam;
+    s->rr = rr;
+    s->alpha = s->beta = 0.0;
+    s->it = 0;
+    s->maxit = maxit;
+    s->status = HC_OK;
+    s->done = (bb == 0.0 || rr <= s->tol2) ? 1 : 0;
+  }
+}
+
+// alpha = gam / (p, q), or stop on a non-positive or non-finite (p, q)
+__global__ void __launch_bounds__(AV_THREADS)
+k_pcgd_alpha(const double * __restrict__ part, int nb, PcgDev * s)
+{
+  __shared__ double sh[AV_THREADS];
+  if (s->done)
+    return;
+  double pq = 0.0;
+  for (int k = 0; k < nb; k++)
+    pq += fixed_sum(part + k * AV_BLOCKS, AV_BLOCKS, sh);
+  if (threadIdx.x == 0)
+  {
+    if (!isfinite(pq) || !(pq > 0.0))
+    {
+      s->status = isfinite(pq) ? HC_ENOTCONV : HC_ENAN;
+      s->done = 1;
+    }
+    else
+      s->alpha = s->gam / pq;
+  }
+}
+
+// after the update: convergence test on (r, r), beta = (r, z)_new / (r, z)_old
+__global__ void __launch_bounds__(AV_THREADS)
+k_pcgd_beta(const double * __restrict__ part, int nb, PcgDev * s)
+{
+  __shared__ double sh[AV_THREADS];
+  if (s->done)
+    return;
+  double gnew = 0.0, rr = 0.0;
+  for (int k = 0; k < nb; k++)
+  {
+    gnew += fixed_sum(part + (2 + 2 * k) * AV_BLOCKS, AV_BLOCKS, sh);
+    rr += fixed_sum(part + (3 + 2 * k) * AV_BLOCKS, AV_BLOCKS, sh);
+  }
+  if (threadIdx.x == 0)
+  {
+    s->it++;
+    s->rr = rr;
+    if (!isfinite(gnew) || !isfinite(rr))
+    {
+      s->status = HC_ENAN;
+      s->done = 1;
+    }
+    else if (rr <= s->tol2)
+      s->done = 1;
+    else if (s->it >= s->maxit)
+    {
+      s->status = HC_ENOTCONV;
+      s->done = 1;
+    }
+    else
+    {
+      s->beta = gnew / s->gam;
+      s->gam = gnew;
+    }
+  }
+}
+
+// x += alpha p, r -= alpha q, z = d r with alpha from the state (nothing once done)
+__global__ void __launch_bounds__(AV_THREADS)
+k_pcgd_update(int64_t n, const PcgDev * __restrict__ s, const double * __restrict__ p, const double * __restrict__ q,
+              const double * __restrict__ d, double * __restrict__ x, double * __restrict__ r,
+              double * __restrict__ z, double * __restrict__ out)
+{
+  if (s->done)
+    return;
+  __shared__ double sh[AV_THREADS];
+  const double alpha = s->alpha;
+  double rz = 0.0, rr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * AV_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * AV_THREADS)
+  {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    const double zi = d[i] * ri;
+    r[i] = ri;
+    z[i] = zi;
+    rz += ri * zi;
+    rr += ri * ri;
+  }
+  rz = av_block_sum(rz, sh);
+  rr = av_block_sum(rr, sh);
+  if (threadIdx.x == 0)
+  {
+    out[blockIdx.x] = rz;
+    out[AV_BLOCKS + blockIdx.x] = rr;
+  }
+}
+
+// p = z + beta p (nothing once done)
+__global__ void
+k_pcgd_xpby(int64_t n, const PcgDev * __restrict__ s, const double * __restrict__ z, double * __restrict__ p)
+{
+  if (s->done)
+    return;
+  const double beta = s->beta;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = z[i] + beta * p[i];
 }
@@ -136,47 +253,6 @@ k_av_resid(int64_t n, const double * __restrict__ b, const double * __restrict__
     out[blockIdx.x] = rr;
 }
 
-// Device buffers of one solve, released on every path out.
-struct AvBuffers
-{
-  std::vector<void *> p;
-  ~AvBuffers()
-  {
-    for (void * q : p)
-      (void)hipFree(q);
-  }
-  double * get(size_t n)
-  {
-    void * q = nullptr;
-    if (hipMalloc(&q, n * sizeof(double)) != hipSuccess)
-      return nullptr;
-    p.push_back(q);
-    return (double *)q;
-  }
-};
-
-double
-host_sum(const double * v, int n)
-{
-  double s = 0.0;
-  for (int i = 0; i < n; i++)
-    s += v[i];
-  return s;
-}
-
-} // namespace
-
-// Copy nsums arrays of AV_BLOCKS block partials to the host (stream-ordered).
-static int
-read_partials(double * dev, double * host, int nsums, hipStream_t st)
-{
-  if (hipMemcpyAsync(host, dev, sizeof(double) * AV_BLOCKS * nsums, hipMemcpyDeviceToHost, st) != hipSuccess)
-    return HC_EHIP;
-  if (hipStreamSynchronize(st) != hipSuccess)
-    return HC_EHIP;
-  return HC_OK;
-}
-
 __global__ void
 k_fill_value(int64_t n, double v, double * __restrict__ y)
 {
@@ -184,65 +260,90 @@ k_fill_value(int64_t n, double v, double * __restrict__ y)
     y[i] = v;
 }
 
-// Preconditioned CG on one vector of n entries: apply(p, q) computes q = S p,
-// dinv the diagonal preconditioner; x holds the initial guess.  Stops when
-// ||r|| <= rtol ||b|| (recursive residual).  r, z, p, q: n-entry work vectors;
-// part: 2 * AV_BLOCKS device doubles; h: as many host doubles.
+// Pieces carved out of one device block, 256-byte aligned.
+struct Carver
+{
+  char * p;
+  double * take(int64_t n)
+  {
+    double * v = (double *)p;
+    p += align256(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    return v;
+  }
+};
+
+size_t
+bytes_of(std::initializer_list<int64_t> counts)
+{
+  size_t b = 0;
+  for (int64_t n : counts)
+    b += align256(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  return b;
+}
+
+} // namespace
+
+// One vector block of a device-driven PCG (nb = 1 or 2 blocks: the AV system
+// has an ND block and a node block).
+struct PcgBlock
+{
+  int64_t n;
+  const double * dinv;
+  const double * b;
+  double *x, *r, *z, *p, *q;
+};
+
+// Preconditioned CG with a diagonal preconditioner over nb vector blocks:
+// apply(in0, in1, out0, out1) computes out = S in.  Scalars stay on the
+// device (fixed-order partial sums, bitwise reproducible); the host reads the
+// state every PCG_CHUNK iterations, the kernels of the iterations after the
+// stopping point do nothing.  Stops when ||r|| <= rtol ||b|| on the recursive
+// residual.  part: PCG_PARTS device doubles; s: device state; hs: pinned
+// host copy.
 template <typename Apply>
 static int
-pcg_one(int64_t n, Apply && apply, const double * dinv, const double * b, double * x, double rtol, int maxit,
-        double * r, double * z, double * p, double * q, double * part, double * h, hipStream_t st, int & iters,
-        bool & conv, int & status)
+pcg_dev(int nb, const PcgBlock * bl, Apply && apply, double rtol, int maxit, double * part, PcgDev * s,
+        PcgDev * hs, hipStream_t st, int & iters, bool & conv, int & status)
 {
   const dim3 G(AV_BLOCKS), T(AV_THREADS);
-  iters = 0;
-  status = HC_OK;
-  int rc = apply(x, q);
-  if (rc)
+  if (int rc = apply(bl[0].x, nb > 1 ? bl[1].x : nullptr, bl[0].q, nb > 1 ? bl[1].q : nullptr))
     return rc;
-  hipLaunchKernelGGL(k_av_dot, G, T, 0, st, n, b, b, part);
-  if ((rc = read_partials(part, h, 1, st)))
-    return fail(rc, "PCG: reduction failed");
-  const double bb = host_sum(h, AV_BLOCKS);
-  hipLaunchKernelGGL(k_av_start, G, T, 0, st, n, b, (const double *)q, dinv, r, z, p, part);
-  if ((rc = read_partials(part, h, 2, st)))
-    return fail(rc, "PCG: reduction failed");
-  double gam = host_sum(h, AV_BLOCKS), rr = host_sum(h + AV_BLOCKS, AV_BLOCKS);
-  const double tol2 = rtol * rtol * bb;
-  conv = bb == 0.0 || rr <= tol2;
-  while (!conv && iters < maxit)
+  for (int k = 0; k < nb; k++)
   {
-    if ((rc = apply(p, q)))
-      return rc;
-    hipLaunchKernelGGL(k_av_dot, G, T, 0, st, n, (const double *)p, (const double *)q, part);
-    if ((rc = read_partials(part, h, 1, st)))
-      return fail(rc, "PCG: reduction failed");
-    const double pq = host_sum(h, AV_BLOCKS);
-    if (!std::isfinite(pq) || !(pq > 0.0))
-    {
-      status = std::isfinite(pq) ? HC_ENOTCONV : HC_ENAN;
-      break;
-    }
-    hipLaunchKernelGGL(k_av_update, G, T, 0, st, n, gam / pq, (const double *)p, (const double *)q, dinv, x, r, z,
-                       part);
-    if ((rc = read_partials(part, h, 2, st)))
-      return fail(rc, "PCG: reduction failed");
-    const double gnew = host_sum(h, AV_BLOCKS);
-    rr = host_sum(h + AV_BLOCKS, AV_BLOCKS);
-    iters++;
-    if (!std::isfinite(gnew) || !std::isfinite(rr))
-    {
-      status = HC_ENAN;
-      break;
-    }
-    conv = rr <= tol2;
-    if (conv)
-      break;
-    hipLaunchKernelGGL(k_av_xpby, dim3(2048), dim3(256), 0, st, n, (const double *)z, gnew / gam, p);
-    gam = gnew;
+    hipLaunchKernelGGL(k_av_dot, G, T, 0, st, bl[k].n, bl[k].b, bl[k].b, part + k * AV_BLOCKS);
+    hipLaunchKernelGGL(k_av_start, G, T, 0, st, bl[k].n, bl[k].b, (const double *)bl[k].q, bl[k].dinv, bl[k].r,
+                       bl[k].z, bl[k].p, part + (2 + 2 * k) * AV_BLOCKS);
   }
-  if (status == HC_OK && !conv)
-    status = HC_ENOTCONV;
+  hipLaunchKernelGGL(k_pcgd_start, dim3(1), T, 0, st, (const double *)part, nb, rtol, maxit, s);
+  HC_TRY_HIP(hipGetLastError());
+  for (;;)
+  {
+    HC_TRY_HIP(hipMemcpyAsync(hs, s, sizeof(PcgDev), hipMemcpyDeviceToHost, st));
+    HC_TRY_HIP(hipStreamSynchronize(st));
+    if (hs->done)
+      break;
+    for (int c = 0; c < PCG_CHUNK; c++)
+    {
+      if (int rc = apply(bl[0].p, nb > 1 ? bl[1].p : nullptr, bl[0].q, nb > 1 ? bl[1].q : nullptr))
+        return rc;
+      for (int k = 0; k < nb; k++)
+        hipLaunchKernelGGL(k_av_dot, G, T, 0, st, bl[k].n, (const double *)bl[k].p, (const double *)bl[k].q,
+                           part + k * AV_BLOCKS);
+      hipLaunchKernelGGL(k_pcgd_alpha, dim3(1), T, 0, st, (const double *)part, nb, s);
+      for (int k = 0; k < nb; k++)
+        hipLaunchKernelGGL(k_pcgd_update, G, T, 0, st, bl[k].n, (const PcgDev *)s, (const double *)bl[k].p,
+                           (const double *)bl[k].q, bl[k].dinv, bl[k].x, bl[k].r, bl[k].z,
+                           part + (2 + 2 * k) * AV_BLOCKS);
+      hipLaunchKernelGGL(k_pcgd_beta, dim3(1), T, 0, st, (const double *)part, nb, s);
+      for (int k = 0; k < nb; k++)
+        hipLaunchKernelGGL(k_pcgd_xpby, dim3(2048), dim3(256), 0, st, bl[k].n, (const PcgDev *)s,
+                           (const double *)bl[k].z, bl[k].p);
+      HC_TRY_HIP(hipGetLastError());
+    }
+  }
+  iters = hs->it;
+  status = hs->status;
+  conv = status == HC_OK;
   return HC_OK;
 }
 
@@ -278,17 +379,30 @@ hc_op_div_free_source(hc_op_t op, const double * j, double * src, double * lf, d
   hipStream_t st = (hipStream_t)stream;
   const int64_t N = op->N, NN = op->ag.NN, E = op->E;
   const int Qp = op->p + 1;
-  hc::AvBuffers buf;
-  double * qdu = buf.get((size_t)12 * E * Qp * Qp * Qp);
-  double *c0 = buf.get(E), *c1 = buf.get(E), *dM = buf.get(N);
-  double *r = buf.get(N), *z = buf.get(N), *p = buf.get(N), *q = buf.get(N), *t = buf.get(N);
-  double *rn = buf.get(NN), *zn = buf.get(NN), *pn = buf.get(NN), *qn = buf.get(NN), *dS = buf.get(NN),
-         *Qv = buf.get(NN), *bn = buf.get(NN);
-  double * part = buf.get(2 * AV_BLOCKS);
-  if (!qdu || !c0 || !c1 || !dM || !r || !z || !p || !q || !t || !rn || !zn || !pn || !qn || !dS || !Qv || !bn ||
-      !part)
-    return hc::fail(HC_ENOMEM, "hc_op_div_free_source: device allocation failed");
-  std::vector<double> h(2 * AV_BLOCKS);
+  const int64_t NQ = (int64_t)12 * E * Qp * Qp * Qp;
+  // work vectors, kept with the operator: unit-mass factors and the two
+  // diagonals (mesh only, computed once), the ND and node PCG vectors
+  if (!op->dfs_mem.base)
+  {
+    const size_t need = hc::bytes_of({NQ, E, E, N, N, N, N, N, N, NN, NN, NN, NN, NN, NN, NN, hc::PCG_PARTS}) +
+                        hc::align256(sizeof(hc::PcgDev));
+    if ((rc = op->dfs_mem.alloc(need)))
+      return rc;
+    op->dfs_ready = false;
+  }
+  if (!op->av_host && hipHostMalloc((void **)&op->av_host, sizeof(hc::PcgDev)) != hipSuccess)
+  {
+    op->av_host = nullptr;
+    return hc::fail(HC_ENOMEM, "hc_op_div_free_source: pinned host allocation failed");
+  }
+  hc::Carver cv{(char *)op->dfs_mem.base};
+  double *qdu = cv.take(NQ), *c0 = cv.take(E), *c1 = cv.take(E), *dM = cv.take(N);
+  double *r = cv.take(N), *z = cv.take(N), *p = cv.take(N), *q = cv.take(N), *t = cv.take(N);
+  double *rn = cv.take(NN), *zn = cv.take(NN), *pn = cv.take(NN), *qn = cv.take(NN), *dS = cv.take(NN),
+         *Qv = cv.take(NN), *bn = cv.take(NN);
+  double * part = cv.take(hc::PCG_PARTS);
+  hc::PcgDev * sdev = (hc::PcgDev *)cv.p;
+  hc::PcgDev * shost = (hc::PcgDev *)op->av_host;
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return hc::fail(HC_EHIP, "hc_op_div_free_source: event creation failed");
@@ -305,12 +419,18 @@ hc_op_div_free_source(hc_op_t op, const double * j, double * src, double * lf, d
   const hc::ApplyPlan & pl = op->plan64;
   hc::Geom g0 = pl.g;
   g0.pec = 0;
-  // unit-coefficient mass factors on the operator's geometry
-  hipLaunchKernelGGL(hc::k_fill_value, dim3(256), dim3(256), 0, st, E, 0.0, c0);
-  hipLaunchKernelGGL(hc::k_fill_value, dim3(256), dim3(256), 0, st, E, 1.0, c1);
-  HC_TRY_HIP(hc::pa_setup(op->g, op->tb64, op->d_vert, op->h, c0, c1, 1.0, qdu, st));
-  HC_TRY_HIP(hc::diagonal(g0, op->tb64, qdu, dM, st));
-  HC_TRY_HIP(hc::invert_diag(dM, dM, nullptr, N, st));
+  if (!op->dfs_ready)
+  {
+    // unit-coefficient mass factors on the operator's geometry, their
+    // diagonal and the node diagonal of G^T M G
+    hipLaunchKernelGGL(hc::k_fill_value, dim3(256), dim3(256), 0, st, E, 0.0, c0);
+    hipLaunchKernelGGL(hc::k_fill_value, dim3(256), dim3(256), 0, st, E, 1.0, c1);
+    HC_TRY_HIP(hc::pa_setup(op->g, op->tb64, op->d_vert, op->h, c0, c1, 1.0, qdu, st));
+    HC_TRY_HIP(hc::diagonal(g0, op->tb64, qdu, dM, st));
+    HC_TRY_HIP(hc::invert_diag(dM, dM, nullptr, N, st));
+    HC_TRY_HIP(hc::aux_diag_inv(g0, op->ag, op->tb64, qdu, dS, dS, nullptr, st));
+    op->dfs_ready = true;
+  }
   auto mass = [&](const double * x, double * y) -> int {
     HC_TRY_HIP(hc::apply_scaled<double>(g0, pl.segs, op->tb64, qdu, x, y, op->ws.part64, 1.0, 0.0, pl.cand,
                                         op->desc_dev, st));
@@ -320,8 +440,12 @@ hc_op_div_free_source(hc_op_t op, const double * j, double * src, double * lf, d
   bool c1ok = false, c2ok = true;
   // g = M^-1 j
   hipLaunchKernelGGL(hc::k_fill_value, dim3(2048), dim3(256), 0, st, N, 0.0, src);
-  if ((rc = hc::pcg_one(N, mass, dM, j, src, rtol, maxit, r, z, p, q, part, h.data(), st, it1, c1ok, s1)))
-    return rc;
+  {
+    const hc::PcgBlock blk{N, dM, j, src, r, z, p, q};
+    auto ap = [&](const double * x0, const double *, double * y0, double *) -> int { return mass(x0, y0); };
+    if ((rc = hc::pcg_dev(1, &blk, ap, rtol, maxit, part, sdev, shost, st, it1, c1ok, s1)))
+      return rc;
+  }
   hipLaunchKernelGGL(hc::k_fill_value, dim3(2048), dim3(256), 0, st, NN, 0.0, Qv);
   if (project && s1 == HC_OK)
   {
@@ -329,17 +453,17 @@ hc_op_div_free_source(hc_op_t op, const double * j, double * src, double * lf, d
     if ((rc = mass(src, t)))
       return rc;
     HC_TRY_HIP(hc::grad_transpose(op->ag, t, bn, st));
-    HC_TRY_HIP(hc::aux_diag_inv(g0, op->ag, op->tb64, qdu, dS, dS, nullptr, st));
-    auto stiff = [&](const double * x, double * y) -> int {
-      HC_TRY_HIP(hc::grad_apply(op->ag, x, t, st));
+    auto stiff = [&](const double * x0, const double *, double * y0, double *) -> int {
+      HC_TRY_HIP(hc::grad_apply(op->ag, x0, t, st));
       int e = mass(t, q);
       if (e)
         return e;
-      HC_TRY_HIP(hc::grad_transpose(op->ag, q, y, st));
+      HC_TRY_HIP(hc::grad_transpose(op->ag, q, y0, st));
       return HC_OK;
     };
     // the stiffness apply uses t and q: the node PCG's own vectors are rn..qn
-    if ((rc = hc::pcg_one(NN, stiff, dS, bn, Qv, rtol, maxit, rn, zn, pn, qn, part, h.data(), st, it2, c2ok, s2)))
+    const hc::PcgBlock blk{NN, dS, bn, Qv, rn, zn, pn, qn};
+    if ((rc = hc::pcg_dev(1, &blk, stiff, rtol, maxit, part, sdev, shost, st, it2, c2ok, s2)))
       return rc;
     // g <- g - G Q
     HC_TRY_HIP(hc::grad_apply(op->ag, Qv, t, st));
@@ -386,13 +510,22 @@ hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a, dou
     return hc::fail(HC_EINVAL, "hc_op_av_solve: rtol must be > 0 and maxit >= 1");
   hipStream_t st = (hipStream_t)stream;
   const int64_t N = op->N, NN = op->ag.NN;
-  hc::AvBuffers buf;
-  double *ra = buf.get(N), *za = buf.get(N), *pa = buf.get(N), *qa = buf.get(N);
-  double *rv = buf.get(NN), *zv = buf.get(NN), *pv = buf.get(NN), *qv = buf.get(NN), *vinv = buf.get(NN);
-  double * part = buf.get(4 * AV_BLOCKS);
-  if (!ra || !za || !pa || !qa || !rv || !zv || !pv || !qv || !vinv || !part)
-    return hc::fail(HC_ENOMEM, "hc_op_av_solve: device allocation failed");
-  std::vector<double> h(4 * AV_BLOCKS);
+  int rc;
+  if (!op->av_mem.base &&
+      (rc = op->av_mem.alloc(hc::bytes_of({N, N, N, N, NN, NN, NN, NN, NN, hc::PCG_PARTS}) +
+                             hc::align256(sizeof(hc::PcgDev)))))
+    return rc;
+  if (!op->av_host && hipHostMalloc((void **)&op->av_host, sizeof(hc::PcgDev)) != hipSuccess)
+  {
+    op->av_host = nullptr;
+    return hc::fail(HC_ENOMEM, "hc_op_av_solve: pinned host allocation failed");
+  }
+  hc::Carver cv{(char *)op->av_mem.base};
+  double *ra = cv.take(N), *za = cv.take(N), *pa = cv.take(N), *qa = cv.take(N);
+  double *rv = cv.take(NN), *zv = cv.take(NN), *pv = cv.take(NN), *qv = cv.take(NN), *vinv = cv.take(NN);
+  double * part = cv.take(hc::PCG_PARTS);
+  hc::PcgDev * sdev = (hc::PcgDev *)cv.p;
+  hc::PcgDev * shost = (hc::PcgDev *)op->av_host;
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return hc::fail(HC_EHIP, "hc_op_av_solve: event creation failed");
@@ -411,70 +544,28 @@ hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a, dou
   g0.pec = 0;
   if (hc::aux_diag_inv(g0, op->ag, op->tb64, op->qd64, vinv, vinv, nullptr, st) != hipSuccess)
     return hc::fail(HC_EHIP, "hc_op_av_solve: node diagonal failed");
-  const dim3 G(AV_BLOCKS), T(AV_THREADS);
-  // r = b - AV x, z = D^-1 r, p = z
-  int rc = hc_op_av_apply(op, a, v, qa, qv, stream);
-  if (rc)
-    return rc;
-  hipLaunchKernelGGL(hc::k_av_dot, G, T, 0, st, N, ba, ba, part);
-  hipLaunchKernelGGL(hc::k_av_dot, G, T, 0, st, NN, bv, bv, part + AV_BLOCKS);
-  if ((rc = hc::read_partials(part, h.data(), 2, st)))
-    return hc::fail(rc, "hc_op_av_solve: reduction failed");
-  const double bb = hc::host_sum(h.data(), AV_BLOCKS) + hc::host_sum(h.data() + AV_BLOCKS, AV_BLOCKS);
-  hipLaunchKernelGGL(hc::k_av_start, G, T, 0, st, N, ba, qa, op->dinv64, ra, za, pa, part);
-  hipLaunchKernelGGL(hc::k_av_start, G, T, 0, st, NN, bv, qv, vinv, rv, zv, pv, part + 2 * AV_BLOCKS);
-  if ((rc = hc::read_partials(part, h.data(), 4, st)))
-    return hc::fail(rc, "hc_op_av_solve: reduction failed");
-  double gam = hc::host_sum(h.data(), AV_BLOCKS) + hc::host_sum(h.data() + 2 * AV_BLOCKS, AV_BLOCKS);
-  double rr = hc::host_sum(h.data() + AV_BLOCKS, AV_BLOCKS) + hc::host_sum(h.data() + 3 * AV_BLOCKS, AV_BLOCKS);
-  const double tol2 = rtol * rtol * bb;
+  const hc::PcgBlock blk[2] = {{N, op->dinv64, ba, a, ra, za, pa, qa}, {NN, vinv, bv, v, rv, zv, pv, qv}};
+  auto ap = [&](const double * x0, const double * x1, double * y0, double * y1) -> int {
+    return hc_op_av_apply(op, x0, x1, y0, y1, stream);
+  };
   int it = 0, status = HC_OK;
-  bool conv = bb == 0.0 || rr <= tol2;
-  while (!conv && it < maxit)
-  {
-    if ((rc = hc_op_av_apply(op, pa, pv, qa, qv, stream)))
-      return rc;
-    hipLaunchKernelGGL(hc::k_av_dot, G, T, 0, st, N, pa, qa, part);
-    hipLaunchKernelGGL(hc::k_av_dot, G, T, 0, st, NN, pv, qv, part + AV_BLOCKS);
-    if ((rc = hc::read_partials(part, h.data(), 2, st)))
-      return hc::fail(rc, "hc_op_av_solve: reduction failed");
-    const double pq = hc::host_sum(h.data(), AV_BLOCKS) + hc::host_sum(h.data() + AV_BLOCKS, AV_BLOCKS);
-    if (!std::isfinite(pq) || !(pq > 0.0))
-    {
-      status = std::isfinite(pq) ? HC_ENOTCONV : HC_ENAN;
-      break;
-    }
-    const double alpha = gam / pq;
-    hipLaunchKernelGGL(hc::k_av_update, G, T, 0, st, N, alpha, (const double *)pa, (const double *)qa,
-                       (const double *)op->dinv64, a, ra, za, part);
-    hipLaunchKernelGGL(hc::k_av_update, G, T, 0, st, NN, alpha, (const double *)pv, (const double *)qv,
-                       (const double *)vinv, v, rv, zv, part + 2 * AV_BLOCKS);
-    if ((rc = hc::read_partials(part, h.data(), 4, st)))
-      return hc::fail(rc, "hc_op_av_solve: reduction failed");
-    const double gnew = hc::host_sum(h.data(), AV_BLOCKS) + hc::host_sum(h.data() + 2 * AV_BLOCKS, AV_BLOCKS);
-    rr = hc::host_sum(h.data() + AV_BLOCKS, AV_BLOCKS) + hc::host_sum(h.data() + 3 * AV_BLOCKS, AV_BLOCKS);
-    it++;
-    if (!std::isfinite(gnew) || !std::isfinite(rr))
-    {
-      status = HC_ENAN;
-      break;
-    }
-    conv = rr <= tol2;
-    if (conv)
-      break;
-    hipLaunchKernelGGL(hc::k_av_xpby, dim3(2048), dim3(256), 0, st, N, (const double *)za, gnew / gam, pa);
-    hipLaunchKernelGGL(hc::k_av_xpby, dim3(2048), dim3(256), 0, st, NN, (const double *)zv, gnew / gam, pv);
-    gam = gnew;
-  }
+  bool conv = false;
+  if ((rc = hc::pcg_dev(2, blk, ap, rtol, maxit, part, sdev, shost, st, it, conv, status)))
+    return rc;
   // true residual of the final iterate
   if ((rc = hc_op_av_apply(op, a, v, qa, qv, stream)))
     return rc;
+  const dim3 G(AV_BLOCKS), T(AV_THREADS);
   hipLaunchKernelGGL(hc::k_av_resid, G, T, 0, st, N, ba, qa, part);
   hipLaunchKernelGGL(hc::k_av_resid, G, T, 0, st, NN, bv, qv, part + AV_BLOCKS);
   (void)hipEventRecord(e1, st);
-  if ((rc = hc::read_partials(part, h.data(), 2, st)))
-    return hc::fail(rc, "hc_op_av_solve: reduction failed");
-  const double tr = hc::host_sum(h.data(), AV_BLOCKS) + hc::host_sum(h.data() + AV_BLOCKS, AV_BLOCKS);
+  std::vector<double> h(2 * AV_BLOCKS);
+  HC_TRY_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * 2 * AV_BLOCKS, hipMemcpyDeviceToHost, st));
+  HC_TRY_HIP(hipStreamSynchronize(st));
+  double tr = 0.0;
+  for (double x : h)
+    tr += x;
+  const double bb = shost->bb;
   const double rel = bb > 0.0 ? std::sqrt(tr / bb) : 0.0;
   conv = conv && status == HC_OK && rel <= rtol * (1.0 + 1e-6);
   if (status == HC_OK && !conv)

@@ -197,6 +197,12 @@ struct hc_op_s
   // geometric multigrid hierarchy of HC_SOLVE_MG (hc_mg.hip), built on first
   // use and rebuilt when setup_version (counts the re-assemblies) moved on
   hc::MgHier * mg = nullptr;
+  // work vectors of hc_op_av_solve and hc_op_div_free_source (hc_av.hip),
+  // allocated on first use and kept; the unit-mass factors of the latter
+  // depend on the mesh alone and are computed once
+  hc::DevBlock av_mem, dfs_mem;
+  double * av_host = nullptr; // pinned copy of the solver state
+  bool dfs_ready = false;
   int setup_version = 0;
 
   template <typename T>

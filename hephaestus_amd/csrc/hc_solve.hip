@@ -644,6 +644,8 @@ hc_op_s::~hc_op_s()
 {
   delete engine;
   hc::mg_free(mg);
+  if (av_host)
+    (void)hipHostFree(av_host);
   for (void * v : allocs)
     (void)hipFree(v);
 }

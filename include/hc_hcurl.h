@@ -223,7 +223,9 @@ int hc_op_av_apply(hc_op_t op, const double * a, const double * v, double * ya, 
  * DOFs; diag(G^T M G)^-1 on the nodes).  (a, v) hold the initial guess and
  * receive the solution; its gauge component stays that of the guess.  Stops
  * when ||b - AV x|| <= rtol ||b|| (FP64, both blocks); stats->rel_res is the
- * true residual of the result.  Allocates its work vectors per call. */
+ * true residual of the result.  Its work vectors are allocated on the first
+ * call and kept with the operator; the PCG scalars stay on the device (the
+ * host reads the solver state every 8 iterations). */
 int hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a, double * v, double rtol,
                    int maxit, hc_solve_stats * stats, void * stream);
 /* DivFreeSource::Apply (src/sources/div_free_source.cpp:71-114) for a source
@@ -234,8 +236,9 @@ int hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a,
  * an empty BCMap); then lf += M g.  src receives g, Q (optional, H1 nodes) the
  * potential.  Both solves: Jacobi-PCG to rtol (the reference: its default
  * GMRES).  stats: iterations of both solves summed; lf is left unchanged when
- * a solve fails.  Allocates its work vectors and the unit-mass quadrature
- * factors per call. */
+ * a solve fails.  Its work vectors and the unit-mass quadrature factors
+ * (which depend on the mesh alone) are set up on the first call and kept with
+ * the operator; PCG scalars on the device as in hc_op_av_solve. */
 int hc_op_div_free_source(hc_op_t op, const double * j, double * src, double * lf, double * Q, int project,
                           double rtol, int maxit, hc_solve_stats * stats, void * stream);
 /* Right-hand sides per work item of the multi-RHS apply and solve (0 =
