@@ -7,11 +7,13 @@
 // semi-definite: its energy is dt (curl a, nu curl a) + (a0 + G v, sigma
 // (a0 + G v)), zero exactly on the gauge fields (-G phi, phi).  The reference
 // solves it with HypreGMRES + BoomerAMG (problem_builder_base.cpp:206-218);
-// here it is preconditioned CG with the block-Jacobi preconditioner
-// diag(A_AA)^-1 (1 on PEC DOFs) and diag(G^T M G)^-1 on the nodes, which
-// converges on consistent right-hand sides (the gauge component of the
-// iterate is left as the initial guess has it).  Reductions: per-block
-// partial sums on the device, summed on the host in a fixed order, so a solve
+// here it is preconditioned CG with a block-diagonal preconditioner: the
+// multigrid V-cycles of hc_mg.hip on the ND block and on the node block
+// G^T M G where the mesh has the hierarchies (with exact blocks the
+// preconditioned operator has the eigenvalues 0, 1 and 2 only on the free
+// DOFs, since K G = 0 makes A G = M G there), else block Jacobi diag(A_AA)^-1 (1 on PEC DOFs) and
+// diag(G^T M G)^-1; it converges on consistent right-hand sides.  Reductions:
+// per-block partial sums on the device, summed in a fixed order, so a solve
 // is bitwise reproducible.
 #include "hc_aux.hpp"
 #include "hc_kernels.hpp"
@@ -211,19 +213,40 @@ k_pcgd_update(int64_t n, const PcgDev * __restrict__ s, const double * __restric
   {
     x[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
-    const double zi = d[i] * ri;
     r[i] = ri;
-    z[i] = zi;
-    rz += ri * zi;
+    if (d)
+    {
+      const double zi = d[i] * ri;
+      z[i] = zi;
+      rz += ri * zi;
+    }
     rr += ri * ri;
   }
   rz = av_block_sum(rz, sh);
   rr = av_block_sum(rr, sh);
   if (threadIdx.x == 0)
   {
-    out[blockIdx.x] = rz;
+    out[blockIdx.x] = rz; // (without d the caller's (r, z) dot overwrites it)
     out[AV_BLOCKS + blockIdx.x] = rr;
   }
+}
+
+// r = b - q; out[AV_BLOCKS + b] = block parts of (r, r)
+__global__ void __launch_bounds__(AV_THREADS)
+k_pcgd_rstart(int64_t n, const double * __restrict__ b, const double * __restrict__ q, double * __restrict__ r,
+              double * __restrict__ out)
+{
+  __shared__ double sh[AV_THREADS];
+  double rr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * AV_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * AV_THREADS)
+  {
+    const double ri = b[i] - q[i];
+    r[i] = ri;
+    rr += ri * ri;
+  }
+  rr = av_block_sum(rr, sh);
+  if (threadIdx.x == 0)
+    out[AV_BLOCKS + blockIdx.x] = rr;
 }
 
 // p = z + beta p (nothing once done)
@@ -293,26 +316,53 @@ struct PcgBlock
   double *x, *r, *z, *p, *q;
 };
 
-// Preconditioned CG with a diagonal preconditioner over nb vector blocks:
-// apply(in0, in1, out0, out1) computes out = S in.  Scalars stay on the
+// Preconditioned CG over nb vector blocks: apply(in0, in1, out0, out1)
+// computes out = S in; the preconditioner is diagonal (blk.dinv) or, with
+// use_prec, prec(r0, r1, z0, z1) (a fixed symmetric positive definite
+// operator, launched without host synchronisation).  Scalars stay on the
 // device (fixed-order partial sums, bitwise reproducible); the host reads the
 // state every PCG_CHUNK iterations, the kernels of the iterations after the
 // stopping point do nothing.  Stops when ||r|| <= rtol ||b|| on the recursive
 // residual.  part: PCG_PARTS device doubles; s: device state; hs: pinned
 // host copy.
-template <typename Apply>
+struct NoPrec
+{
+  int operator()(const double *, const double *, double *, double *) const { return HC_OK; }
+};
+
+template <typename Apply, typename Prec = NoPrec>
 static int
 pcg_dev(int nb, const PcgBlock * bl, Apply && apply, double rtol, int maxit, double * part, PcgDev * s,
-        PcgDev * hs, hipStream_t st, int & iters, bool & conv, int & status)
+        PcgDev * hs, hipStream_t st, int & iters, bool & conv, int & status, bool use_prec = false,
+        Prec && prec = Prec{})
 {
   const dim3 G(AV_BLOCKS), T(AV_THREADS);
+  auto precond = [&]() -> int {
+    if (int rc = prec(bl[0].r, nb > 1 ? bl[1].r : nullptr, bl[0].z, nb > 1 ? bl[1].z : nullptr))
+      return rc;
+    for (int k = 0; k < nb; k++)
+      hipLaunchKernelGGL(k_av_dot, G, T, 0, st, bl[k].n, (const double *)bl[k].r, (const double *)bl[k].z,
+                         part + (2 + 2 * k) * AV_BLOCKS);
+    return HC_OK;
+  };
   if (int rc = apply(bl[0].x, nb > 1 ? bl[1].x : nullptr, bl[0].q, nb > 1 ? bl[1].q : nullptr))
     return rc;
   for (int k = 0; k < nb; k++)
   {
     hipLaunchKernelGGL(k_av_dot, G, T, 0, st, bl[k].n, bl[k].b, bl[k].b, part + k * AV_BLOCKS);
-    hipLaunchKernelGGL(k_av_start, G, T, 0, st, bl[k].n, bl[k].b, (const double *)bl[k].q, bl[k].dinv, bl[k].r,
-                       bl[k].z, bl[k].p, part + (2 + 2 * k) * AV_BLOCKS);
+    if (use_prec)
+      hipLaunchKernelGGL(k_pcgd_rstart, G, T, 0, st, bl[k].n, bl[k].b, (const double *)bl[k].q, bl[k].r,
+                         part + (2 + 2 * k) * AV_BLOCKS);
+    else
+      hipLaunchKernelGGL(k_av_start, G, T, 0, st, bl[k].n, bl[k].b, (const double *)bl[k].q, bl[k].dinv, bl[k].r,
+                         bl[k].z, bl[k].p, part + (2 + 2 * k) * AV_BLOCKS);
+  }
+  if (use_prec)
+  {
+    if (int rc = precond())
+      return rc;
+    for (int k = 0; k < nb; k++)
+      HC_TRY_HIP(hipMemcpyAsync(bl[k].p, bl[k].z, sizeof(double) * bl[k].n, hipMemcpyDeviceToDevice, st));
   }
   hipLaunchKernelGGL(k_pcgd_start, dim3(1), T, 0, st, (const double *)part, nb, rtol, maxit, s);
   HC_TRY_HIP(hipGetLastError());
@@ -332,8 +382,11 @@ pcg_dev(int nb, const PcgBlock * bl, Apply && apply, double rtol, int maxit, dou
       hipLaunchKernelGGL(k_pcgd_alpha, dim3(1), T, 0, st, (const double *)part, nb, s);
       for (int k = 0; k < nb; k++)
         hipLaunchKernelGGL(k_pcgd_update, G, T, 0, st, bl[k].n, (const PcgDev *)s, (const double *)bl[k].p,
-                           (const double *)bl[k].q, bl[k].dinv, bl[k].x, bl[k].r, bl[k].z,
+                           (const double *)bl[k].q, use_prec ? nullptr : bl[k].dinv, bl[k].x, bl[k].r, bl[k].z,
                            part + (2 + 2 * k) * AV_BLOCKS);
+      if (use_prec)
+        if (int rc = precond())
+          return rc;
       hipLaunchKernelGGL(k_pcgd_beta, dim3(1), T, 0, st, (const double *)part, nb, s);
       for (int k = 0; k < nb; k++)
         hipLaunchKernelGGL(k_pcgd_xpby, dim3(2048), dim3(256), 0, st, bl[k].n, (const PcgDev *)s,
@@ -526,6 +579,9 @@ hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a, dou
   double * part = cv.take(hc::PCG_PARTS);
   hc::PcgDev * sdev = (hc::PcgDev *)cv.p;
   hc::PcgDev * shost = (hc::PcgDev *)op->av_host;
+  // block preconditioner: multigrid on both blocks where the mesh has the
+  // hierarchies (built on first use, outside the timed region), else block Jacobi
+  const bool mg = hc::mg_av_ready(op, st);
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return hc::fail(HC_EHIP, "hc_op_av_solve: event creation failed");
@@ -548,9 +604,12 @@ hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a, dou
   auto ap = [&](const double * x0, const double * x1, double * y0, double * y1) -> int {
     return hc_op_av_apply(op, x0, x1, y0, y1, stream);
   };
+  auto pr = [&](const double * r0, const double * r1, double * z0, double * z1) -> int {
+    return hc::mg_av_precond(op, r0, r1, z0, z1, st);
+  };
   int it = 0, status = HC_OK;
   bool conv = false;
-  if ((rc = hc::pcg_dev(2, blk, ap, rtol, maxit, part, sdev, shost, st, it, conv, status)))
+  if ((rc = hc::pcg_dev(2, blk, ap, rtol, maxit, part, sdev, shost, st, it, conv, status, mg, pr)))
     return rc;
   // true residual of the final iterate
   if ((rc = hc_op_av_apply(op, a, v, qa, qv, stream)))
@@ -576,7 +635,7 @@ hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a, dou
   {
     *stats = hc_solve_stats{};
     stats->iterations = it;
-    stats->outer_iterations = 1;
+    stats->outer_iterations = mg ? 2 : 1; // 2: multigrid block preconditioner, 1: block Jacobi
     stats->converged = conv ? 1 : 0;
     stats->status = status;
     stats->rel_res = rel;

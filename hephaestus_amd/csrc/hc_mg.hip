@@ -25,6 +25,7 @@
 // V-cycle (the same polynomial smoother before and after an exact coarse
 // solve) is a fixed symmetric positive definite operator, so plain PCG
 // applies.
+#include "hc_aux.hpp"
 #include "hc_op.hpp"
 
 #include <rocblas/rocblas.h>
@@ -336,9 +337,14 @@ struct MgLevel
   Csr1Dev P[3][2], R[3][2];
 };
 
+struct NodeHier;
+void node_free(NodeHier * n);
+
 struct MgHier
 {
   std::vector<MgLevel> lev;
+  NodeHier * nodes = nullptr; // node-space hierarchy of the AV preconditioner (built on demand)
+  int nodes_state = 0;        // 0 not tried, 1 built, -1 not available
   int version = -1;
   DevBlock mem;        // level vectors, temporaries, transfer matrices
   double * tmp1 = nullptr;
@@ -354,6 +360,7 @@ struct MgHier
   int nsmooth = MG_SMOOTH;   // smoothing steps before and after the coarse correction
   ~MgHier()
   {
+    node_free(nodes);
     for (auto & l : lev)
       if (l.owned && l.op)
         hc_op_destroy(l.op);
@@ -951,6 +958,350 @@ mg_levels(hc_op_t op, hipStream_t st, int * nlev, int64_t * ncoarse)
   *nlev = (int)op->mg->lev.size();
   *ncoarse = op->mg->NL;
   return HC_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Node-space multigrid for the AV formulation's scalar-potential block
+// S = G^T M_sigma G (M_sigma: the operator's mass term without PEC
+// elimination).  Its levels are the node spaces of the ND hierarchy's levels
+// (Q_p Lagrange on the closed points, nested like the ND spaces), transfers
+// are closed 1-D interpolations along every axis (the nodal counterpart of
+// the ND transfers: G_f P_nodes = P_ND G_c), the smoother is the Chebyshev
+// iteration with Jacobi, and the coarsest level, singular with the constants
+// as kernel (V has no essential nodes), is solved by the dense inverse of
+// S_c + alpha 1 1^T (alpha = trace(S_c) / n_c^2), which inverts S_c on the
+// complement of the constants.
+// ---------------------------------------------------------------------------
+
+struct NodeLevel
+{
+  hc_op_t op = nullptr; // the ND level's operator (borrowed)
+  Geom g0{};            // its geometry without PEC
+  int64_t NN = 0, N = 0;
+  double *x = nullptr, *b = nullptr, *r = nullptr, *z = nullptr, *d = nullptr, *dinv = nullptr;
+  double *t1 = nullptr, *t2 = nullptr; // ND-size temporaries of the S apply
+  double lam = 1.0;
+  Csr1Dev P[3], R[3]; // from the next coarser level, per axis
+};
+
+struct NodeHier
+{
+  std::vector<NodeLevel> lev;
+  DevBlock mem, sinv_mem;
+  double * Sinv = nullptr;
+  int64_t NC = 0;
+};
+
+void
+node_free(NodeHier * n)
+{
+  delete n;
+}
+
+namespace
+{
+
+constexpr int64_t NODE_COARSE_MAX = 4096; // largest coarsest node level (dense inverse)
+constexpr double NODE_CHEB_RANGE = 20.0;
+
+__global__ void
+k_add_const(double * __restrict__ A, int64_t n, double v)
+{
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    A[k] += v;
+}
+
+__global__ void
+k_trace_part(const double * __restrict__ A, int64_t n, double * __restrict__ out)
+{
+  // one workgroup, fixed order
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256)
+    s += A[i * n + i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1)
+  {
+    if ((int)threadIdx.x < k)
+      sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    out[0] = sh[0];
+}
+
+// y = S x on a node level
+int
+node_apply(NodeLevel & L, const double * x, double * y, hipStream_t st)
+{
+  hc_op_t op = L.op;
+  HC_TRY_HIP(grad_apply(op->ag, x, L.t1, st));
+  HC_TRY_HIP(apply_scaled<double>(L.g0, op->plan64.segs, op->tb64, op->qd64, L.t1, L.t2, op->ws.part64, 1.0, 0.0,
+                                  op->plan64.cand, op->desc_dev, st));
+  HC_TRY_HIP(grad_transpose(op->ag, L.t2, y, st));
+  return HC_OK;
+}
+
+// y (level l) += P x (level l+1) or y (level l+1) = P^T x (level l)
+hipError_t
+node_transfer(MgHier & h, NodeHier & nh, int l, bool prolong, const double * x, double * y, hipStream_t st)
+{
+  const NodeLevel & f = nh.lev[l];
+  const NodeLevel & c = nh.lev[l + 1];
+  int d[3];
+  for (int a = 0; a < 3; a++)
+    d[a] = prolong ? c.op->ag.M[a] : f.op->ag.M[a];
+  for (int a = 0; a < 3; a++)
+  {
+    const Csr1Dev & m = prolong ? f.P[a] : f.R[a];
+    double * out = (a == 2) ? y : (a == 0 ? h.tmp1 : h.tmp2);
+    const double * in = (a == 0) ? x : (a == 1 ? h.tmp1 : h.tmp2);
+    const int64_t plane = (int64_t)(a == 0 ? m.rows : d[0]) * (a == 1 ? m.rows : d[1]);
+    const int o2 = a == 2 ? m.rows : d[2];
+    hipLaunchKernelGGL(k_pass1d, dim3((unsigned)((plane + 255) / 256), (unsigned)o2), dim3(256), 0, st, in, out, d[0],
+                       d[1], d[2], a, m.rows, m.rp, m.ci, m.w, 1.0, (a == 2 && prolong) ? 1 : 0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      return e;
+    d[a] = m.rows;
+  }
+  return hipSuccess;
+}
+
+int
+node_smooth(MgHier & h, NodeLevel & L, bool zero_start, hipStream_t st)
+{
+  const double hi = 1.1 * L.lam, lo = hi / NODE_CHEB_RANGE;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma1 = theta / delta;
+  double rho0 = 1.0 / sigma1;
+  for (int s = 0; s < h.nsmooth; s++)
+  {
+    const bool first = zero_start && s == 0;
+    if (first)
+      HC_TRY_HIP(hipMemcpyAsync(L.r, L.b, sizeof(double) * L.NN, hipMemcpyDeviceToDevice, st));
+    else
+    {
+      if (int rc = node_apply(L, L.x, L.r, st))
+        return rc;
+      HC_TRY_HIP(axpby(L.NN, 1.0, L.b, -1.0, L.r, st));
+    }
+    HC_TRY_HIP(mul(L.NN, L.dinv, L.r, L.z, st));
+    double a, b;
+    if (s == 0)
+    {
+      a = 1.0 / theta;
+      b = 0.0;
+    }
+    else
+    {
+      const double rho1 = 1.0 / (2.0 * sigma1 - rho0);
+      a = 2.0 * rho1 / delta;
+      b = rho1 * rho0;
+      rho0 = rho1;
+    }
+    hipLaunchKernelGGL(k_cheb_update, dim3(grid_for(L.NN)), dim3(256), 0, st, L.NN, a, b, L.z, L.d, L.x,
+                       (s == 0 ? 1 : 0) | (first ? 2 : 0));
+    HC_TRY_HIP(hipGetLastError());
+  }
+  return HC_OK;
+}
+
+int
+node_vcycle(MgHier & h, NodeHier & nh, int l, hipStream_t st)
+{
+  NodeLevel & L = nh.lev[l];
+  if (l == (int)nh.lev.size() - 1)
+  {
+    const double one = 1.0, zero = 0.0;
+    if (rocblas_set_stream(h.blas, st) != rocblas_status_success ||
+        rocblas_dgemv(h.blas, rocblas_operation_transpose, (rocblas_int)L.NN, (rocblas_int)L.NN, &one, nh.Sinv,
+                      (rocblas_int)L.NN, L.b, 1, &zero, L.x, 1) != rocblas_status_success)
+      return fail(HC_EHIP, "multigrid: node coarse solve (rocblas_dgemv) failed");
+    return HC_OK;
+  }
+  if (int rc = node_smooth(h, L, true, st))
+    return rc;
+  if (int rc = node_apply(L, L.x, L.r, st))
+    return rc;
+  HC_TRY_HIP(axpby(L.NN, 1.0, L.b, -1.0, L.r, st));
+  HC_TRY_HIP(node_transfer(h, nh, l, false, L.r, nh.lev[l + 1].b, st));
+  if (int rc = node_vcycle(h, nh, l + 1, st))
+    return rc;
+  HC_TRY_HIP(node_transfer(h, nh, l, true, nh.lev[l + 1].x, L.x, st));
+  return node_smooth(h, L, false, st);
+}
+
+// Node hierarchy over the ND levels (the ND hierarchy must be built and have a
+// dense coarsest level); HC_OK when built, a status when not available.
+int
+node_build(MgHier & h, hipStream_t st)
+{
+  if (h.coarse_iter)
+    return fail(HC_EINVAL, "multigrid: no node hierarchy for an inexact coarsest level");
+  const int nl = (int)h.lev.size();
+  const int64_t NC = h.lev.back().op->ag.NN;
+  if (NC > NODE_COARSE_MAX)
+    return fail(HC_EINVAL, "multigrid: coarsest node level too large for its dense inverse");
+  auto * nh = new NodeHier();
+  std::unique_ptr<NodeHier> guard(nh);
+  size_t bytes = 0;
+  std::vector<Csr1> mats;
+  for (int l = 0; l < nl; l++)
+  {
+    NodeLevel L;
+    L.op = h.lev[l].op;
+    L.g0 = L.op->plan64.g;
+    L.g0.pec = 0;
+    L.NN = L.op->ag.NN;
+    L.N = L.op->N;
+    if (int rc = L.op->ensure_ws(1, false))
+      return rc;
+    nh->lev.push_back(L);
+    bytes += 6 * align256(sizeof(double) * L.NN) + 2 * align256(sizeof(double) * L.N);
+  }
+  for (int l = 0; l + 1 < nl; l++)
+    for (int a = 0; a < 3; a++)
+    {
+      const MgLevel & f = h.lev[l];
+      const MgLevel & c = h.lev[l + 1];
+      Csr1 P = prolong_1d(c.n[a], c.op->p, f.n[a], f.op->p, true);
+      Csr1 R = transpose(P);
+      bytes += csr_bytes(P) + csr_bytes(R);
+      mats.push_back(std::move(P));
+      mats.push_back(std::move(R));
+    }
+  if (int rc = nh->mem.alloc(bytes))
+    return rc;
+  char * q = (char *)nh->mem.base;
+  size_t mi = 0;
+  for (int l = 0; l + 1 < nl; l++)
+    for (int a = 0; a < 3; a++)
+    {
+      nh->lev[l].P[a] = upload(mats[mi++], q);
+      nh->lev[l].R[a] = upload(mats[mi++], q);
+    }
+  for (auto & L : nh->lev)
+  {
+    double ** v[6] = {&L.x, &L.b, &L.r, &L.z, &L.d, &L.dinv};
+    for (double ** pp : v)
+    {
+      *pp = (double *)q;
+      q += align256(sizeof(double) * L.NN);
+    }
+    L.t1 = (double *)q;
+    q += align256(sizeof(double) * L.N);
+    L.t2 = (double *)q;
+    q += align256(sizeof(double) * L.N);
+    HC_TRY_HIP(aux_diag_inv(L.g0, L.op->ag, L.op->tb64, L.op->qd64, L.dinv, L.dinv, nullptr, st));
+  }
+  // lambda_max(D^-1 S) per smoothed level by power iteration
+  for (int l = 0; l + 1 < nl; l++)
+  {
+    NodeLevel & L = nh->lev[l];
+    HC_TRY_HIP(fill_uniform(nullptr, 93, L.x, L.NN, st));
+    double lam = 1.0;
+    for (int it = 0; it < MG_POWER_ITERS; it++)
+    {
+      double ab, nn;
+      HC_TRY_HIP(dot2(h, L.x, L.x, L.NN, ab, nn, st));
+      HC_TRY_HIP(axpby(L.NN, 1.0 / std::sqrt(std::max(nn, 1e-300)), L.x, 0.0, L.x, st));
+      if (int rc = node_apply(L, L.x, L.r, st))
+        return rc;
+      HC_TRY_HIP(mul(L.NN, L.dinv, L.r, L.z, st));
+      double xr, xx, xz, zz;
+      HC_TRY_HIP(dot2(h, L.x, L.r, L.NN, xr, xx, st)); // (x, S x)
+      HC_TRY_HIP(dot2(h, L.r, L.z, L.NN, xz, zz, st)); // (S x, D^-1 S x)
+      lam = xr > 0 ? xz / xr : 1.0;
+      HC_TRY_HIP(hipMemcpyAsync(L.x, L.z, sizeof(double) * L.NN, hipMemcpyDeviceToDevice, st));
+    }
+    L.lam = lam;
+  }
+  // coarsest: dense S_c by probing, + alpha 1 1^T, Cholesky inverse
+  {
+    NodeLevel & C = nh->lev.back();
+    const int64_t n = C.NN;
+    if (int rc = nh->sinv_mem.alloc(sizeof(double) * (n * n + n + 64)))
+      return rc;
+    nh->Sinv = (double *)nh->sinv_mem.base;
+    double * e = nh->Sinv + n * n;
+    int * info = (int *)(e + n);
+    double * tr = (double *)(info + 16);
+    for (int64_t j = 0; j < n; j++)
+    {
+      hipLaunchKernelGGL(k_identity_cols, dim3(grid_for(n)), dim3(256), 0, st, e, n, (int)j, 1);
+      HC_TRY_HIP(hipGetLastError());
+      if (int rc = node_apply(C, e, nh->Sinv + j * n, st))
+        return rc;
+    }
+    hipLaunchKernelGGL(k_trace_part, dim3(1), dim3(256), 0, st, (const double *)nh->Sinv, n, tr);
+    double htr = 0.0;
+    HC_TRY_HIP(hipMemcpyAsync(&htr, tr, sizeof(double), hipMemcpyDeviceToHost, st));
+    HC_TRY_HIP(hipStreamSynchronize(st));
+    if (!(htr > 0.0) || !std::isfinite(htr))
+      return fail(HC_EINVAL, "multigrid: node coarsest level has no positive trace");
+    hipLaunchKernelGGL(k_add_const, dim3(grid_for(n * n)), dim3(256), 0, st, nh->Sinv, n * n, htr / ((double)n * n));
+    HC_TRY_HIP(hipGetLastError());
+    if (rocblas_set_stream(h.blas, st) != rocblas_status_success ||
+        rocsolver_dpotrf(h.blas, rocblas_fill_lower, (rocblas_int)n, nh->Sinv, (rocblas_int)n, info) !=
+            rocblas_status_success ||
+        rocsolver_dpotri(h.blas, rocblas_fill_lower, (rocblas_int)n, nh->Sinv, (rocblas_int)n, info + 1) !=
+            rocblas_status_success)
+      return fail(HC_EHIP, "multigrid: rocsolver Cholesky of the node coarsest level failed");
+    int hinfo[2] = {0, 0};
+    HC_TRY_HIP(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
+    HC_TRY_HIP(hipStreamSynchronize(st));
+    if (hinfo[0] || hinfo[1])
+      return fail(HC_EHIP, "multigrid: node coarsest matrix is not positive definite");
+    hipLaunchKernelGGL(k_symmetrize_lower, dim3(grid_for(n * n)), dim3(256), 0, st, nh->Sinv, n);
+    HC_TRY_HIP(hipGetLastError());
+    nh->NC = n;
+  }
+  h.nodes = guard.release();
+  return HC_OK;
+}
+
+} // namespace
+
+// Whether the AV block preconditioner diag(V-cycle of A, node V-cycle of S)
+// is available for op (builds both hierarchies on first use).
+bool
+mg_av_ready(hc_op_t op, hipStream_t st)
+{
+  if (!op->box || mg_build(op, st) != HC_OK)
+    return false;
+  MgHier & h = *op->mg;
+  if (h.nodes_state == 0)
+    h.nodes_state = node_build(h, st) == HC_OK ? 1 : -1;
+  return h.nodes_state == 1;
+}
+
+// za = V-cycle of the ND system on ra, zv = node V-cycle of S on rv (both
+// fixed linear operators: a symmetric positive definite block preconditioner)
+int
+mg_av_precond(hc_op_t op, const double * ra, const double * rv, double * za, double * zv, hipStream_t st)
+{
+  MgHier & h = *op->mg;
+  MgLevel & L = h.lev[0];
+  double * const b0 = L.b;
+  double * const x0 = L.x;
+  L.b = (double *)ra;
+  L.x = za;
+  int rc = vcycle(h, 0, st);
+  L.b = b0;
+  L.x = x0;
+  if (rc)
+    return rc;
+  NodeLevel & N0 = h.nodes->lev[0];
+  double * const nb0 = N0.b;
+  double * const nx0 = N0.x;
+  N0.b = (double *)rv;
+  N0.x = zv;
+  rc = node_vcycle(h, *h.nodes, 0, st);
+  N0.b = nb0;
+  N0.x = nx0;
+  return rc;
 }
 
 } // namespace hc

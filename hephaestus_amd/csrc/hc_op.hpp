@@ -359,6 +359,10 @@ void geom_for(const hc_op_s * op, const TileCandidate * c, Geom & g, PlaneSegs &
 int mg_pcg_solve(hc_op_t op, const double * b, double * x, double rtol, int maxit, hc_solve_stats * stats,
                  hipStream_t st);
 int mg_levels(hc_op_t op, hipStream_t st, int * nlev, int64_t * ncoarse);
+// AV block preconditioner (hc_mg.hip): availability (builds the ND and node
+// hierarchies) and application za = V-cycle(ra), zv = node V-cycle(rv).
+bool mg_av_ready(hc_op_t op, hipStream_t st);
+int mg_av_precond(hc_op_t op, const double * ra, const double * rv, double * za, double * zv, hipStream_t st);
 
 // Graph capture of the engine's steady-state rounds (process-wide switch,
 // off by default; hc_engine_set_graphs).

@@ -219,9 +219,15 @@ int hc_op_av_apply(hc_op_t op, const double * a, const double * v, double * ya, 
  * monolithic matrix, src/problem_builders/problem_builder_base.cpp:206-218,
  * for AVFormulation): the block operator of hc_op_av_apply is symmetric
  * positive semi-definite (zero on the gauge fields (-G phi, phi)), so this is
- * PCG with block-Jacobi preconditioning (diag(A)^-1 on the ND DOFs, 1 on PEC
- * DOFs; diag(G^T M G)^-1 on the nodes).  (a, v) hold the initial guess and
- * receive the solution; its gauge component stays that of the guess.  Stops
+ * PCG with a block-diagonal preconditioner: on box meshes with the multigrid
+ * hierarchy of HC_SOLVE_MG (dense coarsest level), the V-cycle of that
+ * hierarchy on the ND block and a node-space V-cycle on G^T M G (stats->
+ * outer_iterations = 2); elsewhere block Jacobi (diag(A)^-1 on the ND DOFs, 1
+ * on PEC DOFs; diag(G^T M G)^-1 on the nodes; outer_iterations = 1).  With
+ * exact blocks the preconditioned operator has the eigenvalues 0 (gauge), 1
+ * and 2 on the free DOFs (K G = 0 makes A G = M G).  (a, v) hold the initial
+ * guess and
+ * receive the solution (its gauge component is not determined).  Stops
  * when ||b - AV x|| <= rtol ||b|| (FP64, both blocks); stats->rel_res is the
  * true residual of the result.  Its work vectors are allocated on the first
  * call and kept with the operator; the PCG scalars stay on the device (the
