@@ -456,6 +456,29 @@ hc_op_div_free_source(hc_op_t op, const double * j, double * src, double * lf, d
   double * part = cv.take(hc::PCG_PARTS);
   hc::PcgDev * sdev = (hc::PcgDev *)cv.p;
   hc::PcgDev * shost = (hc::PcgDev *)op->av_host;
+  // the projection's node Laplacian G^T M G: preconditioned by the node-space
+  // V-cycle of a unit-mass operator on the same box mesh (built once, outside the timed region), else Jacobi
+  if (project && op->box && op->dfs_mg == 0)
+  {
+    op->dfs_mg = -1;
+    hc_mesh_desc md{};
+    md.nx = op->g.n[0];
+    md.ny = op->g.n[1];
+    md.nz = op->g.n[2];
+    md.vert = nullptr;
+    for (int a = 0; a < 3; a++)
+    {
+      md.x0[a] = op->x0[a];
+      md.len[a] = op->h[a] * op->g.n[a];
+    }
+    md.pec_faces = 0;
+    std::vector<double> zero(E, 0.0), one(E, 1.0);
+    if (!op->dfs_op && hc_op_create(&md, op->p, zero.data(), one.data(), HC_PREC_FP64, &op->dfs_op) != HC_OK)
+      op->dfs_op = nullptr;
+    if (op->dfs_op && hc_op_setup(op->dfs_op, 1.0, stream) == HC_OK && hc::mg_av_ready(op->dfs_op, st))
+      op->dfs_mg = 1;
+  }
+  const bool node_mg = project && op->dfs_mg == 1;
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return hc::fail(HC_EHIP, "hc_op_div_free_source: event creation failed");
@@ -516,7 +539,10 @@ hc_op_div_free_source(hc_op_t op, const double * j, double * src, double * lf, d
     };
     // the stiffness apply uses t and q: the node PCG's own vectors are rn..qn
     const hc::PcgBlock blk{NN, dS, bn, Qv, rn, zn, pn, qn};
-    if ((rc = hc::pcg_dev(1, &blk, stiff, rtol, maxit, part, sdev, shost, st, it2, c2ok, s2)))
+    auto npr = [&](const double * r0, const double *, double * z0, double *) -> int {
+      return hc::mg_node_precond(op->dfs_op, r0, z0, st);
+    };
+    if ((rc = hc::pcg_dev(1, &blk, stiff, rtol, maxit, part, sdev, shost, st, it2, c2ok, s2, node_mg, npr)))
       return rc;
     // g <- g - G Q
     HC_TRY_HIP(hc::grad_apply(op->ag, Qv, t, st));

@@ -1277,6 +1277,21 @@ mg_av_ready(hc_op_t op, hipStream_t st)
   return h.nodes_state == 1;
 }
 
+int
+mg_node_precond(hc_op_t op, const double * r, double * z, hipStream_t st)
+{
+  MgHier & h = *op->mg;
+  NodeLevel & N0 = h.nodes->lev[0];
+  double * const nb0 = N0.b;
+  double * const nx0 = N0.x;
+  N0.b = (double *)r;
+  N0.x = z;
+  const int rc = node_vcycle(h, *h.nodes, 0, st);
+  N0.b = nb0;
+  N0.x = nx0;
+  return rc;
+}
+
 // za = V-cycle of the ND system on ra, zv = node V-cycle of S on rv (both
 // fixed linear operators: a symmetric positive definite block preconditioner)
 int

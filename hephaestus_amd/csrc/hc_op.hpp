@@ -203,6 +203,10 @@ struct hc_op_s
   hc::DevBlock av_mem, dfs_mem;
   double * av_host = nullptr; // pinned copy of the solver state
   bool dfs_ready = false;
+  // unit-mass operator on the same box mesh (nu = 0, sigma = 1) whose node
+  // hierarchy preconditions the Helmholtz projection of hc_op_div_free_source
+  hc_op_t dfs_op = nullptr;
+  int dfs_mg = 0; // 0 not tried, 1 available, -1 not available
   int setup_version = 0;
 
   template <typename T>
@@ -363,6 +367,8 @@ int mg_levels(hc_op_t op, hipStream_t st, int * nlev, int64_t * ncoarse);
 // hierarchies) and application za = V-cycle(ra), zv = node V-cycle(rv).
 bool mg_av_ready(hc_op_t op, hipStream_t st);
 int mg_av_precond(hc_op_t op, const double * ra, const double * rv, double * za, double * zv, hipStream_t st);
+// z = node V-cycle of G^T M G on r (op's node hierarchy, after mg_av_ready)
+int mg_node_precond(hc_op_t op, const double * r, double * z, hipStream_t st);
 
 // Graph capture of the engine's steady-state rounds (process-wide switch,
 // off by default; hc_engine_set_graphs).

@@ -646,6 +646,8 @@ hc_op_s::~hc_op_s()
   hc::mg_free(mg);
   if (av_host)
     (void)hipHostFree(av_host);
+  if (dfs_op)
+    hc_op_destroy(dfs_op);
   for (void * v : allocs)
     (void)hipFree(v);
 }

@@ -240,8 +240,9 @@ int hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a,
  * the Helmholtz projection g <- g - G Q, (G^T M G) Q = G^T M g (Q up to a
  * constant, no boundary conditions, as the reference's HelmholtzProjector with
  * an empty BCMap); then lf += M g.  src receives g, Q (optional, H1 nodes) the
- * potential.  Both solves: Jacobi-PCG to rtol (the reference: its default
- * GMRES).  stats: iterations of both solves summed; lf is left unchanged when
+ * potential.  Both solves: PCG to rtol (the reference: its default GMRES),
+ * Jacobi-preconditioned, the projection's node Laplacian on box meshes by
+ * the node-space multigrid V-cycle of a unit-mass operator on the same mesh.  stats: iterations of both solves summed; lf is left unchanged when
  * a solve fails.  Its work vectors and the unit-mass quadrature factors
  * (which depend on the mesh alone) are set up on the first call and kept with
  * the operator; PCG scalars on the device as in hc_op_av_solve. */
