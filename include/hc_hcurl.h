@@ -231,7 +231,7 @@ int hc_op_av_apply(hc_op_t op, const double * a, const double * v, double * ya, 
  * when ||b - AV x|| <= rtol ||b|| (FP64, both blocks); stats->rel_res is the
  * true residual of the result.  Its work vectors are allocated on the first
  * call and kept with the operator; the PCG scalars stay on the device (the
- * host reads the solver state every 8 iterations). */
+ * host reads the solver state every few iterations, at most 8). */
 int hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a, double * v, double rtol,
                    int maxit, hc_solve_stats * stats, void * stream);
 /* DivFreeSource::Apply (src/sources/div_free_source.cpp:71-114) for a source
