@@ -280,8 +280,12 @@ public:
   // first instance of that order.  rhs_block: right-hand sides per work item
   // of the batched apply (0: default).
   // mode: HC_SOLVE_FP64 or HC_SOLVE_FP32_IR, optionally | HC_SOLVE_AUX
+  // st: the stream of the solves that follow; the table uploads (and the read
+  // of the iteration counter) are ordered on it, not on the null stream, so
+  // that engines on other streams are not synchronised with it.
   int set(const std::vector<EngineInst> & inst, int mode, int rhs_block = 0, bool force_batched = false,
-          const TileCandidate * const * cands64 = nullptr, const TileCandidate * const * cands32 = nullptr);
+          const TileCandidate * const * cands64 = nullptr, const TileCandidate * const * cands32 = nullptr,
+          hipStream_t st = nullptr);
   // One round: phase moves, FP64 residuals and refinement updates, inner
   // starts, then `iters` PCG iterations of every slot in an inner solve.
   // graph: replay a captured hipGraph of this round's launches (captured on

@@ -194,7 +194,7 @@ hc_server_s::set_engine()
     e.fp64 = r->fp64;
     inst.push_back(e);
   }
-  return engine.set(inst, cfg.mode, 0, true, cands64, cands32);
+  return engine.set(inst, cfg.mode, 0, true, cands64, cands32, st);
 }
 
 int

@@ -68,7 +68,7 @@ Engine::max_outer() const
 
 int
 Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool force_batched,
-            const TileCandidate * const * cands64, const TileCandidate * const * cands32)
+            const TileCandidate * const * cands64, const TileCandidate * const * cands32, hipStream_t st)
 {
   drop_graphs(); // captured rounds hold the old tables' sizes
   inst_ = inst;
@@ -126,7 +126,10 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
   // the iteration counter survives a rebuild (slots may be mid-solve)
   int git_host[2] = {0, 0};
   if (git_)
-    HC_TRY_HIP(hipMemcpy(git_host, git_, sizeof(git_host), hipMemcpyDeviceToHost));
+  {
+    HC_TRY_HIP(hipMemcpyAsync(git_host, git_, sizeof(git_host), hipMemcpyDeviceToHost, st));
+    HC_TRY_HIP(hipStreamSynchronize(st));
+  }
   if (mem_.bytes < total)
   {
     int rc = mem_.alloc(total);
@@ -312,10 +315,11 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
       groups_[k].push_back(G);
     }
   }
-  HC_TRY_HIP(hipMemcpy(d_slots, hs.data(), sizeof(SolveSlot) * nslots_, hipMemcpyHostToDevice));
-  HC_TRY_HIP(hipMemcpy(d_blk0, hblk.data(), sizeof(int) * (nslots_ + 1), hipMemcpyHostToDevice));
-  HC_TRY_HIP(hipMemcpy(d_desc, hd.data(), sizeof(ApplyDesc) * hd.size(), hipMemcpyHostToDevice));
-  HC_TRY_HIP(hipMemcpy(git_, git_host, sizeof(git_host), hipMemcpyHostToDevice));
+  HC_TRY_HIP(hipMemcpyAsync(d_slots, hs.data(), sizeof(SolveSlot) * nslots_, hipMemcpyHostToDevice, st));
+  HC_TRY_HIP(hipMemcpyAsync(d_blk0, hblk.data(), sizeof(int) * (nslots_ + 1), hipMemcpyHostToDevice, st));
+  HC_TRY_HIP(hipMemcpyAsync(d_desc, hd.data(), sizeof(ApplyDesc) * hd.size(), hipMemcpyHostToDevice, st));
+  HC_TRY_HIP(hipMemcpyAsync(git_, git_host, sizeof(git_host), hipMemcpyHostToDevice, st));
+  HC_TRY_HIP(hipStreamSynchronize(st)); // the host tables above are about to go out of scope
   if (host_cap_ < nslots_)
   {
     if (host_)
@@ -582,7 +586,7 @@ Engine::solve(int mode, hipStream_t st_in, std::vector<SlotStats> & stats, doubl
         c64[p] = cands64_[p];
         c32[p] = cands32_[p];
       }
-      if ((rc = set(keep, mode_ | (aux_ ? HC_SOLVE_AUX : 0), rhs_block_, force_batched_, c64, c32)))
+      if ((rc = set(keep, mode_ | (aux_ ? HC_SOLVE_AUX : 0), rhs_block_, force_batched_, c64, c32, st)))
         return rc;
       orig.swap(keep_orig);
     }
@@ -595,7 +599,7 @@ Engine::solve(int mode, hipStream_t st_in, std::vector<SlotStats> & stats, doubl
         c64[p] = cands64_[p];
         c32[p] = cands32_[p];
       }
-      if ((rc = set(cur, mode_ | (aux_ ? HC_SOLVE_AUX : 0), rhs_block_, force_batched_, c64, c32)))
+      if ((rc = set(cur, mode_ | (aux_ ? HC_SOLVE_AUX : 0), rhs_block_, force_batched_, c64, c32, st)))
         return rc;
     }
   }
@@ -614,7 +618,7 @@ Engine::solve(int mode, hipStream_t st_in, std::vector<SlotStats> & stats, doubl
     c64[p] = cands64_[p];
     c32[p] = cands32_[p];
   }
-  return set(all, mode_ | (aux_ ? HC_SOLVE_AUX : 0), rhs_block_, force_batched_, c64, c32);
+  return set(all, mode_ | (aux_ ? HC_SOLVE_AUX : 0), rhs_block_, force_batched_, c64, c32, st);
 }
 
 } // namespace hc
@@ -792,7 +796,7 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
   if (!op->engine)
     op->engine = new hc::Engine();
   std::vector<hc::EngineInst> inst{{op, nrhs, b, x, rtol, maxit, &op->ws, 0}};
-  if ((rc = op->engine->set(inst, mode, op->rhs_block)))
+  if ((rc = op->engine->set(inst, mode, op->rhs_block, false, nullptr, nullptr, (hipStream_t)stream)))
     return rc;
   std::vector<hc::SlotStats> s;
   double ms = 0;
@@ -875,7 +879,7 @@ hc_batch_solve(hc_batch_t bt, int mode, hc_solve_stats * stats, void * stream)
       const auto & e = bt->entries[i];
       inst.push_back({e.op, e.nrhs, e.b, e.x, e.rtol, e.maxit, &bt->ws[i], 0});
     }
-    if ((rc = bt->engine.set(inst, mode, bt->rhs_block, true)))
+    if ((rc = bt->engine.set(inst, mode, bt->rhs_block, true, nullptr, nullptr, (hipStream_t)stream)))
       return rc;
     bt->dirty = false;
     bt->mode = mode;
