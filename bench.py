@@ -308,7 +308,7 @@ def like_for_like(res, kind):
     with the same preconditioner, and over the plain FP64 Jacobi path."""
     sps = lambda k: res[k]["solves_per_s"] if k in res and res[k].get("solves_per_s") else None
     out = {}
-    for pre, base in (("", "fp64_single"), ("_aux", "fp64_aux_single")):
+    for pre, base in (("", "fp64_single"), ("_aux", "fp64_aux_single"), ("_mg", "fp64_mg_single")):
         for prec in ("fp64", "fp32_ir"):
             key = f"{prec}{pre}_{kind}"
             if sps(key) and sps(base):
@@ -358,13 +358,15 @@ def solves_uneven(torch, H, ops, bs, args):
     """64 uneven instances, coil right-hand sides, rtol 1e-8: one batched
     FP32 PCG + FP64 refinement solve (FP64 continuation per instance where FP32
     stalls) against the plain FP64 path solving the instances one at a time;
-    each also with the gradient-space correction of the preconditioner (_aux)."""
+    each also with the gradient-space correction of the preconditioner (_aux);
+    multigrid one at a time and as a batch (worker streams, _mg)."""
     res = {"workload": f"{len(ops)} cubes n~U{{8..40}}, p~U{{1,2,3}}, two regions sigma/dt=10^U[-6,4] "
                        f"(seed 4+i), coil RHS (seed 1000+i); {sum(o.ndofs for o in ops)} DOFs in total",
            "rtol": 1e-8}
     x_refs = [fp64_reference(H, op, b, args) for op, b in zip(ops, bs)]
     for name, mode in (("fp64_batched", H.SOLVE_FP64), ("fp64_aux_batched", H.SOLVE_FP64 | H.SOLVE_AUX),
-                       ("fp32_ir_batched", H.SOLVE_FP32_IR), ("fp32_ir_aux_batched", H.SOLVE_FP32_IR | H.SOLVE_AUX)):
+                       ("fp32_ir_batched", H.SOLVE_FP32_IR), ("fp32_ir_aux_batched", H.SOLVE_FP32_IR | H.SOLVE_AUX),
+                       ("fp64_mg_batched", H.SOLVE_FP64 | H.SOLVE_MG)):
         xs = [torch.zeros_like(b) for b in bs]
         bt = H.Batch()
         for op, b, x in zip(ops, bs, xs):

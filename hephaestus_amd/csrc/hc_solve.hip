@@ -8,6 +8,12 @@
 // table of instances (hc_apply.hip), each gated by the slots' done flags.
 #include "hc_op.hpp"
 
+#include <chrono>
+#include <condition_variable>
+#include <mutex>
+#include <set>
+#include <thread>
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -809,6 +815,146 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
   return status_rc(out, "hc_pcg_solve");
 }
 
+// HC_SOLVE_MG batches: the multigrid PCG of every right-hand side (the
+// gradient-space PCG for an operator whose mesh has no hierarchy), run by
+// MG_BATCH_WORKERS host threads with their own streams; a worker takes the
+// next right-hand side whose operator no other worker is using, so solves of
+// different operators overlap on the device.  Each result is that of the same
+// solve run alone.  stats[i].solve_ms: the batch's wall time.
+constexpr int MG_BATCH_WORKERS = 4;
+
+static int
+mg_batch_solve(hc_batch_t bt, hc_solve_stats * stats, hipStream_t caller)
+{
+  HC_TRY_HIP(hipStreamSynchronize(caller)); // b and x as the caller's stream left them
+  struct Task
+  {
+    int entry, rhs;
+  };
+  std::vector<Task> tasks;
+  for (int i = 0; i < (int)bt->entries.size(); i++)
+    for (int r = 0; r < bt->entries[i].nrhs; r++)
+      tasks.push_back({i, r});
+  std::vector<hc_solve_stats> ts(tasks.size());
+  std::vector<char> done(tasks.size(), 0);
+  std::mutex mu;
+  std::condition_variable cv;
+  std::set<hc_op_t> busy;
+  size_t next_free = 0; // tasks before it are all taken
+  int err = HC_OK;
+  std::string err_msg;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto worker = [&]() {
+    (void)hipSetDevice(0);
+    hipStream_t st = nullptr;
+    if (hipStreamCreate(&st) != hipSuccess)
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (!err)
+      {
+        err = hc::fail(HC_EHIP, "hc_batch_solve: hipStreamCreate failed");
+        err_msg = hc_last_error();
+      }
+      cv.notify_all();
+      return;
+    }
+    for (;;)
+    {
+      int k = -1;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;)
+        {
+          if (err || next_free >= tasks.size())
+            break;
+          for (size_t j = next_free; j < tasks.size(); j++)
+            if (!done[j] && !busy.count(bt->entries[tasks[j].entry].op))
+            {
+              k = (int)j;
+              break;
+            }
+          if (k >= 0)
+            break;
+          cv.wait(lk);
+        }
+        if (k < 0)
+          break;
+        done[k] = 1; // taken
+        busy.insert(bt->entries[tasks[k].entry].op);
+        while (next_free < tasks.size() && done[next_free])
+          next_free++;
+      }
+      const auto & e = bt->entries[tasks[k].entry];
+      hc_op_t op = e.op;
+      const int64_t off = (int64_t)tasks[k].rhs * op->N;
+      int nlev = 0;
+      int64_t nc = 0;
+      hc_solve_stats s{};
+      int rc = hc::mg_levels(op, st, &nlev, &nc) == HC_OK
+                   ? hc::mg_pcg_solve(op, e.b + off, e.x + off, e.rtol, e.maxit, &s, st)
+                   : hc_pcg_solve(op, e.b + off, e.x + off, 1, e.rtol, e.maxit, HC_SOLVE_FP64 | HC_SOLVE_AUX, &s, st);
+      if (rc == HC_ENOTCONV || rc == HC_ENAN)
+        rc = HC_OK; // reported in the statistics
+      if (!rc && hipStreamSynchronize(st) != hipSuccess)
+        rc = hc::fail(HC_EHIP, "hc_batch_solve: stream synchronisation failed");
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        ts[k] = s;
+        busy.erase(op);
+        if (rc && !err)
+        {
+          err = rc;
+          err_msg = hc_last_error();
+        }
+      }
+      cv.notify_all();
+      if (rc)
+        break;
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    cv.notify_all();
+  };
+  std::vector<std::thread> pool;
+  for (int w = 0; w < MG_BATCH_WORKERS; w++)
+    pool.emplace_back(worker);
+  for (auto & t : pool)
+    t.join();
+  if (err)
+    return hc::fail(err, "hc_batch_solve: " + err_msg);
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  hc_solve_stats all{};
+  all.converged = 1;
+  size_t pos = 0;
+  for (size_t i = 0; i < bt->entries.size(); i++)
+  {
+    hc_solve_stats s{};
+    s.converged = 1;
+    for (int r = 0; r < bt->entries[i].nrhs; r++, pos++)
+    {
+      const hc_solve_stats & t = ts[pos];
+      s.iterations = std::max(s.iterations, t.iterations);
+      s.outer_iterations = std::max(s.outer_iterations, t.outer_iterations);
+      s.rel_res = std::max(s.rel_res, t.rel_res);
+      if (!t.converged)
+        s.converged = 0;
+      if (t.status && !s.status)
+        s.status = t.status;
+    }
+    s.solve_ms = ms;
+    if (stats)
+      stats[i] = s;
+    all.iterations = std::max(all.iterations, s.iterations);
+    all.rel_res = std::max(all.rel_res, s.rel_res);
+    if (!s.converged)
+      all.converged = 0;
+    if (s.status && !all.status)
+      all.status = s.status;
+  }
+  all.solve_ms = ms;
+  return status_rc(all, "hc_batch_solve");
+}
+
 int
 hc_batch_create(hc_batch_t * out)
 {
@@ -849,6 +995,12 @@ hc_batch_solve(hc_batch_t bt, int mode, hc_solve_stats * stats, void * stream)
     return hc::fail(HC_EINVAL, "hc_batch_solve: null handle");
   if (bt->entries.empty())
     return hc::fail(HC_EINVAL, "hc_batch_solve: empty batch");
+  if (mode & HC_SOLVE_MG)
+  {
+    if (hc::prec_mode(mode) != HC_SOLVE_FP64 || (mode & ~(HC_SOLVE_MG | HC_SOLVE_AUX)))
+      return hc::fail(HC_EINVAL, "hc_batch_solve: HC_SOLVE_MG runs in FP64 (mode HC_SOLVE_FP64 | HC_SOLVE_MG)");
+    return mg_batch_solve(bt, stats, (hipStream_t)stream);
+  }
   const int pm = hc::prec_mode(mode);
   if ((pm != HC_SOLVE_FP64 && pm != HC_SOLVE_FP32_IR) || (mode & ~(HC_SOLVE_AUX | 0xF)))
     return hc::fail(HC_EINVAL, "hc_batch_solve: unknown mode");

@@ -311,7 +311,13 @@ int hc_mg_transfer(hc_op_t op, int level, int dir, const double * x, double * y,
  *
  * hc_batch_add: b, x are device arrays of nrhs*N doubles of op (x = initial
  * guess); returns the entry index (>= 0) or a negative status.
- * hc_batch_solve: stats (optional) receives one entry per added instance. */
+ * hc_batch_solve: stats (optional) receives one entry per added instance.
+ * Mode HC_SOLVE_FP64 | HC_SOLVE_MG: every right-hand side is solved by the
+ * multigrid PCG of hc_pcg_solve (the gradient-space PCG where a mesh has no
+ * hierarchy) by 4 host workers with their own streams, each taking the next
+ * right-hand side whose operator no other worker is using (the caller's
+ * stream is synchronised first; the call returns when all are done; solve_ms
+ * is the batch's wall time). */
 int hc_batch_create(hc_batch_t * out);
 int hc_batch_add(hc_batch_t batch, hc_op_t op, const double * b, double * x, int nrhs, double rtol, int maxit);
 /* Right-hand sides per work item of the batched apply (0 = automatic). */

@@ -83,7 +83,7 @@ def test_multi_rhs_apply(torch_cuda, hc, O, p, n, rb):
         assert _rel(Y32[r], Yh[r]) <= 2e-6
 
 
-@pytest.mark.parametrize("mode", ["fp64", "fp32_ir", "fp64_aux", "fp32_ir_aux"])
+@pytest.mark.parametrize("mode", ["fp64", "fp32_ir", "fp64_aux", "fp32_ir_aux", "fp64_mg"])
 def test_multi_rhs_solve(torch_cuda, hc, O, mode):
     torch = torch_cuda
     n, p = (8, 7, 6), 2
@@ -91,6 +91,8 @@ def test_multi_rhs_solve(torch_cuda, hc, O, mode):
     md = hc.SOLVE_FP64 if mode.startswith("fp64") else hc.SOLVE_FP32_IR
     if mode.endswith("_aux"):
         md |= hc.SOLVE_AUX
+    if mode.endswith("_mg"):
+        md |= hc.SOLVE_MG
     m = O.Mesh(*n)
     sig = _two_region(n, 100.0, 0.1)
     op = hc.Operator(n, p, 1.0, sig).setup(1.0)
@@ -128,12 +130,14 @@ SPECS = [((8, 8, 8), 1, 1e2, 1e-2), ((6, 9, 7), 2, 10.0, 1.0), ((5, 5, 6), 3, 1e
          ((10, 6, 5), 1, 1.0, 1.0), ((7, 7, 7), 2, 1e2, 1e-2)]
 
 
-@pytest.mark.parametrize("mode", ["fp32_ir", "fp64", "fp64_aux", "fp32_ir_aux"])
+@pytest.mark.parametrize("mode", ["fp32_ir", "fp64", "fp64_aux", "fp32_ir_aux", "fp64_mg"])
 def test_batch_uneven_mix_invariance(torch_cuda, hc, O, mode):
     torch = torch_cuda
     md = hc.SOLVE_FP64 if mode.startswith("fp64") else hc.SOLVE_FP32_IR
     if mode.endswith("_aux"):
         md |= hc.SOLVE_AUX
+    if mode.endswith("_mg"):
+        md |= hc.SOLVE_MG  # multigrid solves on worker streams
     print(f"case uneven batch {mode} ({len(SPECS)} instances)", flush=True)
     ops = _instances(hc, SPECS)
     rtol = 1e-8
@@ -172,13 +176,14 @@ SPAN = [((8, 9, 8), 1, 1e4, 1e-6), ((10, 8, 9), 2, 1e-6, 1e4), ((8, 8, 7), 3, 1e
         ((12, 10, 8), 1, 1e-6, 1e-6), ((9, 7, 10), 2, 1e4, 1e4), ((7, 8, 8), 3, 1e-6, 1e4)]
 
 
-@pytest.mark.parametrize("mode", ["fp64", "fp32_ir_aux"])
+@pytest.mark.parametrize("mode", ["fp64", "fp32_ir_aux", "fp64_mg"])
 def test_batch_sigma_span(torch_cuda, hc, O, mode):
     """Uneven batch whose instances span sigma/dt 1e-6 and 1e4, coil right-hand
     sides: every instance converges, its true residual recomputed by the oracle
     meets rtol 1e-8, and it is bitwise the same solved alone."""
     torch = torch_cuda
-    md = hc.SOLVE_FP64 if mode == "fp64" else hc.SOLVE_FP32_IR | hc.SOLVE_AUX
+    md = {"fp64": hc.SOLVE_FP64, "fp32_ir_aux": hc.SOLVE_FP32_IR | hc.SOLVE_AUX,
+          "fp64_mg": hc.SOLVE_FP64 | hc.SOLVE_MG}[mode]
     print(f"case sigma span {mode} ({len(SPAN)} instances)", flush=True)
     ops = _instances(hc, SPAN)
     rtol = 1e-8
