@@ -32,6 +32,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -55,6 +56,7 @@ constexpr double MG_COARSE_RTOL = 0.2;     // larger coarsest levels: inexact PC
 constexpr int MG_COARSE_MAXIT = 200;
 constexpr int MG_POWER_ITERS = 15;      // power iterations for the smoother's lambda_max
 constexpr int DOT_BLOCKS = 512;
+std::atomic<int> g_mg_graphs{1}; // replay the finest-level V-cycle from a captured graph
 constexpr int DOT_THREADS = 256;
 
 // ---------------------------------------------------------------------------
@@ -157,6 +159,29 @@ k_symmetrize_lower(double * __restrict__ A, int64_t n)
     if (i < j)
       A[k] = A[j + i * n];
   }
+}
+
+// y = A x for a dense symmetric n x n matrix stored full, column-major: row i
+// of A is its column i (contiguous).  One workgroup per row, fixed-order sum.
+__global__ void __launch_bounds__(256)
+k_dense_symv(const double * __restrict__ A, int64_t n, const double * __restrict__ x, double * __restrict__ y)
+{
+  __shared__ double sh[256];
+  const int64_t i = blockIdx.x;
+  const double * __restrict__ col = A + i * n;
+  double s = 0.0;
+  for (int64_t j = threadIdx.x; j < n; j += 256)
+    s += col[j] * x[j];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1)
+  {
+    if ((int)threadIdx.x < k)
+      sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    y[i] = sh[0];
 }
 
 // Identity columns j0 .. j0 + nb - 1 of an n x n matrix (column-major, nb columns).
@@ -358,6 +383,15 @@ struct MgHier
   int64_t NL = 0;
   rocblas_handle blas = nullptr;
   int nsmooth = MG_SMOOTH;   // smoothing steps before and after the coarse correction
+  // the finest-level V-cycle captured as a graph for fixed input and output
+  // vectors (replayed on the solve's stream); rebuilt when a level operator's
+  // workspace moved
+  hipStream_t cap = nullptr;
+  hipGraphExec_t vc = nullptr;
+  const double * vc_in = nullptr;
+  double * vc_out = nullptr;
+  std::vector<void *> vc_ws;
+  int vc_seen = 0; // eager V-cycles with the current pointers (capture from the second)
   ~MgHier()
   {
     node_free(nodes);
@@ -368,6 +402,10 @@ struct MgHier
       (void)hipHostFree(hpart);
     if (blas)
       rocblas_destroy_handle(blas);
+    if (vc)
+      (void)hipGraphExecDestroy(vc);
+    if (cap)
+      (void)hipStreamDestroy(cap);
   }
 };
 
@@ -533,12 +571,10 @@ vcycle(MgHier & h, int l, hipStream_t st)
   }
   if (l == (int)h.lev.size() - 1)
   {
-    // the inverse is stored full (symmetric): x = A^-T b reads it by columns
-    const double one = 1.0, zero = 0.0;
-    if (rocblas_set_stream(h.blas, st) != rocblas_status_success ||
-        rocblas_dgemv(h.blas, rocblas_operation_transpose, (rocblas_int)L.N, (rocblas_int)L.N, &one, h.Ainv,
-                      (rocblas_int)L.N, L.b, 1, &zero, L.x, 1) != rocblas_status_success)
-      return fail(HC_EHIP, "multigrid: coarse solve (rocblas_dgemv) failed");
+    // the inverse is stored full (symmetric): x = A^-1 b reads it by columns
+    hipLaunchKernelGGL(k_dense_symv, dim3((unsigned)L.N), dim3(256), 0, st, (const double *)h.Ainv, L.N,
+                       (const double *)L.b, L.x);
+    HC_TRY_HIP(hipGetLastError());
     return HC_OK;
   }
   if (int rc = smooth(h, L, true, st)) // pre-smoothing from x = 0
@@ -550,6 +586,69 @@ vcycle(MgHier & h, int l, hipStream_t st)
     return rc;
   HC_TRY_HIP(transfer(h, l, true, h.lev[l + 1].x, L.x, st));
   return smooth(h, L, false, st); // post-smoothing
+}
+
+// z = V-cycle(r) on the finest level.  With graphs on and an exact coarsest
+// solve the launches are captured once per (r, z, level workspaces) -- on the
+// second call, after an eager one has made every lazy allocation -- and
+// replayed on st; the replay runs the same kernels with the same arguments in
+// the same order as the eager cycle.
+int
+vcycle_top(MgHier & h, double * rin, double * zout, hipStream_t st)
+{
+  MgLevel & L = h.lev[0];
+  auto eager = [&](hipStream_t s) -> int {
+    double * const b0 = L.b;
+    double * const x0 = L.x;
+    L.b = rin;
+    L.x = zout;
+    const int rc = vcycle(h, 0, s);
+    L.b = b0;
+    L.x = x0;
+    return rc;
+  };
+  if (!g_mg_graphs.load() || h.coarse_iter)
+    return eager(st);
+  std::vector<void *> ws;
+  for (auto & l : h.lev)
+    ws.push_back(l.op->ws_mem.base);
+  const bool same = h.vc_in == rin && h.vc_out == zout && ws == h.vc_ws;
+  if (!same)
+  {
+    if (h.vc)
+      (void)hipGraphExecDestroy(h.vc);
+    h.vc = nullptr;
+    h.vc_in = rin;
+    h.vc_out = zout;
+    h.vc_ws = ws;
+    h.vc_seen = 0;
+  }
+  if (!h.vc)
+  {
+    if (h.vc_seen++ == 0)
+      return eager(st);
+    if (!h.cap)
+      HC_TRY_HIP(hipStreamCreate(&h.cap));
+    HC_TRY_HIP(hipStreamBeginCapture(h.cap, hipStreamCaptureModeThreadLocal));
+    const int rc = eager(h.cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h.cap, &g);
+    if (rc || e != hipSuccess)
+    {
+      if (g)
+        (void)hipGraphDestroy(g);
+      return rc ? rc : fail(HC_EHIP, std::string("multigrid: V-cycle capture: ") + hipGetErrorString(e));
+    }
+    const hipError_t ei = hipGraphInstantiate(&h.vc, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess)
+    {
+      h.vc = nullptr;
+      return fail(HC_EHIP, std::string("multigrid: V-cycle graph instantiation: ") + hipGetErrorString(ei));
+    }
+  }
+  HC_TRY_HIP(hipGraphLaunch(h.vc, st));
+  return HC_OK;
 }
 
 Csr1Dev
@@ -854,16 +953,7 @@ mg_pcg_solve(hc_op_t op, const double * b, double * x, double rtol, int maxit, h
   double rr = 0.0;
   // the V-cycle reads the finest level's b and writes its x: point them at
   // the PCG's residual and preconditioned residual for the call
-  auto precond = [&](double * rin, double * zout) -> int {
-    double * const b0 = L.b;
-    double * const x0 = L.x;
-    L.b = rin;
-    L.x = zout;
-    const int rc = vcycle(h, 0, st);
-    L.b = b0;
-    L.x = x0;
-    return rc;
-  };
+  auto precond = [&](double * rin, double * zout) -> int { return vcycle_top(h, rin, zout, st); };
   for (outer = 1; outer <= 1 + MAX_RESTARTS && !conv && status == HC_OK; outer++)
   {
     // r = b - A x
@@ -1114,11 +1204,9 @@ node_vcycle(MgHier & h, NodeHier & nh, int l, hipStream_t st)
   NodeLevel & L = nh.lev[l];
   if (l == (int)nh.lev.size() - 1)
   {
-    const double one = 1.0, zero = 0.0;
-    if (rocblas_set_stream(h.blas, st) != rocblas_status_success ||
-        rocblas_dgemv(h.blas, rocblas_operation_transpose, (rocblas_int)L.NN, (rocblas_int)L.NN, &one, nh.Sinv,
-                      (rocblas_int)L.NN, L.b, 1, &zero, L.x, 1) != rocblas_status_success)
-      return fail(HC_EHIP, "multigrid: node coarse solve (rocblas_dgemv) failed");
+    hipLaunchKernelGGL(k_dense_symv, dim3((unsigned)L.NN), dim3(256), 0, st, (const double *)nh.Sinv, L.NN,
+                       (const double *)L.b, L.x);
+    HC_TRY_HIP(hipGetLastError());
     return HC_OK;
   }
   if (int rc = node_smooth(h, L, true, st))
@@ -1322,6 +1410,13 @@ mg_av_precond(hc_op_t op, const double * ra, const double * rv, double * za, dou
 } // namespace hc
 
 extern "C" {
+
+int
+hc_mg_set_graphs(int enable)
+{
+  hc::g_mg_graphs.store(enable ? 1 : 0);
+  return HC_OK;
+}
 
 int
 hc_mg_levels(hc_op_t op, int * nlevels, int64_t * coarse_ndofs, void * stream)

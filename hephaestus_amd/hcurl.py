@@ -43,7 +43,7 @@ EXPORTS = [
     "hc_op_nnodes", "hc_op_av_apply", "hc_op_av_solve", "hc_op_div_free_source",
     "hc_server_request_bytes", "hc_server_create", "hc_server_set_tile", "hc_server_submit", "hc_server_poll",
     "hc_server_info", "hc_server_destroy", "hc_op_apply_noalloc", "hc_noalloc_scratch_doubles",
-    "hc_mg_levels", "hc_mg_level_op", "hc_mg_transfer",
+    "hc_mg_levels", "hc_mg_level_op", "hc_mg_transfer", "hc_mg_set_graphs",
 ]
 
 
@@ -127,6 +127,7 @@ def load():
     lib.hc_mg_levels.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int64), vp]
     lib.hc_mg_level_op.argtypes = [vp, C.c_int, C.POINTER(vp)]
     lib.hc_mg_transfer.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
+    lib.hc_mg_set_graphs.argtypes = [C.c_int]
     lib.hc_op_destroy.argtypes = [vp]
     lib.hc_last_error.restype = C.c_char_p
     lib.hc_fill_uniform.argtypes = [vp, C.c_uint64, vp, C.c_int64, vp]
@@ -195,6 +196,11 @@ def _nrhs(v, n, dtype):
 def set_graphs(enable):
     """Process-wide: replay the solve engine's steady-state rounds from captured hipGraphs."""
     _check(load().hc_engine_set_graphs(1 if enable else 0), "hc_engine_set_graphs")
+
+
+def set_mg_graphs(enable):
+    """Process-wide: replay multigrid V-cycles from captured hipGraphs (default on)."""
+    _check(load().hc_mg_set_graphs(1 if enable else 0), "hc_mg_set_graphs")
 
 
 def tile_list(order):
