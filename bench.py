@@ -265,7 +265,8 @@ def solves_multi_rhs_32(torch, H, args):
     """32^3 p=2 two-region, 32 coil right-hand sides (seeds 3..34): plain FP64
     Jacobi-PCG one right-hand side at a time vs one batched FP32 PCG + FP64
     refinement solve of all 32, both to a true relative residual of 1e-8; each
-    also with the gradient-space correction of the preconditioner (_aux)."""
+    also with the gradient-space correction of the preconditioner (_aux), and
+    multigrid one at a time and with all 32 through one V-cycle (_mg)."""
     n, p, nr = 32, 2, 32
     op = H.Operator(n, p, 1.0, two_region(n)).setup(1.0)
     op.autotune(reps=5)
@@ -292,7 +293,8 @@ def solves_multi_rhs_32(torch, H, args):
         res[name] = {"sample_rhs": k, "iterations_mean": float(np.mean(its)), "max_true_rel_res": worst,
                      "max_err_vs_fp64_ref": err, "solves_per_s": round(1000.0 * k / tot_ms, 4)}
     for name, mode in (("fp64_batched", H.SOLVE_FP64), ("fp64_aux_batched", H.SOLVE_FP64 | H.SOLVE_AUX),
-                       ("fp32_ir_batched", H.SOLVE_FP32_IR), ("fp32_ir_aux_batched", H.SOLVE_FP32_IR | H.SOLVE_AUX)):
+                       ("fp32_ir_batched", H.SOLVE_FP32_IR), ("fp32_ir_aux_batched", H.SOLVE_FP32_IR | H.SOLVE_AUX),
+                       ("fp64_mg_batched", H.SOLVE_FP64 | H.SOLVE_MG)):
         X = torch.zeros_like(B)
         st = op.solve(B, X, rtol=1e-8, maxit=args.solve_maxit, mode=mode, raise_on_fail=False)
         res[name] = solve_entry(st, nr)

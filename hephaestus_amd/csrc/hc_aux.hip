@@ -2,6 +2,8 @@
 // quadrature data, and the standalone preconditioner apply (the batched PCG
 // kernels use the same device functions, hc_aux.hpp).
 #include "hc_aux.hpp"
+
+#include <algorithm>
 #include "hc_kernels.hpp"
 
 namespace hc
@@ -172,8 +174,10 @@ term_ratio_blocks(const double * qd, int64_t E, int p, double * out, hipStream_t
 // Standalone z = B r: t = Dg^-1 G^T r at the nodes, then z = D^-1 r + G t.
 __global__ void
 k_aux_nodes_single(AuxGeom a, const double * __restrict__ ginv, const double * __restrict__ r,
-                   double * __restrict__ t)
+                   double * __restrict__ t, int64_t sr, int64_t st_)
 {
+  r += blockIdx.y * sr; // right-hand side blockIdx.y (strides 0 for one)
+  t += blockIdx.y * st_;
   with_order(a, [&](auto pc) {
     constexpr int P = decltype(pc)::value;
     const DgTab<P> tab = load_dg<P>(a);
@@ -187,8 +191,11 @@ k_aux_nodes_single(AuxGeom a, const double * __restrict__ ginv, const double * _
 
 __global__ void
 k_aux_z_single(AuxGeom a, const double * __restrict__ dinv, const double * __restrict__ r,
-               const double * __restrict__ t, double * __restrict__ z)
+               const double * __restrict__ t, double * __restrict__ z, int64_t sr, int64_t st_)
 {
+  r += blockIdx.y * sr;
+  z += blockIdx.y * sr;
+  t += blockIdx.y * st_;
   with_order(a, [&](auto pc) {
     constexpr int P = decltype(pc)::value;
     const DgTab<P> tab = load_dg<P>(a);
@@ -200,10 +207,13 @@ k_aux_z_single(AuxGeom a, const double * __restrict__ dinv, const double * __res
 
 hipError_t
 aux_precond_apply(const AuxGeom & a, const double * dinv, const double * ginv, const double * r, double * t,
-                  double * z, hipStream_t st)
+                  double * z, hipStream_t st, int nrhs, int64_t sn, int64_t snn)
 {
-  hipLaunchKernelGGL(k_aux_nodes_single, dim3(2048), dim3(256), 0, st, a, ginv, r, t);
-  hipLaunchKernelGGL(k_aux_z_single, dim3(4096), dim3(256), 0, st, a, dinv, r, (const double *)t, z);
+  // one right-hand side: the full grids; several: fewer blocks per right-hand side
+  const unsigned gn = nrhs > 1 ? std::max(64u, 2048u / (unsigned)nrhs) : 2048u;
+  const unsigned gz = nrhs > 1 ? std::max(64u, 4096u / (unsigned)nrhs) : 4096u;
+  hipLaunchKernelGGL(k_aux_nodes_single, dim3(gn, nrhs), dim3(256), 0, st, a, ginv, r, t, sn, snn);
+  hipLaunchKernelGGL(k_aux_z_single, dim3(gz, nrhs), dim3(256), 0, st, a, dinv, r, (const double *)t, z, sn, snn);
   return hipGetLastError();
 }
 

@@ -275,6 +275,6 @@ hipError_t grad_apply(const AuxGeom & a, const double * v, double * y, hipStream
 hipError_t grad_transpose(const AuxGeom & a, const double * w, double * yv, hipStream_t st);
 // z = D^-1 r + G Dg^-1 G^T r for one vector (t: node scratch, NN entries).
 hipError_t aux_precond_apply(const AuxGeom & a, const double * dinv, const double * ginv, const double * r, double * t,
-                             double * z, hipStream_t st);
+                             double * z, hipStream_t st, int nrhs = 1, int64_t sn = 0, int64_t snn = 0);
 
 } // namespace hc

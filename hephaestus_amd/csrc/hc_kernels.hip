@@ -536,8 +536,9 @@ k_fill_zero(double * __restrict__ v, int64_t n)
 }
 
 __global__ void
-k_zero_ess(Geom g, double * __restrict__ v)
+k_zero_ess(Geom g, double * __restrict__ v, int64_t sv)
 {
+  v += blockIdx.y * sv; // right-hand side blockIdx.y
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.off[3]; i += (int64_t)gridDim.x * blockDim.x)
   {
     int c;
@@ -562,7 +563,7 @@ rhs_coil(const Geom & g, const CoilParams & cp, const double * vert, const doubl
     hipLaunchKernelGGL(k_rhs_coil, dim3(div_up(nsub, 64)), dim3(64), 0, st, g, cp, vert, h[0], h[1], h[2], x0[0],
                        x0[1], x0[2], col, b);
   }
-  hipLaunchKernelGGL(k_zero_ess, dim3(1024), dim3(256), 0, st, g, b);
+  hipLaunchKernelGGL(k_zero_ess, dim3(1024), dim3(256), 0, st, g, b, (int64_t)0);
   return hipGetLastError();
 }
 
@@ -600,9 +601,10 @@ mul(int64_t n, const double * d, const double * x, double * y, hipStream_t st)
 }
 
 hipError_t
-zero_ess(const Geom & g, double * v, hipStream_t st)
+zero_ess(const Geom & g, double * v, hipStream_t st, int nrhs)
 {
-  hipLaunchKernelGGL(k_zero_ess, dim3(1024), dim3(256), 0, st, g, v);
+  hipLaunchKernelGGL(k_zero_ess, dim3(nrhs > 1 ? std::max(64, 1024 / nrhs) : 1024, nrhs), dim3(256), 0, st, g, v,
+                     (int64_t)g.off[3]);
   return hipGetLastError();
 }
 

@@ -202,7 +202,7 @@ hipError_t axpby(int64_t n, double a, const double * x, double b, double * y, hi
 // y = d * x (entrywise)
 hipError_t mul(int64_t n, const double * d, const double * x, double * y, hipStream_t st);
 // v_i = 0 on the PEC DOFs of g
-hipError_t zero_ess(const Geom & g, double * v, hipStream_t st);
+hipError_t zero_ess(const Geom & g, double * v, hipStream_t st, int nrhs = 1); // nrhs vectors at stride N
 // b_i = int J . phi_i (zero on PEC DOFs)
 hipError_t rhs_coil(const Geom & g, const CoilParams & cp, const double * vert, const double h[3],
                     const double x0[3], double * b, hipStream_t st);

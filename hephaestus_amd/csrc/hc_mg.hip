@@ -56,7 +56,11 @@ constexpr double MG_COARSE_RTOL = 0.2;     // larger coarsest levels: inexact PC
 constexpr int MG_COARSE_MAXIT = 200;
 constexpr int MG_POWER_ITERS = 15;      // power iterations for the smoother's lambda_max
 constexpr int DOT_BLOCKS = 512;
-std::atomic<int> g_mg_graphs{1}; // replay the finest-level V-cycle from a captured graph
+// replay the finest-level V-cycle from a captured graph (off by default: a
+// capture must not overlap synchronous HIP calls of other host threads, such
+// as the hierarchy builds of the worker threads of the multigrid server and
+// batch, which therefore always launch eagerly)
+std::atomic<int> g_mg_graphs{0};
 constexpr int DOT_THREADS = 256;
 
 // ---------------------------------------------------------------------------
@@ -70,8 +74,10 @@ constexpr int DOT_THREADS = 256;
 __global__ void __launch_bounds__(256)
 k_pass1d(const double * __restrict__ in, double * __restrict__ out, int d0, int d1, int d2, int axis, int m,
          const int * __restrict__ rp, const int * __restrict__ ci, const double * __restrict__ w, double scale,
-         int accumulate)
+         int accumulate, int64_t sin, int64_t sout)
 {
+  in += blockIdx.z * sin; // right-hand side blockIdx.z (strides 0 for one)
+  out += blockIdx.z * sout;
   const unsigned o0 = axis == 0 ? m : d0, o1 = axis == 1 ? m : d1;
   const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= o0 * o1)
@@ -134,6 +140,67 @@ k_xr_update(int64_t n, double alpha, const double * __restrict__ p, const double
   }
 }
 
+// Block partials of (a, b) and (a, a) per right-hand side blockIdx.y (vectors
+// at stride n; the same blocks and order as k_dot2 for each of them).
+__global__ void __launch_bounds__(DOT_THREADS)
+k_dot2_k(const double * __restrict__ a, const double * __restrict__ b, int64_t n, double * __restrict__ part)
+{
+  __shared__ double sh[2][DOT_THREADS];
+  a += blockIdx.y * n;
+  b += blockIdx.y * n;
+  double s0 = 0.0, s1 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * DOT_THREADS + threadIdx.x; i < n; i += (int64_t)DOT_BLOCKS * DOT_THREADS)
+  {
+    const double ai = a[i];
+    s0 += ai * b[i];
+    s1 += ai * ai;
+  }
+  sh[0][threadIdx.x] = s0;
+  sh[1][threadIdx.x] = s1;
+  __syncthreads();
+  for (int s = DOT_THREADS / 2; s > 0; s >>= 1)
+  {
+    if ((int)threadIdx.x < s)
+    {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+  {
+    double * o = part + 2 * ((int64_t)blockIdx.y * DOT_BLOCKS + blockIdx.x);
+    o[0] = sh[0][0];
+    o[1] = sh[1][0];
+  }
+}
+
+// x += c p, r -= c q with c = coef[rhs] (vectors at stride n, rhs blockIdx.y)
+__global__ void
+k_xr_update_k(int64_t n, const double * __restrict__ coef, const double * __restrict__ p,
+              const double * __restrict__ q, double * __restrict__ x, double * __restrict__ r)
+{
+  const double c = coef[blockIdx.y];
+  if (c == 0.0)
+    return;
+  const int64_t o = blockIdx.y * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+  {
+    x[o + i] += c * p[o + i];
+    r[o + i] -= c * q[o + i];
+  }
+}
+
+// p = z + c p with c = coef[rhs]
+__global__ void
+k_zpbp_k(int64_t n, const double * __restrict__ coef, const double * __restrict__ z, double * __restrict__ p)
+{
+  const double c = coef[blockIdx.y];
+  const int64_t o = blockIdx.y * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[o + i] = z[o + i] + c * p[o + i];
+}
+
 // Chebyshev step: d = a z + b d (d = a z with flag 1: d not read); x = x + d
 // (x = d with flag 2: x not read)
 __global__ void
@@ -166,6 +233,8 @@ k_symmetrize_lower(double * __restrict__ A, int64_t n)
 __global__ void __launch_bounds__(256)
 k_dense_symv(const double * __restrict__ A, int64_t n, const double * __restrict__ x, double * __restrict__ y)
 {
+  x += blockIdx.y * n; // right-hand side blockIdx.y
+  y += blockIdx.y * n;
   __shared__ double sh[256];
   const int64_t i = blockIdx.x;
   const double * __restrict__ col = A + i * n;
@@ -364,11 +433,14 @@ struct MgLevel
 
 struct NodeHier;
 void node_free(NodeHier * n);
+struct MgMulti;
+void multi_free(MgMulti * m);
 
 struct MgHier
 {
   std::vector<MgLevel> lev;
   NodeHier * nodes = nullptr; // node-space hierarchy of the AV preconditioner (built on demand)
+  MgMulti * multi = nullptr;  // work vectors of multi-RHS solves (allocated on demand)
   int nodes_state = 0;        // 0 not tried, 1 built, -1 not available
   int version = -1;
   DevBlock mem;        // level vectors, temporaries, transfer matrices
@@ -395,6 +467,7 @@ struct MgHier
   ~MgHier()
   {
     node_free(nodes);
+    multi_free(multi);
     for (auto & l : lev)
       if (l.owned && l.op)
         hc_op_destroy(l.op);
@@ -449,10 +522,19 @@ dot2(MgHier & h, const double * a, const double * b, int64_t n, double & ab, dou
 // y (level l) += P x (level l+1)   (prolong = true), or
 // y (level l+1) = R x (level l)    (prolong = false)
 hipError_t
-transfer(MgHier & h, int l, bool prolong, const double * x, double * y, hipStream_t st)
+transfer(MgHier & h, int l, bool prolong, const double * x, double * y, hipStream_t st, int nrhs = 1,
+         double * tmp1 = nullptr, double * tmp2 = nullptr)
 {
   const MgLevel & f = h.lev[l];
   const MgLevel & c = h.lev[l + 1];
+  // nrhs vectors at strides N_in / N_out; the intermediates of every
+  // right-hand side at stride N of the finest level
+  if (!tmp1)
+  {
+    tmp1 = h.tmp1;
+    tmp2 = h.tmp2;
+  }
+  const int64_t sx = prolong ? c.N : f.N, sy = prolong ? f.N : c.N, sT = h.lev[0].N;
   int64_t offf = 0, offc = 0;
   for (int comp = 0; comp < 3; comp++)
   {
@@ -469,13 +551,13 @@ transfer(MgHier & h, int l, bool prolong, const double * x, double * y, hipStrea
     {
       const int kind = (a == comp) ? 0 : 1;
       const Csr1Dev & m = prolong ? f.P[a][kind] : f.R[a][kind];
-      double * out = (a == 2) ? dst : (a == 0 ? h.tmp1 : h.tmp2);
-      const double * in = (a == 0) ? src : (a == 1 ? h.tmp1 : h.tmp2);
+      double * out = (a == 2) ? dst : (a == 0 ? tmp1 : tmp2);
+      const double * in = (a == 0) ? src : (a == 1 ? tmp1 : tmp2);
       const int64_t plane = (int64_t)(a == 0 ? m.rows : d[0]) * (a == 1 ? m.rows : d[1]);
       const int o2 = a == 2 ? m.rows : d[2];
-      hipLaunchKernelGGL(k_pass1d, dim3((unsigned)((plane + 255) / 256), (unsigned)o2), dim3(256), 0, st, in, out,
-                         d[0], d[1], d[2], a, m.rows, m.rp, m.ci, m.w, a == 2 ? scale : 1.0,
-                         (a == 2 && prolong) ? 1 : 0);
+      hipLaunchKernelGGL(k_pass1d, dim3((unsigned)((plane + 255) / 256), (unsigned)o2, (unsigned)nrhs), dim3(256), 0,
+                         st, in, out, d[0], d[1], d[2], a, m.rows, m.rp, m.ci, m.w, a == 2 ? scale : 1.0,
+                         (a == 2 && prolong) ? 1 : 0, a == 0 ? sx : sT, a == 2 ? sy : sT);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess)
         return e;
@@ -485,7 +567,7 @@ transfer(MgHier & h, int l, bool prolong, const double * x, double * y, hipStrea
     offc += (int64_t)dc[0] * dc[1] * dc[2];
   }
   if (!prolong)
-    return zero_ess(c.op->g, y, st);
+    return zero_ess(c.op->g, y, st, nrhs);
   return hipSuccess;
 }
 
@@ -594,7 +676,7 @@ vcycle(MgHier & h, int l, hipStream_t st)
 // replayed on st; the replay runs the same kernels with the same arguments in
 // the same order as the eager cycle.
 int
-vcycle_top(MgHier & h, double * rin, double * zout, hipStream_t st)
+vcycle_top(MgHier & h, double * rin, double * zout, hipStream_t st, bool allow_graph)
 {
   MgLevel & L = h.lev[0];
   auto eager = [&](hipStream_t s) -> int {
@@ -607,7 +689,7 @@ vcycle_top(MgHier & h, double * rin, double * zout, hipStream_t st)
     L.x = x0;
     return rc;
   };
-  if (!g_mg_graphs.load() || h.coarse_iter)
+  if (!allow_graph || !g_mg_graphs.load() || h.coarse_iter)
     return eager(st);
   std::vector<void *> ws;
   for (auto & l : h.lev)
@@ -628,7 +710,7 @@ vcycle_top(MgHier & h, double * rin, double * zout, hipStream_t st)
     if (h.vc_seen++ == 0)
       return eager(st);
     if (!h.cap)
-      HC_TRY_HIP(hipStreamCreate(&h.cap));
+      HC_TRY_HIP(hipStreamCreateWithFlags(&h.cap, hipStreamNonBlocking));
     HC_TRY_HIP(hipStreamBeginCapture(h.cap, hipStreamCaptureModeThreadLocal));
     const int rc = eager(h.cap);
     hipGraph_t g = nullptr;
@@ -649,6 +731,187 @@ vcycle_top(MgHier & h, double * rin, double * zout, hipStream_t st)
   }
   HC_TRY_HIP(hipGraphLaunch(h.vc, st));
   return HC_OK;
+}
+
+} // namespace
+
+// Work vectors of a solve of k right-hand sides at once (every vector holds
+// the k right-hand sides one after another).
+struct MgMulti
+{
+  int k = 0;
+  DevBlock mem;
+  std::vector<double *> x, b, r, z, d; // per level, k N_l each
+  double *tmp1 = nullptr, *tmp2 = nullptr;
+  double *R = nullptr, *P = nullptr, *Q = nullptr, *Z = nullptr;
+  double *part = nullptr, *coef = nullptr;
+  double *hpart = nullptr, *hcoef = nullptr; // pinned
+  ~MgMulti()
+  {
+    if (hpart)
+      (void)hipHostFree(hpart);
+    if (hcoef)
+      (void)hipHostFree(hcoef);
+  }
+};
+
+void
+multi_free(MgMulti * m)
+{
+  delete m;
+}
+
+namespace
+{
+
+int
+mm_ensure(MgHier & h, int k)
+{
+  if (h.multi && h.multi->k >= k)
+    return HC_OK;
+  multi_free(h.multi);
+  h.multi = nullptr;
+  auto * mm = new MgMulti();
+  std::unique_ptr<MgMulti> guard(mm);
+  mm->k = k;
+  const int64_t N0 = h.lev[0].N;
+  size_t bytes = 0;
+  for (auto & L : h.lev)
+    bytes += 5 * align256(sizeof(double) * L.N * k);
+  bytes += 6 * align256(sizeof(double) * N0 * k) + align256(sizeof(double) * 2 * DOT_BLOCKS * k) +
+           align256(sizeof(double) * k);
+  if (int rc = mm->mem.alloc(bytes))
+    return rc;
+  char * q = (char *)mm->mem.base;
+  auto take = [&](int64_t n) {
+    double * v = (double *)q;
+    q += align256(sizeof(double) * n);
+    return v;
+  };
+  for (auto & L : h.lev)
+  {
+    mm->x.push_back(take(L.N * k));
+    mm->b.push_back(take(L.N * k));
+    mm->r.push_back(take(L.N * k));
+    mm->z.push_back(take(L.N * k));
+    mm->d.push_back(take(L.N * k));
+  }
+  mm->tmp1 = take(N0 * k);
+  mm->tmp2 = take(N0 * k);
+  mm->R = take(N0 * k);
+  mm->P = take(N0 * k);
+  mm->Q = take(N0 * k);
+  mm->Z = take(N0 * k);
+  mm->part = take(2 * DOT_BLOCKS * k);
+  mm->coef = take(k);
+  HC_TRY_HIP(hipHostMalloc((void **)&mm->hpart, sizeof(double) * 2 * DOT_BLOCKS * k));
+  HC_TRY_HIP(hipHostMalloc((void **)&mm->hcoef, sizeof(double) * k));
+  for (auto & L : h.lev)
+    if (int rc = L.op->ensure_ws(k, false))
+      return rc;
+  h.multi = guard.release();
+  return HC_OK;
+}
+
+// per right-hand side (a, b) and (a, a) of k vectors of n entries
+hipError_t
+dot2_k(MgMulti & mm, const double * a, const double * b, int64_t n, int k, std::vector<double> & ab,
+       std::vector<double> & aa, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_dot2_k, dim3(DOT_BLOCKS, k), dim3(DOT_THREADS), 0, st, a, b, n, mm.part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return e;
+  if ((e = hipMemcpyAsync(mm.hpart, mm.part, sizeof(double) * 2 * DOT_BLOCKS * k, hipMemcpyDeviceToHost, st)) !=
+      hipSuccess)
+    return e;
+  if ((e = hipStreamSynchronize(st)) != hipSuccess)
+    return e;
+  ab.assign(k, 0.0);
+  aa.assign(k, 0.0);
+  for (int r = 0; r < k; r++)
+    for (int i = 0; i < DOT_BLOCKS; i++)
+    {
+      ab[r] += mm.hpart[2 * ((int64_t)r * DOT_BLOCKS + i)];
+      aa[r] += mm.hpart[2 * ((int64_t)r * DOT_BLOCKS + i) + 1];
+    }
+  return hipSuccess;
+}
+
+int
+smooth_k(MgHier & h, MgMulti & mm, int l, bool zero_start, int k, hipStream_t st)
+{
+  MgLevel & L = h.lev[l];
+  const int64_t n = L.N * (int64_t)k;
+  double *x = mm.x[l], *b = mm.b[l], *r = mm.r[l], *z = mm.z[l], *d = mm.d[l];
+  double theta = 0, delta = 0, sigma1 = 0, rho0 = 0;
+  if (L.cheb > 1.0)
+  {
+    const double hi = 1.1 * L.lam, lo = hi / L.cheb;
+    theta = 0.5 * (hi + lo);
+    delta = 0.5 * (hi - lo);
+    sigma1 = theta / delta;
+    rho0 = 1.0 / sigma1;
+  }
+  for (int s = 0; s < h.nsmooth; s++)
+  {
+    const bool first = zero_start && s == 0;
+    if (first)
+      HC_TRY_HIP(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    else
+    {
+      if (int rc = hc_op_apply(L.op, x, r, k, st))
+        return rc;
+      HC_TRY_HIP(axpby(n, 1.0, b, -1.0, r, st));
+    }
+    HC_TRY_HIP(aux_precond_apply(L.op->ag, L.op->dinv64, L.op->ginv64, r, L.op->ws.t64, z, st, k, L.N,
+                                 L.op->ws.NNC));
+    if (L.cheb > 1.0)
+    {
+      double a, bb;
+      if (s == 0)
+      {
+        a = 1.0 / theta;
+        bb = 0.0;
+      }
+      else
+      {
+        const double rho1 = 1.0 / (2.0 * sigma1 - rho0);
+        a = 2.0 * rho1 / delta;
+        bb = rho1 * rho0;
+        rho0 = rho1;
+      }
+      hipLaunchKernelGGL(k_cheb_update, dim3(grid_for(n)), dim3(256), 0, st, n, a, bb, z, d, x,
+                         (s == 0 ? 1 : 0) | (first ? 2 : 0));
+      HC_TRY_HIP(hipGetLastError());
+    }
+    else
+      HC_TRY_HIP(axpby(n, L.omega, z, first ? 0.0 : 1.0, x, st));
+  }
+  return HC_OK;
+}
+
+int
+vcycle_k(MgHier & h, MgMulti & mm, int l, int k, hipStream_t st)
+{
+  MgLevel & L = h.lev[l];
+  if (l == (int)h.lev.size() - 1)
+  {
+    hipLaunchKernelGGL(k_dense_symv, dim3((unsigned)L.N, (unsigned)k), dim3(256), 0, st, (const double *)h.Ainv, L.N,
+                       (const double *)mm.b[l], mm.x[l]);
+    HC_TRY_HIP(hipGetLastError());
+    return HC_OK;
+  }
+  if (int rc = smooth_k(h, mm, l, true, k, st))
+    return rc;
+  if (int rc = hc_op_apply(L.op, mm.x[l], mm.r[l], k, st))
+    return rc;
+  HC_TRY_HIP(axpby(L.N * (int64_t)k, 1.0, mm.b[l], -1.0, mm.r[l], st));
+  HC_TRY_HIP(transfer(h, l, false, mm.r[l], mm.b[l + 1], st, k, mm.tmp1, mm.tmp2));
+  if (int rc = vcycle_k(h, mm, l + 1, k, st))
+    return rc;
+  HC_TRY_HIP(transfer(h, l, true, mm.x[l + 1], mm.x[l], st, k, mm.tmp1, mm.tmp2));
+  return smooth_k(h, mm, l, false, k, st);
 }
 
 Csr1Dev
@@ -925,7 +1188,7 @@ mg_build(hc_op_t op, hipStream_t st)
 // residual, restarted from the current x up to MAX_RESTARTS times).
 int
 mg_pcg_solve(hc_op_t op, const double * b, double * x, double rtol, int maxit, hc_solve_stats * stats,
-             hipStream_t st)
+             hipStream_t st, bool allow_graph)
 {
   constexpr int MAX_RESTARTS = 4;
   if (int rc = mg_build(op, st))
@@ -953,7 +1216,7 @@ mg_pcg_solve(hc_op_t op, const double * b, double * x, double rtol, int maxit, h
   double rr = 0.0;
   // the V-cycle reads the finest level's b and writes its x: point them at
   // the PCG's residual and preconditioned residual for the call
-  auto precond = [&](double * rin, double * zout) -> int { return vcycle_top(h, rin, zout, st); };
+  auto precond = [&](double * rin, double * zout) -> int { return vcycle_top(h, rin, zout, st, allow_graph); };
   for (outer = 1; outer <= 1 + MAX_RESTARTS && !conv && status == HC_OK; outer++)
   {
     // r = b - A x
@@ -1037,6 +1300,167 @@ mg_pcg_solve(hc_op_t op, const double * b, double * x, double rtol, int maxit, h
   if (!conv)
     return fail(HC_ENOTCONV, "hc_pcg_solve (multigrid): not converged after " + std::to_string(iters) +
                                  " iterations, true relative residual " + std::to_string(bb > 0 ? std::sqrt(rr / bb) : 0.0));
+  return HC_OK;
+}
+
+// PCG with the multigrid V-cycle on k right-hand sides at once (columns of
+// B, X at stride N): every level operation runs as one launch over the k
+// vectors (the operator applies as one multi-RHS launch), each right-hand
+// side keeps its own PCG scalars and stops on its own (its later updates are
+// zero), with the stopping rule of the one-RHS solve.  Each right-hand side's
+// arithmetic is that of the same solve alone.  Returns HC_EINVAL through
+// `handled` = false when the hierarchy has an inexact coarsest level (the
+// caller then solves the right-hand sides one at a time).
+int
+mg_pcg_solve_multi(hc_op_t op, const double * B, double * X, int k, double rtol, int maxit, hc_solve_stats * stats,
+                   hipStream_t st, bool & handled)
+{
+  constexpr int MAX_RESTARTS = 4;
+  handled = false;
+  if (int rc = mg_build(op, st))
+    return rc;
+  MgHier & h = *op->mg;
+  if (h.coarse_iter)
+    return HC_OK;
+  handled = true;
+  if (int rc = mm_ensure(h, k))
+    return rc;
+  MgMulti & mm = *h.multi;
+  const int64_t N = op->N, n = N * (int64_t)k;
+  double *R = mm.R, *P = mm.P, *Q = mm.Q, *Z = mm.Z;
+  hipEvent_t e0, e1;
+  HC_TRY_HIP(hipEventCreate(&e0));
+  HC_TRY_HIP(hipEventCreate(&e1));
+  HC_TRY_HIP(hipEventRecord(e0, st));
+  std::vector<double> bb, dummy, rr, gam, gnew, pq;
+  HC_TRY_HIP(dot2_k(mm, B, B, N, k, dummy, bb, st));
+  std::vector<double> tol2(k);
+  for (int r = 0; r < k; r++)
+    tol2[r] = rtol * rtol * bb[r];
+  std::vector<int> iters(k, 0), active(k, 0), stat(k, HC_OK);
+  std::vector<char> conv(k, 0);
+  // the V-cycle's finest level reads R and writes Z
+  auto precond = [&]() -> int {
+    double * const b0 = mm.b[0];
+    double * const x0 = mm.x[0];
+    mm.b[0] = R;
+    mm.x[0] = Z;
+    const int rc = vcycle_k(h, mm, 0, k, st);
+    mm.b[0] = b0;
+    mm.x[0] = x0;
+    return rc;
+  };
+  auto set_coef = [&](const std::vector<double> & c) -> hipError_t {
+    for (int r = 0; r < k; r++)
+      mm.hcoef[r] = c[r];
+    return hipMemcpyAsync(mm.coef, mm.hcoef, sizeof(double) * k, hipMemcpyHostToDevice, st);
+  };
+  auto resid = [&]() -> int {
+    if (int rc = hc_op_apply(op, X, R, k, st))
+      return rc;
+    HC_TRY_HIP(axpby(n, 1.0, B, -1.0, R, st));
+    HC_TRY_HIP(dot2_k(mm, R, R, N, k, dummy, rr, st));
+    return HC_OK;
+  };
+  const dim3 vg(std::max(1, std::min(grid_for(N), 2048 / std::max(1, k / 4))), k);
+  int outer = 0;
+  for (outer = 1; outer <= 1 + MAX_RESTARTS; outer++)
+  {
+    if (int rc = resid())
+      return rc;
+    bool any = false;
+    for (int r = 0; r < k; r++)
+    {
+      conv[r] = rr[r] <= tol2[r];
+      active[r] = !conv[r] && stat[r] == HC_OK && iters[r] < maxit;
+      any = any || active[r];
+    }
+    if (!any)
+      break;
+    if (int rc = precond())
+      return rc;
+    HC_TRY_HIP(dot2_k(mm, R, Z, N, k, gam, dummy, st));
+    HC_TRY_HIP(hipMemcpyAsync(P, Z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    for (;;)
+    {
+      if (int rc = hc_op_apply(op, P, Q, k, st))
+        return rc;
+      HC_TRY_HIP(dot2_k(mm, P, Q, N, k, pq, dummy, st));
+      std::vector<double> alpha(k, 0.0);
+      for (int r = 0; r < k; r++)
+        if (active[r])
+        {
+          if (!std::isfinite(pq[r]) || !std::isfinite(gam[r]))
+          {
+            stat[r] = HC_ENAN;
+            active[r] = 0;
+          }
+          else
+            alpha[r] = gam[r] / pq[r];
+        }
+      HC_TRY_HIP(set_coef(alpha));
+      hipLaunchKernelGGL(k_xr_update_k, vg, dim3(256), 0, st, N, (const double *)mm.coef, (const double *)P,
+                         (const double *)Q, X, R);
+      HC_TRY_HIP(hipGetLastError());
+      HC_TRY_HIP(dot2_k(mm, R, R, N, k, dummy, rr, st));
+      bool any_active = false;
+      for (int r = 0; r < k; r++)
+        if (active[r])
+        {
+          iters[r]++;
+          if (rr[r] <= tol2[r] || iters[r] >= maxit)
+            active[r] = 0;
+          any_active = any_active || active[r];
+        }
+      if (!any_active)
+        break;
+      if (int rc = precond())
+        return rc;
+      HC_TRY_HIP(dot2_k(mm, R, Z, N, k, gnew, dummy, st));
+      std::vector<double> beta(k, 0.0);
+      for (int r = 0; r < k; r++)
+        if (active[r])
+          beta[r] = gnew[r] / gam[r];
+      HC_TRY_HIP(set_coef(beta));
+      hipLaunchKernelGGL(k_zpbp_k, vg, dim3(256), 0, st, N, (const double *)mm.coef, (const double *)Z, P);
+      HC_TRY_HIP(hipGetLastError());
+      for (int r = 0; r < k; r++)
+        if (active[r])
+          gam[r] = gnew[r];
+    }
+  }
+  // true residuals of the results
+  if (int rc = resid())
+    return rc;
+  HC_TRY_HIP(hipEventRecord(e1, st));
+  HC_TRY_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  HC_TRY_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  hc_solve_stats all{};
+  all.converged = 1;
+  int status = HC_OK;
+  for (int r = 0; r < k; r++)
+  {
+    conv[r] = rr[r] <= tol2[r];
+    all.iterations = std::max(all.iterations, iters[r]);
+    all.rel_res = std::max(all.rel_res, bb[r] > 0 ? std::sqrt(rr[r] / bb[r]) : 0.0);
+    if (!conv[r])
+      all.converged = 0;
+    if (stat[r] && !status)
+      status = stat[r];
+  }
+  all.outer_iterations = std::min(outer, 1 + MAX_RESTARTS);
+  all.status = status ? status : (all.converged ? HC_OK : HC_ENOTCONV);
+  all.solve_ms = ms;
+  if (stats)
+    *stats = all;
+  if (status)
+    return fail(status, "hc_pcg_solve (multigrid): NaN met in a reduction");
+  if (!all.converged)
+    return fail(HC_ENOTCONV, "hc_pcg_solve (multigrid): not every right-hand side converged, true relative residual " +
+                                 std::to_string(all.rel_res));
   return HC_OK;
 }
 
@@ -1151,7 +1575,8 @@ node_transfer(MgHier & h, NodeHier & nh, int l, bool prolong, const double * x, 
     const int64_t plane = (int64_t)(a == 0 ? m.rows : d[0]) * (a == 1 ? m.rows : d[1]);
     const int o2 = a == 2 ? m.rows : d[2];
     hipLaunchKernelGGL(k_pass1d, dim3((unsigned)((plane + 255) / 256), (unsigned)o2), dim3(256), 0, st, in, out, d[0],
-                       d[1], d[2], a, m.rows, m.rp, m.ci, m.w, 1.0, (a == 2 && prolong) ? 1 : 0);
+                       d[1], d[2], a, m.rows, m.rp, m.ci, m.w, 1.0, (a == 2 && prolong) ? 1 : 0, (int64_t)0,
+                       (int64_t)0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
       return e;

@@ -364,9 +364,14 @@ void geom_for(const hc_op_s * op, const TileCandidate * c, Geom & g, PlaneSegs &
 
 // PCG with the geometric multigrid preconditioner (hc_mg.hip), FP64, one
 // right-hand side; mg_levels reports the hierarchy (built if needed).
+// allow_graph = false: never capture the V-cycle (callers on worker threads)
 int mg_pcg_solve(hc_op_t op, const double * b, double * x, double rtol, int maxit, hc_solve_stats * stats,
-                 hipStream_t st);
+                 hipStream_t st, bool allow_graph = true);
 int mg_levels(hc_op_t op, hipStream_t st, int * nlev, int64_t * ncoarse);
+// k right-hand sides at once (handled = false: hierarchy with an inexact
+// coarsest level, not solved here)
+int mg_pcg_solve_multi(hc_op_t op, const double * B, double * X, int k, double rtol, int maxit,
+                       hc_solve_stats * stats, hipStream_t st, bool & handled);
 // AV block preconditioner (hc_mg.hip): availability (builds the ND and node
 // hierarchies) and application za = V-cycle(ra), zv = node V-cycle(rv).
 bool mg_av_ready(hc_op_t op, hipStream_t st);

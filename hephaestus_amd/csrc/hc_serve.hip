@@ -427,7 +427,7 @@ hc_server_s::mg_request(Request * r, hipStream_t ws)
   int64_t nc = 0;
   const bool has_mg = hc::mg_levels(op, ws, &nlev, &nc) == HC_OK;
   hc_solve_stats s{};
-  int rc = has_mg ? hc::mg_pcg_solve(op, r->f, r->k, cfg.rtol, cfg.maxit, &s, ws)
+  int rc = has_mg ? hc::mg_pcg_solve(op, r->f, r->k, cfg.rtol, cfg.maxit, &s, ws, false)
                   : hc_pcg_solve(op, r->f, r->k, 1, cfg.rtol, cfg.maxit, HC_SOLVE_FP64 | HC_SOLVE_AUX, &s, ws);
   if (rc && rc != HC_ENOTCONV && rc != HC_ENAN)
     return rc;

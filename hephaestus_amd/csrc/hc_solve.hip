@@ -765,6 +765,14 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
     // the hybrid smoother carries the gradient-space correction)
     if (hc::prec_mode(mode & ~HC_SOLVE_MG) != HC_SOLVE_FP64 || (mode & ~(HC_SOLVE_MG | HC_SOLVE_AUX)))
       return hc::fail(HC_EINVAL, "hc_pcg_solve: HC_SOLVE_MG runs in FP64 (mode HC_SOLVE_FP64 | HC_SOLVE_MG)");
+    if (nrhs > 1)
+    {
+      // all right-hand sides through one V-cycle per iteration
+      bool handled = false;
+      const int rc = hc::mg_pcg_solve_multi(op, b, x, nrhs, rtol, maxit, stats, (hipStream_t)stream, handled);
+      if (handled || rc)
+        return rc;
+    }
     hc_solve_stats all{};
     all.converged = 1;
     int rc_all = HC_OK;
@@ -891,7 +899,7 @@ mg_batch_solve(hc_batch_t bt, hc_solve_stats * stats, hipStream_t caller)
       int64_t nc = 0;
       hc_solve_stats s{};
       int rc = hc::mg_levels(op, st, &nlev, &nc) == HC_OK
-                   ? hc::mg_pcg_solve(op, e.b + off, e.x + off, e.rtol, e.maxit, &s, st)
+                   ? hc::mg_pcg_solve(op, e.b + off, e.x + off, e.rtol, e.maxit, &s, st, false)
                    : hc_pcg_solve(op, e.b + off, e.x + off, 1, e.rtol, e.maxit, HC_SOLVE_FP64 | HC_SOLVE_AUX, &s, st);
       if (rc == HC_ENOTCONV || rc == HC_ENAN)
         rc = HC_OK; // reported in the statistics

@@ -199,7 +199,7 @@ def set_graphs(enable):
 
 
 def set_mg_graphs(enable):
-    """Process-wide: replay multigrid V-cycles from captured hipGraphs (default on)."""
+    """Process-wide: replay multigrid V-cycles from captured hipGraphs (default off)."""
     _check(load().hc_mg_set_graphs(1 if enable else 0), "hc_mg_set_graphs")
 
 

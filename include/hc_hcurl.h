@@ -296,10 +296,13 @@ int hc_pcg_solve(hc_op_t op,
  * y on level l+1 = P^T x with PEC entries zeroed, x on level l). */
 int hc_mg_levels(hc_op_t op, int * nlevels, int64_t * coarse_ndofs, void * stream);
 /* Process-wide: replay the finest-level V-cycle of HC_SOLVE_MG solves from a
- * captured hipGraph (default on; hierarchies with an exact coarsest solve).
- * The replay launches the same kernels with the same arguments in the same
- * order, so results are bitwise those of eager launches.  No reference
- * counterpart. */
+ * captured hipGraph (default off; hierarchies with an exact coarsest solve,
+ * solves called by the caller's thread -- the multigrid server and batch
+ * workers always launch eagerly).  The replay launches the same kernels with
+ * the same arguments in the same order, so results are bitwise those of eager
+ * launches.  A capture must not overlap synchronous HIP calls (allocations,
+ * blocking copies) of other host threads: enable it only where solves do not
+ * run beside such calls.  No reference counterpart. */
 int hc_mg_set_graphs(int enable);
 int hc_mg_level_op(hc_op_t op, int level, hc_op_t * level_op);
 int hc_mg_transfer(hc_op_t op, int level, int dir, const double * x, double * y, void * stream);
