@@ -823,28 +823,22 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
   return status_rc(out, "hc_pcg_solve");
 }
 
-// HC_SOLVE_MG batches: the multigrid PCG of every right-hand side (the
-// gradient-space PCG for an operator whose mesh has no hierarchy), run by
-// MG_BATCH_WORKERS host threads with their own streams; a worker takes the
-// next right-hand side whose operator no other worker is using, so solves of
-// different operators overlap on the device.  Each result is that of the same
-// solve run alone.  stats[i].solve_ms: the batch's wall time.
+// HC_SOLVE_MG batches: the multigrid PCG of every entry (its right-hand sides
+// through one V-cycle per iteration; the gradient-space PCG for an operator
+// whose mesh has no hierarchy), run by MG_BATCH_WORKERS host threads with
+// their own streams; a worker takes the next entry whose operator no other
+// worker is using, so solves of different operators overlap on the device.
+// Each result is that of the same solve run alone.  stats[i].solve_ms: the
+// batch's wall time.
 constexpr int MG_BATCH_WORKERS = 4;
 
 static int
 mg_batch_solve(hc_batch_t bt, hc_solve_stats * stats, hipStream_t caller)
 {
   HC_TRY_HIP(hipStreamSynchronize(caller)); // b and x as the caller's stream left them
-  struct Task
-  {
-    int entry, rhs;
-  };
-  std::vector<Task> tasks;
-  for (int i = 0; i < (int)bt->entries.size(); i++)
-    for (int r = 0; r < bt->entries[i].nrhs; r++)
-      tasks.push_back({i, r});
-  std::vector<hc_solve_stats> ts(tasks.size());
-  std::vector<char> done(tasks.size(), 0);
+  const size_t ntask = bt->entries.size(); // one task per entry
+  std::vector<hc_solve_stats> ts(ntask);
+  std::vector<char> done(ntask, 0);
   std::mutex mu;
   std::condition_variable cv;
   std::set<hc_op_t> busy;
@@ -873,10 +867,10 @@ mg_batch_solve(hc_batch_t bt, hc_solve_stats * stats, hipStream_t caller)
         std::unique_lock<std::mutex> lk(mu);
         for (;;)
         {
-          if (err || next_free >= tasks.size())
+          if (err || next_free >= ntask)
             break;
-          for (size_t j = next_free; j < tasks.size(); j++)
-            if (!done[j] && !busy.count(bt->entries[tasks[j].entry].op))
+          for (size_t j = next_free; j < ntask; j++)
+            if (!done[j] && !busy.count(bt->entries[j].op))
             {
               k = (int)j;
               break;
@@ -888,19 +882,45 @@ mg_batch_solve(hc_batch_t bt, hc_solve_stats * stats, hipStream_t caller)
         if (k < 0)
           break;
         done[k] = 1; // taken
-        busy.insert(bt->entries[tasks[k].entry].op);
-        while (next_free < tasks.size() && done[next_free])
+        busy.insert(bt->entries[k].op);
+        while (next_free < ntask && done[next_free])
           next_free++;
       }
-      const auto & e = bt->entries[tasks[k].entry];
+      const auto & e = bt->entries[k];
       hc_op_t op = e.op;
-      const int64_t off = (int64_t)tasks[k].rhs * op->N;
       int nlev = 0;
       int64_t nc = 0;
       hc_solve_stats s{};
-      int rc = hc::mg_levels(op, st, &nlev, &nc) == HC_OK
-                   ? hc::mg_pcg_solve(op, e.b + off, e.x + off, e.rtol, e.maxit, &s, st, false)
-                   : hc_pcg_solve(op, e.b + off, e.x + off, 1, e.rtol, e.maxit, HC_SOLVE_FP64 | HC_SOLVE_AUX, &s, st);
+      int rc = HC_OK;
+      if (hc::mg_levels(op, st, &nlev, &nc) != HC_OK)
+        rc = hc_pcg_solve(op, e.b, e.x, e.nrhs, e.rtol, e.maxit, HC_SOLVE_FP64 | HC_SOLVE_AUX, &s, st);
+      else
+      {
+        bool handled = false;
+        if (e.nrhs > 1)
+          rc = hc::mg_pcg_solve_multi(op, e.b, e.x, e.nrhs, e.rtol, e.maxit, &s, st, handled);
+        if (!handled && !rc)
+        {
+          // one right-hand side (or an inexact coarsest level): one solve each
+          s = hc_solve_stats{};
+          s.converged = 1;
+          for (int r = 0; r < e.nrhs && !rc; r++)
+          {
+            hc_solve_stats s1{};
+            const int64_t off = (int64_t)r * op->N;
+            rc = hc::mg_pcg_solve(op, e.b + off, e.x + off, e.rtol, e.maxit, &s1, st, false);
+            if (rc == HC_ENOTCONV || rc == HC_ENAN)
+              rc = HC_OK;
+            s.iterations = std::max(s.iterations, s1.iterations);
+            s.outer_iterations = std::max(s.outer_iterations, s1.outer_iterations);
+            s.rel_res = std::max(s.rel_res, s1.rel_res);
+            if (!s1.converged)
+              s.converged = 0;
+            if (s1.status && !s.status)
+              s.status = s1.status;
+          }
+        }
+      }
       if (rc == HC_ENOTCONV || rc == HC_ENAN)
         rc = HC_OK; // reported in the statistics
       if (!rc && hipStreamSynchronize(st) != hipSuccess)
@@ -933,22 +953,9 @@ mg_batch_solve(hc_batch_t bt, hc_solve_stats * stats, hipStream_t caller)
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   hc_solve_stats all{};
   all.converged = 1;
-  size_t pos = 0;
-  for (size_t i = 0; i < bt->entries.size(); i++)
+  for (size_t i = 0; i < ntask; i++)
   {
-    hc_solve_stats s{};
-    s.converged = 1;
-    for (int r = 0; r < bt->entries[i].nrhs; r++, pos++)
-    {
-      const hc_solve_stats & t = ts[pos];
-      s.iterations = std::max(s.iterations, t.iterations);
-      s.outer_iterations = std::max(s.outer_iterations, t.outer_iterations);
-      s.rel_res = std::max(s.rel_res, t.rel_res);
-      if (!t.converged)
-        s.converged = 0;
-      if (t.status && !s.status)
-        s.status = t.status;
-    }
+    hc_solve_stats s = ts[i];
     s.solve_ms = ms;
     if (stats)
       stats[i] = s;
