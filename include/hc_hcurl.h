@@ -95,12 +95,18 @@ extern "C" {
  * even and the coefficients are constant on each 2x2x2 block of children;
  * then the order drops to 1 on the same mesh; then, at order 1 and only
  * while a level exceeds 12000 DOFs, the mesh halves with the coefficients
- * averaged (same PEC faces and dt throughout) -- a mesh that stays above
- * 12000 DOFs is refused.  On every level two pre- and two post-smoothing
- * steps of Chebyshev iteration with the hybrid smoother D^-1 + G Dg^-1 G^T
- * (edge Jacobi plus node Jacobi through the discrete gradient), the
- * transfers of the nested ND spaces, and a dense Cholesky solve on the
- * coarsest level.  The geometric counterpart, on structured meshes, of the
+ * averaged (same PEC faces and dt throughout).  On every level two pre- and
+ * two post-smoothing steps of Chebyshev iteration with the hybrid smoother
+ * D^-1 + G Dg^-1 G^T (edge Jacobi plus node Jacobi through the discrete
+ * gradient), the transfers of the nested ND spaces, and a dense Cholesky
+ * solve on the coarsest level; a coarsest level still above 12000 DOFs (odd
+ * element counts at order 1) is solved by PCG with the gradient-space
+ * correction to rtol 0.2 instead, under a flexible CG outer iteration, and a
+ * mesh that has only such a level (order 1, odd counts, above 12000 DOFs) is
+ * refused (HC_EINVAL).  With nrhs > 1 (exact coarsest level) the right-hand
+ * sides go through one multi-RHS V-cycle per iteration, each with its own PCG
+ * scalars and stopping test; the results equal those of one solve each up
+ * to round-off of the multi-RHS kernels.  The geometric counterpart, on structured meshes, of the
  * multigrid auxiliary-space preconditioner HypreAMS that
  * HCurlFormulation::ConstructJacobianPreconditioner sets
  * (src/formulations/HCurl/hcurl_formulation.cpp:69-79).  Built on first use
@@ -321,12 +327,12 @@ int hc_mg_transfer(hc_op_t op, int level, int dir, const double * x, double * y,
  * hc_batch_add: b, x are device arrays of nrhs*N doubles of op (x = initial
  * guess); returns the entry index (>= 0) or a negative status.
  * hc_batch_solve: stats (optional) receives one entry per added instance.
- * Mode HC_SOLVE_FP64 | HC_SOLVE_MG: every right-hand side is solved by the
- * multigrid PCG of hc_pcg_solve (the gradient-space PCG where a mesh has no
- * hierarchy) by 4 host workers with their own streams, each taking the next
- * right-hand side whose operator no other worker is using (the caller's
- * stream is synchronised first; the call returns when all are done; solve_ms
- * is the batch's wall time). */
+ * Mode HC_SOLVE_FP64 | HC_SOLVE_MG: every entry is solved by the multigrid
+ * PCG of hc_pcg_solve (its right-hand sides through one multi-RHS V-cycle per
+ * iteration; the gradient-space PCG where a mesh has no hierarchy) by 4 host
+ * workers with their own streams, each taking the next entry whose operator
+ * no other worker is using (the caller's stream is synchronised first; the
+ * call returns when all are done; solve_ms is the batch's wall time). */
 int hc_batch_create(hc_batch_t * out);
 int hc_batch_add(hc_batch_t batch, hc_op_t op, const double * b, double * x, int nrhs, double rtol, int maxit);
 /* Right-hand sides per work item of the batched apply (0 = automatic). */
