@@ -18,6 +18,7 @@
 // neighbouring tile go to a partials buffer that k_assemble_planes sums in a
 // fixed order -- results are bitwise reproducible.  LDS strides are padded so
 // that the 16 lanes of each element hit distinct banks.
+#include "hc_basis.hpp"
 #include "hc_kernels.hpp"
 
 #include <type_traits>
@@ -67,11 +68,19 @@ wave_lds_sync()
 // instances of the device table descs[]; a work item loops over a group of rb
 // right-hand sides with the tile's quadrature data held in registers (read
 // from HBM once per group).
-template <int P, typename T, int TX, int TY, int TZ, int MODE, int SP, int BATCH>
+// EF = 1: element factors (box meshes, ApplyDesc::ef) instead of the
+// quadrature-point factors.
+template <int P, typename T, int TX, int TY, int TZ, int MODE, int SP, int BATCH, int EF>
 __global__ void __launch_bounds__((Tile<P, TX, TY, TZ, SP>::NT))
-k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables<T> tb)
+k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables<T> tb_arg)
 {
   using C = Tile<P, TX, TY, TZ, SP>;
+  // the 1D tables of order P as compile-time constants (the same values as
+  // tb_arg): every use has a static index once the loops are unrolled, so
+  // the compiler materialises each value where it is used instead of holding
+  // all of them in scalar registers through the kernel
+  (void)tb_arg;
+  constexpr basis::Tab<T> tb = basis::tab<P, T>();
   constexpr int QH = C::QH, NPP = C::NPP;
   constexpr int O = C::O, D = C::D, Q = C::Q, QQ = C::QQ;
   constexpr int SQX = C::SQX, LA = C::LA, SDZ = C::SDZ, LB = C::LB, SA = C::SA, SB = C::SB;
@@ -150,14 +159,25 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   const int ilx = tid % TX, idy = (tid / TX) % D, idz = (tid / (TX * D)) % D;
   const int ily = (tid / (TX * D * D)) % TY, ilz = tid / (TX * D * D * TY);
 
-  // ---- loads of one tile: the pencil's 12 Q quadrature factors, the item's
-  // x rows of the three components (PEC DOFs read as zero) ----
-  T qf[QH][12];
+  // ---- loads of one tile: the pencil's 12 Q quadrature factors (or its
+  // element's two factors), the item's x rows of the three components (PEC
+  // DOFs read as zero) ----
+  T qf[EF ? 1 : QH][EF ? 1 : 12];
+  T fe[2]; // element factors dt nu_e, sigma_e
   T xr0[O], xr1[D], xr2[D];
   auto load_qf = [&](const int * e0_, const int * ne_) {
     const bool pin = pact && pl0 < ne_[0] && pl1 < ne_[1] && pl2 < ne_[2];
-    // inactive lanes read element 0 (in bounds) and discard the values
+    // Inactive lanes read element 0 (in bounds, finite values) and keep them:
+    // their U, Cu are zero or never reach an output (the x^T rows sum valid
+    // elements only; quadrature points past Q meet zero table entries).
     const int64_t e = pin ? ((int64_t)(e0_[2] + pl2) * g.n[1] + (e0_[1] + pl1)) * g.n[0] + (e0_[0] + pl0) : 0;
+    if constexpr (EF)
+    {
+      const gptr<const T> ce = gmem((const T *)ad.ef.qc) + 2 * e;
+      fe[0] = ce[0];
+      fe[1] = ce[1];
+      return;
+    }
     const gptr<const T> qe = qd + e * (12 * Q * QQ) + (pin ? pqy * Q + pqx : 0);
 #pragma unroll
     for (int j = 0; j < QH; j++)
@@ -167,10 +187,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
       const gptr<const T> qq = qe + (ld ? qz : 0) * 12 * QQ;
 #pragma unroll
       for (int k = 0; k < 12; k++)
-      {
-        const T v = __builtin_nontemporal_load(qq + k * QQ);
-        qf[j][k] = ld ? v : T(0);
-      }
+        qf[j][k] = __builtin_nontemporal_load(qq + k * QQ);
     }
   };
   auto load_x = [&](const int * e0_, const int * ne_) {
@@ -235,6 +252,8 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
     }
     load_qf(e0, ne);
   }
+  else if (EF)
+    load_qf(e0, ne);
 
   for (int rhs = r_begin; rhs < r_end; rhs++)
   {
@@ -448,10 +467,44 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
     __syncthreads();
 
   // ======================= quadrature points =======================
-  if (MODE == 0)
+  if (MODE == 0 && !EF)
     load_qf(e0, ne);
   // term weights (batched kernel only; the single-instance kernel applies A)
   const T msc = TABLE ? (T)ad.mscale : T(1), csc = TABLE ? (T)ad.cscale : T(1);
+  if constexpr (EF)
+  {
+    // factor of component a at (qx, qy, qz): sigma_e w mk[a] (mass) and
+    // dt nu_e w ck[a] (curl), w = w_qx w_qy w_qz
+    auto qw_sel = [&](int q) {
+      T v = (T)tb.qw[0];
+#pragma unroll
+      for (int k = 1; k < Q; k++)
+        v = q == k ? (T)tb.qw[k] : v;
+      return v;
+    };
+    const T wxy = qw_sel(pqx) * qw_sel(pqy);
+    const T sgm = TABLE ? fe[1] * msc : fe[1], cnu = TABLE ? fe[0] * csc : fe[0];
+    const T mk0 = (T)ad.ef.mk[0], mk1 = (T)ad.ef.mk[1], mk2 = (T)ad.ef.mk[2];
+    const T ck0 = (T)ad.ef.ck[0], ck1 = (T)ad.ef.ck[1], ck2 = (T)ad.ef.ck[2];
+    with_ph([&](auto phc) {
+      constexpr int PH = decltype(phc)::value;
+#pragma unroll
+      for (int j = 0; j < QH; j++)
+      {
+        const int qz = PH * QH + j; // past Q: zero weight, and U, Cu are zero there
+        const T w = wxy * (T)tb.qw[qz];
+        const T sm = w * sgm, sc = w * cnu;
+        U[0][j] = (sm * mk0) * U[0][j];
+        U[1][j] = (sm * mk1) * U[1][j];
+        U[2][j] = (sm * mk2) * U[2][j];
+        Cu[0][j] = (sc * ck0) * Cu[0][j];
+        Cu[1][j] = (sc * ck1) * Cu[1][j];
+        Cu[2][j] = (sc * ck2) * Cu[2][j];
+      }
+    });
+  }
+  else
+  {
 #pragma unroll
   for (int j = 0; j < QH; j++)
   {
@@ -472,6 +525,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
     Cu[0][j] = qf[j][6] * c0 + qf[j][7] * c1 + qf[j][8] * c2;
     Cu[1][j] = qf[j][7] * c0 + qf[j][9] * c1 + qf[j][10] * c2;
     Cu[2][j] = qf[j][8] * c0 + qf[j][10] * c1 + qf[j][11] * c2;
+  }
   }
 
   // ======================= backward =======================
@@ -723,14 +777,25 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
       }
     }
   };
-  for (int it = tid; it < NRN; it += NT)
+  // items padded so that each component starts on a wave boundary (a wave
+  // runs one component's code)
+  constexpr int XP1 = (NR0 + 63) / 64 * 64, XP2 = XP1 + (NR1 - NR0 + 63) / 64 * 64;
+  constexpr int XPN = XP2 + (NRN - NR1 + 63) / 64 * 64;
+  for (int i = tid; i < XPN; i += NT)
   {
-    if (it < NR0)
-      xrow(std::integral_constant<int, 0>{}, it, it);
-    else if (it < NR1)
-      xrow(std::integral_constant<int, 1>{}, it, it - NR0);
-    else
-      xrow(std::integral_constant<int, 2>{}, it, it - NR1);
+    const int iw = __builtin_amdgcn_readfirstlane(i - (i & 63)); // the wave's first item
+    if (iw < XP1)
+    {
+      if (i < NR0)
+        xrow(std::integral_constant<int, 0>{}, i, i);
+    }
+    else if (iw < XP2)
+    {
+      if (i - XP1 < NR1 - NR0)
+        xrow(std::integral_constant<int, 1>{}, NR0 + i - XP1, i - XP1);
+    }
+    else if (i - XP2 < NRN - NR1)
+      xrow(std::integral_constant<int, 2>{}, NR1 + i - XP2, i - XP2);
   }
   __syncthreads();
   // second pass: x-faces between the tile's element columns (closed-x components);
@@ -750,84 +815,79 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   // tile to the partials (slot bit0 = lower-numbered closed axis, bit1 = the
   // other; set when this tile holds the face as its upper face), PEC DOFs copy
   // x (written by the tile that holds them as a lower face) ----
-  // descriptor-table instances: segment offsets held in scalar registers; the
-  // by-value instance reads them where used (fewer live registers, measured
-  // faster)
-  SegOffsets eos{};
-  if constexpr (TABLE)
-    eos = seg_offsets(ad.segs);
-  auto wout = [&](auto cc, int i, int r) {
+  // Each component's items start on a wave boundary, so a wave runs the code
+  // of one component; the tile's bases, strides and face flags are
+  // wave-uniform (scalar) and a DOF costs a few compares, two 24-bit
+  // multiply-adds and one select of its destination.
+  constexpr int NB0 = OB1, NB1 = OB2 - OB1, NB2 = OBN - OB2;
+  constexpr int WP1 = (NB0 + 63) / 64 * 64, WP2 = WP1 + (NB1 + 63) / 64 * 64, WPN = WP2 + (NB2 + 63) / 64 * 64;
+  // items [wbeg, wbeg + nb) of the padded item space hold component c; the
+  // thread takes those congruent to tid modulo NT (one loop per component, so
+  // that only one component's tile values are live at a time)
+  auto wloop = [&](auto cc, int wbeg, int nb) {
     constexpr int c = decltype(cc)::value;
-    constexpr int B0 = (c == 0) ? TX * P : TX * P + 1;
-    constexpr int B1 = (c == 1) ? TY * P : TY * P + 1;
+    constexpr int a1 = (c == 0) ? 1 : 0, a2 = (c == 2) ? 1 : 2; // closed axes, a1 < a2
+    constexpr int BB0 = (c == 0) ? TX * P : TX * P + 1;
+    constexpr int BB1 = (c == 1) ? TY * P : TY * P + 1;
     constexpr int Tl[3] = {TX * P, TY * P, TZ * P};
-    int l[3];
-    l[0] = r % B0;
-    l[1] = (r / B0) % B1;
-    l[2] = r / (B0 * B1);
-    int gi[3], dims[3];
-    bool ok = true, ess = false, owner = true;
-    int slot = 0, bit = 1, alo = 3;
+    constexpr int OBc = (c == 0) ? 0 : ((c == 1) ? OB1 : OB2);
+    // in-plane axes (first, second) of the plane normal to a1 and to a2
+    constexpr int p10 = (a1 == 0) ? 1 : 0, p11 = 2;
+    constexpr int p20 = 0, p21 = (a2 == 2) ? 1 : 2;
+    int dims[3], o[3];
 #pragma unroll
     for (int a = 0; a < 3; a++)
     {
-      gi[a] = e0[a] * P + l[a];
       dims[a] = g.n[a] * P + (a != c ? 1 : 0);
-      ok = ok && gi[a] < dims[a];
-      if (a == c)
+      o[a] = e0[a] * P;
+    }
+    const int64_t ybase = g.off[c] + ((int64_t)o[2] * dims[1] + o[1]) * dims[0] + o[0];
+    for (int i = wbeg + ((tid - wbeg) % NT + NT) % NT; i < wbeg + nb; i += NT)
+    {
+      const int j = i - wbeg;
+      const int l[3] = {j % BB0, (j / BB0) % BB1, j / (BB0 * BB1)};
+      if (l[0] >= dims[0] - o[0] || l[1] >= dims[1] - o[1] || l[2] >= dims[2] - o[2])
         continue;
-      ess = ess || (gi[a] == 0 && ((pec >> (2 * a)) & 1)) || (gi[a] == dims[a] - 1 && ((pec >> (2 * a + 1)) & 1));
-      const bool lo = (l[a] == 0 && tpos[a] > 0);
-      const bool hi = (l[a] == Tl[a] && tpos[a] < g.nt[a] - 1);
-      if ((lo || hi) && alo == 3)
-        alo = a; // lowest axis of an interior tile plane through the DOF
-      if (hi)
+      const uint32_t off = __umul24(__umul24(l[2], dims[1]) + l[1], dims[0]) + l[0];
+      const bool lo1 = l[a1] == 0 && tpos[a1] > 0, hi1 = l[a1] == Tl[a1] && tpos[a1] < g.nt[a1] - 1;
+      const bool lo2 = l[a2] == 0 && tpos[a2] > 0, hi2 = l[a2] == Tl[a2] && tpos[a2] < g.nt[a2] - 1;
+      const bool ess = (l[a1] == 0 && tpos[a1] == 0 && ((pec >> (2 * a1)) & 1)) ||
+                       (l[a1] == dims[a1] - 1 - o[a1] && ((pec >> (2 * a1 + 1)) & 1)) ||
+                       (l[a2] == 0 && tpos[a2] == 0 && ((pec >> (2 * a2)) & 1)) ||
+                       (l[a2] == dims[a2] - 1 - o[a2] && ((pec >> (2 * a2 + 1)) & 1));
+      if (ess)
       {
-        owner = false;
-        slot |= bit;
+        if (!hi1 && !hi2)
+          y[ybase + off] = x[ybase + off];
+        continue;
       }
-      bit <<= 1;
-    }
-    if (!ok)
-      return;
-    // index within the component (each component holds fewer than 2^32 DOFs)
-    const uint32_t li = ((uint32_t)gi[2] * (uint32_t)dims[1] + (uint32_t)gi[1]) * (uint32_t)dims[0] + (uint32_t)gi[0];
-    const int64_t gid = g.off[c] + (int64_t)li;
-    if (ess)
-    {
-      if (owner)
-        y[gid] = x[gid];
-    }
-    else if (alo != 3)
-    {
-      // compact entry of the segment (c, alo): row (plane k, position along
-      // the second in-plane axis), entry along the first in-plane axis; the
-      // plane through the DOF is the tile's lower (l = 0) or upper face
-      const int k = ((alo == 0) ? tpos[0] + (l[0] != 0) : ((alo == 1) ? tpos[1] + (l[1] != 0) : tpos[2] + (l[2] != 0))) - 1;
-      const int j0 = (alo == 0) ? gi[1] : gi[0];
-      const int j1 = (alo == 2) ? gi[1] : gi[2];
-      const int n0 = (alo == 0) ? dims[1] : dims[0];
-      const int n1 = (alo == 2) ? dims[1] : dims[2];
-      int64_t eo;
-      if constexpr (TABLE)
-        eo = seg_offset<c>(eos, alo);
+      const T v = sOut[OBc + j];
+      if (lo1 || hi1 || lo2 || hi2)
+      {
+        // compact entry of the segment (c, alo), alo the lowest axis with an
+        // interior tile plane through the DOF: row (plane k, position along
+        // the second in-plane axis), entry along the first; k = the tile's
+        // plane index along alo minus one, plus one for its upper face
+        const int64_t ps = part_stride(ad);
+        int64_t pid;
+        if (lo1 || hi1)
+          pid = ad.segs.eoff[c][a1] +
+                ((int64_t)(tpos[a1] - 1 + (hi1 ? 1 : 0)) * dims[p11] + o[p11] + l[p11]) * dims[p10] + o[p10] +
+                l[p10] + (hi1 ? ps : 0) + (hi2 ? 2 * ps : 0);
+        else
+          pid = ad.segs.eoff[c][a2] +
+                ((int64_t)(tpos[a2] - 1 + (hi2 ? 1 : 0)) * dims[p21] + o[p21] + l[p21]) * dims[p20] + o[p20] +
+                l[p20] + (hi2 ? 2 * ps : 0);
+        part[pid] = v;
+      }
       else
-        eo = (alo == 0) ? ad.segs.eoff[c][0] : ((alo == 1) ? ad.segs.eoff[c][1] : ad.segs.eoff[c][2]);
-      const int64_t pid = eo + ((int64_t)k * n1 + j1) * n0 + j0;
-      part[slot * part_stride(ad) + pid] = sOut[i];
+        y[ybase + off] = v;
     }
-    else
-      y[gid] = sOut[i];
   };
-  for (int i = tid; i < OBN; i += NT)
-  {
-    if (i < OB1)
-      wout(std::integral_constant<int, 0>{}, i, i);
-    else if (i < OB2)
-      wout(std::integral_constant<int, 1>{}, i, i - OB1);
-    else
-      wout(std::integral_constant<int, 2>{}, i, i - OB2);
-  }
+  wloop(std::integral_constant<int, 0>{}, 0, NB0);
+  wloop(std::integral_constant<int, 1>{}, WP1, NB1);
+  wloop(std::integral_constant<int, 2>{}, WP2, NB2);
+  (void)WPN;
   if (GROUP && rhs + 1 < r_end)
     __syncthreads(); // the next rhs overwrites the LDS stages
   }
@@ -845,32 +905,30 @@ launch_one(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, 
   if (nwork <= 0)
     return hipSuccess;
   // BATCH variant: 0 single instance, 1 table with rhs groups, 2 table with one rhs per item
-  const int bv = !descs ? 0 : (grouped ? 1 : 2);
+  const int bv = !descs ? 0 : ((grouped & 1) ? 1 : 2);
+  const bool ef = descs ? (grouped & LAUNCH_EF) != 0 : d0.ef.qc != nullptr;
+  auto go = [&](auto kern, const auto & t) {
+    hipLaunchKernelGGL(kern, dim3(nwork), dim3(C::NT), 0, st, d0, descs, ndesc, t);
+  };
   if (prec == 64)
   {
     const Tables<double> & t = *(const Tables<double> *)tb;
     if (bv == 0)
-      hipLaunchKernelGGL((k_apply_tile<P, double, TX, TY, TZ, MODE, SP, 0>), dim3(nwork), dim3(C::NT), 0, st, d0,
-                         descs, ndesc, t);
+      ef ? go(k_apply_tile<P, double, TX, TY, TZ, MODE, SP, 0, 1>, t) : go(k_apply_tile<P, double, TX, TY, TZ, MODE, SP, 0, 0>, t);
     else if (bv == 1)
-      hipLaunchKernelGGL((k_apply_tile<P, double, TX, TY, TZ, MODE, SP, 1>), dim3(nwork), dim3(C::NT), 0, st, d0,
-                         descs, ndesc, t);
+      ef ? go(k_apply_tile<P, double, TX, TY, TZ, MODE, SP, 1, 1>, t) : go(k_apply_tile<P, double, TX, TY, TZ, MODE, SP, 1, 0>, t);
     else
-      hipLaunchKernelGGL((k_apply_tile<P, double, TX, TY, TZ, MODE, SP, 2>), dim3(nwork), dim3(C::NT), 0, st, d0,
-                         descs, ndesc, t);
+      ef ? go(k_apply_tile<P, double, TX, TY, TZ, MODE, SP, 2, 1>, t) : go(k_apply_tile<P, double, TX, TY, TZ, MODE, SP, 2, 0>, t);
   }
   else
   {
     const Tables<float> & t = *(const Tables<float> *)tb;
     if (bv == 0)
-      hipLaunchKernelGGL((k_apply_tile<P, float, TX, TY, TZ, MODE, SP, 0>), dim3(nwork), dim3(C::NT), 0, st, d0,
-                         descs, ndesc, t);
+      ef ? go(k_apply_tile<P, float, TX, TY, TZ, MODE, SP, 0, 1>, t) : go(k_apply_tile<P, float, TX, TY, TZ, MODE, SP, 0, 0>, t);
     else if (bv == 1)
-      hipLaunchKernelGGL((k_apply_tile<P, float, TX, TY, TZ, MODE, SP, 1>), dim3(nwork), dim3(C::NT), 0, st, d0,
-                         descs, ndesc, t);
+      ef ? go(k_apply_tile<P, float, TX, TY, TZ, MODE, SP, 1, 1>, t) : go(k_apply_tile<P, float, TX, TY, TZ, MODE, SP, 1, 0>, t);
     else
-      hipLaunchKernelGGL((k_apply_tile<P, float, TX, TY, TZ, MODE, SP, 2>), dim3(nwork), dim3(C::NT), 0, st, d0,
-                         descs, ndesc, t);
+      ef ? go(k_apply_tile<P, float, TX, TY, TZ, MODE, SP, 2, 1>, t) : go(k_apply_tile<P, float, TX, TY, TZ, MODE, SP, 2, 0>, t);
   }
   return hipGetLastError();
 }
@@ -890,7 +948,7 @@ static const TileCandidate g_cands[] = {
     HC_QP(2, 4, 2, 2), HC_QP(2, 4, 4, 2),
     HC_CAND(3, 4, 2, 2, 1, 2), HC_CAND(3, 4, 2, 2, 1, 1), HC_CAND(3, 4, 2, 2, 0, 2), HC_CAND(3, 4, 4, 1, 1, 2),
     HC_CAND(3, 2, 2, 4, 1, 2), HC_CAND(3, 2, 2, 2, 1, 1), HC_CAND(3, 4, 2, 1, 1, 1), HC_CAND(3, 8, 2, 1, 1, 2),
-    HC_CAND(3, 8, 1, 2, 1, 2), HC_CAND(3, 8, 2, 1, 1, 1),
+    HC_CAND(3, 8, 1, 2, 1, 2), HC_CAND(3, 8, 2, 1, 1, 1), HC_CAND(3, 4, 2, 1, 1, 2), HC_CAND(3, 2, 2, 2, 1, 2),
 };
 #undef HC_CAND
 #undef HC_QP
@@ -920,9 +978,9 @@ default_tile(int p)
 
 template <typename T>
 hipError_t
-apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T * qd, const T * x, T * y,
-         T * part, const int * done, hipStream_t st, const TileCandidate * cand, int nrhs, int done_stride, int rb,
-         ApplyDesc * desc_dev)
+apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T * qd, const ElemFactors & ef,
+         const T * x, T * y, T * part, const int * done, hipStream_t st, const TileCandidate * cand, int nrhs,
+         int done_stride, int rb, ApplyDesc * desc_dev)
 {
   if (!cand)
   {
@@ -939,6 +997,7 @@ apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T *
   d.segs = segs;
   d.prows = (int)segs.start[segs.nseg];
   d.qd = qd;
+  d.ef = ef;
   d.x = x;
   d.y = y;
   d.part = part;
@@ -957,7 +1016,8 @@ apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T *
     d.rb = rb < nrhs ? rb : nrhs;
     if ((e = hipMemcpyAsync(desc_dev, &d, sizeof(d), hipMemcpyHostToDevice, st)) != hipSuccess)
       return e;
-    e = cand->launch(d, desc_dev, 1, d.ntiles * ((nrhs + d.rb - 1) / d.rb), 1, &tb, sizeof(T) == 8 ? 64 : 32, st);
+    e = cand->launch(d, desc_dev, 1, d.ntiles * ((nrhs + d.rb - 1) / d.rb), 1 | (ef.qc ? LAUNCH_EF : 0), &tb,
+                     sizeof(T) == 8 ? 64 : 32, st);
   }
   else
     e = cand->launch(d, nullptr, 1, d.ntiles * nrhs, 0, &tb, sizeof(T) == 8 ? 64 : 32, st);
@@ -968,15 +1028,16 @@ apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T *
 
 template <typename T>
 hipError_t
-apply_scaled(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T * qd, const T * x, T * y,
-             T * part, double mscale, double cscale, const TileCandidate * cand, ApplyDesc * desc_dev,
-             hipStream_t st)
+apply_scaled(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T * qd, const ElemFactors & ef,
+             const T * x, T * y, T * part, double mscale, double cscale, const TileCandidate * cand,
+             ApplyDesc * desc_dev, hipStream_t st)
 {
   ApplyDesc d{};
   d.g = g;
   d.segs = segs;
   d.prows = (int)segs.start[segs.nseg];
   d.qd = qd;
+  d.ef = ef;
   d.x = x;
   d.y = y;
   d.part = part;
@@ -989,13 +1050,13 @@ apply_scaled(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const
   hipError_t e = hipMemcpyAsync(desc_dev, &d, sizeof(d), hipMemcpyHostToDevice, st);
   if (e != hipSuccess)
     return e;
-  e = cand->launch(d, desc_dev, 1, d.ntiles, 0, &tb, sizeof(T) == 8 ? 64 : 32, st);
+  e = cand->launch(d, desc_dev, 1, d.ntiles, ef.qc ? LAUNCH_EF : 0, &tb, sizeof(T) == 8 ? 64 : 32, st);
   if (e != hipSuccess)
     return e;
   return assemble_planes<T>(d, nullptr, 1, d.prows, st);
 }
 template hipError_t apply_scaled<double>(const Geom &, const PlaneSegs &, const Tables<double> &, const double *,
-                                         const double *, double *, double *, double, double,
+                                         const ElemFactors &, const double *, double *, double *, double, double,
                                          const TileCandidate *, ApplyDesc *, hipStream_t);
 
 int
@@ -1015,11 +1076,11 @@ default_rhs_block(int ntiles, int nrhs)
 }
 
 template hipError_t apply_op<double>(const Geom &, const PlaneSegs &, const Tables<double> &, const double *,
-                                     const double *, double *, double *, const int *, hipStream_t,
-                                     const TileCandidate *, int, int, int, ApplyDesc *);
+                                     const ElemFactors &, const double *, double *, double *, const int *,
+                                     hipStream_t, const TileCandidate *, int, int, int, ApplyDesc *);
 template hipError_t apply_op<float>(const Geom &, const PlaneSegs &, const Tables<float> &, const float *,
-                                    const float *, float *, float *, const int *, hipStream_t, const TileCandidate *,
-                                    int, int, int, ApplyDesc *);
+                                    const ElemFactors &, const float *, float *, float *, const int *, hipStream_t,
+                                    const TileCandidate *, int, int, int, ApplyDesc *);
 
 template <typename T>
 hipError_t

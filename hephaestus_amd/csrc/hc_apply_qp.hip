@@ -540,6 +540,7 @@ launch_qp(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, i
   using C = QpTile<P, TX, TY, TZ>;
   if (nwork <= 0)
     return hipSuccess;
+  (void)grouped; // this kernel reads the quadrature-point factors (always present)
   if (prec == 64 && descs)
     hipLaunchKernelGGL((k_apply_qp<P, double, TX, TY, TZ, true>), dim3(nwork), dim3(C::NT), 0, st, d0, descs, ndesc,
                        *(const Tables<double> *)tb);

@@ -185,7 +185,7 @@ hc_op_div_free_source(hc_op_t op, const double * j, double * src, double * lf, d
     op->dfs_ready = true;
   }
   auto mass = [&](const double * x, double * y) -> int {
-    HC_TRY_HIP(hc::apply_scaled<double>(g0, pl.segs, op->tb64, qdu, x, y, op->ws.part64, 1.0, 0.0, pl.cand,
+    HC_TRY_HIP(hc::apply_scaled<double>(g0, pl.segs, op->tb64, qdu, hc::ElemFactors{}, x, y, op->ws.part64, 1.0, 0.0, pl.cand,
                                         op->desc_dev, st));
     return HC_OK;
   };

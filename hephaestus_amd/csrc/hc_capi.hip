@@ -2,6 +2,7 @@
 // See that header for the reference interface each entry point replaces.
 #include "hc_op.hpp"
 #include "hc_aux.hpp"
+#include "hc_basis.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -38,118 +39,34 @@ using hc::fail;
 namespace
 {
 
-// ---------------------------------------------------------------------------
-// 1D bases: Gauss-Legendre points/weights and Gauss-Lobatto points on [0,1],
-// Lagrange values/derivatives.
-// ---------------------------------------------------------------------------
-void
-legendre_pd(int n, double x, double & P, double & dP)
-{
-  double p0 = 1.0, p1 = x;
-  if (n == 0)
-  {
-    P = 1.0;
-    dP = 0.0;
-    return;
-  }
-  for (int k = 2; k <= n; k++)
-  {
-    const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
-    p0 = p1;
-    p1 = p2;
-  }
-  P = p1;
-  dP = n * (x * p1 - p0) / (x * x - 1.0);
-}
+// 1D bases (hc_basis.hpp): the same constexpr functions the tile kernels
+// fold at compile time.
+using hc::basis::gauss_legendre01;
+using hc::basis::gauss_lobatto01;
+using hc::basis::lagrange;
 
-void
-gauss_legendre01(int n, double * x, double * w)
-{
-  for (int i = 0; i < n; i++)
-  {
-    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), P, dP;
-    for (int it = 0; it < 100; it++)
-    {
-      legendre_pd(n, z, P, dP);
-      const double dz = P / dP;
-      z -= dz;
-      if (std::fabs(dz) < 1e-16)
-        break;
-    }
-    legendre_pd(n, z, P, dP);
-    x[n - 1 - i] = 0.5 * (z + 1.0);
-    w[n - 1 - i] = 1.0 / ((1.0 - z * z) * dP * dP);
-  }
-}
-
-void
-gauss_lobatto01(int n, double * x)
-{
-  const int p = n - 1;
-  x[0] = 0.0;
-  x[p] = 1.0;
-  for (int i = 1; i < p; i++)
-  {
-    double z = -std::cos(M_PI * i / p), P, dP;
-    for (int it = 0; it < 100; it++)
-    {
-      legendre_pd(p, z, P, dP);
-      const double d2P = (2.0 * z * dP - p * (p + 1.0) * P) / (1.0 - z * z);
-      const double dz = dP / d2P;
-      z -= dz;
-      if (std::fabs(dz) < 1e-16)
-        break;
-    }
-    x[i] = 0.5 * (z + 1.0);
-  }
-}
-
-void
-lagrange(int n, const double * nodes, double t, double * val, double * der)
-{
-  for (int j = 0; j < n; j++)
-  {
-    double v = 1.0, d = 0.0;
-    for (int m = 0; m < n; m++)
-      if (m != j)
-        v *= (t - nodes[m]) / (nodes[j] - nodes[m]);
-    for (int k = 0; k < n; k++)
-    {
-      if (k == j)
-        continue;
-      double prod = 1.0 / (nodes[j] - nodes[k]);
-      for (int m = 0; m < n; m++)
-        if (m != j && m != k)
-          prod *= (t - nodes[m]) / (nodes[j] - nodes[m]);
-      d += prod;
-    }
-    val[j] = v;
-    if (der)
-      der[j] = d;
-  }
-}
+// the tables of orders 1..PMAX, evaluated at compile time (the values the
+// tile kernels use)
+constexpr hc::basis::Tab<double> kTabs[hc::PMAX + 1] = {hc::basis::Tab<double>{}, hc::basis::make_tab(1),
+                                                         hc::basis::make_tab(2), hc::basis::make_tab(3)};
+static_assert(hc::PMAX == 3, "kTabs lists orders 1..3");
 
 void
 make_tables(int p, hc::Tables<double> & t)
 {
   std::memset(&t, 0, sizeof(t));
-  const int Q = p + 1, D = p + 1;
-  double on[hc::PMAX], ow[hc::PMAX], cn[hc::PMAX + 1];
-  gauss_legendre01(p, on, ow);
-  gauss_lobatto01(p + 1, cn);
-  gauss_legendre01(Q, t.qp, t.qw);
-  for (int q = 0; q < Q; q++)
+  const hc::basis::Tab<double> & k = kTabs[p];
+  for (int i = 0; i < hc::QMAXC * hc::PMAX; i++)
+    t.Bo[i] = k.Bo[i];
+  for (int i = 0; i < hc::QMAXC * hc::QMAXC; i++)
   {
-    double v[hc::PMAX + 1], d[hc::PMAX + 1];
-    lagrange(p, on, t.qp[q], v, nullptr);
-    for (int j = 0; j < p; j++)
-      t.Bo[q * p + j] = v[j];
-    lagrange(D, cn, t.qp[q], v, d);
-    for (int j = 0; j < D; j++)
-    {
-      t.Bc[q * D + j] = v[j];
-      t.Gc[q * D + j] = d[j];
-    }
+    t.Bc[i] = k.Bc[i];
+    t.Gc[i] = k.Gc[i];
+  }
+  for (int i = 0; i < hc::QMAXC; i++)
+  {
+    t.qw[i] = k.qw[i];
+    t.qp[i] = k.qp[i];
   }
 }
 
@@ -228,7 +145,7 @@ apply_t(hc_op_t op, const T * x, T * y, int nrhs, const int * done, hipStream_t 
   T * part = (T *)(f32 ? (void *)op->ws.part32 : (void *)op->ws.part64);
   const int ntiles = pl.g.nt[0] * pl.g.nt[1] * pl.g.nt[2];
   const int rb = nrhs == 1 ? 1 : (op->rhs_block > 0 ? std::min(op->rhs_block, nrhs) : hc::default_rhs_block(ntiles, nrhs));
-  HC_TRY_HIP(hc::apply_op<T>(pl.g, pl.segs, tb, qd, x, y, part, done, st, pl.cand, nrhs, hc::STATE_STRIDE_INT, rb,
+  HC_TRY_HIP(hc::apply_op<T>(pl.g, pl.segs, tb, qd, op->ef(f32), x, y, part, done, st, pl.cand, nrhs, hc::STATE_STRIDE_INT, rb,
                              f32 ? op->desc_dev + 1 : op->desc_dev));
   return HC_OK;
 }
@@ -373,7 +290,22 @@ hc_op_create(const hc_mesh_desc * mesh, int order, const double * nu_e, const do
       return rc;
     }
   }
-  if (!op->box)
+  if (op->box)
+  {
+    // element factors: J = diag(h), |det J| = h0 h1 h2
+    if ((rc = op->alloc(op->qc64, 2 * op->E)) || (op->qd32 && (rc = op->alloc(op->qc32, 2 * op->E))))
+    {
+      delete op;
+      return rc;
+    }
+    const double det = op->h[0] * (op->h[1] * op->h[2]);
+    for (int a = 0; a < 3; a++)
+    {
+      op->ck[a] = op->h[a] * op->h[a] / det;
+      op->mk[a] = det / (op->h[a] * op->h[a]);
+    }
+  }
+  else
   {
     const int64_t nv = (int64_t)(g.n[0] + 1) * (g.n[1] + 1) * (g.n[2] + 1) * 3;
     if ((rc = op->alloc(op->d_vert, nv)))
@@ -431,6 +363,8 @@ hc_op_setup(hc_op_t op, double dt, void * stream)
   const int Q = op->p + 1;
   if (op->qd32)
     HC_TRY_HIP(hc::cast_f32(op->qd64, op->qd32, op->E * 12 * Q * Q * Q, st));
+  if (op->qc64)
+    HC_TRY_HIP(hc::elem_factors(op->E, op->d_nu, op->d_sig, dt, op->qc64, op->qc32, st));
   HC_TRY_HIP(hc::diagonal(op->g, op->tb64, op->qd64, op->diag, st));
   HC_TRY_HIP(hc::invert_diag(op->diag, op->dinv64, op->dinv32, op->N, st));
   // node diagonal of the gradient-space correction (computed in place)
@@ -511,7 +445,7 @@ hc_op_apply_curl(hc_op_t op, const double * x, double * y, void * stream)
   if (rc)
     return rc;
   // the quadrature data hold dt * nu: scale the curl term back by 1/dt
-  HC_TRY_HIP(hc::apply_scaled<double>(op->plan64.g, op->plan64.segs, op->tb64, op->qd64, x, y, op->ws.part64, 0.0,
+  HC_TRY_HIP(hc::apply_scaled<double>(op->plan64.g, op->plan64.segs, op->tb64, op->qd64, op->ef(false), x, y, op->ws.part64, 0.0,
                                       1.0 / op->last_dt, op->plan64.cand, op->desc_dev, (hipStream_t)stream));
   return HC_OK;
 }
@@ -549,11 +483,11 @@ hc_op_av_apply(hc_op_t op, const double * a, const double * v, double * ya, doub
   // s = a0 + G v
   HC_TRY_HIP(hc::av_gather(pl.g, op->ag, a, v, s, st));
   // ya = M s (mass term alone, no elimination), yv = G^T M s
-  HC_TRY_HIP(hc::apply_scaled<double>(g0, pl.segs, op->tb64, op->qd64, s, ya, op->ws.part64, 1.0, 0.0, pl.cand,
+  HC_TRY_HIP(hc::apply_scaled<double>(g0, pl.segs, op->tb64, op->qd64, op->ef(false), s, ya, op->ws.part64, 1.0, 0.0, pl.cand,
                                       op->desc_dev, st));
   HC_TRY_HIP(hc::grad_transpose(op->ag, ya, yv, st));
   // k = dt K a0 on the free DOFs (the quadrature data hold dt nu); ya += k there, ya = a on PEC DOFs
-  HC_TRY_HIP(hc::apply_scaled<double>(pl.g, pl.segs, op->tb64, op->qd64, a, k, op->ws.part64, 0.0, 1.0, pl.cand,
+  HC_TRY_HIP(hc::apply_scaled<double>(pl.g, pl.segs, op->tb64, op->qd64, op->ef(false), a, k, op->ws.part64, 0.0, 1.0, pl.cand,
                                       op->desc_dev, st));
   HC_TRY_HIP(hc::av_finish(pl.g, op->ag, a, k, ya, st));
   return HC_OK;
@@ -566,6 +500,23 @@ hc_op_set_rhs_block(hc_op_t op, int rhs_block)
     return fail(HC_EINVAL, "hc_op_set_rhs_block: bad argument");
   op->rhs_block = rhs_block;
   return HC_OK;
+}
+
+int
+hc_op_set_factor_mode(hc_op_t op, int mode)
+{
+  if (!op || (mode != 0 && mode != 1))
+    return fail(HC_EINVAL, "hc_op_set_factor_mode: null handle or mode other than 0 and 1");
+  op->ef_mode = mode;
+  return HC_OK;
+}
+
+int
+hc_op_factor_mode(hc_op_t op)
+{
+  if (!op)
+    return fail(HC_EINVAL, "hc_op_factor_mode: null handle");
+  return op->ef(false).qc ? 1 : 0;
 }
 
 int
@@ -764,6 +715,7 @@ hc_op_profile_apply(hc_op_t op, int prec_bits, const void * x, void * y, int rep
   d.segs = pl.segs;
   d.prows = (int)pl.segs.start[pl.segs.nseg];
   d.qd = f32 ? (const void *)op->qd32 : (const void *)op->qd64;
+  d.ef = op->ef(f32);
   d.x = x;
   d.y = y;
   d.part = f32 ? (void *)op->ws.part32 : (void *)op->ws.part64;

@@ -363,6 +363,32 @@ pa_setup(const Geom & g, const Tables<double> & tb, const double * vert, const d
   return hipGetLastError();
 }
 
+// Element factors of a box mesh: qc[2e] = dt nu_e, qc[2e+1] = sigma_e (and
+// their FP32 copies when qc32 is set).
+__global__ void
+k_elem_factors(int64_t E, const double * __restrict__ nu, const double * __restrict__ sig, double dt,
+               double * __restrict__ qc, float * __restrict__ qc32)
+{
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x)
+  {
+    const double a = dt * nu[e], b = sig[e];
+    qc[2 * e] = a;
+    qc[2 * e + 1] = b;
+    if (qc32)
+    {
+      qc32[2 * e] = (float)a;
+      qc32[2 * e + 1] = (float)b;
+    }
+  }
+}
+
+hipError_t
+elem_factors(int64_t E, const double * nu, const double * sig, double dt, double * qc, float * qc32, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_elem_factors, dim3(1024), dim3(256), 0, st, E, nu, sig, dt, qc, qc32);
+  return hipGetLastError();
+}
+
 hipError_t
 cast_f32(const double * a, float * b, int64_t n, hipStream_t st)
 {

@@ -33,11 +33,26 @@ struct PlaneSegs
 // single-instance kernel uses rb = 1).  The items of instance i occupy
 // [w0, w0 + ntiles*ceil(nrhs/rb)) of the launch's flattened work range.
 // Vectors of rhs r start at r*N.
+// Element factors of an axis-aligned box mesh.  With J = diag(h) the
+// quadrature-point factors reduce to mass_a = sigma_e w_q |det J| / h_a^2 and
+// curl_a = dt nu_e w_q h_a^2 / |det J| (no off-diagonal terms), so a box
+// operator needs two numbers per element, qc[2e] = dt nu_e, qc[2e+1] =
+// sigma_e (element type of the launch), plus the axis constants
+// ck[a] = h_a^2 / |det J| and mk[a] = |det J| / h_a^2 and the 1D weights.
+// qc == nullptr: the quadrature-point factors qd are used.
+struct ElemFactors
+{
+  const void * qc = nullptr;
+  double ck[3] = {0, 0, 0};
+  double mk[3] = {0, 0, 0};
+};
+
 struct ApplyDesc
 {
   Geom g;
   PlaneSegs segs;
   const void * qd; // quadrature data (element type of the launch)
+  ElemFactors ef;  // element factors (box meshes), used instead of qd when set
   const void * x;
   void * y;
   void * part;     // 4 slots per rhs: slot s of compact plane entry j at s*pstride + j
@@ -158,6 +173,9 @@ xcd_remap(int b, int nb)
 // A compiled tile variant of the apply kernel (shape + load mode).
 // descs == nullptr: the single instance d0; otherwise ndesc instances.
 // grouped != 0 (with descs): work items are groups of rb right-hand sides.
+// grouped bit 0: rhs groups; bit 1 (LAUNCH_EF): every instance of descs carries
+// element factors (the single instance d0 is read from d0.ef.qc).
+constexpr int LAUNCH_EF = 2;
 using TileLaunch = hipError_t (*)(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, int grouped,
                                   const void * tables, int prec_bits, hipStream_t st);
 struct TileCandidate
@@ -188,6 +206,10 @@ hipError_t pa_setup(const Geom & g,
                     double * qd,
                     hipStream_t st);
 hipError_t cast_f32(const double * a, float * b, int64_t n, hipStream_t st);
+// Element factors of a box mesh (ElemFactors::qc): dt nu_e, sigma_e per
+// element, FP64 and (qc32 != nullptr) FP32.
+hipError_t elem_factors(int64_t E, const double * nu, const double * sig, double dt, double * qc, float * qc32,
+                        hipStream_t st);
 
 // Coil-like current J = curl psi, psi = sum_k a_k exp(-|x - c_k|^2 / (2 s_k^2)),
 // with its Gauss rule (p + 2 points) and the ND basis at those points.
@@ -218,6 +240,7 @@ hipError_t apply_op(const Geom & g,
                     const PlaneSegs & segs,
                     const Tables<T> & tb,
                     const T * qd,
+                    const ElemFactors & ef,
                     const T * x,
                     T * y,
                     T * part,
@@ -234,9 +257,9 @@ int default_rhs_block(int ntiles, int nrhs);
 // i.e. dt K) with the batched kernel; its descriptor is staged in desc_dev.
 // part: 4 N scratch.
 template <typename T>
-hipError_t apply_scaled(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T * qd, const T * x,
-                        T * y, T * part, double mscale, double cscale, const TileCandidate * cand,
-                        ApplyDesc * desc_dev, hipStream_t st);
+hipError_t apply_scaled(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T * qd,
+                        const ElemFactors & ef, const T * x, T * y, T * part, double mscale, double cscale,
+                        const TileCandidate * cand, ApplyDesc * desc_dev, hipStream_t st);
 
 // Fill the plane-assembly fields of a descriptor (segments, rows per rhs).
 void plane_rows(ApplyDesc & d);

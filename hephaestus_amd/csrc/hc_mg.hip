@@ -26,6 +26,7 @@
 // solve) is a fixed symmetric positive definite operator, so plain PCG
 // applies.
 #include "hc_aux.hpp"
+#include "hc_basis.hpp"
 #include "hc_op.hpp"
 
 #include <rocblas/rocblas.h>
@@ -286,61 +287,18 @@ lagrange1(int n, const double * nodes, int j, double t)
   return v;
 }
 
-// Gauss-Legendre / Gauss-Lobatto points on [0,1] (Newton on P_n / P'_{n-1})
+// Gauss-Legendre / Gauss-Lobatto points on [0,1] (hc_basis.hpp, the points
+// of the operator's tables)
 void
 points01(int n, bool lobatto, double * x)
 {
-  auto leg = [](int k, double z, double & P, double & dP) {
-    double p0 = 1.0, p1 = z;
-    if (k == 0)
-    {
-      P = 1.0;
-      dP = 0.0;
-      return;
-    }
-    for (int j = 2; j <= k; j++)
-    {
-      const double p2 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p0) / j;
-      p0 = p1;
-      p1 = p2;
-    }
-    P = p1;
-    dP = k * (z * p1 - p0) / (z * z - 1.0);
-  };
-  if (!lobatto)
+  if (lobatto)
   {
-    for (int i = 0; i < n; i++)
-    {
-      double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), P, dP;
-      for (int it = 0; it < 100; it++)
-      {
-        leg(n, z, P, dP);
-        const double dz = P / dP;
-        z -= dz;
-        if (std::fabs(dz) < 1e-16)
-          break;
-      }
-      x[n - 1 - i] = 0.5 * (z + 1.0);
-    }
+    basis::gauss_lobatto01(n, x);
     return;
   }
-  const int p = n - 1;
-  x[0] = 0.0;
-  x[p] = 1.0;
-  for (int i = 1; i < p; i++)
-  {
-    double z = -std::cos(M_PI * i / p), P, dP;
-    for (int it = 0; it < 100; it++)
-    {
-      leg(p, z, P, dP);
-      const double d2P = (2.0 * z * dP - p * (p + 1.0) * P) / (1.0 - z * z);
-      const double dz = dP / d2P;
-      z -= dz;
-      if (std::fabs(dz) < 1e-16)
-        break;
-    }
-    x[i] = 0.5 * (z + 1.0);
-  }
+  double w[QMAXC + 1] = {};
+  basis::gauss_legendre01(n, x, w);
 }
 
 struct Csr1
@@ -1552,7 +1510,7 @@ node_apply(NodeLevel & L, const double * x, double * y, hipStream_t st)
 {
   hc_op_t op = L.op;
   HC_TRY_HIP(grad_apply(op->ag, x, L.t1, st));
-  HC_TRY_HIP(apply_scaled<double>(L.g0, op->plan64.segs, op->tb64, op->qd64, L.t1, L.t2, op->ws.part64, 1.0, 0.0,
+  HC_TRY_HIP(apply_scaled<double>(L.g0, op->plan64.segs, op->tb64, op->qd64, op->ef(false), L.t1, L.t2, op->ws.part64, 1.0, 0.0,
                                   op->plan64.cand, op->desc_dev, st));
   HC_TRY_HIP(grad_transpose(op->ag, L.t2, y, st));
   return HC_OK;

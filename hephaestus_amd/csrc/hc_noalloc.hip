@@ -111,6 +111,7 @@ hc_op_apply_noalloc(hc_op_t op, const double * x, double * y, void * stream)
   d.segs = pl.segs;
   d.prows = (int)pl.segs.start[pl.segs.nseg];
   d.qd = op->qd64;
+  d.ef = op->ef(false);
   d.x = x;
   d.y = y;
   d.part = part;

@@ -174,6 +174,26 @@ struct hc_op_s
   double * d_sig = nullptr;
   double * qd64 = nullptr;
   float * qd32 = nullptr;
+  // element factors of a box mesh (dt nu_e, sigma_e per element) and the axis
+  // constants: the apply reads them instead of qd64/qd32 while ef_mode is 1
+  double * qc64 = nullptr;
+  float * qc32 = nullptr;
+  double ck[3] = {0, 0, 0}, mk[3] = {0, 0, 0};
+  int ef_mode = 1;
+  hc::ElemFactors ef(bool f32) const
+  {
+    hc::ElemFactors f;
+    if (ef_mode && qc64)
+    {
+      f.qc = f32 ? (const void *)qc32 : (const void *)qc64;
+      for (int a = 0; a < 3; a++)
+      {
+        f.ck[a] = ck[a];
+        f.mk[a] = mk[a];
+      }
+    }
+    return f;
+  }
   double * diag = nullptr;
   double * dinv64 = nullptr;
   float * dinv32 = nullptr;
@@ -318,7 +338,8 @@ private:
     ApplyDesc single;  // by-value descriptor (batched == false)
     const ApplyDesc * descs; // device
     int ndesc, nwork, npwork;
-    int grouped; // some instance of the table groups rb > 1 right-hand sides per work item
+    int grouped; // bit 0: some instance groups rb > 1 right-hand sides per work item;
+                 // LAUNCH_EF: every instance reads element factors
     const TileCandidate * cand;
     const void * tables; // Tables<double> or Tables<float>
     int prec;

@@ -224,7 +224,7 @@ hc_server_s::start_request(Request * r)
   hc::Geom g;
   hc::PlaneSegs segs;
   hc::geom_for(op, cand, g, segs);
-  HC_TRY_HIP(hc::apply_scaled<double>(g, segs, op->tb64, op->qd64, r->u, r->f, r->ws.part64, 0.0,
+  HC_TRY_HIP(hc::apply_scaled<double>(g, segs, op->tb64, op->qd64, op->ef(false), r->u, r->f, r->ws.part64, 0.0,
                                       1.0 / op->last_dt, cand, op->desc_dev, st));
   HC_TRY_HIP(hc::axpby(N, 1.0, r->bsrc, -1.0, r->f, st));
   HC_TRY_HIP(hc::zero_ess(op->g, r->f, st));

@@ -269,6 +269,7 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
       G.ndesc = (int)ids.size();
       G.grouped = 0;
       int w0 = 0, pw0 = 0;
+      bool all_ef = true; // every instance has element factors: one launch reads them all
       for (int i : ids)
       {
         const EngineInst & e = inst_[i];
@@ -281,6 +282,9 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
         d.nrhs = e.nrhs;
         d.rb = G.batched ? (rhs_block > 0 ? std::min(rhs_block, e.nrhs) : default_rhs_block(d.ntiles, e.nrhs)) : 1;
         d.qd = prec32 ? (const void *)op->qd32 : (const void *)op->qd64;
+        d.ef = op->ef(prec32);
+        if (!d.ef.qc)
+          all_ef = false;
         if (k == K_PQ32)
         {
           d.x = w.p32;
@@ -316,6 +320,13 @@ Engine::set(const std::vector<EngineInst> & inst, int mode, int rhs_block, bool 
           G.single = d;
         hd[dpos++] = d;
       }
+      if (all_ef)
+        G.grouped |= LAUNCH_EF;
+      else // mixed: every instance reads its quadrature-point factors
+        for (size_t q = dpos - ids.size(); q < dpos; q++)
+          hd[q].ef = ElemFactors{};
+      if (!G.batched)
+        G.single = hd[dpos - 1];
       G.nwork = w0;
       G.npwork = pw0;
       groups_[k].push_back(G);

@@ -44,6 +44,7 @@ EXPORTS = [
     "hc_server_request_bytes", "hc_server_create", "hc_server_set_tile", "hc_server_submit", "hc_server_poll",
     "hc_server_info", "hc_server_destroy", "hc_op_apply_noalloc", "hc_noalloc_scratch_doubles",
     "hc_mg_levels", "hc_mg_level_op", "hc_mg_transfer", "hc_mg_set_graphs",
+    "hc_op_set_factor_mode", "hc_op_factor_mode",
 ]
 
 
@@ -122,6 +123,8 @@ def load():
     lib.hc_op_diagonal.argtypes = [vp, vp, vp]
     lib.hc_op_precond_apply.argtypes = [vp, C.c_int, vp, vp, vp]
     lib.hc_op_term_ratio.argtypes = [vp, C.POINTER(C.c_double)]
+    lib.hc_op_set_factor_mode.argtypes = [vp, C.c_int]
+    lib.hc_op_factor_mode.argtypes = [vp]
     lib.hc_engine_set_graphs.argtypes = [C.c_int]
     lib.hc_pcg_solve.argtypes = [vp, vp, vp, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(SolveStats), vp]
     lib.hc_mg_levels.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int64), vp]
@@ -354,6 +357,19 @@ class Operator:
         d = torch.empty(self.ndofs, dtype=torch.float64, device="cuda")
         _check(_lib.hc_op_diagonal(self._h, _ptr(d), _stream(stream)), "hc_op_diagonal")
         return d
+
+    def set_factor_mode(self, mode):
+        """1 (default): element factors on box meshes; 0: quadrature-point factors."""
+        _check(_lib.hc_op_set_factor_mode(self._h, int(mode)), "hc_op_set_factor_mode")
+        return self
+
+    @property
+    def factor_mode(self):
+        """The factor storage the apply kernels read (1: element factors, 0: quadrature points)."""
+        r = _lib.hc_op_factor_mode(self._h)
+        if r < 0:
+            _check(r, "hc_op_factor_mode")
+        return int(r)
 
     @property
     def term_ratio(self):

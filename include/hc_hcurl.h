@@ -176,6 +176,20 @@ int hc_op_precond_apply(hc_op_t op, int mode, const double * r, double * z, void
  * refinement).  Needs hc_op_setup. */
 int hc_op_term_ratio(hc_op_t op, double * ratio);
 
+/* Factor storage read by the apply kernels.  mode 1 (default): on an
+ * axis-aligned box mesh (hc_mesh_desc.vert == NULL) the kernels read two
+ * factors per element, dt nu_e and sigma_e, and form the quadrature-point
+ * factors from them (J = diag(h): mass sigma_e w |det J| / h_a^2, curl
+ * dt nu_e w h_a^2 / |det J|, no off-diagonal terms); other meshes read the
+ * 12 (p+1)^3 quadrature-point factors per element of the partial assembly.
+ * mode 0: the quadrature-point factors on every mesh.  The two agree to
+ * round-off (a few units in the last place of each factor).  No counterpart
+ * in the reference (MFEM's partial assembly stores the quadrature-point
+ * factors).  hc_op_factor_mode: the storage the kernels read (0 or 1) or a
+ * negative status. */
+int hc_op_set_factor_mode(hc_op_t op, int mode);
+int hc_op_factor_mode(hc_op_t op);
+
 /* Process-wide switch (default off): batch and single solves replay their
  * steady-state rounds (64 iterations) from a captured hipGraph, one per table
  * set, instead of launching every kernel from the host.  No counterpart in the

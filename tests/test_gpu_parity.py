@@ -281,3 +281,88 @@ def test_errors_are_reported(torch_cuda, hc):
     b[3] = float("nan")
     st = op.solve(b, torch.zeros_like(x), rtol=1e-8, maxit=100, raise_on_fail=False)
     assert st["rc"] == -5
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_element_factors(torch_cuda, hc, oracle, p):
+    """Box meshes apply from two factors per element (dt nu_e, sigma_e) and the
+    axis constants: every tile variant against the oracle (FP64 1e-13, FP32
+    2e-6) and against the quadrature-point factors of the same operator
+    (1e-14), partial PEC, several right-hand sides in one launch bitwise equal
+    to single applies, the curl-only term weights; trilinear meshes keep the
+    quadrature-point factors."""
+    torch = torch_cuda
+    from hephaestus_amd import hcurl
+    n = {1: (11, 9, 6), 2: (9, 7, 5), 3: (9, 5, 7)}[p]
+    print(f"case element factors p={p} n={n}", flush=True)
+    rng = np.random.default_rng(700 + p)
+    pec = 0x19
+    m = oracle.Mesh(*n, pec_faces=pec)
+    E = m.nelem
+    nu, sig = rng.uniform(0.5, 2.0, E), 10.0 ** rng.uniform(-4, 4, E)
+    dt = 0.37
+    qd = oracle.pa_setup(m, p, dt * nu, sig)
+    ess = oracle.ess_mask(m, p)
+    N = oracle.ndofs(m, p)
+    X = np.stack([oracle.rand_uniform(51 + r, N) for r in range(3)])
+    Y = np.stack([oracle.pa_apply(m, p, qd, ess, X[r]) for r in range(3)])
+    op = hc.Operator(n, p, nu, sig, pec_faces=pec).setup(dt)
+    assert op.factor_mode == 1
+    xd = _dev(torch, X)
+    for t in hcurl.tile_list(p):
+        op.set_tile(64, *t)
+        op.set_tile(32, *t)
+        op.set_factor_mode(1)
+        y = op.apply(xd)
+        y32 = op.apply_f32(xd.float())
+        k1 = op.apply_curl(xd[0])
+        for r in range(3):
+            assert _rel(y[r].cpu().numpy(), Y[r]) <= 1e-13, (t, r)
+            assert _rel(y32[r].double().cpu().numpy(), Y[r]) <= 2e-6, (t, r)
+            assert torch.equal(op.apply(xd[r]), y[r]), (t, r)
+        op.set_factor_mode(0)
+        assert op.factor_mode == 0
+        yq = op.apply(xd)
+        kq = op.apply_curl(xd[0])
+        assert _rel(y.cpu().numpy(), yq.cpu().numpy()) <= 1e-14, t
+        assert _rel(k1.cpu().numpy(), kq.cpu().numpy()) <= 1e-14, t
+    op.set_factor_mode(1)
+    op.close()
+    vert = oracle.Mesh(*n).box_vertices() + 0.04 * rng.standard_normal(((n[0] + 1) * (n[1] + 1) * (n[2] + 1), 3)) / max(n)
+    opt = hc.Operator(n, p, nu, sig, vert=vert).setup(dt)
+    assert opt.factor_mode == 0
+    opt.close()
+
+
+def test_element_factors_mixed_batch(torch_cuda, hc, oracle):
+    """A batch holding a box and a trilinear instance of one order runs both
+    from their quadrature-point factors; each solution meets rtol on the
+    oracle's operator and agrees with the same instance solved alone."""
+    torch = torch_cuda
+    p, n = 2, (7, 6, 5)
+    print("case element factors mixed batch", flush=True)
+    rng = np.random.default_rng(77)
+    E = n[0] * n[1] * n[2]
+    nu, sig = rng.uniform(0.5, 2.0, E), rng.uniform(0.5, 2.0, E)
+    vert = oracle.Mesh(*n).box_vertices() + 0.04 * rng.standard_normal(((n[0] + 1) * (n[1] + 1) * (n[2] + 1), 3)) / max(n)
+    ops = [hc.Operator(n, p, nu, sig).setup(1.0), hc.Operator(n, p, nu, sig, vert=vert).setup(1.0)]
+    bs = [op.rhs_coil(90 + i) for i, op in enumerate(ops)]
+    xs = [torch.zeros_like(b) for b in bs]
+    bt = hc.Batch()
+    for op, b, x in zip(ops, bs, xs):
+        bt.add(op, b, x, rtol=1e-10, maxit=5000)
+    _, st = bt.solve(mode=hc.SOLVE_FP64)
+    assert all(s["converged"] for s in st), st
+    for i, (op, b, x) in enumerate(zip(ops, bs, xs)):
+        m = oracle.Mesh(*n, vert=vert if i else None)
+        qd = oracle.pa_setup(m, p, nu, sig)
+        ess = oracle.ess_mask(m, p)
+        bh, xh = b.cpu().numpy(), x.cpu().numpy()
+        r = bh - oracle.pa_apply(m, p, qd, ess, xh)
+        assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(bh), i
+        x1 = torch.zeros_like(b)
+        op.solve(b, x1, rtol=1e-10, maxit=5000, mode=hc.SOLVE_FP64)
+        assert _rel(x1.cpu().numpy(), xh) <= 1e-8, i
+    bt.close()
+    for op in ops:
+        op.close()
