@@ -571,7 +571,7 @@ def main():
     # element factors (the default on box meshes, used by every solve below):
     # two factors per element instead of 12 (p+1)^3, same operator to round-off
     op.set_factor_mode(1)
-    tune_ef = op.autotune(reps=5)
+    tune_ef = op.autotune(reps=10)
     y_q = y.clone()
     op.set_factor_mode(0)
     op.apply(x, y_q)

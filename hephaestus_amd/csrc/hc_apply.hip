@@ -22,6 +22,7 @@
 #include "hc_kernels.hpp"
 
 #include <type_traits>
+#include <utility>
 
 namespace hc
 {
@@ -54,6 +55,23 @@ struct Tile
   static constexpr bool WS = (SP == 1) && (64 % QQ == 0) && (NP % 64 == 0) && (64 % (D * D * TX) == 0);
 };
 
+// Move the listed entries of a (wave-uniform) array into vector registers:
+// an empty asm statement that takes and returns each value in a VGPR, so the
+// compiler keeps them there instead of re-creating constants in scalar
+// registers at every use.  Static indices only (the array stays in registers).
+template <typename V>
+__device__ inline void
+pin_one(V & v)
+{
+  asm volatile("" : "+v"(v));
+}
+template <typename A, size_t... I>
+__device__ inline void
+pin_vgpr(A & a, std::index_sequence<I...>)
+{
+  (pin_one(a[I]), ...);
+}
+
 // Order the LDS traffic of one wave (all lanes of a wavefront run in lockstep,
 // so completing its own LDS operations is enough).
 __device__ inline void
@@ -70,21 +88,40 @@ wave_lds_sync()
 // from HBM once per group).
 // EF = 1: element factors (box meshes, ApplyDesc::ef) instead of the
 // quadrature-point factors.
-template <int P, typename T, int TX, int TY, int TZ, int MODE, int SP, int BATCH, int EF>
+template <int P, typename T, int TX, int TY, int TZ, int MODE, int SP, int BATCH, int EF, int VT = 1>
 __global__ void __launch_bounds__((Tile<P, TX, TY, TZ, SP>::NT))
 k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tables<T> tb_arg)
 {
   using C = Tile<P, TX, TY, TZ, SP>;
   // the 1D tables of order P as compile-time constants (the same values as
-  // tb_arg): every use has a static index once the loops are unrolled, so
-  // the compiler materialises each value where it is used instead of holding
-  // all of them in scalar registers through the kernel
+  // tb_arg): every use has a static index once the loops are unrolled, so no
+  // table occupies scalar registers through the kernel
   (void)tb_arg;
-  constexpr basis::Tab<T> tb = basis::tab<P, T>();
   constexpr int QH = C::QH, NPP = C::NPP;
   constexpr int O = C::O, D = C::D, Q = C::Q, QQ = C::QQ;
   constexpr int SQX = C::SQX, LA = C::LA, SDZ = C::SDZ, LB = C::LB, SA = C::SA, SB = C::SB;
   constexpr int NT = C::NT, TE = C::TE;
+  constexpr basis::Tab<T> tb_c = basis::tab<P, T>();
+  // FP64: gfx950 has no 64-bit instruction literals, so each use of a double
+  // constant costs two scalar moves into a register pair (rematerialised
+  // whenever scalar registers run short).  With VREG the order-P tables are
+  // moved into vector registers once per wave and every contraction reads
+  // them there: 88 more VGPRs at order 3, which fit where a SIMD holds two
+  // waves of these kernels (one qz group, the LDS of two workgroups per CU)
+  // and no quadrature-point factors are held (element factors); every batch
+  // form of such a tile does the same, so their results agree.  Measured at
+  // 48^3 order 3: 177 -> 156 us on the (4, 2, 2) tile; with two qz groups, or
+  // with the quadrature-point factors, the extra registers cost occupancy
+  // and the kernels slow down.  FP32 constants are literals of the VOP2
+  // encoding.
+  constexpr bool VREG = VT != 0 && sizeof(T) == 8 && SP == 1 && EF;
+  basis::Tab<T> tb = tb_c;
+  if constexpr (VREG)
+  {
+    pin_vgpr(tb.Bo, std::make_index_sequence<Q * O>{});
+    pin_vgpr(tb.Bc, std::make_index_sequence<Q * D>{});
+    pin_vgpr(tb.Gc, std::make_index_sequence<Q * D>{});
+  }
   // barrier between stages whose LDS exchanges may cross waves
   auto stage_sync = [&]() {
     if constexpr (C::WS)
@@ -476,10 +513,14 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
     // factor of component a at (qx, qy, qz): sigma_e w mk[a] (mass) and
     // dt nu_e w ck[a] (curl), w = w_qx w_qy w_qz
     auto qw_sel = [&](int q) {
-      T v = (T)tb.qw[0];
-#pragma unroll
-      for (int k = 1; k < Q; k++)
-        v = q == k ? (T)tb.qw[k] : v;
+      T v = tb_c.qw[0];
+      if constexpr (Q > 1)
+        v = q == 1 ? tb_c.qw[1] : v;
+      if constexpr (Q > 2)
+        v = q == 2 ? tb_c.qw[2] : v;
+      if constexpr (Q > 3)
+        v = q == 3 ? tb_c.qw[3] : v;
+      static_assert(Q <= 4, "weights of at most four points");
       return v;
     };
     const T wxy = qw_sel(pqx) * qw_sel(pqy);
@@ -492,7 +533,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
       for (int j = 0; j < QH; j++)
       {
         const int qz = PH * QH + j; // past Q: zero weight, and U, Cu are zero there
-        const T w = wxy * (T)tb.qw[qz];
+        const T w = wxy * tb_c.qw[qz];
         const T sm = w * sgm, sc = w * cnu;
         U[0][j] = (sm * mk0) * U[0][j];
         U[1][j] = (sm * mk1) * U[1][j];
