@@ -88,4 +88,55 @@ div_up(int64_t a, int64_t b)
   return (int)((a + b - 1) / b);
 }
 
+// Sum over the 64 lanes of a wavefront by a butterfly of cross-lane
+// exchanges.  In every step the two lanes of a pair add the same two operands,
+// so all lanes end with the same value and the summation order is fixed.
+__device__ inline double
+wave_sum(double v)
+{
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+    v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// Sum over a workgroup of NT threads (a multiple of 64) of K values per
+// thread: wavefront sums, then the NT / 64 wave sums in wave order.  Every
+// thread receives the results; sh (K * NT / 64 doubles) is free again on
+// return.
+template <int NT, int K>
+__device__ inline void
+group_sum(double (&v)[K], double * sh)
+{
+  constexpr int NW = NT / 64;
+  static_assert(NT % 64 == 0, "whole wavefronts");
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; k++)
+  {
+    v[k] = wave_sum(v[k]);
+    if ((threadIdx.x & 63) == 0)
+      sh[k * NW + w] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++)
+  {
+    double r = sh[k * NW];
+#pragma unroll
+    for (int i = 1; i < NW; i++)
+      r += sh[k * NW + i];
+    v[k] = r;
+  }
+  __syncthreads();
+}
+template <int NT>
+__device__ inline double
+group_sum(double v, double * sh)
+{
+  double a[1] = {v};
+  group_sum<NT, 1>(a, sh);
+  return a[0];
+}
+
 } // namespace hc

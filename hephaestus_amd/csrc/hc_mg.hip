@@ -38,6 +38,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 namespace hc
@@ -110,22 +111,12 @@ k_dot2(const double * __restrict__ a, const double * __restrict__ b, int64_t n, 
     s0 += ai * b[i];
     s1 += ai * ai;
   }
-  sh[0][threadIdx.x] = s0;
-  sh[1][threadIdx.x] = s1;
-  __syncthreads();
-  for (int s = DOT_THREADS / 2; s > 0; s >>= 1)
-  {
-    if ((int)threadIdx.x < s)
-    {
-      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
-      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
-    }
-    __syncthreads();
-  }
+  double v[2] = {s0, s1};
+  group_sum<DOT_THREADS, 2>(v, &sh[0][0]);
   if (threadIdx.x == 0)
   {
-    part[2 * blockIdx.x] = sh[0][0];
-    part[2 * blockIdx.x + 1] = sh[1][0];
+    part[2 * blockIdx.x] = v[0];
+    part[2 * blockIdx.x + 1] = v[1];
   }
 }
 
@@ -156,23 +147,13 @@ k_dot2_k(const double * __restrict__ a, const double * __restrict__ b, int64_t n
     s0 += ai * b[i];
     s1 += ai * ai;
   }
-  sh[0][threadIdx.x] = s0;
-  sh[1][threadIdx.x] = s1;
-  __syncthreads();
-  for (int s = DOT_THREADS / 2; s > 0; s >>= 1)
-  {
-    if ((int)threadIdx.x < s)
-    {
-      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
-      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
-    }
-    __syncthreads();
-  }
+  double v[2] = {s0, s1};
+  group_sum<DOT_THREADS, 2>(v, &sh[0][0]);
   if (threadIdx.x == 0)
   {
     double * o = part + 2 * ((int64_t)blockIdx.y * DOT_BLOCKS + blockIdx.x);
-    o[0] = sh[0][0];
-    o[1] = sh[1][0];
+    o[0] = v[0];
+    o[1] = v[1];
   }
 }
 
@@ -242,16 +223,9 @@ k_dense_symv(const double * __restrict__ A, int64_t n, const double * __restrict
   double s = 0.0;
   for (int64_t j = threadIdx.x; j < n; j += 256)
     s += col[j] * x[j];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1)
-  {
-    if ((int)threadIdx.x < k)
-      sh[threadIdx.x] += sh[threadIdx.x + k];
-    __syncthreads();
-  }
+  s = group_sum<256>(s, sh);
   if (threadIdx.x == 0)
-    y[i] = sh[0];
+    y[i] = s;
 }
 
 // Identity columns j0 .. j0 + nb - 1 of an n x n matrix (column-major, nb columns).
@@ -924,12 +898,20 @@ estimate_lambda(MgHier & h, MgLevel & L, hipStream_t st, double & lam)
   return HC_OK;
 }
 
+// Hierarchy builds run one at a time in the process: a build is one-time
+// setup (level operators, their autotuning, rocSOLVER on the coarsest level,
+// power iterations), and the multigrid server and batch workers otherwise
+// start several of them together on their first requests.  Solves on built
+// hierarchies run concurrently.
+std::mutex g_build_mu;
+
 // the hierarchy of op (built on first use; rebuilt when op was set up again)
 int
 mg_build(hc_op_t op, hipStream_t st)
 {
   if (op->mg && op->mg->version == op->setup_version)
     return HC_OK;
+  std::lock_guard<std::mutex> lk(g_build_mu);
   mg_free(op->mg);
   op->mg = nullptr;
   if (!op->box)
@@ -1492,16 +1474,9 @@ k_trace_part(const double * __restrict__ A, int64_t n, double * __restrict__ out
   double s = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += 256)
     s += A[i * n + i];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1)
-  {
-    if ((int)threadIdx.x < k)
-      sh[threadIdx.x] += sh[threadIdx.x + k];
-    __syncthreads();
-  }
+  s = group_sum<256>(s, sh);
   if (threadIdx.x == 0)
-    out[0] = sh[0];
+    out[0] = s;
 }
 
 // y = S x on a node level

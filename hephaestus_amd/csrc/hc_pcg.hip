@@ -33,30 +33,13 @@ slot_blocks(int64_t n)
   return red_blocks(n);
 }
 
-// Block-wide sum of K values per thread (fixed tree order).
+// Block-wide sum of K values per thread (wavefront butterflies, then the
+// wave sums in order: a fixed order).
 template <int K>
 __device__ inline void
 block_sum(double (&v)[K], double * sh)
 {
-  const int t = threadIdx.x;
-#pragma unroll
-  for (int k = 0; k < K; k++)
-    sh[k * RED_THREADS + t] = v[k];
-  __syncthreads();
-  for (int s = RED_THREADS / 2; s > 0; s >>= 1)
-  {
-    if (t < s)
-    {
-#pragma unroll
-      for (int k = 0; k < K; k++)
-        sh[k * RED_THREADS + t] += sh[k * RED_THREADS + t + s];
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int k = 0; k < K; k++)
-    v[k] = sh[k * RED_THREADS];
-  __syncthreads();
+  group_sum<RED_THREADS, K>(v, sh);
 }
 
 // Reduce the nb block partials of slots k0.. (stride 4) in a fixed order;

@@ -23,18 +23,7 @@ constexpr int AV_BLOCKS = 1024;
 __device__ inline double
 av_block_sum(double v, double * sh)
 {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int s = AV_THREADS / 2; s > 0; s >>= 1)
-  {
-    if (t < s)
-      sh[t] += sh[t + s];
-    __syncthreads();
-  }
-  const double r = sh[0];
-  __syncthreads(); // sh is reused by the caller's next reduction
-  return r;
+  return group_sum<AV_THREADS>(v, sh); // sh is free again on return
 }
 
 // out[b] = block b's part of x.y
