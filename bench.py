@@ -588,7 +588,8 @@ def main():
         "fp64_us": round(ms_ef * 1e3, 2), "fp64_GDOF_s": round(N / ms_ef / 1e6, 3),
         "tile_kernel_us": round(prof_ef["tile_us"], 2), "assembly_kernel_us": round(prof_ef["planes_us"], 2),
         "fp32_us": round(ms_ef32 * 1e3, 2), "fp32_GDOF_s": round(N / ms_ef32 / 1e6, 3),
-        "algorithmic_bytes": B_ef, "GB_s": round(B_ef / ms_ef / 1e6, 1),
+        "algorithmic_bytes": B_ef, "bytes_rule": "B_min = 16 N + 16 E (no per-quadrature-point data streamed)",
+        "GB_s": round(B_ef / ms_ef / 1e6, 1), "frac_hbm": round(B_ef / ms_ef / 1e6 / HBM_PEAK_GBS, 4),
         "rel_diff_vs_quadrature_point_factors": diff_ef,
         "bound": "instruction issue (VALU + SALU per wave), not HBM"}
     print(f"[bench] element-factor apply: FP64 {ms_ef * 1e3:.1f} us = {N / ms_ef / 1e6:.1f} GDOF/s "
