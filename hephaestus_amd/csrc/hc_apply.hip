@@ -114,11 +114,14 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   // with the quadrature-point factors, the extra registers cost occupancy
   // and the kernels slow down.  FP32 constants are literals of the VOP2
   // encoding.
-  constexpr bool VREG = VT != 0 && sizeof(T) == 8 && SP == 1 && EF;
+  // With the quadrature-point factors the order-3 tiles hold 170 VGPRs
+  // already: only the closed-basis tables (64 VGPRs) are moved there.
+  constexpr bool VREG = VT != 0 && sizeof(T) == 8 && SP == 1 && (EF || (P == 3 && MODE == 1 && BATCH != 1));
   basis::Tab<T> tb = tb_c;
   if constexpr (VREG)
   {
-    pin_vgpr(tb.Bo, std::make_index_sequence<Q * O>{});
+    if constexpr (EF)
+      pin_vgpr(tb.Bo, std::make_index_sequence<Q * O>{});
     pin_vgpr(tb.Bc, std::make_index_sequence<Q * D>{});
     pin_vgpr(tb.Gc, std::make_index_sequence<Q * D>{});
   }
