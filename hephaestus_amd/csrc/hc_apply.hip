@@ -154,6 +154,8 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
   gptr<const T> x = gmem((const T *)ad.x);
   gptr<T> y = gmem((T *)ad.y);
   gptr<T> part = gmem((T *)ad.part);
+  gptr<const T> rbv = gmem((const T *)ad.resid); // residual mode when set (wave-uniform)
+  const bool resid = ad.resid != nullptr;
   __shared__ T sA[TE * SA];
   __shared__ T sB[TE * SB];
 
@@ -301,6 +303,8 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
     continue;
   x = gmem((const T *)ad.x) + (int64_t)rhs * g.off[3];
   y = gmem((T *)ad.y) + (int64_t)rhs * g.off[3];
+  if (resid)
+    rbv = gmem((const T *)ad.resid) + (int64_t)rhs * g.off[3];
   part = gmem((T *)ad.part) + (int64_t)rhs * 4 * part_stride(ad);
   if (!x_ready)
     load_x(e0, ne);
@@ -902,7 +906,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
       if (ess)
       {
         if (!hi1 && !hi2)
-          y[ybase + off] = x[ybase + off];
+          y[ybase + off] = resid ? rbv[ybase + off] - x[ybase + off] : x[ybase + off];
         continue;
       }
       const T v = sOut[OBc + j];
@@ -925,7 +929,7 @@ k_apply_tile(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc, Tabl
         part[pid] = v;
       }
       else
-        y[ybase + off] = v;
+        y[ybase + off] = resid ? rbv[ybase + off] - v : v;
     }
   };
   wloop(std::integral_constant<int, 0>{}, 0, NB0);
@@ -1024,7 +1028,7 @@ template <typename T>
 hipError_t
 apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T * qd, const ElemFactors & ef,
          const T * x, T * y, T * part, const int * done, hipStream_t st, const TileCandidate * cand, int nrhs,
-         int done_stride, int rb, ApplyDesc * desc_dev)
+         int done_stride, int rb, ApplyDesc * desc_dev, const T * resid_b)
 {
   if (!cand)
   {
@@ -1042,6 +1046,7 @@ apply_op(const Geom & g, const PlaneSegs & segs, const Tables<T> & tb, const T *
   d.prows = (int)segs.start[segs.nseg];
   d.qd = qd;
   d.ef = ef;
+  d.resid = resid_b;
   d.x = x;
   d.y = y;
   d.part = part;
@@ -1121,10 +1126,11 @@ default_rhs_block(int ntiles, int nrhs)
 
 template hipError_t apply_op<double>(const Geom &, const PlaneSegs &, const Tables<double> &, const double *,
                                      const ElemFactors &, const double *, double *, double *, const int *,
-                                     hipStream_t, const TileCandidate *, int, int, int, ApplyDesc *);
+                                     hipStream_t, const TileCandidate *, int, int, int, ApplyDesc *,
+                                     const double *);
 template hipError_t apply_op<float>(const Geom &, const PlaneSegs &, const Tables<float> &, const float *,
                                     const ElemFactors &, const float *, float *, float *, const int *, hipStream_t,
-                                    const TileCandidate *, int, int, int, ApplyDesc *);
+                                    const TileCandidate *, int, int, int, ApplyDesc *, const float *);
 
 template <typename T>
 hipError_t

@@ -205,6 +205,38 @@ k_aux_z_single(AuxGeom a, const double * __restrict__ dinv, const double * __res
   });
 }
 
+// One smoothing step after t = Dg^-1 G^T r: z = D^-1 r + G t, then the
+// Chebyshev (or damped) update d = ca z + cb d, x += d, in one pass
+// (flags 1: fresh d, d = ca z; 2: fresh x, x = d).  The same arithmetic as z
+// followed by the separate update, with z kept in a register.
+__global__ void
+k_aux_z_cheb(AuxGeom a, const double * __restrict__ dinv, const double * __restrict__ r,
+             const double * __restrict__ t, double ca, double cb, double * __restrict__ d, double * __restrict__ x,
+             int flags)
+{
+  with_order(a, [&](auto pc) {
+    constexpr int P = decltype(pc)::value;
+    const DgTab<P> tab = load_dg<P>(a);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.off[3];
+         i += (int64_t)gridDim.x * blockDim.x)
+    {
+      const double z = dinv[i] * r[i] + aux_g<P>(a, tab, t, i);
+      const double di = (flags & 1) ? ca * z : ca * z + cb * d[i];
+      d[i] = di;
+      x[i] = (flags & 2) ? di : x[i] + di;
+    }
+  });
+}
+
+hipError_t
+aux_smooth_step(const AuxGeom & a, const double * dinv, const double * ginv, const double * r, double * t,
+                double ca, double cb, double * d, double * x, int flags, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_aux_nodes_single, dim3(2048u, 1), dim3(256), 0, st, a, ginv, r, t, (int64_t)0, (int64_t)0);
+  hipLaunchKernelGGL(k_aux_z_cheb, dim3(4096), dim3(256), 0, st, a, dinv, r, (const double *)t, ca, cb, d, x, flags);
+  return hipGetLastError();
+}
+
 hipError_t
 aux_precond_apply(const AuxGeom & a, const double * dinv, const double * ginv, const double * r, double * t,
                   double * z, hipStream_t st, int nrhs, int64_t sn, int64_t snn)

@@ -274,6 +274,11 @@ hipError_t grad_apply(const AuxGeom & a, const double * v, double * y, hipStream
 // yv = G^T w at every node (no essential nodes).
 hipError_t grad_transpose(const AuxGeom & a, const double * w, double * yv, hipStream_t st);
 // z = D^-1 r + G Dg^-1 G^T r for one vector (t: node scratch, NN entries).
+// One step of the Chebyshev iteration with the hybrid smoother B = D^-1 + G Dg^-1 G^T:
+// t = Dg^-1 G^T r, then d = ca B r + cb d and x += d in one pass (flags 1:
+// fresh d; 2: fresh x, x = d).
+hipError_t aux_smooth_step(const AuxGeom & a, const double * dinv, const double * ginv, const double * r, double * t,
+                           double ca, double cb, double * d, double * x, int flags, hipStream_t st);
 hipError_t aux_precond_apply(const AuxGeom & a, const double * dinv, const double * ginv, const double * r, double * t,
                              double * z, hipStream_t st, int nrhs = 1, int64_t sn = 0, int64_t snn = 0);
 

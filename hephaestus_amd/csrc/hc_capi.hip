@@ -130,10 +130,12 @@ plan_of(hc_op_t op)
   return sizeof(T) == 8 ? op->plan64 : op->plan32;
 }
 
-// y = A x for nrhs right-hand sides (rhs r at r*N) in the precision of T.
+// y = A x for nrhs right-hand sides (rhs r at r*N) in the precision of T;
+// with resid_b, y = resid_b - A x (the kernels form the difference as they
+// store).
 template <typename T>
 int
-apply_t(hc_op_t op, const T * x, T * y, int nrhs, const int * done, hipStream_t st)
+apply_t(hc_op_t op, const T * x, T * y, int nrhs, const int * done, hipStream_t st, const T * resid_b = nullptr)
 {
   const bool f32 = sizeof(T) == 4;
   int rc = op->ensure_ws(nrhs, f32);
@@ -146,7 +148,7 @@ apply_t(hc_op_t op, const T * x, T * y, int nrhs, const int * done, hipStream_t 
   const int ntiles = pl.g.nt[0] * pl.g.nt[1] * pl.g.nt[2];
   const int rb = nrhs == 1 ? 1 : (op->rhs_block > 0 ? std::min(op->rhs_block, nrhs) : hc::default_rhs_block(ntiles, nrhs));
   HC_TRY_HIP(hc::apply_op<T>(pl.g, pl.segs, tb, qd, op->ef(f32), x, y, part, done, st, pl.cand, nrhs, hc::STATE_STRIDE_INT, rb,
-                             f32 ? op->desc_dev + 1 : op->desc_dev));
+                             f32 ? op->desc_dev + 1 : op->desc_dev, resid_b));
   return HC_OK;
 }
 
@@ -564,6 +566,15 @@ hc_op_diagonal(hc_op_t op, double * d, void * stream)
 }
 
 } // extern "C"
+
+namespace hc
+{
+int
+op_residual(hc_op_t op, const double * b, const double * x, double * r, hipStream_t st)
+{
+  return apply_t<double>(op, x, r, 1, nullptr, st, b);
+}
+} // namespace hc
 
 namespace
 {

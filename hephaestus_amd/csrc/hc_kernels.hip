@@ -169,6 +169,7 @@ k_assemble_planes(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc)
   const int64_t PS = part_stride(d);
   const T * part = (const T *)d.part + (int64_t)rhs * 4 * PS;
   T * y = (T *)d.y + (int64_t)rhs * N;
+  const T * rb = d.resid ? (const T *)d.resid + (int64_t)rhs * N : nullptr;
   const int a2 = 3 - c - a; // the other closed axis of component c
   const int step2 = g.T[a2] * g.p;
   const int dlo = (c == 0) ? 1 : 0;
@@ -205,7 +206,7 @@ k_assemble_planes(ApplyDesc d0, const ApplyDesc * __restrict__ descs, int ndesc)
       v += part[2 * PS + pid];
     if (on_lo && on_hi)
       v += part[3 * PS + pid];
-    y[gid] = v;
+    y[gid] = rb ? rb[gid] - v : v;
   }
 }
 

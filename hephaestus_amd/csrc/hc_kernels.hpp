@@ -53,6 +53,7 @@ struct ApplyDesc
   PlaneSegs segs;
   const void * qd; // quadrature data (element type of the launch)
   ElemFactors ef;  // element factors (box meshes), used instead of qd when set
+  const void * resid = nullptr; // residual mode: y = resid - A x (vector at the rhs offset of y)
   const void * x;
   void * y;
   void * part;     // 4 slots per rhs: slot s of compact plane entry j at s*pstride + j
@@ -250,7 +251,8 @@ hipError_t apply_op(const Geom & g,
                     int nrhs = 1,
                     int done_stride = 0,
                     int rb = 1,
-                    ApplyDesc * desc_dev = nullptr);
+                    ApplyDesc * desc_dev = nullptr,
+                    const T * resid_b = nullptr);
 // Right-hand sides per work item for a batch of nrhs on ntiles tiles.
 int default_rhs_block(int ntiles, int nrhs);
 // y = (mscale M + cscale K') x (K' = the curl part of the quadrature data,

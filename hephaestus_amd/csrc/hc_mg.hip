@@ -512,10 +512,7 @@ residual(MgLevel & L, const double * b, const double * x, double * r, hipStream_
     HC_TRY_HIP(hipMemcpyAsync(r, b, sizeof(double) * L.N, hipMemcpyDeviceToDevice, st));
     return HC_OK;
   }
-  if (int rc = hc_op_apply(L.op, x, r, 1, st))
-    return rc;
-  HC_TRY_HIP(axpby(L.N, 1.0, b, -1.0, r, st));
-  return HC_OK;
+  return op_residual(L.op, b, x, r, st);
 }
 
 // h.nsmooth steps of the smoother on a level, from x = 0 when zero_start.
@@ -543,8 +540,6 @@ smooth(MgHier & h, MgLevel & L, bool zero_start, hipStream_t st)
     const bool first = zero_start && s == 0;
     if (int rc = residual(L, L.b, first ? nullptr : L.x, L.r, st))
       return rc;
-    if (int rc = hc_op_precond_apply(L.op, HC_SOLVE_FP64 | HC_SOLVE_AUX, L.r, L.z, st))
-      return rc;
     if (L.cheb > 1.0)
     {
       double a, b;
@@ -560,12 +555,18 @@ smooth(MgHier & h, MgLevel & L, bool zero_start, hipStream_t st)
         b = rho1 * rho0;
         rho0 = rho1;
       }
-      hipLaunchKernelGGL(k_cheb_update, dim3(grid_for(L.N)), dim3(256), 0, st, L.N, a, b, L.z, L.d, L.x,
-                         (s == 0 ? 1 : 0) | (first ? 2 : 0));
-      HC_TRY_HIP(hipGetLastError());
+      // z = B r and the update in one pass after the node stage
+      if (int rc = L.op->ensure_ws(1, false))
+        return rc;
+      HC_TRY_HIP(aux_smooth_step(L.op->ag, L.op->dinv64, L.op->ginv64, L.r, L.op->ws.t64, a, b, L.d, L.x,
+                                 (s == 0 ? 1 : 0) | (first ? 2 : 0), st));
     }
     else
+    {
+      if (int rc = hc_op_precond_apply(L.op, HC_SOLVE_FP64 | HC_SOLVE_AUX, L.r, L.z, st))
+        return rc;
       HC_TRY_HIP(axpby(L.N, L.omega, L.z, first ? 0.0 : 1.0, L.x, st));
+    }
   }
   return HC_OK;
 }
