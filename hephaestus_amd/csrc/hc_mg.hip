@@ -99,6 +99,53 @@ k_pass1d(const double * __restrict__ in, double * __restrict__ out, int d0, int 
   out[o] = accumulate ? out[o] + s : s;
 }
 
+// The same pass for the three components of an ND transfer in one launch
+// (blockIdx.z = 3 rhs + component): each component has its own dims, 1-D
+// matrix, scale and input/output vectors; the per-entry arithmetic is that
+// of k_pass1d.
+struct Pass3
+{
+  const double * in[3];
+  double * out[3];
+  int d[3][3]; // input dims per component
+  int m[3];    // rows of the component's 1-D matrix
+  const int * rp[3];
+  const int * ci[3];
+  const double * w[3];
+  double scale[3];
+  int64_t sin, sout; // right-hand-side strides
+  int axis, accumulate;
+};
+
+__global__ void __launch_bounds__(256)
+k_pass3(Pass3 q)
+{
+  const int comp = (int)(blockIdx.z % 3), rhs = (int)(blockIdx.z / 3);
+  const int axis = q.axis, m = q.m[comp];
+  const int d0 = q.d[comp][0], d1 = q.d[comp][1], d2 = q.d[comp][2];
+  const unsigned o0 = axis == 0 ? m : d0, o1 = axis == 1 ? m : d1, o2 = axis == 2 ? m : d2;
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= o0 * o1 || blockIdx.y >= o2)
+    return;
+  const double * __restrict__ in = q.in[comp] + rhs * q.sin;
+  double * __restrict__ out = q.out[comp] + rhs * q.sout;
+  const int * __restrict__ rp = q.rp[comp];
+  const int * __restrict__ ci = q.ci[comp];
+  const double * __restrict__ w = q.w[comp];
+  const int i1 = (int)(t / o0);
+  const int i0 = (int)(t - (unsigned)i1 * o0);
+  const int i2 = (int)blockIdx.y;
+  const int row = axis == 0 ? i0 : (axis == 1 ? i1 : i2);
+  const int64_t stride = axis == 0 ? 1 : (axis == 1 ? (int64_t)d0 : (int64_t)d0 * d1);
+  const int64_t base = ((int64_t)(axis == 2 ? 0 : i2) * d1 + (axis == 1 ? 0 : i1)) * d0 + (axis == 0 ? 0 : i0);
+  double s = 0.0;
+  for (int k = rp[row]; k < rp[row + 1]; k++)
+    s += w[k] * in[base + ci[k] * stride];
+  s *= q.scale[comp];
+  const int64_t o = ((int64_t)i2 * o1 + i1) * o0 + i0;
+  out[o] = q.accumulate ? out[o] + s : s;
+}
+
 // Block partials of (a, b) and (a, a) (fixed grid and order).
 __global__ void __launch_bounds__(DOT_THREADS)
 k_dot2(const double * __restrict__ a, const double * __restrict__ b, int64_t n, double * __restrict__ part)
@@ -467,6 +514,14 @@ transfer(MgHier & h, int l, bool prolong, const double * x, double * y, hipStrea
     tmp2 = h.tmp2;
   }
   const int64_t sx = prolong ? c.N : f.N, sy = prolong ? f.N : c.N, sT = h.lev[0].N;
+  // per component: source, destination, and its intermediates at the
+  // component's offset on the finer level (every intermediate of a component
+  // is at most the size of its fine-level block, so the blocks do not meet)
+  const double * src[3];
+  double * dst[3];
+  double *t1[3], *t2[3];
+  int d[3][3];
+  double scale[3];
   int64_t offf = 0, offc = 0;
   for (int comp = 0; comp < 3; comp++)
   {
@@ -474,29 +529,46 @@ transfer(MgHier & h, int l, bool prolong, const double * x, double * y, hipStrea
     comp_dims(f.n, f.op->p, comp, df);
     comp_dims(c.n, c.op->p, comp, dc);
     // covariant DOF (h times the tangential component): h_f / h_c along the component's axis
-    const double scale = (double)c.n[comp] / f.n[comp];
-    const double * src = x + (prolong ? offc : offf);
-    double * dst = y + (prolong ? offf : offc);
-    int d[3];
-    std::memcpy(d, prolong ? dc : df, sizeof(d));
-    for (int a = 0; a < 3; a++)
+    scale[comp] = (double)c.n[comp] / f.n[comp];
+    src[comp] = x + (prolong ? offc : offf);
+    dst[comp] = y + (prolong ? offf : offc);
+    t1[comp] = tmp1 + offf;
+    t2[comp] = tmp2 + offf;
+    std::memcpy(d[comp], prolong ? dc : df, sizeof(d[comp]));
+    offf += (int64_t)df[0] * df[1] * df[2];
+    offc += (int64_t)dc[0] * dc[1] * dc[2];
+  }
+  // pass a over every component: x (src -> t1), y (t1 -> t2), z (t2 -> dst)
+  for (int a = 0; a < 3; a++)
+  {
+    Pass3 q{};
+    int64_t plane = 0, o2 = 0;
+    for (int comp = 0; comp < 3; comp++)
     {
       const int kind = (a == comp) ? 0 : 1;
       const Csr1Dev & m = prolong ? f.P[a][kind] : f.R[a][kind];
-      double * out = (a == 2) ? dst : (a == 0 ? tmp1 : tmp2);
-      const double * in = (a == 0) ? src : (a == 1 ? tmp1 : tmp2);
-      const int64_t plane = (int64_t)(a == 0 ? m.rows : d[0]) * (a == 1 ? m.rows : d[1]);
-      const int o2 = a == 2 ? m.rows : d[2];
-      hipLaunchKernelGGL(k_pass1d, dim3((unsigned)((plane + 255) / 256), (unsigned)o2, (unsigned)nrhs), dim3(256), 0,
-                         st, in, out, d[0], d[1], d[2], a, m.rows, m.rp, m.ci, m.w, a == 2 ? scale : 1.0,
-                         (a == 2 && prolong) ? 1 : 0, a == 0 ? sx : sT, a == 2 ? sy : sT);
-      hipError_t e = hipGetLastError();
-      if (e != hipSuccess)
-        return e;
-      d[a] = m.rows;
+      q.in[comp] = (a == 0) ? src[comp] : (a == 1 ? t1[comp] : t2[comp]);
+      q.out[comp] = (a == 2) ? dst[comp] : (a == 0 ? t1[comp] : t2[comp]);
+      for (int b = 0; b < 3; b++)
+        q.d[comp][b] = d[comp][b];
+      q.m[comp] = m.rows;
+      q.rp[comp] = m.rp;
+      q.ci[comp] = m.ci;
+      q.w[comp] = m.w;
+      q.scale[comp] = a == 2 ? scale[comp] : 1.0;
+      plane = std::max(plane, (int64_t)(a == 0 ? m.rows : d[comp][0]) * (a == 1 ? m.rows : d[comp][1]));
+      o2 = std::max(o2, (int64_t)(a == 2 ? m.rows : d[comp][2]));
+      d[comp][a] = m.rows;
     }
-    offf += (int64_t)df[0] * df[1] * df[2];
-    offc += (int64_t)dc[0] * dc[1] * dc[2];
+    q.axis = a;
+    q.accumulate = (a == 2 && prolong) ? 1 : 0;
+    q.sin = a == 0 ? sx : sT;
+    q.sout = a == 2 ? sy : sT;
+    hipLaunchKernelGGL(k_pass3, dim3((unsigned)((plane + 255) / 256), (unsigned)o2, (unsigned)(3 * nrhs)), dim3(256),
+                       0, st, q);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      return e;
   }
   if (!prolong)
     return zero_ess(c.op->g, y, st, nrhs);
