@@ -115,6 +115,7 @@ struct Pass3
   double scale[3];
   int64_t sin, sout; // right-hand-side strides
   int axis, accumulate;
+  int zero_pec; // PEC bits (HC_FACE_*) whose entries of the output are set to zero, or 0
 };
 
 __global__ void __launch_bounds__(256)
@@ -143,6 +144,21 @@ k_pass3(Pass3 q)
     s += w[k] * in[base + ci[k] * stride];
   s *= q.scale[comp];
   const int64_t o = ((int64_t)i2 * o1 + i1) * o0 + i0;
+  if (q.zero_pec)
+  {
+    // PEC entries of the output (a restriction's last pass): tangential DOFs
+    // on a PEC face of one of the component's closed axes
+    const int ii[3] = {i0, i1, i2};
+    const unsigned od[3] = {o0, o1, o2};
+    bool ess = false;
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+      if (b != comp)
+        ess = ess || (ii[b] == 0 && ((q.zero_pec >> (2 * b)) & 1)) ||
+              ((unsigned)ii[b] == od[b] - 1 && ((q.zero_pec >> (2 * b + 1)) & 1));
+    if (ess)
+      s = 0.0;
+  }
   out[o] = q.accumulate ? out[o] + s : s;
 }
 
@@ -562,6 +578,7 @@ transfer(MgHier & h, int l, bool prolong, const double * x, double * y, hipStrea
     }
     q.axis = a;
     q.accumulate = (a == 2 && prolong) ? 1 : 0;
+    q.zero_pec = (a == 2 && !prolong) ? c.op->g.pec : 0;
     q.sin = a == 0 ? sx : sT;
     q.sout = a == 2 ? sy : sT;
     hipLaunchKernelGGL(k_pass3, dim3((unsigned)((plane + 255) / 256), (unsigned)o2, (unsigned)(3 * nrhs)), dim3(256),
@@ -570,9 +587,7 @@ transfer(MgHier & h, int l, bool prolong, const double * x, double * y, hipStrea
     if (e != hipSuccess)
       return e;
   }
-  if (!prolong)
-    return zero_ess(c.op->g, y, st, nrhs);
-  return hipSuccess;
+  return hipSuccess; // (a restriction's PEC entries are zeroed by its last pass)
 }
 
 // r = b - A x on level l (x == nullptr: r = b)
