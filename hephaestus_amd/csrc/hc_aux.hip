@@ -212,8 +212,12 @@ k_aux_z_single(AuxGeom a, const double * __restrict__ dinv, const double * __res
 __global__ void
 k_aux_z_cheb(AuxGeom a, const double * __restrict__ dinv, const double * __restrict__ r,
              const double * __restrict__ t, double ca, double cb, double * __restrict__ d, double * __restrict__ x,
-             int flags)
+             int flags, int64_t sr, int64_t st_)
 {
+  r += blockIdx.y * sr; // right-hand side blockIdx.y (strides 0 for one)
+  d += blockIdx.y * sr;
+  x += blockIdx.y * sr;
+  t += blockIdx.y * st_;
   with_order(a, [&](auto pc) {
     constexpr int P = decltype(pc)::value;
     const DgTab<P> tab = load_dg<P>(a);
@@ -230,10 +234,15 @@ k_aux_z_cheb(AuxGeom a, const double * __restrict__ dinv, const double * __restr
 
 hipError_t
 aux_smooth_step(const AuxGeom & a, const double * dinv, const double * ginv, const double * r, double * t,
-                double ca, double cb, double * d, double * x, int flags, hipStream_t st)
+                double ca, double cb, double * d, double * x, int flags, hipStream_t st, int nrhs, int64_t sn,
+                int64_t snn)
 {
-  hipLaunchKernelGGL(k_aux_nodes_single, dim3(2048u, 1), dim3(256), 0, st, a, ginv, r, t, (int64_t)0, (int64_t)0);
-  hipLaunchKernelGGL(k_aux_z_cheb, dim3(4096), dim3(256), 0, st, a, dinv, r, (const double *)t, ca, cb, d, x, flags);
+  // the grids of aux_precond_apply (the per-entry arithmetic does not depend on them)
+  const unsigned gn = nrhs > 1 ? std::max(64u, 2048u / (unsigned)nrhs) : 2048u;
+  const unsigned gz = nrhs > 1 ? std::max(64u, 4096u / (unsigned)nrhs) : 4096u;
+  hipLaunchKernelGGL(k_aux_nodes_single, dim3(gn, nrhs), dim3(256), 0, st, a, ginv, r, t, sn, snn);
+  hipLaunchKernelGGL(k_aux_z_cheb, dim3(gz, nrhs), dim3(256), 0, st, a, dinv, r, (const double *)t, ca, cb, d, x,
+                     flags, sn, snn);
   return hipGetLastError();
 }
 

@@ -278,7 +278,8 @@ hipError_t grad_transpose(const AuxGeom & a, const double * w, double * yv, hipS
 // t = Dg^-1 G^T r, then d = ca B r + cb d and x += d in one pass (flags 1:
 // fresh d; 2: fresh x, x = d).
 hipError_t aux_smooth_step(const AuxGeom & a, const double * dinv, const double * ginv, const double * r, double * t,
-                           double ca, double cb, double * d, double * x, int flags, hipStream_t st);
+                           double ca, double cb, double * d, double * x, int flags, hipStream_t st, int nrhs = 1,
+                           int64_t sn = 0, int64_t snn = 0);
 hipError_t aux_precond_apply(const AuxGeom & a, const double * dinv, const double * ginv, const double * r, double * t,
                              double * z, hipStream_t st, int nrhs = 1, int64_t sn = 0, int64_t snn = 0);
 

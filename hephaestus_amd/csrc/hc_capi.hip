@@ -570,9 +570,9 @@ hc_op_diagonal(hc_op_t op, double * d, void * stream)
 namespace hc
 {
 int
-op_residual(hc_op_t op, const double * b, const double * x, double * r, hipStream_t st)
+op_residual(hc_op_t op, const double * b, const double * x, double * r, hipStream_t st, int nrhs)
 {
-  return apply_t<double>(op, x, r, 1, nullptr, st, b);
+  return apply_t<double>(op, x, r, nrhs, nullptr, st, b);
 }
 } // namespace hc
 

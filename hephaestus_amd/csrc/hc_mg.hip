@@ -878,14 +878,8 @@ smooth_k(MgHier & h, MgMulti & mm, int l, bool zero_start, int k, hipStream_t st
     const bool first = zero_start && s == 0;
     if (first)
       HC_TRY_HIP(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    else
-    {
-      if (int rc = hc_op_apply(L.op, x, r, k, st))
-        return rc;
-      HC_TRY_HIP(axpby(n, 1.0, b, -1.0, r, st));
-    }
-    HC_TRY_HIP(aux_precond_apply(L.op->ag, L.op->dinv64, L.op->ginv64, r, L.op->ws.t64, z, st, k, L.N,
-                                 L.op->ws.NNC));
+    else if (int rc = op_residual(L.op, b, x, r, st, k))
+      return rc;
     if (L.cheb > 1.0)
     {
       double a, bb;
@@ -901,12 +895,16 @@ smooth_k(MgHier & h, MgMulti & mm, int l, bool zero_start, int k, hipStream_t st
         bb = rho1 * rho0;
         rho0 = rho1;
       }
-      hipLaunchKernelGGL(k_cheb_update, dim3(grid_for(n)), dim3(256), 0, st, n, a, bb, z, d, x,
-                         (s == 0 ? 1 : 0) | (first ? 2 : 0));
-      HC_TRY_HIP(hipGetLastError());
+      // z = B r and the update in one pass after the node stage
+      HC_TRY_HIP(aux_smooth_step(L.op->ag, L.op->dinv64, L.op->ginv64, r, L.op->ws.t64, a, bb, d, x,
+                                 (s == 0 ? 1 : 0) | (first ? 2 : 0), st, k, L.N, L.op->ws.NNC));
     }
     else
+    {
+      HC_TRY_HIP(aux_precond_apply(L.op->ag, L.op->dinv64, L.op->ginv64, r, L.op->ws.t64, z, st, k, L.N,
+                                   L.op->ws.NNC));
       HC_TRY_HIP(axpby(n, L.omega, z, first ? 0.0 : 1.0, x, st));
+    }
   }
   return HC_OK;
 }

@@ -388,9 +388,9 @@ void geom_for(const hc_op_s * op, const TileCandidate * c, Geom & g, PlaneSegs &
 // allow_graph = false: never capture the V-cycle (callers on worker threads)
 int mg_pcg_solve(hc_op_t op, const double * b, double * x, double rtol, int maxit, hc_solve_stats * stats,
                  hipStream_t st, bool allow_graph = true);
-// r = b - A x (FP64, one right-hand side) with the apply kernels forming the
-// difference as they store (no separate pass over r).
-int op_residual(hc_op_t op, const double * b, const double * x, double * r, hipStream_t st);
+// r = b - A x (FP64, nrhs right-hand sides at stride N) with the apply
+// kernels forming the difference as they store (no separate pass over r).
+int op_residual(hc_op_t op, const double * b, const double * x, double * r, hipStream_t st, int nrhs = 1);
 int mg_levels(hc_op_t op, hipStream_t st, int * nlev, int64_t * ncoarse);
 // k right-hand sides at once (handled = false: hierarchy with an inexact
 // coarsest level, not solved here)
