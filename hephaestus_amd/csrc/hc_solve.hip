@@ -912,7 +912,7 @@ mg_batch_solve(hc_batch_t bt, hc_solve_stats * stats, hipStream_t caller)
           rc = hc::mg_pcg_solve_multi(op, e.b, e.x, e.nrhs, e.rtol, e.maxit, &s, st, handled);
         if (!handled && !rc)
         {
-          // one right-hand side (or an inexact coarsest level): one solve each
+          // one right-hand side: one solve
           s = hc_solve_stats{};
           s.converged = 1;
           for (int r = 0; r < e.nrhs && !rc; r++)

@@ -100,11 +100,12 @@ extern "C" {
  * D^-1 + G Dg^-1 G^T (edge Jacobi plus node Jacobi through the discrete
  * gradient), the transfers of the nested ND spaces, and a dense Cholesky
  * solve on the coarsest level; a coarsest level still above 12000 DOFs (odd
- * element counts at order 1) is solved by PCG with the gradient-space
- * correction to rtol 0.2 instead, under a flexible CG outer iteration, and a
- * mesh that has only such a level (order 1, odd counts, above 12000 DOFs) is
- * refused (HC_EINVAL).  With nrhs > 1 (exact coarsest level) the right-hand
- * sides go through one multi-RHS V-cycle per iteration, each with its own PCG
+ * element counts at order 1) is approximated instead by 16 steps of the same
+ * Chebyshev iteration on [lambda_max / 100, 1.1 lambda_max] (a fixed
+ * polynomial, so the V-cycle stays a fixed SPD preconditioner), and a mesh
+ * that has only such a level (order 1, odd counts, above 12000 DOFs) is
+ * refused (HC_EINVAL).  With nrhs > 1 the right-hand sides go through one
+ * multi-RHS V-cycle per iteration, each with its own PCG
  * scalars and stopping test; the results equal those of one solve each up
  * to round-off of the multi-RHS kernels.  The geometric counterpart, on structured meshes, of the
  * multigrid auxiliary-space preconditioner HypreAMS that

@@ -208,7 +208,7 @@ MIX_MG = [(16, 2, 1e2, 1e-2), (12, 3, 10.0, 0.1), (17, 1, 1.0, 1.0), (9, 3, 50.0
 
 def test_server_multigrid_multilevel_mix(torch_cuda, hc):
     """Multigrid workers on operators with real hierarchies (halved meshes, an
-    order-1 level, an inexact coarsest solve at 12^3 p3 and 9^3 p3, a 17^3
+    order-1 level, a Chebyshev-smoothed coarsest level at 12^3 p3 and 9^3 p3, a 17^3
     order-1 mesh without a hierarchy served by the gradient-space PCG): every
     request converges, several run at once, and each is bitwise the same
     request served alone."""
