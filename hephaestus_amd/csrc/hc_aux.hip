@@ -271,20 +271,75 @@ aux_precond_apply(const AuxGeom & a, const double * dinv, const double * ginv, c
 // M G is exact (G maps into the ND space) and K G = 0, so the mixed and
 // diffusion blocks are M G and G^T M G.
 // ---------------------------------------------------------------------------
+// Row segments of the ND DOFs: one work item per element edge along a
+// component's open axis (the p DOFs that share the p + 1 nodes of that edge),
+// items of a component in the memory order of its DOFs with the open axis
+// counted in elements.
+struct RowItems
+{
+  int64_t ioff[4]; // first item of each component, ioff[3] = N / p
+  int dim[3][2];   // items along the first two axes of each component
+  double rd[3][2]; // their reciprocals (fast_divmod)
+};
+
+static RowItems
+row_items(const AuxGeom & a)
+{
+  RowItems ri{};
+  for (int c = 0; c < 4; c++)
+    ri.ioff[c] = a.off[c] / a.p;
+  for (int c = 0; c < 3; c++)
+    for (int d = 0; d < 2; d++)
+    {
+      ri.dim[c][d] = (d == c) ? a.n[d] : a.cd[c][d];
+      ri.rd[c][d] = 1.0 / (double)ri.dim[c][d];
+    }
+  return ri;
+}
+
+// s = a0 + G v (ADD_A, a0 = a with its PEC DOFs zeroed) or s = G v, one row
+// segment per item: the p + 1 node values of the edge are loaded once for
+// its p DOFs; each DOF sums its p + 1 terms in node order, as aux_g_at does.
+template <bool ADD_A>
 __global__ void
-k_av_gather(Geom g, AuxGeom a, const double * __restrict__ av, const double * __restrict__ v,
-            double * __restrict__ s)
+k_g_rows(Geom g, AuxGeom a, RowItems ri, const double * __restrict__ av, const double * __restrict__ v,
+         double * __restrict__ s)
 {
   with_order(a, [&](auto pc) {
     constexpr int P = decltype(pc)::value;
     const DgTab<P> tab = load_dg<P>(a);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.off[3];
-         i += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ri.ioff[3];
+         t += (int64_t)gridDim.x * blockDim.x)
     {
-      int c, i0, i1, i2;
-      aux_dof_coords(a, i, c, i0, i1, i2);
-      const double a0 = is_ess(g, c, i0, i1, i2) ? 0.0 : av[i];
-      s[i] = a0 + aux_g_at<P>(a, tab, v, c, i0, i1, i2);
+      const int c = (t < ri.ioff[1]) ? 0 : ((t < ri.ioff[2]) ? 1 : 2);
+      int i[3], rest;
+      fast_divmod((int)(t - ri.ioff[c]), ri.dim[c][0], ri.rd[c][0], rest, i[0]);
+      fast_divmod(rest, ri.dim[c][1], ri.rd[c][1], i[2], i[1]);
+      i[c] *= P; // first DOF of the edge, and its first node along c
+      const int64_t d0 = a.off[c] + ((int64_t)i[2] * a.cd[c][1] + i[1]) * a.cd[c][0] + i[0];
+      const int64_t dstride = (c == 0) ? 1 : ((c == 1) ? (int64_t)a.cd[1][0] : (int64_t)a.cd[2][0] * a.cd[2][1]);
+      const int64_t nstride = (c == 0) ? 1 : ((c == 1) ? (int64_t)a.M[0] : (int64_t)a.M[0] * a.M[1]);
+      const double * tl = v + ((int64_t)i[2] * a.M[1] + i[1]) * a.M[0] + i[0];
+      double nv[P + 1];
+#pragma unroll
+      for (int m = 0; m <= P; m++)
+        nv[m] = tl[m * nstride];
+      const bool ess = ADD_A && is_ess(g, c, i[0], i[1], i[2]); // the same for the edge's DOFs
+#pragma unroll
+      for (int l = 0; l < P; l++)
+      {
+        double gv = 0.0;
+#pragma unroll
+        for (int m = 0; m <= P; m++)
+          gv += tab.d[l][m] * nv[m];
+        if constexpr (ADD_A)
+        {
+          const double a0 = ess ? 0.0 : av[d0 + l * dstride];
+          s[d0 + l * dstride] = a0 + gv;
+        }
+        else
+          s[d0 + l * dstride] = gv;
+      }
     }
   });
 }
@@ -315,29 +370,18 @@ k_grad_transpose(AuxGeom a, const double * __restrict__ w, double * __restrict__
   });
 }
 
-__global__ void
-k_grad_apply(AuxGeom a, const double * __restrict__ v, double * __restrict__ y)
-{
-  with_order(a, [&](auto pc) {
-    constexpr int P = decltype(pc)::value;
-    const DgTab<P> tab = load_dg<P>(a);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.off[3];
-         i += (int64_t)gridDim.x * blockDim.x)
-      y[i] = aux_g<P>(a, tab, v, i);
-  });
-}
-
 hipError_t
 grad_apply(const AuxGeom & a, const double * v, double * y, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_grad_apply, dim3(4096), dim3(256), 0, st, a, v, y);
+  Geom g{}; // unused without ADD_A
+  hipLaunchKernelGGL(k_g_rows<false>, dim3(4096), dim3(256), 0, st, g, a, row_items(a), nullptr, v, y);
   return hipGetLastError();
 }
 
 hipError_t
 av_gather(const Geom & g, const AuxGeom & a, const double * av, const double * v, double * s, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_av_gather, dim3(4096), dim3(256), 0, st, g, a, av, v, s);
+  hipLaunchKernelGGL(k_g_rows<true>, dim3(4096), dim3(256), 0, st, g, a, row_items(a), av, v, s);
   return hipGetLastError();
 }
 
