@@ -988,13 +988,16 @@ launch_one(const ApplyDesc & d0, const ApplyDesc * descs, int ndesc, int nwork, 
 #define HC_QP(P, TX, TY, TZ) {P, {TX, TY, TZ, 0}, MODE_QP, &launch_qp<P, TX, TY, TZ>}
 
 static const TileCandidate g_cands[] = {
-    // first entry of each order = default before tuning
+    // first entry of each order = default before tuning (order 3: one qz group,
+    // whose FP64 kernels hold the order tables in vector registers -- the
+    // fastest element-factor tile on box meshes, within a few per cent of the
+    // best with quadrature-point factors)
     HC_CAND(1, 4, 4, 4, 1, 2), HC_CAND(1, 4, 4, 4, 1, 1), HC_CAND(1, 8, 4, 2, 1, 1), HC_CAND(1, 8, 4, 2, 0, 2),
     HC_CAND(1, 8, 8, 1, 0, 1), HC_CAND(1, 16, 2, 2, 1, 1), HC_QP(1, 4, 4, 4), HC_QP(1, 8, 4, 2),
     HC_CAND(2, 4, 2, 2, 1, 1), HC_CAND(2, 4, 2, 2, 1, 2), HC_CAND(2, 4, 2, 2, 0, 2), HC_CAND(2, 4, 4, 2, 1, 1),
     HC_CAND(2, 4, 4, 2, 0, 2), HC_CAND(2, 8, 2, 2, 0, 1), HC_CAND(2, 8, 2, 2, 1, 1), HC_CAND(2, 8, 4, 1, 1, 1),
     HC_QP(2, 4, 2, 2), HC_QP(2, 4, 4, 2),
-    HC_CAND(3, 4, 2, 2, 1, 2), HC_CAND(3, 4, 2, 2, 1, 1), HC_CAND(3, 4, 2, 2, 0, 2), HC_CAND(3, 4, 4, 1, 1, 2),
+    HC_CAND(3, 4, 2, 2, 1, 1), HC_CAND(3, 4, 2, 2, 1, 2), HC_CAND(3, 4, 2, 2, 0, 2), HC_CAND(3, 4, 4, 1, 1, 2),
     HC_CAND(3, 2, 2, 4, 1, 2), HC_CAND(3, 2, 2, 2, 1, 1), HC_CAND(3, 4, 2, 1, 1, 1), HC_CAND(3, 8, 2, 1, 1, 2),
     HC_CAND(3, 8, 1, 2, 1, 2), HC_CAND(3, 8, 2, 1, 1, 1), HC_CAND(3, 4, 2, 1, 1, 2), HC_CAND(3, 2, 2, 2, 1, 2),
 };
