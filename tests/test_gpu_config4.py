@@ -1,13 +1,17 @@
-"""The uneven batch at its full size (needs an MI355X): 64 cubes, edge
-U{8..40} elements, order U{1,2,3}, sigma/dt of the inner region
-[0.25, 0.75]^3 and of the outer region each 10^U[-6,4] (instance i draws from
-splitmix64 with seed 4 + i), coil right-hand sides (seed 1000 + i), 53.6 M
-DOFs in total, solved as one multigrid batch.
+"""The uneven batch and the serving stream at their full size (needs an
+MI355X): 64 cubes, edge U{8..40} elements, order U{1,2,3}, sigma/dt of the
+inner region [0.25, 0.75]^3 and of the outer region each 10^U[-6,4] (instance
+i draws from splitmix64 with seed 4 + i), coil right-hand sides (seed
+1000 + i), 53.6 M DOFs in total; solved as one multigrid batch, and served as
+128 backward-Euler step requests drawn from the 64 operators with Poisson
+arrivals of mean 200 req/s (seed 5) by the multigrid server.
 
 Bars: every instance converges to rtol 1e-8 in its true residual, recomputed
 here by a separate FP64 apply (the apply itself is pinned against the CPU
 oracle by the parity tests); a fixed sample of instances solved alone through
-the batch interface gives bitwise the batch's solution and iteration count.
+the batch interface gives bitwise the batch's solution and iteration count,
+and a fixed sample of requests served alone gives bitwise the result they had
+in the stream.
 """
 import numpy as np
 import pytest
@@ -54,7 +58,19 @@ def _two_region(n, inner, outer):
     return np.where((X & Y & Z).ravel(), inner, outer).astype(np.float64)
 
 
-def test_uneven_64_multigrid_batch(env):
+@pytest.fixture(scope="module")
+def mix(env):
+    torch, H = env
+    ops, bs = [], []
+    for i, (n, p, s_in, s_out) in enumerate(_specs()):
+        op = H.Operator(n, p, 1.0, _two_region(n, s_in, s_out)).setup(1.0)
+        ops.append(op)
+        bs.append(op.rhs_coil(1000 + i))
+    torch.cuda.synchronize()
+    return ops, bs
+
+
+def test_uneven_64_multigrid_batch(env, mix):
     torch, H = env
     specs = _specs()
     sig = [s for (_, _, a, b) in specs for s in (a, b)]
@@ -62,12 +78,7 @@ def test_uneven_64_multigrid_batch(env):
     assert min(sig) < 1e-5 and max(sig) > 1e3
     assert {p for (_, p, _, _) in specs} == {1, 2, 3}
     assert min(n for (n, _, _, _) in specs) <= 10 and max(n for (n, _, _, _) in specs) >= 38
-    ops, bs = [], []
-    for i, (n, p, s_in, s_out) in enumerate(specs):
-        op = H.Operator(n, p, 1.0, _two_region(n, s_in, s_out)).setup(1.0)
-        ops.append(op)
-        bs.append(op.rhs_coil(1000 + i))
-    torch.cuda.synchronize()
+    ops, bs = mix
     print(f"case uneven 64: {sum(o.ndofs for o in ops)} DOFs, multigrid batch", flush=True)
     mode = H.SOLVE_FP64 | H.SOLVE_MG
     xs = [torch.zeros_like(b) for b in bs]
@@ -89,3 +100,46 @@ def test_uneven_64_multigrid_batch(env):
         b1.close()
         assert s1[0]["iterations"] == st[i]["iterations"], (i, s1[0]["iterations"], st[i]["iterations"])
         assert torch.equal(x1, xs[i]), i
+
+
+def _serve(H, torch, reqs, arrivals):
+    srv = H.Server(16 << 30, mode=H.SOLVE_FP64 | H.SOLVE_MG, rtol=RTOL, maxit=5000, round_iters=16)
+    for i, (op, b, u) in enumerate(reqs):
+        srv.submit(op, b, u, 1.0, tag=i, arrival_ms=float(arrivals[i]))
+    res = {}
+    while len(res) < len(reqs):
+        for r in srv.poll(wait_ms=2000):
+            res[r["tag"]] = r
+    info = srv.info()
+    srv.close()
+    torch.cuda.synchronize()
+    return res, info
+
+
+def test_serving_stream_128_multigrid(env, mix):
+    torch, H = env
+    ops, bs = mix
+    nreq = 128
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / 200.0, nreq)) * 1000.0
+    pick = rng.integers(0, len(ops), nreq)
+    us = [torch.zeros_like(bs[k]) for k in pick]
+    print(f"case serving stream: {nreq} requests, Poisson 200 req/s, multigrid server", flush=True)
+    res, info = _serve(H, torch, [(ops[k], bs[k], u) for k, u in zip(pick, us)], arrivals)
+    assert info["completed"] == nreq
+    for i in range(nreq):
+        r = res[i]
+        assert r["converged"] and r["arrival_ms"] >= arrivals[i] - 1e-6, (i, r)
+        # u_n = 0, dt = 1: the served state solves A u = b
+        op, b = ops[pick[i]], bs[pick[i]]
+        rel = float(torch.linalg.norm(b - op.apply(us[i])) / torch.linalg.norm(b))
+        assert rel <= RTOL * (1 + 1e-6), (i, rel)
+    # requests overlapped on the device
+    span = {t: (r["arrival_ms"] + r["queue_ms"], r["arrival_ms"] + r["latency_ms"]) for t, r in res.items()}
+    assert any(span[i][0] < span[j][0] < span[i][1] for i in span for j in span if i != j)
+    print("case serving stream: requests alone vs in the stream", flush=True)
+    for i in range(0, nreq, 16):
+        u1 = torch.zeros_like(bs[pick[i]])
+        r1, _ = _serve(H, torch, [(ops[pick[i]], bs[pick[i]], u1)], [0.0])
+        assert r1[0]["iterations"] == res[i]["iterations"], i
+        assert torch.equal(u1, us[i]), f"request {i} differs alone vs in the stream"
