@@ -4,7 +4,8 @@ inner region [0.25, 0.75]^3 and of the outer region each 10^U[-6,4] (instance
 i draws from splitmix64 with seed 4 + i), coil right-hand sides (seed
 1000 + i), 53.6 M DOFs in total; solved as one multigrid batch, and served as
 128 backward-Euler step requests drawn from the 64 operators with Poisson
-arrivals of mean 200 req/s (seed 5) by the multigrid server.
+arrivals of mean 200 req/s (seed 5), by the multigrid server and by the
+FP32 PCG + FP64 refinement server with the gradient-space correction.
 
 Bars: every instance converges to rtol 1e-8 in its true residual, recomputed
 here by a separate FP64 apply (the apply itself is pinned against the CPU
@@ -102,8 +103,8 @@ def test_uneven_64_multigrid_batch(env, mix):
         assert torch.equal(x1, xs[i]), i
 
 
-def _serve(H, torch, reqs, arrivals):
-    srv = H.Server(16 << 30, mode=H.SOLVE_FP64 | H.SOLVE_MG, rtol=RTOL, maxit=5000, round_iters=16)
+def _serve(H, torch, reqs, arrivals, mode):
+    srv = H.Server(16 << 30, mode=mode, rtol=RTOL, maxit=20000, round_iters=16)
     for i, (op, b, u) in enumerate(reqs):
         srv.submit(op, b, u, 1.0, tag=i, arrival_ms=float(arrivals[i]))
     res = {}
@@ -116,16 +117,18 @@ def _serve(H, torch, reqs, arrivals):
     return res, info
 
 
-def test_serving_stream_128_multigrid(env, mix):
+@pytest.mark.parametrize("mode", ["fp64_mg", "fp32_ir_aux"])
+def test_serving_stream_128(env, mix, mode):
     torch, H = env
     ops, bs = mix
+    md = H.SOLVE_FP64 | H.SOLVE_MG if mode == "fp64_mg" else H.SOLVE_FP32_IR | H.SOLVE_AUX
     nreq = 128
     rng = np.random.default_rng(5)
     arrivals = np.cumsum(rng.exponential(1.0 / 200.0, nreq)) * 1000.0
     pick = rng.integers(0, len(ops), nreq)
     us = [torch.zeros_like(bs[k]) for k in pick]
-    print(f"case serving stream: {nreq} requests, Poisson 200 req/s, multigrid server", flush=True)
-    res, info = _serve(H, torch, [(ops[k], bs[k], u) for k, u in zip(pick, us)], arrivals)
+    print(f"case serving stream {mode}: {nreq} requests, Poisson 200 req/s", flush=True)
+    res, info = _serve(H, torch, [(ops[k], bs[k], u) for k, u in zip(pick, us)], arrivals, md)
     assert info["completed"] == nreq
     for i in range(nreq):
         r = res[i]
@@ -140,6 +143,6 @@ def test_serving_stream_128_multigrid(env, mix):
     print("case serving stream: requests alone vs in the stream", flush=True)
     for i in range(0, nreq, 16):
         u1 = torch.zeros_like(bs[pick[i]])
-        r1, _ = _serve(H, torch, [(ops[pick[i]], bs[pick[i]], u1)], [0.0])
+        r1, _ = _serve(H, torch, [(ops[pick[i]], bs[pick[i]], u1)], [0.0], md)
         assert r1[0]["iterations"] == res[i]["iterations"], i
         assert torch.equal(u1, us[i]), f"request {i} differs alone vs in the stream"
