@@ -1239,7 +1239,7 @@ mg_pcg_solve(hc_op_t op, const double * b, double * x, double rtol, int maxit, h
   double bb, dummy;
   HC_TRY_HIP(dot2(h, b, b, N, dummy, bb, st));
   const double tol2 = rtol * rtol * bb;
-  int iters = 0, outer = 0, status = HC_OK;
+  int iters = 0, outer = 0, passes = 0, status = HC_OK; // passes: PCG runs from a true residual
   bool conv = false;
   double rr = 0.0;
   // the V-cycle reads the finest level's b and writes its x: point them at
@@ -1257,6 +1257,7 @@ mg_pcg_solve(hc_op_t op, const double * b, double * x, double rtol, int maxit, h
       conv = true;
       break;
     }
+    passes++;
     if (int rc = precond(r, zz))
       return rc;
     HC_TRY_HIP(hipMemcpyAsync(pv, zz, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
@@ -1309,7 +1310,7 @@ mg_pcg_solve(hc_op_t op, const double * b, double * x, double rtol, int maxit, h
   {
     hc_solve_stats s{};
     s.iterations = iters;
-    s.outer_iterations = std::min(outer, 1 + MAX_RESTARTS);
+    s.outer_iterations = passes;
     s.converged = conv ? 1 : 0;
     s.status = status != HC_OK ? status : (conv ? HC_OK : HC_ENOTCONV);
     s.rel_res = bb > 0 ? std::sqrt(rr / bb) : 0.0;
@@ -1382,8 +1383,8 @@ mg_pcg_solve_multi(hc_op_t op, const double * B, double * X, int k, double rtol,
     return HC_OK;
   };
   const dim3 vg(std::max(1, std::min(grid_for(N), 2048 / std::max(1, k / 4))), k);
-  int outer = 0;
-  for (outer = 1; outer <= 1 + MAX_RESTARTS; outer++)
+  int passes = 0; // PCG runs from a true residual
+  for (int outer = 1; outer <= 1 + MAX_RESTARTS; outer++)
   {
     if (int rc = resid())
       return rc;
@@ -1396,6 +1397,7 @@ mg_pcg_solve_multi(hc_op_t op, const double * B, double * X, int k, double rtol,
     }
     if (!any)
       break;
+    passes++;
     if (int rc = precond())
       return rc;
     HC_TRY_HIP(dot2_k(mm, R, Z, N, k, gam, dummy, st));
@@ -1470,7 +1472,7 @@ mg_pcg_solve_multi(hc_op_t op, const double * B, double * X, int k, double rtol,
     if (stat[r] && !status)
       status = stat[r];
   }
-  all.outer_iterations = std::min(outer, 1 + MAX_RESTARTS);
+  all.outer_iterations = passes;
   all.status = status ? status : (all.converged ? HC_OK : HC_ENOTCONV);
   all.solve_ms = ms;
   if (stats)
