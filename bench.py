@@ -60,7 +60,7 @@ def measured_traffic(n, p):
         return None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return int(float(line.split(":", 1)[1]))
+            return int(float(line.split(":", 1)[1].split()[0]))
     return None
 
 
