@@ -306,6 +306,14 @@ k_identity_cols(double * __restrict__ e, int64_t n, int j0, int nb)
   }
 }
 
+// A_jj *= 1 + delta (diagonal boost of a dense n x n matrix)
+__global__ void
+k_boost_diag(double * __restrict__ A, int64_t n, double delta)
+{
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+    A[j * n + j] *= 1.0 + delta;
+}
+
 int
 grid_for(int64_t n)
 {
@@ -1178,26 +1186,44 @@ mg_build(hc_op_t op, hipStream_t st)
     DevBlock eblk;
     if (int rc = eblk.alloc(sizeof(double) * n * nb))
       return rc;
-    for (int j0 = 0; j0 < n; j0 += nb)
-    {
-      const int k = (int)std::min<int64_t>(nb, n - j0);
-      hipLaunchKernelGGL(k_identity_cols, dim3(grid_for(n * k)), dim3(256), 0, st, (double *)eblk.base, n, j0, k);
-      HC_TRY_HIP(hipGetLastError());
-      if (int rc = hc_op_apply(C.op, (const double *)eblk.base, h->Ainv + (int64_t)j0 * n, k, st))
-        return rc;
-    }
     if (rocblas_create_handle(&h->blas) != rocblas_status_success)
       return fail(HC_EHIP, "multigrid: rocblas_create_handle failed");
     if (rocblas_set_stream(h->blas, st) != rocblas_status_success)
       return fail(HC_EHIP, "multigrid: rocblas_set_stream failed");
-    if (rocsolver_dpotrf(h->blas, rocblas_fill_lower, (rocblas_int)n, h->Ainv, (rocblas_int)n, info) !=
-            rocblas_status_success ||
-        rocsolver_dpotri(h->blas, rocblas_fill_lower, (rocblas_int)n, h->Ainv, (rocblas_int)n, info + 1) !=
-            rocblas_status_success)
-      return fail(HC_EHIP, "multigrid: rocsolver Cholesky of the coarsest level failed");
+    // A coarsest level with a tiny sigma (a gradient-dominated null space
+    // near h^3 sigma against a curl part near 1 / h) is SPD but can be so
+    // ill-conditioned that the rounding of its probed columns (which depends
+    // on the level's tile variant) leaves a Cholesky pivot negative; then
+    // the probe is repeated with the diagonal raised by a relative 1e-12,
+    // 1e-10, 1e-8 (the coarsest solve is part of a preconditioner).  The
+    // first attempt is the plain matrix.
+    constexpr double BOOST[4] = {0.0, 1e-12, 1e-10, 1e-8};
     int hinfo[2] = {0, 0};
-    HC_TRY_HIP(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
-    HC_TRY_HIP(hipStreamSynchronize(st));
+    for (int attempt = 0; attempt < 4; attempt++)
+    {
+      for (int j0 = 0; j0 < n; j0 += nb)
+      {
+        const int k = (int)std::min<int64_t>(nb, n - j0);
+        hipLaunchKernelGGL(k_identity_cols, dim3(grid_for(n * k)), dim3(256), 0, st, (double *)eblk.base, n, j0, k);
+        HC_TRY_HIP(hipGetLastError());
+        if (int rc = hc_op_apply(C.op, (const double *)eblk.base, h->Ainv + (int64_t)j0 * n, k, st))
+          return rc;
+      }
+      if (BOOST[attempt] > 0.0)
+      {
+        hipLaunchKernelGGL(k_boost_diag, dim3(grid_for(n)), dim3(256), 0, st, h->Ainv, n, BOOST[attempt]);
+        HC_TRY_HIP(hipGetLastError());
+      }
+      if (rocsolver_dpotrf(h->blas, rocblas_fill_lower, (rocblas_int)n, h->Ainv, (rocblas_int)n, info) !=
+              rocblas_status_success ||
+          rocsolver_dpotri(h->blas, rocblas_fill_lower, (rocblas_int)n, h->Ainv, (rocblas_int)n, info + 1) !=
+              rocblas_status_success)
+        return fail(HC_EHIP, "multigrid: rocsolver Cholesky of the coarsest level failed");
+      HC_TRY_HIP(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
+      HC_TRY_HIP(hipStreamSynchronize(st));
+      if (!hinfo[0] && !hinfo[1])
+        break;
+    }
     if (hinfo[0] || hinfo[1])
       return fail(HC_EHIP, "multigrid: coarsest-level matrix is not positive definite (info " +
                                std::to_string(hinfo[0]) + ", " + std::to_string(hinfo[1]) + ")");
