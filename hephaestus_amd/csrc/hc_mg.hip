@@ -1356,9 +1356,9 @@ mg_pcg_solve(hc_op_t op, const double * b, double * x, double rtol, int maxit, h
 // vectors (the operator applies as one multi-RHS launch), each right-hand
 // side keeps its own PCG scalars and stops on its own (its later updates are
 // zero), with the stopping rule of the one-RHS solve.  Each right-hand side's
-// arithmetic is that of the same solve alone.  Returns HC_EINVAL through
-// `handled` = false when the hierarchy has an inexact coarsest level (the
-// caller then solves the right-hand sides one at a time).
+// arithmetic is that of the same solve alone.  `handled` stays false when
+// the hierarchy cannot be built (the returned error; the caller then solves
+// the right-hand sides another way).
 int
 mg_pcg_solve_multi(hc_op_t op, const double * B, double * X, int k, double rtol, int maxit, hc_solve_stats * stats,
                    hipStream_t st, bool & handled)
