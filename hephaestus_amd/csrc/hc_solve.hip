@@ -954,8 +954,8 @@ mg_batch_solve(hc_batch_t bt, hc_solve_stats * stats, hipStream_t caller)
     (void)hipStreamDestroy(st);
     cv.notify_all();
   };
-  std::vector<std::thread> pool;
-  for (int w = 0; w < MG_BATCH_WORKERS; w++)
+  std::vector<std::thread> pool; // no more workers (and streams) than tasks
+  for (size_t w = 0; w < std::min<size_t>(MG_BATCH_WORKERS, ntask); w++)
     pool.emplace_back(worker);
   for (auto & t : pool)
     t.join();
