@@ -581,14 +581,27 @@ hc_server_create(const hc_server_config * cfg, hc_server_t * out)
   }
   s->t0 = Clock::now();
   s->mg = (cfg->mode & HC_SOLVE_MG) != 0;
-  if (s->mg)
+  // (no C++ exception may cross the C ABI: a thread that cannot be started
+  // is reported as an error; with some multigrid workers running, they serve)
+  try
   {
-    const int nw = cfg->max_active > 0 ? cfg->max_active : MG_WORKERS_DEFAULT;
-    for (int i = 0; i < nw; i++)
-      s->workers.emplace_back([s] { s->mg_worker(); });
+    if (s->mg)
+    {
+      const int nw = cfg->max_active > 0 ? cfg->max_active : MG_WORKERS_DEFAULT;
+      for (int i = 0; i < nw; i++)
+        s->workers.emplace_back([s] { s->mg_worker(); });
+    }
+    else
+      s->th = std::thread([s] { s->loop(); });
   }
-  else
-    s->th = std::thread([s] { s->loop(); });
+  catch (const std::exception & ex)
+  {
+    if (!s->mg || s->workers.empty())
+    {
+      hc_server_destroy(s);
+      return hc::fail(HC_EHIP, std::string("hc_server_create: cannot start a thread: ") + ex.what());
+    }
+  }
   *out = s;
   return HC_OK;
 }

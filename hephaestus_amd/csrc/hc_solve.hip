@@ -954,9 +954,36 @@ mg_batch_solve(hc_batch_t bt, hc_solve_stats * stats, hipStream_t caller)
     (void)hipStreamDestroy(st);
     cv.notify_all();
   };
+  // no C++ exception may cross the C ABI: a worker body reports what it
+  // throws as an error, and when no thread can be started the calling thread
+  // runs the worker itself
+  auto guarded = [&]() {
+    try
+    {
+      worker();
+    }
+    catch (const std::exception & ex)
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (!err)
+      {
+        err = hc::fail(HC_EHIP, std::string("hc_batch_solve: ") + ex.what());
+        err_msg = hc_last_error();
+      }
+      cv.notify_all();
+    }
+  };
   std::vector<std::thread> pool; // no more workers (and streams) than tasks
-  for (size_t w = 0; w < std::min<size_t>(MG_BATCH_WORKERS, ntask); w++)
-    pool.emplace_back(worker);
+  try
+  {
+    for (size_t w = 0; w < std::min<size_t>(MG_BATCH_WORKERS, ntask); w++)
+      pool.emplace_back(guarded);
+  }
+  catch (const std::exception &)
+  {
+  }
+  if (pool.empty() && ntask > 0)
+    guarded();
   for (auto & t : pool)
     t.join();
   if (err)
