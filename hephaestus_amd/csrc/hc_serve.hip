@@ -533,6 +533,28 @@ hc_server_s::mg_worker()
   cv_work.notify_all();
 }
 
+// Runs a server thread's body; what it throws becomes the server's error
+// (the thread ends; polls then report the error).
+template <typename F>
+static void
+guarded(hc_server_t s, F && body)
+{
+  try
+  {
+    body();
+  }
+  catch (const std::exception & ex)
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->error)
+    {
+      s->error = hc::fail(HC_EHIP, std::string("server thread: ") + ex.what());
+      s->error_msg = hc_last_error();
+    }
+    s->cv_done.notify_all();
+  }
+}
+
 extern "C" {
 
 int64_t
@@ -589,10 +611,10 @@ hc_server_create(const hc_server_config * cfg, hc_server_t * out)
     {
       const int nw = cfg->max_active > 0 ? cfg->max_active : MG_WORKERS_DEFAULT;
       for (int i = 0; i < nw; i++)
-        s->workers.emplace_back([s] { s->mg_worker(); });
+        s->workers.emplace_back([s] { guarded(s, [s] { s->mg_worker(); }); });
     }
     else
-      s->th = std::thread([s] { s->loop(); });
+      s->th = std::thread([s] { guarded(s, [s] { s->loop(); }); });
   }
   catch (const std::exception & ex)
   {
