@@ -98,6 +98,7 @@ using hc::AV_THREADS;
 extern "C" int
 hc_op_div_free_source(hc_op_t op, const double * j, double * src, double * lf, double * Q, int project,
                       double rtol, int maxit, hc_solve_stats * stats, void * stream)
+try
 {
   if (!op || !j || !src || !lf)
     return hc::fail(HC_EINVAL, "hc_op_div_free_source: null argument");
@@ -253,10 +254,15 @@ hc_op_div_free_source(hc_op_t op, const double * j, double * src, double * lf, d
                                               : "hc_op_div_free_source: tolerance not reached in maxit");
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 extern "C" int
 hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a, double * v, double rtol, int maxit,
                hc_solve_stats * stats, void * stream)
+try
 {
   if (!op || !ba || !bv || !a || !v)
     return hc::fail(HC_EINVAL, "hc_op_av_solve: null argument");
@@ -348,4 +354,8 @@ hc_op_av_solve(hc_op_t op, const double * ba, const double * bv, double * a, dou
     return hc::fail(status, status == HC_ENAN ? "hc_op_av_solve: NaN/Inf in a reduction"
                                                      : "hc_op_av_solve: tolerance not reached in maxit");
   return HC_OK;
+}
+catch (...)
+{
+  return hc::fail_exception();
 }

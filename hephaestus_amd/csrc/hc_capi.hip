@@ -32,6 +32,27 @@ last_error()
   return g_last_error.c_str();
 }
 
+int
+fail_exception()
+{
+  try
+  {
+    throw;
+  }
+  catch (const std::bad_alloc &)
+  {
+    return fail(HC_ENOMEM, "host memory allocation failed");
+  }
+  catch (const std::exception & ex)
+  {
+    return fail(HC_EHIP, std::string("unexpected error: ") + ex.what());
+  }
+  catch (...)
+  {
+    return fail(HC_EHIP, "unexpected error");
+  }
+}
+
 } // namespace hc
 
 using hc::fail;
@@ -178,6 +199,7 @@ hc_last_error(void)
 int
 hc_op_create(const hc_mesh_desc * mesh, int order, const double * nu_e, const double * sigma_e, int precision,
              hc_op_t * out)
+try
 {
   if (!mesh || !out || !nu_e || !sigma_e)
     return fail(HC_EINVAL, "hc_op_create: null argument");
@@ -330,9 +352,14 @@ hc_op_create(const hc_mesh_desc * mesh, int order, const double * nu_e, const do
   *out = op;
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_set_coefficients(hc_op_t op, const double * nu_e, const double * sigma_e)
+try
 {
   if (!op || !nu_e || !sigma_e)
     return fail(HC_EINVAL, "hc_op_set_coefficients: null argument");
@@ -343,9 +370,14 @@ hc_op_set_coefficients(hc_op_t op, const double * nu_e, const double * sigma_e)
   op->coef_dirty = true;
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_setup(hc_op_t op, double dt, void * stream)
+try
 {
   if (!op)
     return fail(HC_EINVAL, "hc_op_setup: null handle");
@@ -388,6 +420,10 @@ hc_op_setup(hc_op_t op, double dt, void * stream)
   op->setup_version++; // a multigrid hierarchy of the old setup is rebuilt on its next use
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int64_t
 hc_op_ndofs(hc_op_t op)
@@ -397,11 +433,16 @@ hc_op_ndofs(hc_op_t op)
 
 int
 hc_op_ess_count(hc_op_t op, int64_t * n_ess)
+try
 {
   if (!op || !n_ess)
     return fail(HC_EINVAL, "hc_op_ess_count: null argument");
   *n_ess = op->n_ess;
   return HC_OK;
+}
+catch (...)
+{
+  return hc::fail_exception();
 }
 
 int64_t
@@ -412,6 +453,7 @@ hc_op_device_bytes(hc_op_t op)
 
 int
 hc_op_apply(hc_op_t op, const double * x, double * y, int nrhs, void * stream)
+try
 {
   if (!op || !x || !y)
     return fail(HC_EINVAL, "hc_op_apply: null argument");
@@ -421,9 +463,14 @@ hc_op_apply(hc_op_t op, const double * x, double * y, int nrhs, void * stream)
     return fail(HC_EINVAL, "hc_op_apply: nrhs must be >= 1");
   return apply_t<double>(op, x, y, nrhs, nullptr, (hipStream_t)stream);
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_apply_f32(hc_op_t op, const float * x, float * y, int nrhs, void * stream)
+try
 {
   if (!op || !x || !y)
     return fail(HC_EINVAL, "hc_op_apply_f32: null argument");
@@ -435,9 +482,14 @@ hc_op_apply_f32(hc_op_t op, const float * x, float * y, int nrhs, void * stream)
     return fail(HC_EINVAL, "hc_op_apply_f32: nrhs must be >= 1");
   return apply_t<float>(op, x, y, nrhs, nullptr, (hipStream_t)stream);
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_apply_curl(hc_op_t op, const double * x, double * y, void * stream)
+try
 {
   if (!op || !x || !y)
     return fail(HC_EINVAL, "hc_op_apply_curl: null argument");
@@ -450,6 +502,10 @@ hc_op_apply_curl(hc_op_t op, const double * x, double * y, void * stream)
   HC_TRY_HIP(hc::apply_scaled<double>(op->plan64.g, op->plan64.segs, op->tb64, op->qd64, op->ef(false), x, y, op->ws.part64, 0.0,
                                       1.0 / op->last_dt, op->plan64.cand, op->desc_dev, (hipStream_t)stream));
   return HC_OK;
+}
+catch (...)
+{
+  return hc::fail_exception();
 }
 
 int64_t
@@ -465,6 +521,7 @@ hc_op_nnodes(hc_op_t op)
 
 int
 hc_op_av_apply(hc_op_t op, const double * a, const double * v, double * ya, double * yv, void * stream)
+try
 {
   if (!op || !a || !v || !ya || !yv)
     return fail(HC_EINVAL, "hc_op_av_apply: null argument");
@@ -494,35 +551,55 @@ hc_op_av_apply(hc_op_t op, const double * a, const double * v, double * ya, doub
   HC_TRY_HIP(hc::av_finish(pl.g, op->ag, a, k, ya, st));
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_set_rhs_block(hc_op_t op, int rhs_block)
+try
 {
   if (!op || rhs_block < 0)
     return fail(HC_EINVAL, "hc_op_set_rhs_block: bad argument");
   op->rhs_block = rhs_block;
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_set_factor_mode(hc_op_t op, int mode)
+try
 {
   if (!op || (mode != 0 && mode != 1))
     return fail(HC_EINVAL, "hc_op_set_factor_mode: null handle or mode other than 0 and 1");
   op->ef_mode = mode;
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_factor_mode(hc_op_t op)
+try
 {
   if (!op)
     return fail(HC_EINVAL, "hc_op_factor_mode: null handle");
   return op->ef(false).qc ? 1 : 0;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_term_ratio(hc_op_t op, double * ratio)
+try
 {
   if (!op || !ratio)
     return fail(HC_EINVAL, "hc_op_term_ratio: null argument");
@@ -531,9 +608,14 @@ hc_op_term_ratio(hc_op_t op, double * ratio)
   *ratio = op->term_ratio;
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_precond_apply(hc_op_t op, int mode, const double * r, double * z, void * stream)
+try
 {
   if (!op || !r || !z)
     return fail(HC_EINVAL, "hc_op_precond_apply: null argument");
@@ -553,9 +635,14 @@ hc_op_precond_apply(hc_op_t op, int mode, const double * r, double * z, void * s
   HC_TRY_HIP(hc::aux_precond_apply(op->ag, op->dinv64, op->ginv64, r, op->ws.t64, z, st));
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_diagonal(hc_op_t op, double * d, void * stream)
+try
 {
   if (!op || !d)
     return fail(HC_EINVAL, "hc_op_diagonal: null argument");
@@ -563,6 +650,10 @@ hc_op_diagonal(hc_op_t op, double * d, void * stream)
     return fail(HC_EINVAL, "hc_op_diagonal: hc_op_setup has not been called");
   HC_TRY_HIP(hipMemcpyAsync(d, op->diag, op->N * sizeof(double), hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return HC_OK;
+}
+catch (...)
+{
+  return hc::fail_exception();
 }
 
 } // extern "C"
@@ -595,6 +686,7 @@ extern "C" {
 
 int
 hc_tile_list(int order, int * out, int max)
+try
 {
   const hc::TileCandidate * c;
   const int n = hc::tile_candidates(order, &c);
@@ -607,9 +699,14 @@ hc_tile_list(int order, int * out, int max)
   }
   return n;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_set_tile(hc_op_t op, int prec_bits, int tx, int ty, int tz, int prefetch)
+try
 {
   if (!op || (prec_bits != 64 && prec_bits != 32))
     return fail(HC_EINVAL, "hc_op_set_tile: bad handle or precision (64 or 32)");
@@ -620,9 +717,14 @@ hc_op_set_tile(hc_op_t op, int prec_bits, int tx, int ty, int tz, int prefetch)
   set_plan(op, prec_bits == 64 ? op->plan64 : op->plan32, c);
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_get_tile(hc_op_t op, int prec_bits, int * tx, int * ty, int * tz, int * prefetch)
+try
 {
   if (!op || (prec_bits != 64 && prec_bits != 32) || !tx || !ty || !tz || !prefetch)
     return fail(HC_EINVAL, "hc_op_get_tile: bad argument");
@@ -633,9 +735,14 @@ hc_op_get_tile(hc_op_t op, int prec_bits, int * tx, int * ty, int * tz, int * pr
   *prefetch = pl.cand->mode;
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_autotune(hc_op_t op, int reps, double * best_us, void * stream)
+try
 {
   if (!op)
     return fail(HC_EINVAL, "hc_op_autotune: null handle");
@@ -695,9 +802,14 @@ hc_op_autotune(hc_op_t op, int reps, double * best_us, void * stream)
   (void)hipEventDestroy(e1);
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_destroy(hc_op_t op)
+try
 {
   if (!op)
     return fail(HC_EINVAL, "hc_op_destroy: null handle");
@@ -705,9 +817,14 @@ hc_op_destroy(hc_op_t op)
   delete op;
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_profile_apply(hc_op_t op, int prec_bits, const void * x, void * y, int reps, double * out_us, void * stream)
+try
 {
   if (!op || !x || !y || !out_us || reps < 1 || (prec_bits != 64 && prec_bits != 32))
     return fail(HC_EINVAL, "hc_op_profile_apply: bad argument");
@@ -762,9 +879,14 @@ hc_op_profile_apply(hc_op_t op, int prec_bits, const void * x, void * y, int rep
   out_us[1] = 1000.0 * t_plane / reps;
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_rhs_coil(hc_op_t op, uint64_t seed, double * b, void * stream)
+try
 {
   if (!op || !b)
     return fail(HC_EINVAL, "hc_op_rhs_coil: null argument");
@@ -803,27 +925,42 @@ hc_op_rhs_coil(hc_op_t op, uint64_t seed, double * b, void * stream)
   HC_TRY_HIP(hc::rhs_coil(op->g, cp, op->d_vert, op->h, op->x0, b, (hipStream_t)stream));
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_axpby(int64_t n, double a, const double * x, double b, double * y, void * stream)
+try
 {
   if (n < 0 || (n > 0 && (!x || !y)))
     return fail(HC_EINVAL, "hc_axpby: bad argument");
   HC_TRY_HIP(hc::axpby(n, a, x, b, y, (hipStream_t)stream));
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_op_zero_ess(hc_op_t op, double * v, void * stream)
+try
 {
   if (!op || !v)
     return fail(HC_EINVAL, "hc_op_zero_ess: null argument");
   HC_TRY_HIP(hc::zero_ess(op->g, v, (hipStream_t)stream));
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_fill_uniform(hc_op_t op, uint64_t seed, double * v, int64_t n, void * stream)
+try
 {
   if (!v || n < 0)
     return fail(HC_EINVAL, "hc_fill_uniform: bad argument");
@@ -831,6 +968,10 @@ hc_fill_uniform(hc_op_t op, uint64_t seed, double * v, int64_t n, void * stream)
     return fail(HC_EINVAL, "hc_fill_uniform: length differs from the operator's DOF count");
   HC_TRY_HIP(hc::fill_uniform(op ? &op->g : nullptr, seed, v, n, (hipStream_t)stream));
   return HC_OK;
+}
+catch (...)
+{
+  return hc::fail_exception();
 }
 
 } // extern "C"

@@ -1878,13 +1878,19 @@ extern "C" {
 
 int
 hc_mg_set_graphs(int enable)
+try
 {
   hc::g_mg_graphs.store(enable ? 1 : 0);
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_mg_levels(hc_op_t op, int * nlevels, int64_t * coarse_ndofs, void * stream)
+try
 {
   if (!op || !nlevels || !coarse_ndofs)
     return hc::fail(HC_EINVAL, "hc_mg_levels: null argument");
@@ -1892,9 +1898,14 @@ hc_mg_levels(hc_op_t op, int * nlevels, int64_t * coarse_ndofs, void * stream)
     return hc::fail(HC_EINVAL, "hc_mg_levels: hc_op_setup has not been called");
   return hc::mg_levels(op, (hipStream_t)stream, nlevels, coarse_ndofs);
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_mg_level_op(hc_op_t op, int level, hc_op_t * level_op)
+try
 {
   if (!op || !level_op)
     return hc::fail(HC_EINVAL, "hc_mg_level_op: null argument");
@@ -1905,9 +1916,14 @@ hc_mg_level_op(hc_op_t op, int level, hc_op_t * level_op)
   *level_op = op->mg->lev[level].op;
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_mg_transfer(hc_op_t op, int level, int dir, const double * x, double * y, void * stream)
+try
 {
   if (!op || !x || !y)
     return hc::fail(HC_EINVAL, "hc_mg_transfer: null argument");
@@ -1922,6 +1938,10 @@ hc_mg_transfer(hc_op_t op, int level, int dir, const double * x, double * y, voi
     HC_TRY_HIP(hipMemsetAsync(y, 0, sizeof(double) * op->mg->lev[level].N, st));
   HC_TRY_HIP(hc::transfer(*op->mg, level, dir == 1, x, y, st));
   return HC_OK;
+}
+catch (...)
+{
+  return hc::fail_exception();
 }
 
 } // extern "C"

@@ -90,6 +90,7 @@ hc_noalloc_scratch_doubles(void)
 
 int
 hc_op_apply_noalloc(hc_op_t op, const double * x, double * y, void * stream)
+try
 {
   if (!op || !x || !y)
     return hc::fail(HC_EINVAL, "hc_op_apply_noalloc: null argument");
@@ -124,6 +125,10 @@ hc_op_apply_noalloc(hc_op_t op, const double * x, double * y, void * stream)
   HC_TRY_HIP(pl.cand->launch(d, nullptr, 1, d.ntiles, 0, &op->tb64, 64, st));
   HC_TRY_HIP(hc::assemble_planes<double>(d, nullptr, 1, d.prows, st));
   return HC_OK;
+}
+catch (...)
+{
+  return hc::fail_exception();
 }
 
 } // extern "C"

@@ -12,6 +12,9 @@ namespace hc
 {
 
 int fail(int code, const std::string & msg);
+// The error code of a C++ exception caught at the C ABI (HC_ENOMEM for an
+// allocation failure, else HC_EHIP), with its message as the last error.
+int fail_exception();
 
 #define HC_TRY_HIP(expr)                                                                                        \
   do                                                                                                             \

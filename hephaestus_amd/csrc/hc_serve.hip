@@ -569,6 +569,7 @@ hc_server_request_bytes(int64_t ndofs, int mode)
 
 int
 hc_server_create(const hc_server_config * cfg, hc_server_t * out)
+try
 {
   if (!cfg || !out)
     return hc::fail(HC_EINVAL, "hc_server_create: null argument");
@@ -627,9 +628,14 @@ hc_server_create(const hc_server_config * cfg, hc_server_t * out)
   *out = s;
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_server_set_tile(hc_server_t s, int order, int prec_bits, int tx, int ty, int tz, int mode)
+try
 {
   if (!s || order < 1 || order > hc::PMAX || (prec_bits != 64 && prec_bits != 32))
     return hc::fail(HC_EINVAL, "hc_server_set_tile: bad argument");
@@ -646,10 +652,15 @@ hc_server_set_tile(hc_server_t s, int order, int prec_bits, int tx, int ty, int 
     }
   return hc::fail(HC_EINVAL, "hc_server_set_tile: no compiled tile variant with this shape");
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_server_submit(hc_server_t s, hc_op_t op, const double * b_src, double * u, double dt, uint64_t tag,
                  double arrival_ms)
+try
 {
   if (!s || !op || !b_src || !u)
     return hc::fail(HC_EINVAL, "hc_server_submit: null argument");
@@ -689,9 +700,14 @@ hc_server_submit(hc_server_t s, hc_op_t op, const double * b_src, double * u, do
   s->cv_work.notify_all();
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_server_poll(hc_server_t s, hc_request_result * out, int max, int wait_ms)
+try
 {
   if (!s || (max > 0 && !out))
     return hc::fail(HC_EINVAL, "hc_server_poll: bad argument");
@@ -709,9 +725,14 @@ hc_server_poll(hc_server_t s, hc_request_result * out, int max, int wait_ms)
   }
   return n;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_server_info(hc_server_t s, hc_server_stats * st)
+try
 {
   if (!s || !st)
     return hc::fail(HC_EINVAL, "hc_server_info: null argument");
@@ -727,9 +748,14 @@ hc_server_info(hc_server_t s, hc_server_stats * st)
   st->elapsed_ms = ms_between(s->t0, Clock::now());
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_server_destroy(hc_server_t s)
+try
 {
   if (!s)
     return hc::fail(HC_EINVAL, "hc_server_destroy: null handle");
@@ -757,6 +783,10 @@ hc_server_destroy(hc_server_t s)
   (void)hipStreamDestroy(s->st);
   delete s;
   return HC_OK;
+}
+catch (...)
+{
+  return hc::fail_exception();
 }
 
 } // extern "C"

@@ -753,14 +753,20 @@ extern "C" {
 
 int
 hc_engine_set_graphs(int enable)
+try
 {
   hc::g_graphs.store(enable ? 1 : 0);
   return HC_OK;
+}
+catch (...)
+{
+  return hc::fail_exception();
 }
 
 int
 hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, int maxit, int mode,
              hc_solve_stats * stats, void * stream)
+try
 {
   if (!op || !b || !x)
     return hc::fail(HC_EINVAL, "hc_pcg_solve: null argument");
@@ -832,6 +838,10 @@ hc_pcg_solve(hc_op_t op, const double * b, double * x, int nrhs, double rtol, in
   if (stats)
     *stats = out;
   return status_rc(out, "hc_pcg_solve");
+}
+catch (...)
+{
+  return hc::fail_exception();
 }
 
 // HC_SOLVE_MG batches: the multigrid PCG of every entry (its right-hand sides
@@ -1010,15 +1020,21 @@ mg_batch_solve(hc_batch_t bt, hc_solve_stats * stats, hipStream_t caller)
 
 int
 hc_batch_create(hc_batch_t * out)
+try
 {
   if (!out)
     return hc::fail(HC_EINVAL, "hc_batch_create: null argument");
   *out = new hc_batch_s();
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_batch_add(hc_batch_t bt, hc_op_t op, const double * b, double * x, int nrhs, double rtol, int maxit)
+try
 {
   if (!bt || !op || !b || !x)
     return hc::fail(HC_EINVAL, "hc_batch_add: null argument");
@@ -1030,9 +1046,14 @@ hc_batch_add(hc_batch_t bt, hc_op_t op, const double * b, double * x, int nrhs, 
   bt->dirty = true;
   return (int)bt->entries.size() - 1;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_batch_set_rhs_block(hc_batch_t bt, int rb)
+try
 {
   if (!bt || rb < 0)
     return hc::fail(HC_EINVAL, "hc_batch_set_rhs_block: bad argument");
@@ -1040,9 +1061,14 @@ hc_batch_set_rhs_block(hc_batch_t bt, int rb)
   bt->dirty = true;
   return HC_OK;
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_batch_solve(hc_batch_t bt, int mode, hc_solve_stats * stats, void * stream)
+try
 {
   if (!bt)
     return hc::fail(HC_EINVAL, "hc_batch_solve: null handle");
@@ -1124,15 +1150,24 @@ hc_batch_solve(hc_batch_t bt, int mode, hc_solve_stats * stats, void * stream)
   all.solve_ms = ms;
   return status_rc(all, "hc_batch_solve");
 }
+catch (...)
+{
+  return hc::fail_exception();
+}
 
 int
 hc_batch_destroy(hc_batch_t bt)
+try
 {
   if (!bt)
     return hc::fail(HC_EINVAL, "hc_batch_destroy: null handle");
   (void)hipDeviceSynchronize();
   delete bt;
   return HC_OK;
+}
+catch (...)
+{
+  return hc::fail_exception();
 }
 
 } // extern "C"

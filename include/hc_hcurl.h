@@ -59,7 +59,7 @@ extern "C" {
 
 #define HC_OK 0
 #define HC_EINVAL (-1)   /* bad argument */
-#define HC_ENOMEM (-2)   /* device allocation failed */
+#define HC_ENOMEM (-2)   /* device (or host) allocation failed */
 #define HC_EHIP (-3)     /* HIP runtime error */
 #define HC_ENOTCONV (-4) /* solver did not reach the tolerance in maxit */
 #define HC_ENAN (-5)     /* NaN/Inf met in a reduction */
